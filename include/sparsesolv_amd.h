@@ -1,0 +1,178 @@
+/*
+ * sparsesolv_amd.h -- C ABI of the MI355X (gfx950) ABMC-IC(0) Krylov path.
+ *
+ * The library (libsparsesolv_amd.so) replaces, for one GPU, the hot path of
+ * ksugahar/ngsolve-sparsesolv: ICPreconditioner on its ABMC branch,
+ * CGSolver / COCRSolver driven by it, and the solve_iccg / solve_cocr entry
+ * points.  Each function below names the reference interface it stands in
+ * for (paths relative to that repository).  Plain pointers and sizes only;
+ * every function returns SSA_OK or a negative status, with the message in
+ * ssa_last_error().  Exceptions never cross this boundary.
+ *
+ * Conventions
+ *   - indices are int32 (reference core/types.hpp:21), CSR rows sorted by
+ *     column, column indices unique within a row, diagonal stored;
+ *   - dtype SSA_F64 = double, SSA_C128 = complex<double> as (re, im) pairs;
+ *   - vectors passed to apply / solve are in the caller's (original)
+ *     ordering; `on_device` = 0 for host memory, 1 for device memory of the
+ *     handle's GPU;
+ *   - a handle is not re-entrant (one HIP stream per context), like the
+ *     reference's preconditioner whose apply writes mutable work buffers
+ *     (preconditioners/ic_preconditioner.hpp:556-563).
+ */
+#ifndef SPARSESOLV_AMD_H
+#define SPARSESOLV_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSA_OK 0
+#define SSA_ERR_ARGUMENT (-1)    /* bad argument (reference: invalid_argument / type_error) */
+#define SSA_ERR_RUNTIME (-2)     /* reference std::runtime_error cases (missing diagonal, ...) */
+#define SSA_ERR_DEVICE (-3)      /* HIP runtime failure */
+#define SSA_ERR_UNSUPPORTED (-4) /* option not implemented on this path */
+
+#define SSA_F64 0
+#define SSA_C128 1
+
+#define SSA_PRECISION_FP64 0   /* factor, sweeps and vectors in double / complex<double> */
+#define SSA_PRECISION_MIXED 1  /* factor and sweeps in float / complex<float>, outer FP64 */
+
+#define SSA_METHOD_ICCG 0        /* CG / COCG preconditioned by IC(0) */
+#define SSA_METHOD_ICCOCR 1      /* COCR preconditioned by IC(0) */
+#define SSA_METHOD_CG 2          /* CG without preconditioner */
+#define SSA_METHOD_COCR 3        /* COCR without preconditioner */
+#define SSA_METHOD_NGS_ICCOCR 4  /* COCRSolverNGS loop with the IC preconditioner */
+
+typedef struct ssa_context ssa_context;
+typedef struct ssa_ic ssa_ic;
+
+/* sparsesolv::SolverConfig (core/solver_config.hpp:33-159) plus GPU fields.
+ * Defaults: ssa_config_default. */
+typedef struct ssa_config {
+    double tolerance;                 /* 1e-10 */
+    int max_iterations;               /* 1000 */
+    double abs_tolerance;             /* 1e-12 */
+    int norm_type;                    /* 0 RHS, 1 initial residual, 2 custom */
+    double custom_norm;               /* 1.0 */
+    double shift_parameter;           /* 1.05 */
+    int diagonal_scaling;             /* 0 (unsupported on this path: error) */
+    int auto_shift;                   /* 0 */
+    double shift_increment;           /* 0.05 */
+    double max_shift_value;           /* 5.0 */
+    double min_diagonal_threshold;    /* 1e-6 */
+    double zero_diagonal_replacement; /* 1e-10 */
+    int max_shift_trials;             /* 100 */
+    int divergence_check;             /* 0 none, 1 stagnation count */
+    double divergence_threshold;      /* 1000.0 */
+    int divergence_count;             /* 100 */
+    int save_best_result;             /* 1 */
+    int save_residual_history;        /* 0 */
+    int conjugate;                    /* 0 */
+    int use_abmc;                     /* 0 (this path requires 1) */
+    int abmc_block_size;              /* 4 */
+    int abmc_num_colors;              /* 4 */
+    int abmc_reorder_spmv;            /* 0 (GPU Krylov always runs in ABMC order) */
+    int abmc_use_rcm;                 /* 0 (unsupported: error) */
+    /* GPU-only fields */
+    int precision;                    /* SSA_PRECISION_FP64 */
+    int use_graph;                    /* 1: replay a captured iteration */
+    int poll_interval;                /* iterations between host status polls (0: auto) */
+} ssa_config;
+
+/* sparsesolv::SolverResult (core/types.hpp:29-37) plus timing. */
+typedef struct ssa_result {
+    int converged;
+    int iterations;
+    double final_residual;
+    int history_len;
+    int status;               /* 1 converged, 2 breakdown, 3 diverged, 4 max iterations, 5 rho breakdown */
+    double actual_shift;
+    double solve_seconds;     /* wall time of the solve (device inputs resident) */
+} ssa_result;
+
+typedef struct ssa_ic_info {
+    int n;
+    int nnz;
+    int nnz_l;
+    int num_colors;
+    int num_blocks;
+    int rows_per_lane;        /* regular blocks per lane walk (power of two) */
+    int big_blocks;           /* oversized blocks scheduled by levels */
+    int max_big_rows;
+    int max_levels;
+    int shift_trials;
+    double actual_shift;
+    double setup_seconds;
+    size_t device_bytes;
+    size_t factor_bytes;      /* bytes of one sweep's factor stream (values+indices) */
+    size_t spmv_bytes;        /* bytes of the SpMV operator stream (values+indices) */
+} ssa_ic_info;
+
+void ssa_config_default(ssa_config* cfg);
+const char* ssa_last_error(void);
+
+/* One context per GPU: device selection + a HIP stream. */
+int ssa_context_create(int device, ssa_context** out);
+int ssa_context_destroy(ssa_context* ctx);
+
+/* ICPreconditioner<Scalar>::setup (preconditioners/ic_preconditioner.hpp:69-201):
+ * ABMC schedule, reordering, IC(0) with auto-shift, transpose -- on the host,
+ * then the operator and the factor are uploaded once.  Keeps A (in ABMC
+ * order) for solves.  With ctx == NULL the handle is host-only: the setup runs
+ * and the schedule / factor queries work, nothing touches a GPU, and apply /
+ * spmv / solve return SSA_ERR_ARGUMENT. */
+int ssa_ic_create(ssa_context* ctx, int dtype, int n, const int32_t* row_ptr,
+                  const int32_t* col_idx, const void* values, const ssa_config* cfg,
+                  ssa_ic** out);
+int ssa_ic_destroy(ssa_ic* ic);
+int ssa_ic_info_get(const ssa_ic* ic, ssa_ic_info* out);
+
+/* ICPreconditioner::abmc_ordering (ic_preconditioner.hpp:260-262): ordering[old] = new. */
+int ssa_ic_ordering(const ssa_ic* ic, int32_t* ordering);
+/* ABMCSchedule::color_offsets (nc+1) and block_offsets (nb+1) (core/abmc_ordering.hpp:47-53). */
+int ssa_ic_schedule(const ssa_ic* ic, int32_t* color_offsets, int32_t* block_offsets);
+/* The factor L_ (diagonal last per row) and inv_diag_ (ic_preconditioner.hpp:541-543),
+ * as computed on the host before upload, in ABMC order: row_ptr[n+1], col_idx[nnz_l],
+ * values[nnz_l], inv_diag[n] (scalars of the handle's dtype). */
+int ssa_ic_factor(const ssa_ic* ic, int32_t* row_ptr, int32_t* col_idx, void* values,
+                  void* inv_diag);
+
+/* ICPreconditioner::apply (ic_preconditioner.hpp:215-225): y = M^{-1} x, run on the GPU. */
+int ssa_ic_apply(ssa_ic* ic, const void* x, void* y, int on_device);
+/* SparseMatrixView::multiply (core/sparse_matrix_view.hpp:139-147): y = A x on the GPU. */
+int ssa_spmv(ssa_ic* ic, const void* x, void* y, int on_device);
+
+/* solve_iccg / solve_cocr (sparsesolv.hpp:43-105, :153-169), CGSolver / COCRSolver
+ * without a preconditioner (ngsolve/sparsesolv_solvers.hpp:116-123), and the
+ * COCRSolverNGS loop (ngsolve/sparsesolv_solvers.hpp:273-369) with this IC
+ * preconditioner.  x holds the initial guess on entry (zeroed for
+ * SSA_METHOD_NGS_ICCOCR) and the solution on exit.  `history` receives
+ * result->history_len residuals when cfg->save_residual_history (may be NULL;
+ * needs max_iterations+2 entries). */
+int ssa_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device,
+              const ssa_config* cfg, ssa_result* result, double* history);
+
+/* Kernel timing with HIP events on the handle's stream (no reference
+ * counterpart: measurement hook for the roofline report).  what:
+ *   SSA_PROFILE_SPMV    one SpMV + fused dot over the Krylov vectors;
+ *   SSA_PROFILE_FORWARD the forward sweep (all colour phases);
+ *   SSA_PROFILE_BACKWARD the backward sweep (all colour phases);
+ *   SSA_PROFILE_APPLY   the whole preconditioner apply.
+ * avg_us = mean wall time of one call over `reps` calls; bytes = algorithmic
+ * bytes one call must move (every operand once). */
+#define SSA_PROFILE_SPMV 0
+#define SSA_PROFILE_FORWARD 1
+#define SSA_PROFILE_BACKWARD 2
+#define SSA_PROFILE_APPLY 3
+int ssa_ic_profile(ssa_ic* ic, int what, int reps, double* avg_us, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPARSESOLV_AMD_H */

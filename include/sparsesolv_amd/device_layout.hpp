@@ -1,0 +1,270 @@
+// device_layout.hpp -- HBM layouts of the operator and of the IC factor
+// (host C++17, no device code).
+//
+// Everything lives in ABMC order: the Krylov vectors, the operator A and
+// both triangular factors.  Two layouts are built here.
+//
+// 1. SpmvLayout: SELL-64.  Rows are taken 64 at a time (one wavefront);
+//    slice s stores K_s = max row length entry slots, slot-major, so lane l
+//    of slot k reads val[(off_s + k)*64 + l] -- every load of the wavefront
+//    is one contiguous 512 B (f64) / 1 KiB (c128) segment.  Short rows are
+//    padded with (value 0, column = own row).  Entries keep the CSR column
+//    order of each row.
+//
+// 2. SweepLayout (one per direction).  The ABMC schedule is
+//    colour -> blocks -> rows; blocks of one colour are independent, rows of
+//    a block are sequential.  A regular block (at most BR rows) is owned by
+//    one lane, which walks its rows in processing order (ascending for the
+//    forward sweep L, descending for the backward sweep L^T).  Blocks of a
+//    colour are sorted by their per-row entry counts and packed 64 to a
+//    slice, so padding is small; per slice and row position r there are K_r
+//    entry slots, laid out like SpmvLayout.  A column c >= 0 is a finished
+//    row of an earlier phase (gathered from HBM); c < 0 refers to row
+//    position -1-c of the same block (held in registers).
+//    The ABMC aggregation can leave one oversized last block (the rows left
+//    unassigned once ceil(n/B) blocks exist).  Such a block ("big block")
+//    is scheduled by its own dependency levels: rows grouped level by level,
+//    entries as CSR, internal references encoded -1-p with p the
+//    dependency's position in that level order.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+#include "abmc_ordering.hpp"
+#include "ic_factor.hpp"
+
+namespace sparsesolv_amd {
+
+constexpr int kWave = 64;
+
+template <typename F>
+struct SpmvLayout {
+    index_t n = 0, nslices = 0;
+    std::vector<index_t> slice_off;   // nslices+1, in entry slots
+    std::vector<F> val;               // slots*64
+    std::vector<index_t> col;
+
+    template <typename T>
+    void build(const CsrHost<T>& A)
+    {
+        n = A.n;
+        nslices = (n + kWave - 1) / kWave;
+        slice_off.assign(size_t(nslices) + 1, 0);
+        for (index_t s = 0; s < nslices; ++s) {
+            index_t K = 0;
+            for (index_t r = s * kWave; r < std::min<index_t>(n, (s + 1) * kWave); ++r)
+                K = std::max(K, A.rp[r + 1] - A.rp[r]);
+            slice_off[s + 1] = slice_off[s] + K;
+        }
+        const size_t slots = size_t(slice_off[nslices]) * kWave;
+        val.assign(slots, F(0));
+        col.assign(slots, 0);
+        for (index_t s = 0; s < nslices; ++s) {
+            const index_t K = slice_off[s + 1] - slice_off[s];
+            for (int l = 0; l < kWave; ++l) {
+                const index_t r = s * kWave + l;
+                for (index_t k = 0; k < K; ++k) {
+                    const size_t at = (size_t(slice_off[s]) + k) * kWave + l;
+                    if (r < n && k < A.rp[r + 1] - A.rp[r]) {
+                        val[at] = static_cast<F>(A.v[A.rp[r] + k]);
+                        col[at] = A.ci[A.rp[r] + k];
+                    } else {
+                        col[at] = r < n ? r : 0;
+                    }
+                }
+            }
+        }
+    }
+};
+
+// One triangular sweep (forward over L or backward over L^T).
+template <typename F>
+struct SweepLayout {
+    int BR = 1;                         // rows per lane (power of two)
+    index_t nphases = 0;                // colours, in processing order
+    std::vector<index_t> phase_slices;  // nphases+1: slice ranges
+    std::vector<index_t> lane_row0;     // nslices*64: first row (-1: idle lane)
+    std::vector<index_t> lane_nrows;    // nslices*64
+    std::vector<index_t> slice_slot;    // nslices+1: entry-slot offsets
+    std::vector<std::uint8_t> slice_k;  // nslices*BR: slots per row position
+    std::vector<F> val;                 // slots*64
+    std::vector<index_t> col;
+
+    // big blocks (level scheduled), each owned by one workgroup
+    std::vector<index_t> phase_big;     // nphases+1: big-block ranges
+    std::vector<index_t> big_pos;       // nbig+1: offsets into big_rows
+    std::vector<index_t> big_lvl;       // level offsets, concatenated
+    std::vector<index_t> big_lvl_off;   // nbig+1: offsets into big_lvl
+    std::vector<index_t> big_rows;      // rows in level order
+    std::vector<index_t> big_rp;        // per big row (global position), CSR
+    std::vector<index_t> big_ci;
+    std::vector<F> big_v;
+    index_t max_big_rows = 0;
+    index_t max_levels = 0;
+
+    index_t nslices() const { return phase_slices.empty() ? 0 : phase_slices.back(); }
+
+    // forward: rows of L below the diagonal; backward: rows of U = L^T above.
+    template <typename T>
+    void build(const AbmcSchedule& S, const CsrHost<T>& M, bool forward, int block_size)
+    {
+        const index_t nc = S.num_colors();
+        const index_t nb = S.num_blocks();
+        index_t maxreg = 1;
+        // only the last block can exceed block_size (aggregation stragglers)
+        for (index_t b = 0; b < nb; ++b) {
+            const index_t sz = S.block_offsets[b + 1] - S.block_offsets[b];
+            if (sz <= block_size) maxreg = std::max(maxreg, sz);
+        }
+        BR = 1;
+        while (BR < maxreg) BR *= 2;
+        if (BR > 16) BR = 0;            // wide blocks: everything level scheduled
+        nphases = nc;
+        phase_slices.assign(size_t(nc) + 1, 0);
+        phase_big.assign(size_t(nc) + 1, 0);
+        big_pos.assign(1, 0);
+        big_lvl_off.assign(1, 0);
+        big_lvl.clear(); big_rows.clear(); big_rp.assign(1, 0); big_ci.clear(); big_v.clear();
+        lane_row0.clear(); lane_nrows.clear(); slice_slot.assign(1, 0); slice_k.clear();
+        val.clear(); col.clear();
+        max_big_rows = 0; max_levels = 0;
+
+        // strictly-off-diagonal entries of row i in sweep direction
+        auto row_span = [&](index_t i, index_t& a, index_t& e) {
+            if (forward) { a = M.rp[i]; e = M.rp[i + 1] - 1; }      // diag last
+            else { a = M.rp[i] + 1; e = M.rp[i + 1]; }               // diag first
+        };
+        for (index_t p = 0; p < nc; ++p) {
+            const index_t c = forward ? p : nc - 1 - p;
+            std::vector<index_t> regular, big;
+            for (index_t b = S.color_offsets[c]; b < S.color_offsets[c + 1]; ++b) {
+                const index_t sz = S.block_offsets[b + 1] - S.block_offsets[b];
+                (BR > 0 && sz <= BR ? regular : big).push_back(b);
+            }
+            // per-block entry counts by row position
+            const int br = BR > 0 ? BR : 1;
+            auto count = [&](index_t b, int r) -> index_t {
+                const index_t b0 = S.block_offsets[b], b1 = S.block_offsets[b + 1];
+                if (r >= b1 - b0) return 0;
+                const index_t i = forward ? b0 + r : b1 - 1 - r;
+                index_t a, e;
+                row_span(i, a, e);
+                return e - a;
+            };
+            std::stable_sort(regular.begin(), regular.end(), [&](index_t x, index_t y) {
+                for (int r = 0; r < br; ++r) {
+                    const index_t cx = count(x, r), cy = count(y, r);
+                    if (cx != cy) return cx > cy;
+                }
+                return false;
+            });
+            const index_t nsl = (index_t(regular.size()) + kWave - 1) / kWave;
+            for (index_t s = 0; s < nsl; ++s) {
+                const index_t lo = s * kWave;
+                const index_t hi = std::min<index_t>(index_t(regular.size()), lo + kWave);
+                std::vector<index_t> K(br, 0);
+                for (index_t q = lo; q < hi; ++q)
+                    for (int r = 0; r < br; ++r) K[r] = std::max(K[r], count(regular[q], r));
+                index_t total = 0;
+                for (int r = 0; r < br; ++r) {
+                    if (K[r] > 255) throw std::runtime_error("sweep layout: row too long");
+                    slice_k.push_back(std::uint8_t(K[r]));
+                    total += K[r];
+                }
+                const size_t base = size_t(slice_slot.back()) * kWave;
+                slice_slot.push_back(slice_slot.back() + total);
+                val.resize(base + size_t(total) * kWave, F(0));
+                col.resize(base + size_t(total) * kWave, -1);
+                for (int l = 0; l < kWave; ++l) {
+                    const index_t q = lo + l;
+                    if (q >= hi) { lane_row0.push_back(-1); lane_nrows.push_back(0); continue; }
+                    const index_t b = regular[q];
+                    const index_t b0 = S.block_offsets[b], b1 = S.block_offsets[b + 1];
+                    lane_row0.push_back(forward ? b0 : b1 - 1);
+                    lane_nrows.push_back(b1 - b0);
+                    index_t slot = 0;
+                    for (int r = 0; r < br; ++r) {
+                        if (r < b1 - b0) {
+                            const index_t i = forward ? b0 + r : b1 - 1 - r;
+                            index_t a, e;
+                            row_span(i, a, e);
+                            for (index_t k = a; k < e; ++k) {
+                                const index_t cc = M.ci[k];
+                                const bool internal = cc >= b0 && cc < b1;
+                                const index_t pos = forward ? cc - b0 : b1 - 1 - cc;
+                                const size_t at = base + (size_t(slot) + (k - a)) * kWave + l;
+                                val[at] = static_cast<F>(M.v[k]);
+                                col[at] = internal ? -1 - pos : cc;
+                            }
+                        }
+                        slot += K[r];
+                    }
+                }
+            }
+            phase_slices[p + 1] = phase_slices[p] + nsl;
+
+            // level-scheduled blocks
+            for (index_t b : big) {
+                const index_t b0 = S.block_offsets[b], b1 = S.block_offsets[b + 1];
+                const index_t sz = b1 - b0;
+                std::vector<index_t> level(sz, 0);
+                index_t nlev = 0;
+                for (index_t q = 0; q < sz; ++q) {
+                    const index_t i = forward ? b0 + q : b1 - 1 - q;
+                    index_t a, e;
+                    row_span(i, a, e);
+                    index_t lv = 0;
+                    for (index_t k = a; k < e; ++k) {
+                        const index_t cc = M.ci[k];
+                        if (cc >= b0 && cc < b1) {
+                            const index_t dq = forward ? cc - b0 : b1 - 1 - cc;
+                            lv = std::max(lv, level[dq] + 1);
+                        }
+                    }
+                    level[q] = lv;
+                    nlev = std::max(nlev, lv + 1);
+                }
+                // counting sort of positions by level (stable in q)
+                std::vector<index_t> loff(size_t(nlev) + 1, 0);
+                for (index_t q = 0; q < sz; ++q) loff[level[q] + 1]++;
+                for (index_t v = 0; v < nlev; ++v) loff[v + 1] += loff[v];
+                std::vector<index_t> at(loff.begin(), loff.end() - 1), posof(sz);
+                std::vector<index_t> order(sz);
+                for (index_t q = 0; q < sz; ++q) { posof[q] = at[level[q]]++; order[posof[q]] = q; }
+                for (index_t v = 0; v <= nlev; ++v) big_lvl.push_back(loff[v]);
+                big_lvl_off.push_back(index_t(big_lvl.size()));
+                for (index_t p2 = 0; p2 < sz; ++p2) {
+                    const index_t q = order[p2];
+                    const index_t i = forward ? b0 + q : b1 - 1 - q;
+                    big_rows.push_back(i);
+                    index_t a, e;
+                    row_span(i, a, e);
+                    for (index_t k = a; k < e; ++k) {
+                        const index_t cc = M.ci[k];
+                        const bool internal = cc >= b0 && cc < b1;
+                        const index_t dq = forward ? cc - b0 : b1 - 1 - cc;
+                        big_ci.push_back(internal ? -1 - posof[dq] : cc);
+                        big_v.push_back(static_cast<F>(M.v[k]));
+                    }
+                    big_rp.push_back(index_t(big_ci.size()));
+                }
+                big_pos.push_back(index_t(big_rows.size()));
+                max_big_rows = std::max(max_big_rows, sz);
+                max_levels = std::max(max_levels, nlev);
+            }
+            phase_big[p + 1] = phase_big[p] + index_t(big.size());
+        }
+        if (val.empty()) { val.assign(kWave, F(0)); col.assign(kWave, -1); }
+        if (big_ci.empty()) { big_ci.push_back(0); big_v.push_back(F(0)); }
+        if (big_rows.empty()) big_rows.push_back(0);
+        if (big_lvl.empty()) big_lvl.push_back(0);
+        if (lane_row0.empty()) { lane_row0.assign(kWave, -1); lane_nrows.assign(kWave, 0); }
+        if (slice_k.empty()) slice_k.assign(size_t(BR > 0 ? BR : 1), 0);
+    }
+};
+
+}  // namespace sparsesolv_amd

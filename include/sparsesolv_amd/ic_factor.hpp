@@ -1,0 +1,257 @@
+// ic_factor.hpp -- host side of the IC(0) preconditioner setup (C++17).
+//
+// Follows ICPreconditioner::setup on its ABMC branch
+// (preconditioners/ic_preconditioner.hpp:69-201):
+//   reorder A by the ABMC permutation, rows sorted by new column (:925-989);
+//   keep the lower triangle with the diagonal last (:1075-1115);
+//   factor in place, L_ij -= sum_{k<j} L_ik L_jk / d_k and
+//   d_i = shift*a_ii - sum_{k<i} L_ik^2 / d_k, unconjugated for complex
+//   (:1287-1417), with the auto-shift restart (shift += inc, inc *= 2 after
+//   a colour in which some |d_i| < min_diagonal_threshold with a_ii != 0)
+//   and the pivot clamp |d_i| <= zero_diagonal_replacement -> +replacement;
+//   transpose L into L^T (core/sparse_matrix_csr.hpp:47-72).
+// The level-schedule branch (use_abmc = false) factors the matrix in its
+// own ordering and restarts at the failing row (:1161-1273).
+//
+// The arithmetic is the reference's, term for term and in the same order,
+// so a real factor is bit-identical when both are compiled without fused
+// multiply-add (this header is compiled with -ffp-contract=off).  The search
+// for L(j,k) walks row j against a scatter of row i instead of the
+// reference's linear search; it visits the same k in the same order.
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "abmc_ordering.hpp"
+
+namespace sparsesolv_amd {
+
+// POD mirror of sparsesolv::SolverConfig (core/solver_config.hpp:33-159);
+// the C-ABI struct ssa_config has the same layout.
+struct IcSettings {
+    double shift = 1.05;
+    bool diagonal_scaling = false;
+    bool auto_shift = false;
+    double shift_increment = 0.05;
+    double max_shift_value = 5.0;
+    double min_diagonal_threshold = 1e-6;
+    double zero_diagonal_replacement = 1e-10;
+    int max_shift_trials = 100;
+    bool use_abmc = true;
+    int abmc_block_size = 4;
+    int abmc_num_colors = 4;
+};
+
+template <typename T>
+inline double abs_of(const T& v) { return std::abs(v); }
+
+template <typename T>
+struct CsrHost {
+    index_t n = 0;
+    std::vector<index_t> rp, ci;
+    std::vector<T> v;
+};
+
+// B[perm i][perm j] = A[i][j], each row sorted by column (unique columns).
+template <typename T>
+CsrHost<T> permute_csr(index_t n, const index_t* rp, const index_t* ci, const T* v,
+                       const std::vector<index_t>& perm)
+{
+    CsrHost<T> out;
+    out.n = n;
+    out.rp.assign(size_t(n) + 1, 0);
+    for (index_t i = 0; i < n; ++i) out.rp[perm[i] + 1] = rp[i + 1] - rp[i];
+    for (index_t i = 0; i < n; ++i) out.rp[i + 1] += out.rp[i];
+    out.ci.resize(out.rp[n]);
+    out.v.resize(out.rp[n]);
+    std::vector<std::pair<index_t, index_t>> tmp;
+    for (index_t i = 0; i < n; ++i) {
+        const index_t ni = perm[i];
+        tmp.clear();
+        for (index_t k = rp[i]; k < rp[i + 1]; ++k) tmp.emplace_back(perm[ci[k]], k);
+        std::sort(tmp.begin(), tmp.end(),
+                  [](const auto& a, const auto& b) { return a.first < b.first; });
+        index_t pos = out.rp[ni];
+        for (const auto& e : tmp) { out.ci[pos] = e.first; out.v[pos] = v[e.second]; ++pos; }
+    }
+    return out;
+}
+
+template <typename T>
+struct IcFactorHost {
+    index_t n = 0;
+    AbmcSchedule sched;                 // empty on the level path
+    CsrHost<T> L;                       // lower triangle, diagonal last
+    CsrHost<T> U;                       // L^T, diagonal first
+    std::vector<T> inv_diag;
+    std::vector<double> scaling;        // 1/sqrt|a_ii| if diagonal_scaling
+    double actual_shift = 0.0;
+    int shift_trials = 0;
+    CsrHost<T> A_perm;                  // A in ABMC order (Krylov operator)
+
+    void build(index_t n_, const index_t* rp, const index_t* ci, const T* v,
+               const IcSettings& cfg, bool keep_permuted_matrix)
+    {
+        n = n_;
+        if (cfg.use_abmc) {
+            sched.build(rp, ci, n, cfg.abmc_block_size, cfg.abmc_num_colors);
+            CsrHost<T> P = permute_csr(n, rp, ci, v, sched.ordering);
+            extract_lower(P.n, P.rp.data(), P.ci.data(), P.v.data());
+            if (keep_permuted_matrix) A_perm = std::move(P);
+        } else {
+            extract_lower(n, rp, ci, v);
+            if (keep_permuted_matrix) {
+                A_perm.n = n;
+                A_perm.rp.assign(rp, rp + n + 1);
+                A_perm.ci.assign(ci, ci + rp[n]);
+                A_perm.v.assign(v, v + rp[n]);
+            }
+        }
+        const std::vector<T> original = L.v;
+        if (cfg.diagonal_scaling) scaling_factors();
+        factor(cfg, original);
+        transpose();
+    }
+
+private:
+    void extract_lower(index_t nn, const index_t* rp, const index_t* ci, const T* v)
+    {
+        L.n = nn;
+        L.rp.assign(size_t(nn) + 1, 0);
+        for (index_t i = 0; i < nn; ++i) {
+            index_t c = 0;
+            for (index_t k = rp[i]; k < rp[i + 1]; ++k) c += ci[k] <= i;
+            L.rp[i + 1] = L.rp[i] + c;
+        }
+        L.ci.resize(L.rp[nn]);
+        L.v.resize(L.rp[nn]);
+        for (index_t i = 0; i < nn; ++i) {
+            index_t pos = L.rp[i];
+            for (index_t k = rp[i]; k < rp[i + 1]; ++k)
+                if (ci[k] <= i) { L.ci[pos] = ci[k]; L.v[pos] = v[k]; ++pos; }
+        }
+    }
+
+    void scaling_factors()
+    {
+        scaling.assign(n, 1.0);
+        for (index_t i = 0; i < n; ++i) {
+            const index_t d = L.rp[i + 1] - 1;
+            if (d >= L.rp[i] && L.ci[d] == i) {
+                const double a = abs_of(L.v[d]);
+                scaling[i] = a > 0.0 ? 1.0 / std::sqrt(a) : 1.0;
+            }
+        }
+    }
+
+    void scale_into(const std::vector<T>& original)
+    {
+        for (index_t i = 0; i < n; ++i)
+            for (index_t k = L.rp[i]; k < L.rp[i + 1]; ++k)
+                L.v[k] = original[k] * static_cast<T>(scaling[i]) *
+                         static_cast<T>(scaling[L.ci[k]]);
+    }
+
+    void factor(const IcSettings& cfg, const std::vector<T>& original)
+    {
+        inv_diag.assign(n, T(0));
+        double shift = cfg.shift;
+        double inc = cfg.shift_increment;
+        int trials = 0;
+        const bool abmc = sched.built();
+        const index_t nc = abmc ? sched.num_colors() : 1;
+        // where[c] = position of column c in the current row i, or -1
+        std::vector<index_t> where(n, -1);
+        bool restart = true;
+        while (restart) {
+            restart = false;
+            if (trials > 0) L.v = original;
+            if (cfg.diagonal_scaling) scale_into(original);
+            for (index_t c = 0; c < nc && !restart; ++c) {
+                const index_t r0 = abmc ? sched.block_offsets[sched.color_offsets[c]] : 0;
+                const index_t r1 = abmc ? sched.block_offsets[sched.color_offsets[c + 1]] : n;
+                bool breakdown = false;
+                for (index_t i = r0; i < r1; ++i) {
+                    const index_t rs = L.rp[i], re = L.rp[i + 1];
+                    for (index_t q = rs; q < re; ++q) where[L.ci[q]] = q;
+                    for (index_t kk = rs; kk < re - 1; ++kk) {
+                        const index_t j = L.ci[kk];
+                        if (j >= i) break;
+                        T s = L.v[kk];
+                        // terms k < j common to rows i and j, ascending k
+                        for (index_t jj = L.rp[j]; jj < L.rp[j + 1]; ++jj) {
+                            const index_t k = L.ci[jj];
+                            if (k >= j) break;
+                            const index_t ii = where[k];
+                            if (ii >= 0 && ii < kk)
+                                s -= L.v[ii] * L.v[jj] * inv_diag[k];
+                        }
+                        L.v[kk] = s;
+                    }
+                    for (index_t q = rs; q < re; ++q) where[L.ci[q]] = -1;
+                    const index_t dp = re - 1;
+                    if (dp < rs || L.ci[dp] != i)
+                        throw std::runtime_error(
+                            "IC decomposition failed: missing diagonal at row " + std::to_string(i));
+                    const T orig = L.v[dp];
+                    T s = orig * static_cast<T>(shift);
+                    for (index_t kk = rs; kk < dp; ++kk) {
+                        const index_t k = L.ci[kk];
+                        if (k >= i) break;
+                        s -= L.v[kk] * L.v[kk] * inv_diag[k];
+                    }
+                    L.v[dp] = s;
+                    const double abs_s = abs_of(s);
+                    if (cfg.auto_shift && abs_s < cfg.min_diagonal_threshold && abs_of(orig) > 0.0) {
+                        if (abmc) {
+                            breakdown = true;
+                        } else if (shift < cfg.max_shift_value && trials < cfg.max_shift_trials) {
+                            shift += inc; inc *= 2; ++trials;
+                            restart = true;
+                            break;
+                        }
+                    }
+                    if (abs_s > cfg.zero_diagonal_replacement) {
+                        inv_diag[i] = T(1) / s;
+                    } else {
+                        const T safe = static_cast<T>(cfg.zero_diagonal_replacement);
+                        L.v[dp] = safe;
+                        inv_diag[i] = T(1) / safe;
+                    }
+                }
+                if (abmc && breakdown && shift < cfg.max_shift_value &&
+                    trials < cfg.max_shift_trials) {
+                    shift += inc; inc *= 2; ++trials;
+                    restart = true;
+                }
+            }
+        }
+        actual_shift = shift;
+        shift_trials = trials;
+    }
+
+    void transpose()
+    {
+        U.n = n;
+        const index_t nnz = L.rp[n];
+        U.rp.assign(size_t(n) + 1, 0);
+        U.ci.resize(nnz);
+        U.v.resize(nnz);
+        for (index_t k = 0; k < nnz; ++k) U.rp[L.ci[k] + 1]++;
+        for (index_t i = 0; i < n; ++i) U.rp[i + 1] += U.rp[i];
+        std::vector<index_t> fill(U.rp.begin(), U.rp.end() - 1);
+        for (index_t i = 0; i < n; ++i)
+            for (index_t k = L.rp[i]; k < L.rp[i + 1]; ++k) {
+                const index_t p = fill[L.ci[k]]++;
+                U.ci[p] = i;
+                U.v[p] = L.v[k];
+            }
+    }
+};
+
+}  // namespace sparsesolv_amd

@@ -1,0 +1,360 @@
+// kernels.hpp -- gfx950 kernels of the ABMC-IC(0) Krylov path.
+//
+// All vectors are in ABMC order.  Every kernel of the iteration reads the
+// solver state first and returns at once when the solve has stopped, so a
+// captured iteration can be replayed past convergence without effect.
+//
+// Reductions are deterministic: each workgroup writes one partial (wave
+// butterfly + fixed-order sum over the waves) and a control kernel sums the
+// partials in index order.
+#pragma once
+
+#include "device_math.hpp"
+#include "solver_state.hpp"
+
+namespace ssa {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts)
+constexpr int kWavesPerBlock = kBlock / 64;
+
+// ---------------------------------------------------------------------------
+// SpMV over SELL-64 (one wavefront per 64-row slice), optional fused dot
+// DOT: 0 none, 1 sum x_i y_i, 2 sum conj(x_i) y_i
+// ---------------------------------------------------------------------------
+template <typename T, int DOT>
+__global__ __launch_bounds__(kBlock) void k_spmv(
+    const index_t* __restrict__ slice_off, const T* __restrict__ val,
+    const index_t* __restrict__ col, index_t n, index_t nslices,
+    const T* __restrict__ x, T* __restrict__ y, T* __restrict__ partials,
+    const SolverState* __restrict__ st)
+{
+    if (st && st->status) return;
+    __shared__ T red[kWavesPerBlock];
+    const int lane = threadIdx.x & 63;
+    const index_t s = index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    T d = zero<T>();
+    if (s < nslices) {
+        const index_t o0 = slice_off[s], o1 = slice_off[s + 1];
+        T acc = zero<T>();
+#pragma unroll 4
+        for (index_t k = o0; k < o1; ++k) {
+            const size_t at = size_t(k) * 64 + lane;
+            acc = fma_(val[at], x[col[at]], acc);
+        }
+        const index_t row = s * 64 + lane;
+        if (row < n) {
+            y[row] = acc;
+            if (DOT) d = mul(conj_if(x[row], DOT == 2), acc);
+        }
+    }
+    if (DOT) {
+        T t = block_sum(d, red);
+        if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular sweep, regular blocks: one lane owns one ABMC block and walks
+// its rows in processing order.  FWD: out_i = (in_i - sum L_ik out_k) * d_i^-1
+// (L below the diagonal).  BWD: out_i = in_i - d_i^-1 * sum U_ik out_k
+// (U = L^T above the diagonal).  Columns >= 0 are finished rows of earlier
+// phases; columns < 0 are earlier rows of the same block (registers).
+// ---------------------------------------------------------------------------
+template <typename P, int BR>
+__device__ inline P pick(const P (&loc)[BR], int q)
+{
+    P v = loc[0];
+#pragma unroll
+    for (int j = 1; j < BR; ++j)
+        if (q == j) v = loc[j];
+    return v;
+}
+
+template <typename P, typename F, int BR, bool FWD>
+__global__ __launch_bounds__(kBlock) void k_sweep(
+    index_t s0, index_t s1, const index_t* __restrict__ lane_row0,
+    const index_t* __restrict__ lane_nrows, const index_t* __restrict__ slice_slot,
+    const std::uint8_t* __restrict__ slice_k, const F* __restrict__ val,
+    const index_t* __restrict__ col, const F* __restrict__ invd,
+    const P* __restrict__ in, P* out, const SolverState* __restrict__ st)
+{
+    if (st && st->status) return;
+    const int lane = threadIdx.x & 63;
+    const index_t s = s0 + index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    if (s >= s1) return;
+    const index_t row0 = lane_row0[size_t(s) * 64 + lane];
+    const index_t nr = lane_nrows[size_t(s) * 64 + lane];
+    P loc[BR];
+#pragma unroll
+    for (int r = 0; r < BR; ++r) loc[r] = zero<P>();
+    index_t slot = slice_slot[s];
+    const std::uint8_t* kk = slice_k + size_t(s) * BR;
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+        const int K = kk[r];
+        const bool live = r < nr;
+        const index_t row = FWD ? row0 + r : row0 - r;
+        P acc = (FWD && live) ? in[row] : zero<P>();
+        for (int k = 0; k < K; ++k) {
+            const size_t at = (size_t(slot) + k) * 64 + lane;
+            const index_t c = col[at];
+            const P xv = c >= 0 ? out[c] : pick(loc, -1 - c);
+            acc = fms_(cvt<P>(val[at]), xv, acc);
+        }
+        slot += K;
+        if (live) {
+            const P dinv = cvt<P>(invd[row]);
+            const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, in[row]);
+            loc[r] = res;
+            out[row] = res;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular sweep, oversized blocks: one workgroup per block, rows grouped
+// by dependency level; a workgroup barrier separates levels.  Values of the
+// block's own rows are kept in LDS when `use_lds` (indexed by position in
+// level order), otherwise re-read from `out`.
+// ---------------------------------------------------------------------------
+template <typename P, typename F, bool FWD>
+__global__ __launch_bounds__(1024) void k_sweep_big(
+    index_t b0, const index_t* __restrict__ big_pos, const index_t* __restrict__ big_lvl_off,
+    const index_t* __restrict__ big_lvl, const index_t* __restrict__ big_rows,
+    const index_t* __restrict__ big_rp, const index_t* __restrict__ big_ci,
+    const F* __restrict__ big_v, const F* __restrict__ invd, const P* __restrict__ in,
+    P* out, int use_lds, const SolverState* __restrict__ st)
+{
+    if (st && st->status) return;
+    extern __shared__ unsigned char lds_raw[];
+    P* lds = reinterpret_cast<P*>(lds_raw);
+    const index_t bb = b0 + index_t(blockIdx.x);
+    const index_t p0 = big_pos[bb];
+    const index_t* lv = big_lvl + big_lvl_off[bb];
+    const index_t nlev = big_lvl_off[bb + 1] - big_lvl_off[bb] - 1;
+    for (index_t l = 0; l < nlev; ++l) {
+        for (index_t q = lv[l] + index_t(threadIdx.x); q < lv[l + 1]; q += index_t(blockDim.x)) {
+            const index_t gp = p0 + q;
+            const index_t row = big_rows[gp];
+            P acc = FWD ? in[row] : zero<P>();
+            for (index_t k = big_rp[gp]; k < big_rp[gp + 1]; ++k) {
+                const index_t c = big_ci[k];
+                P xv;
+                if (c >= 0) xv = out[c];
+                else if (use_lds) xv = lds[-1 - c];
+                else xv = out[big_rows[p0 + (-1 - c)]];
+                acc = fms_(cvt<P>(big_v[k]), xv, acc);
+            }
+            const P dinv = cvt<P>(invd[row]);
+            const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, in[row]);
+            out[row] = res;
+            if (use_lds) lds[q] = res;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels (grid-stride; one partial per workgroup)
+// ---------------------------------------------------------------------------
+
+// partial sums of conj?(a_i) b_i
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(index_t n, const T* __restrict__ a,
+                                               const T* __restrict__ b, int conj,
+                                               T* __restrict__ partials,
+                                               const SolverState* __restrict__ st)
+{
+    if (st && st->status) return;
+    __shared__ T red[kWavesPerBlock];
+    T d = zero<T>();
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock)
+        d = fma_(conj_if(a[i], conj != 0), b[i], d);
+    T t = block_sum(d, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// partial sums of |a_i|^2 and (optionally) |b_i|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_norms(index_t n, const T* __restrict__ a,
+                                                 const T* __restrict__ b,
+                                                 double* __restrict__ pa,
+                                                 double* __restrict__ pb)
+{
+    __shared__ double red[kWavesPerBlock];
+    double sa = 0.0, sb = 0.0;
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock) {
+        sa += norm2(a[i]);
+        if (b) sb += norm2(b[i]);
+    }
+    double t = block_sum(sa, red);
+    if (threadIdx.x == 0) pa[blockIdx.x] = t;
+    if (b) {
+        t = block_sum(sb, red);
+        if (threadIdx.x == 0) pb[blockIdx.x] = t;
+    }
+}
+
+// r = b - Ap
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_residual(index_t n, const T* __restrict__ b,
+                                                    const T* __restrict__ Ap, T* __restrict__ r)
+{
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock)
+        r[i] = sub(b[i], Ap[i]);
+}
+
+// dst[i] = src[map[i]]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather(index_t n, const index_t* __restrict__ map,
+                                                  const T* __restrict__ src, T* __restrict__ dst)
+{
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock)
+        dst[i] = src[map[i]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy(index_t n, const T* __restrict__ src,
+                                                T* __restrict__ dst,
+                                                const SolverState* __restrict__ st)
+{
+    if (st && st->status) return;
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock)
+        dst[i] = src[i];
+}
+
+// best_x <- x when the last convergence check improved the best residual
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy_if_improved(index_t n, const T* __restrict__ x,
+                                                            T* __restrict__ best,
+                                                            const SolverState* __restrict__ st)
+{
+    if (!st->improved) return;
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock)
+        best[i] = x[i];
+}
+
+// CG: x += alpha p; r -= alpha Ap; partial |r|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__ x,
+                                                     T* __restrict__ r, const T* __restrict__ p,
+                                                     const T* __restrict__ Ap,
+                                                     double* __restrict__ partials,
+                                                     const SolverState* __restrict__ st)
+{
+    if (st->status) return;
+    __shared__ double red[kWavesPerBlock];
+    const T alpha = from_pair<T>(st->alpha);
+    double s = 0.0;
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock) {
+        x[i] = fma_(alpha, p[i], x[i]);
+        const T ri = fms_(alpha, Ap[i], r[i]);
+        r[i] = ri;
+        s += norm2(ri);
+    }
+    double t = block_sum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// CG: p = z + beta p
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const T* __restrict__ z,
+                                                T* __restrict__ p,
+                                                const SolverState* __restrict__ st)
+{
+    if (st->status) return;
+    const T beta = from_pair<T>(st->beta);
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock)
+        p[i] = fma_(beta, p[i], z[i]);
+}
+
+// COCR: x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict__ x,
+                                                       T* __restrict__ r, T* __restrict__ rt,
+                                                       const T* __restrict__ p,
+                                                       const T* __restrict__ q,
+                                                       const T* __restrict__ qt,
+                                                       double* __restrict__ partials,
+                                                       const SolverState* __restrict__ st)
+{
+    if (st->status) return;
+    __shared__ double red[kWavesPerBlock];
+    const T alpha = from_pair<T>(st->alpha);
+    double s = 0.0;
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock) {
+        x[i] = fma_(alpha, p[i], x[i]);
+        const T ri = fms_(alpha, q[i], r[i]);
+        r[i] = ri;
+        rt[i] = fms_(alpha, qt[i], rt[i]);
+        s += norm2(ri);
+    }
+    double t = block_sum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// COCR: p = rt + beta p; q = t + beta q
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restrict__ rt,
+                                                   const T* __restrict__ t, T* __restrict__ p,
+                                                   T* __restrict__ q,
+                                                   const SolverState* __restrict__ st)
+{
+    if (st->status) return;
+    const T beta = from_pair<T>(st->beta);
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock) {
+        p[i] = fma_(beta, p[i], rt[i]);
+        q[i] = fma_(beta, q[i], t[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Control kernels: one workgroup; sum partials in index order, then thread 0
+// runs the scalar logic of the reference iteration.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T sum_partials(const T* __restrict__ p, int np, T* red)
+{
+    T s = zero<T>();
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s = add(s, p[i]);
+    return block_sum(s, red);
+}
+
+// After r = b - Ax: normaliser, initial history entry.
+__global__ __launch_bounds__(kBlock) void k_ctrl_init(const double* __restrict__ pb,
+                                                      const double* __restrict__ pr, int np,
+                                                      SolverState* st, SolveParams prm,
+                                                      double* __restrict__ hist);
+
+// rz_old (CG) or rho (COCR) from dot partials; stores the value in slot.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ctrl_store(const T* __restrict__ part, int np,
+                                                      SolverState* st, int slot);
+
+// CG alpha = rz / pAp with breakdown handling; COCR alpha = rho / mu.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ctrl_alpha(const T* __restrict__ part, int np,
+                                                      SolverState* st, SolveParams prm,
+                                                      double* __restrict__ hist);
+
+// convergence check after the residual update
+__global__ __launch_bounds__(kBlock) void k_ctrl_check(const double* __restrict__ part, int np,
+                                                       SolverState* st, SolveParams prm,
+                                                       double* __restrict__ hist);
+
+// beta = new / old (CG: rz; COCR: rho with its breakdown test), iter += 1
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ctrl_beta(const T* __restrict__ part, int np,
+                                                     SolverState* st, SolveParams prm);
+
+}  // namespace ssa

@@ -1,0 +1,978 @@
+// solver.hip -- C ABI of libsparsesolv_amd: upload, IC apply, SpMV and the
+// ICCG / COCR drivers on one MI355X.
+//
+// Design (one GPU, one HIP stream per context):
+//  * the host (host_setup.cpp) computes the ABMC schedule and the IC(0)
+//    factor; this file uploads A (ABMC order, SELL-64), the forward and
+//    backward sweep layouts and D^-1 once per ssa_ic_create;
+//  * the Krylov loop runs entirely in ABMC order (b is gathered in, x is
+//    gathered out), so the preconditioner applies no permutation;
+//  * scalars never leave the device: control kernels reduce the
+//    per-workgroup partials in index order and update SolverState; the host
+//    only polls the stop status every few iterations;
+//  * one iteration is captured into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host_plan.hpp"
+#include "kernels.hpp"
+#include "sparsesolv_amd.h"
+
+using ssa::index_t;
+
+// ===========================================================================
+// errors
+// ===========================================================================
+namespace {
+
+thread_local std::string g_last_error;
+
+struct SsaError : std::runtime_error {
+    int code;
+    SsaError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw SsaError(SSA_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename Fn>
+int guarded(Fn&& fn)
+{
+    try {
+        fn();
+        return SSA_OK;
+    } catch (const SsaError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return SSA_ERR_RUNTIME;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SSA_ERR_RUNTIME;
+    }
+}
+
+// owning device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b)
+    {
+        if (b == bytes && p) return;
+        release();
+        if (b == 0) b = 16;
+        HIP_CHECK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <typename V>
+    void upload(const std::vector<V>& v, hipStream_t s)
+    {
+        alloc(v.size() * sizeof(V));
+        if (!v.empty()) HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, s));
+    }
+    template <typename V> V* as() const { return static_cast<V*>(p); }
+};
+
+constexpr int kMaxGrid = 1024;  // vector kernels: partial count bound
+
+inline int vec_grid(index_t n)
+{
+    return int(std::max<index_t>(1, std::min<index_t>((n + ssa::kBlock - 1) / ssa::kBlock, kMaxGrid)));
+}
+
+}  // namespace
+
+struct ssa_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+// ===========================================================================
+// control kernels (declared in kernels.hpp)
+// ===========================================================================
+namespace ssa {
+
+__device__ inline double dabs(double a) { return fabs(a); }
+__device__ inline double dabs(c128 a) { return hypot(a.x, a.y); }
+__device__ inline double ddiv(double a, double b) { return a / b; }
+__device__ inline c128 ddiv(c128 a, c128 b)
+{
+    // Smith's algorithm
+    if (fabs(b.y) <= fabs(b.x)) {
+        const double r = b.y / b.x, d = b.x + b.y * r;
+        return c128{(a.x + a.y * r) / d, (a.y - a.x * r) / d};
+    }
+    const double r = b.x / b.y, d = b.y + b.x * r;
+    return c128{(a.x * r + a.y) / d, (a.y * r - a.x) / d};
+}
+
+// IterativeSolver::check_convergence (solvers/iterative_solver.hpp:217-249);
+// with prm.ngs the COCRSolverNGS test res/rhs_norm <= tol
+// (ngsolve/sparsesolv_solvers.hpp:338-347).
+__device__ inline bool check_convergence(SolverState* s, const SolveParams& prm, double nr,
+                                         double* hist)
+{
+    const double rel = nr / s->normalizer;
+    if (prm.save_history) hist[s->hist_len++] = rel;
+    s->improved = 0;
+    if (prm.save_best && rel < s->best_residual) {
+        s->best_residual = rel;
+        s->improved = 1;
+    }
+    if (prm.ngs) return rel <= prm.tolerance;
+    if (rel < prm.tolerance || nr < prm.abs_tolerance) return true;
+    if (prm.divergence_check) {
+        if (rel >= s->best_residual * prm.divergence_threshold) ++s->bad_count;
+        else s->bad_count = 0;
+    }
+    return false;
+}
+
+__device__ inline void stop(SolverState* s, int status, int converged, int iters, double nr)
+{
+    s->status = status;
+    s->converged = converged;
+    s->result_iters = iters;
+    s->final_norm = nr;
+}
+
+// prepare_iteration (iterative_solver.hpp:135-180)
+__global__ __launch_bounds__(kBlock) void k_ctrl_init(const double* __restrict__ pb,
+                                                      const double* __restrict__ pr, int np,
+                                                      SolverState* st, SolveParams prm,
+                                                      double* __restrict__ hist)
+{
+    __shared__ double red[kWavesPerBlock];
+    const double sb = sum_partials(pb, np, red);
+    const double sr = sum_partials(pr, np, red);
+    if (threadIdx.x != 0) return;
+    const double norm_b = sqrt(sb), init = sqrt(sr);
+    double nz = prm.norm_type == 0 ? norm_b : prm.norm_type == 1 ? init : prm.custom_norm;
+    if (prm.ngs) {
+        // COCRSolverNGS: rhs_norm of the projected rhs, early exit below 1e-30
+        nz = norm_b;
+    } else if (nz < 1e-15) {
+        nz = 1.0;
+    }
+    st->normalizer = nz;
+    st->norm_r = init;
+    st->best_residual = DBL_MAX;
+    st->bad_count = 0;
+    st->improved = 0;
+    st->iter = 0;
+    st->hist_len = 0;
+    st->status = kRunning;
+    st->converged = 0;
+    if (prm.save_history) hist[st->hist_len++] = prm.ngs ? 1.0 : init / nz;
+    if (prm.ngs && norm_b < 1e-30) {
+        stop(st, kConverged, 1, 0, 0.0);
+        st->normalizer = 1.0;
+        return;
+    }
+    if (prm.max_iterations <= 0) stop(st, kMaxIter, 0, prm.max_iterations, init);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ctrl_store(const T* __restrict__ part, int np,
+                                                      SolverState* st, int slot)
+{
+    __shared__ T red[kWavesPerBlock];
+    const T v = sum_partials(part, np, red);
+    if (threadIdx.x != 0) return;
+    double pr[2];
+    to_pair(v, pr);
+    double* dst = slot == 0 ? st->rz : st->denom;
+    dst[0] = pr[0];
+    dst[1] = pr[1];
+}
+
+// CG: alpha = rz / pAp (cg_solver.hpp:100-110); COCR: alpha = rho / mu
+// (cocr_solver.hpp:104-116); breakdown below 1e-30.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ctrl_alpha(const T* __restrict__ part, int np,
+                                                      SolverState* st, SolveParams prm,
+                                                      double* __restrict__ hist)
+{
+    if (st->status) return;
+    __shared__ T red[kWavesPerBlock];
+    const T d = sum_partials(part, np, red);
+    if (threadIdx.x != 0) return;
+    to_pair(d, st->denom);
+    if (dabs(d) < 1e-30) {
+        const double nr = st->norm_r;
+        if (prm.ngs) {
+            stop(st, kBreakdown, 0, st->iter, nr);
+            return;
+        }
+        const bool conv = check_convergence(st, prm, nr, hist);
+        stop(st, kBreakdown, conv ? 1 : 0, conv ? st->iter + 1 : st->iter, nr);
+        return;
+    }
+    const T a = ddiv(from_pair<T>(st->rz), d);
+    to_pair(a, st->alpha);
+}
+
+// after the residual update: ||r||, check_convergence, is_diverging
+__global__ __launch_bounds__(kBlock) void k_ctrl_check(const double* __restrict__ part, int np,
+                                                       SolverState* st, SolveParams prm,
+                                                       double* __restrict__ hist)
+{
+    if (st->status) return;
+    __shared__ double red[kWavesPerBlock];
+    const double s = sum_partials(part, np, red);
+    if (threadIdx.x != 0) return;
+    const double nr = sqrt(s);
+    st->norm_r = nr;
+    if (check_convergence(st, prm, nr, hist)) {
+        stop(st, kConverged, 1, st->iter + 1, nr);
+        return;
+    }
+    if (!prm.ngs && prm.divergence_check && st->bad_count >= prm.divergence_count)
+        stop(st, kDiverged, 0, st->iter + 1, nr);
+}
+
+// CG: beta = rz_new / rz_old (cg_solver.hpp:133-135);
+// COCR: rho breakdown test on the old rho, beta = rho_new / rho
+// (cocr_solver.hpp:149-155).  Then the loop index advances.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ctrl_beta(const T* __restrict__ part, int np,
+                                                     SolverState* st, SolveParams prm)
+{
+    if (st->status) return;
+    __shared__ T red[kWavesPerBlock];
+    const T v = sum_partials(part, np, red);
+    if (threadIdx.x != 0) return;
+    const T old = from_pair<T>(st->rz);
+    if (prm.cocr && dabs(old) < 1e-30) {
+        stop(st, kRhoBreakdown, 0, prm.ngs ? st->iter : st->iter + 1, st->norm_r);
+        return;
+    }
+    to_pair(ddiv(v, old), st->beta);
+    to_pair(v, st->rz);
+    st->iter += 1;
+    if (st->iter >= prm.max_iterations) stop(st, kMaxIter, 0, prm.max_iterations, st->norm_r);
+}
+
+}  // namespace ssa
+
+// ===========================================================================
+// IC handle
+// ===========================================================================
+namespace {
+
+using ssa::SolverState;
+using ssa::SolveParams;
+
+template <typename T> struct HostOf { using type = T; };
+template <> struct HostOf<ssa::c128> { using type = std::complex<double>; };
+template <> struct HostOf<ssa::c64> { using type = std::complex<float>; };
+
+struct DevSweep {
+    DevBuf row0, nrows, slot, k, val, col;
+    DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
+    std::vector<index_t> phase_slices, phase_big;
+    int BR = 1;
+    index_t nphases = 0, max_big_rows = 0, max_levels = 0;
+    size_t stream_bytes = 0;
+
+    template <typename F>
+    void upload(const sparsesolv_amd::SweepLayout<F>& L, hipStream_t s)
+    {
+        row0.upload(L.lane_row0, s);
+        nrows.upload(L.lane_nrows, s);
+        slot.upload(L.slice_slot, s);
+        k.upload(L.slice_k, s);
+        val.upload(L.val, s);
+        col.upload(L.col, s);
+        big_pos.upload(L.big_pos, s);
+        big_lvl_off.upload(L.big_lvl_off, s);
+        big_lvl.upload(L.big_lvl, s);
+        big_rows.upload(L.big_rows, s);
+        big_rp.upload(L.big_rp, s);
+        big_ci.upload(L.big_ci, s);
+        big_v.upload(L.big_v, s);
+        phase_slices = L.phase_slices;
+        phase_big = L.phase_big;
+        BR = L.BR;
+        nphases = L.nphases;
+        max_big_rows = L.max_big_rows;
+        max_levels = L.max_levels;
+        stream_bytes = L.val.size() * sizeof(F) + L.col.size() * 4 + L.big_v.size() * sizeof(F) +
+                       L.big_ci.size() * 4;
+    }
+    size_t bytes() const
+    {
+        return row0.bytes + nrows.bytes + slot.bytes + k.bytes + val.bytes + col.bytes +
+               big_pos.bytes + big_lvl_off.bytes + big_lvl.bytes + big_rows.bytes +
+               big_rp.bytes + big_ci.bytes + big_v.bytes;
+    }
+};
+
+struct IcBase {
+    virtual ~IcBase() = default;
+    virtual void apply(const void* x, void* y, bool on_device) = 0;
+    virtual void spmv(const void* x, void* y, bool on_device) = 0;
+    virtual void solve(int method, const void* b, void* x, bool on_device, const ssa_config& cfg,
+                       ssa_result& res, double* history) = 0;
+    virtual void info(ssa_ic_info& out) const = 0;
+    virtual void ordering(int32_t* out) const = 0;
+    virtual void schedule(int32_t* co, int32_t* bo) const = 0;
+    virtual void factor(int32_t* rp, int32_t* ci, void* vals, void* invd) const = 0;
+    virtual void profile(int what, int reps, double& avg_us, double& bytes) = 0;
+};
+
+template <typename T, typename F>
+struct IcImpl final : IcBase {
+    using HT = typename HostOf<T>::type;
+    using HF = typename HostOf<F>::type;
+    using P = F;  // preconditioner-internal vector type
+
+    ssa_context* ctx;
+    index_t n = 0, nnz = 0, nnz_l = 0, nc = 0, nb = 0;
+    double actual_shift = 0.0, setup_seconds = 0.0;
+    int shift_trials = 0;
+    // host copies kept for queries
+    std::vector<index_t> h_ordering, h_color_offsets, h_block_offsets, h_Lrp, h_Lci;
+    std::vector<HT> h_Lv, h_invd;
+
+    // device: operator, permutations, factor
+    DevBuf A_off, A_val, A_col, perm, rev, invd;
+    index_t A_nslices = 0;
+    size_t spmv_bytes = 0;
+    DevSweep fwd, bwd;
+    // device: work space
+    DevBuf r, z, p, Ap, t, qt, x, best, bvec, tmp, tmp2;
+    DevBuf part_a, part_b, part_c, part_d, state, hist;
+    SolverState* h_state = nullptr;  // pinned
+
+    IcImpl(ssa_context* c, index_t n_, const int32_t* rp, const int32_t* ci, const HT* v,
+           const sparsesolv_amd::IcSettings& st)
+        : ctx(c)
+    {
+        ssa_host::HostPlan<HT, HF> plan;
+        ssa_host::build_plan<HT, HF>(plan, n_, rp, ci, v, st);
+        n = n_;
+        nnz = rp[n_];
+        nnz_l = index_t(plan.fac.L.v.size());
+        nc = plan.fac.sched.num_colors();
+        nb = plan.fac.sched.num_blocks();
+        actual_shift = plan.fac.actual_shift;
+        shift_trials = plan.fac.shift_trials;
+        setup_seconds = plan.seconds;
+        h_ordering = plan.fac.sched.ordering;
+        h_color_offsets = plan.fac.sched.color_offsets;
+        h_block_offsets = plan.fac.sched.block_offsets;
+        h_Lrp = plan.fac.L.rp;
+        h_Lci = plan.fac.L.ci;
+        h_Lv = plan.fac.L.v;
+        h_invd = plan.fac.inv_diag;
+
+        if (!ctx) return;   // host-only handle: schedule and factor queries only
+        hipStream_t s = ctx->stream;
+        A_off.upload(plan.spmv.slice_off, s);
+        A_val.upload(plan.spmv.val, s);
+        A_col.upload(plan.spmv.col, s);
+        A_nslices = plan.spmv.nslices;
+        spmv_bytes = plan.spmv.val.size() * sizeof(HT) + plan.spmv.col.size() * 4;
+        perm.upload(plan.fac.sched.ordering, s);
+        rev.upload(plan.fac.sched.reverse_ordering, s);
+        invd.upload(plan.invd, s);
+        fwd.upload(plan.fwd, s);
+        bwd.upload(plan.bwd, s);
+        HIP_CHECK(hipStreamSynchronize(s));   // host plan is freed on return
+    }
+
+    void need_device() const
+    {
+        if (!ctx)
+            throw SsaError(SSA_ERR_ARGUMENT, "operation needs a handle created with a device context");
+    }
+
+    ~IcImpl() override
+    {
+        if (h_state) (void)hipHostFree(h_state);
+    }
+
+    // ---- building blocks ---------------------------------------------------
+    void ensure_workspace()
+    {
+        need_device();
+        if (!h_state) HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_state), sizeof(SolverState)));
+        const size_t vb = size_t(n) * sizeof(T);
+        for (DevBuf* b : {&r, &z, &p, &Ap, &t, &qt, &x, &best, &bvec, &tmp, &tmp2}) b->alloc(vb);
+        const size_t np = size_t(std::max<index_t>(kMaxGrid, (A_nslices + 3) / 4)) * sizeof(ssa::c128);
+        for (DevBuf* b : {&part_a, &part_b, &part_c, &part_d}) b->alloc(np);
+        state.alloc(sizeof(SolverState));
+    }
+
+    int spmv_grid() const { return int((A_nslices + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock); }
+
+    // y = A x (ABMC order), optional fused dot into part
+    void launch_spmv(const T* xin, T* y, int dot, T* part, const SolverState* st)
+    {
+        const int g = std::max(1, spmv_grid());
+        hipStream_t s = ctx->stream;
+        if (dot == 0)
+            ssa::k_spmv<T, 0><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st);
+        else if (dot == 1)
+            ssa::k_spmv<T, 1><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st);
+        else
+            ssa::k_spmv<T, 2><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    template <bool FWD>
+    void launch_phase(const DevSweep& D, index_t ph, const P* in, P* out, const SolverState* st)
+    {
+        hipStream_t s = ctx->stream;
+        const index_t s0 = D.phase_slices[ph], s1 = D.phase_slices[ph + 1];
+        if (s1 > s0) {
+            const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock);
+#define SSA_SWEEP(BRV)                                                                          \
+    ssa::k_sweep<P, F, BRV, FWD><<<g, ssa::kBlock, 0, s>>>(                                    \
+        s0, s1, D.row0.as<index_t>(), D.nrows.as<index_t>(), D.slot.as<index_t>(),             \
+        D.k.as<std::uint8_t>(), D.val.as<F>(), D.col.as<index_t>(), invd.as<F>(), in, out, st)
+            switch (D.BR) {
+            case 1: SSA_SWEEP(1); break;
+            case 2: SSA_SWEEP(2); break;
+            case 4: SSA_SWEEP(4); break;
+            case 8: SSA_SWEEP(8); break;
+            case 16: SSA_SWEEP(16); break;
+            default: throw SsaError(SSA_ERR_UNSUPPORTED, "sweep: unsupported rows per lane");
+            }
+#undef SSA_SWEEP
+            HIP_CHECK(hipGetLastError());
+        }
+        const index_t b0 = D.phase_big[ph], b1 = D.phase_big[ph + 1];
+        if (b1 > b0) {
+            const size_t lds = size_t(D.max_big_rows) * sizeof(P);
+            const int use_lds = lds <= 64 * 1024 ? 1 : 0;
+            ssa::k_sweep_big<P, F, FWD><<<int(b1 - b0), 1024, use_lds ? lds : 0, s>>>(
+                b0, D.big_pos.as<index_t>(), D.big_lvl_off.as<index_t>(), D.big_lvl.as<index_t>(),
+                D.big_rows.as<index_t>(), D.big_rp.as<index_t>(), D.big_ci.as<index_t>(),
+                D.big_v.as<F>(), invd.as<F>(), in, out, use_lds, st);
+            HIP_CHECK(hipGetLastError());
+        }
+    }
+
+    // out = M^{-1} in (ABMC order); t is the forward result
+    void launch_precond(const P* in, P* out, P* tbuf, const SolverState* st)
+    {
+        for (index_t ph = 0; ph < fwd.nphases; ++ph) launch_phase<true>(fwd, ph, in, tbuf, st);
+        for (index_t ph = 0; ph < bwd.nphases; ++ph) launch_phase<false>(bwd, ph, tbuf, out, st);
+    }
+
+    void launch_gather(const index_t* map, const T* src, T* dst)
+    {
+        ssa::k_gather<T><<<vec_grid(n), ssa::kBlock, 0, ctx->stream>>>(n, map, src, dst);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // ---- public operations -------------------------------------------------
+    void apply(const void* xin, void* yout, bool on_device) override
+    {
+        ensure_workspace();
+        hipStream_t s = ctx->stream;
+        const size_t vb = size_t(n) * sizeof(T);
+        const T* src = static_cast<const T*>(xin);
+        if (!on_device) {
+            HIP_CHECK(hipMemcpyAsync(tmp.p, xin, vb, hipMemcpyHostToDevice, s));
+            src = tmp.as<T>();
+        }
+        launch_gather(rev.as<index_t>(), src, r.as<T>());
+        launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr);
+        T* dst = on_device ? static_cast<T*>(yout) : tmp2.as<T>();
+        launch_gather(perm.as<index_t>(), z.as<T>(), dst);
+        if (!on_device) HIP_CHECK(hipMemcpyAsync(yout, tmp2.p, vb, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    void spmv(const void* xin, void* yout, bool on_device) override
+    {
+        ensure_workspace();
+        hipStream_t s = ctx->stream;
+        const size_t vb = size_t(n) * sizeof(T);
+        const T* src = static_cast<const T*>(xin);
+        if (!on_device) {
+            HIP_CHECK(hipMemcpyAsync(tmp.p, xin, vb, hipMemcpyHostToDevice, s));
+            src = tmp.as<T>();
+        }
+        launch_gather(rev.as<index_t>(), src, r.as<T>());
+        launch_spmv(r.as<T>(), z.as<T>(), 0, nullptr, nullptr);
+        T* dst = on_device ? static_cast<T*>(yout) : tmp2.as<T>();
+        launch_gather(perm.as<index_t>(), z.as<T>(), dst);
+        if (!on_device) HIP_CHECK(hipMemcpyAsync(yout, tmp2.p, vb, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // one Krylov iteration (kernels only; no host synchronisation)
+    void enqueue_iteration(int method, const SolveParams& prm)
+    {
+        hipStream_t s = ctx->stream;
+        SolverState* st = state.as<SolverState>();
+        const int G = vec_grid(n);
+        const int np_spmv = std::max(1, spmv_grid());
+        const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
+                         method == SSA_METHOD_NGS_ICCOCR;
+        double* H = hist.as<double>();
+        if (!prm.cocr) {
+            // CGSolver::do_iterate body (cg_solver.hpp:85-140)
+            launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
+            ssa::k_ctrl_alpha<T><<<1, ssa::kBlock, 0, s>>>(part_a.as<T>(), np_spmv, st, prm, H);
+            ssa::k_cg_update<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st);
+            ssa::k_ctrl_check<<<1, ssa::kBlock, 0, s>>>(part_b.as<double>(), G, st, prm, H);
+            if (prm.save_best)
+                ssa::k_copy_if_improved<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), best.as<T>(), st);
+            if (pre) launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), st);
+            else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st);
+            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), prm.conjugate, part_c.as<T>(), st);
+            ssa::k_ctrl_beta<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, prm);
+            ssa::k_xpby<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), st);
+        } else {
+            // COCRSolver::do_iterate body (cocr_solver.hpp:96-162); z holds rt, Ap holds q
+            if (pre) launch_precond(Ap.as<P>(), qt.as<P>(), t.as<P>(), st);
+            else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st);
+            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, qt.as<T>(), Ap.as<T>(), 0, part_a.as<T>(), st);
+            ssa::k_ctrl_alpha<T><<<1, ssa::kBlock, 0, s>>>(part_a.as<T>(), G, st, prm, H);
+            ssa::k_cocr_update<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), qt.as<T>(), part_b.as<double>(), st);
+            ssa::k_ctrl_check<<<1, ssa::kBlock, 0, s>>>(part_b.as<double>(), G, st, prm, H);
+            if (prm.save_best)
+                ssa::k_copy_if_improved<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), best.as<T>(), st);
+            launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st);
+            ssa::k_ctrl_beta<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), np_spmv, st, prm);
+            ssa::k_cocr_pq<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), st);
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    void solve(int method, const void* bin, void* xio, bool on_device, const ssa_config& cfg,
+               ssa_result& res, double* history) override
+    {
+        ensure_workspace();
+        hipStream_t s = ctx->stream;
+        const size_t vb = size_t(n) * sizeof(T);
+        const int G = vec_grid(n);
+        SolveParams prm{};
+        prm.tolerance = cfg.tolerance;
+        prm.abs_tolerance = cfg.abs_tolerance;
+        prm.custom_norm = cfg.custom_norm;
+        prm.divergence_threshold = cfg.divergence_threshold;
+        prm.max_iterations = cfg.max_iterations;
+        prm.norm_type = cfg.norm_type;
+        prm.save_best = cfg.save_best_result ? 1 : 0;
+        prm.save_history = cfg.save_residual_history ? 1 : 0;
+        prm.divergence_check = cfg.divergence_check ? 1 : 0;
+        prm.divergence_count = cfg.divergence_count;
+        prm.conjugate = cfg.conjugate ? 1 : 0;
+        prm.cocr = (method == SSA_METHOD_ICCOCR || method == SSA_METHOD_COCR ||
+                    method == SSA_METHOD_NGS_ICCOCR) ? 1 : 0;
+        prm.ngs = method == SSA_METHOD_NGS_ICCOCR ? 1 : 0;
+        if (prm.ngs) {
+            prm.save_best = 0;
+            prm.divergence_check = 0;
+        }
+        const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
+                         method == SSA_METHOD_NGS_ICCOCR;
+        hist.alloc(sizeof(double) * size_t(std::max(cfg.max_iterations, 0) + 4));
+        SolverState* st = state.as<SolverState>();
+        double* H = hist.as<double>();
+
+        const auto t0 = std::chrono::steady_clock::now();
+        // inputs into ABMC order: b -> bvec, x0 -> x
+        const T* bsrc = static_cast<const T*>(bin);
+        const T* xsrc = static_cast<const T*>(xio);
+        if (!on_device) {
+            HIP_CHECK(hipMemcpyAsync(tmp.p, bin, vb, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(tmp2.p, xio, vb, hipMemcpyHostToDevice, s));
+            bsrc = tmp.as<T>();
+            xsrc = tmp2.as<T>();
+        }
+        launch_gather(rev.as<index_t>(), bsrc, bvec.as<T>());
+        if (prm.ngs) HIP_CHECK(hipMemsetAsync(x.p, 0, vb, s));
+        else launch_gather(rev.as<index_t>(), xsrc, x.as<T>());
+        HIP_CHECK(hipMemsetAsync(best.p, 0, vb, s));
+        HIP_CHECK(hipMemsetAsync(state.p, 0, sizeof(SolverState), s));
+
+        // prepare_iteration: r = b - A x, norms, normaliser
+        launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
+        ssa::k_residual<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), Ap.as<T>(), r.as<T>());
+        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), r.as<T>(), part_b.as<double>(), part_d.as<double>());
+        ssa::k_ctrl_init<<<1, ssa::kBlock, 0, s>>>(part_b.as<double>(), part_d.as<double>(), G, st, prm, H);
+        HIP_CHECK(hipGetLastError());
+        if (!prm.cocr) {
+            // rz_old = (r, M^{-1} r); p = z (cg_solver.hpp:73-77)
+            if (pre) launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr);
+            else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
+            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), prm.conjugate, part_c.as<T>(), nullptr);
+            ssa::k_ctrl_store<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, 0);
+            ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr);
+        } else {
+            // rt = M^{-1} r; p = rt; q = A p; rho = rt^T q (cocr_solver.hpp:78-88)
+            if (pre) launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr);
+            else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
+            ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr);
+            launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_c.as<T>(), nullptr);
+            ssa::k_ctrl_store<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), std::max(1, spmv_grid()), st, 0);
+        }
+        HIP_CHECK(hipGetLastError());
+
+        // iterations: replay a captured iteration, poll the stop status
+        int poll = cfg.poll_interval > 0 ? cfg.poll_interval : 8;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (cfg.use_graph) {
+            HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            enqueue_iteration(method, prm);
+            HIP_CHECK(hipStreamEndCapture(s, &graph));
+            HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        int done = 0;
+        const int maxit = std::max(cfg.max_iterations, 0);
+        HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        while (h_state->status == ssa::kRunning && done <= maxit) {
+            const int k = std::min(poll, maxit + 1 - done);
+            for (int i = 0; i < k; ++i) {
+                if (exec) HIP_CHECK(hipGraphLaunch(exec, s));
+                else enqueue_iteration(method, prm);
+            }
+            done += k;
+            HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            if (poll < 64) poll *= 2;
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+
+        // build_result (iterative_solver.hpp:262-279)
+        const SolverState S = *h_state;
+        res.converged = S.converged;
+        res.iterations = S.result_iters;
+        res.status = S.status;
+        res.final_residual = S.final_norm / S.normalizer;
+        res.history_len = prm.save_history ? S.hist_len : 0;
+        res.actual_shift = pre ? actual_shift : 0.0;
+        if (!S.converged && prm.save_best) {
+            HIP_CHECK(hipMemcpyAsync(x.p, best.p, vb, hipMemcpyDeviceToDevice, s));
+            res.final_residual = S.best_residual;
+        }
+        T* xdst = on_device ? static_cast<T*>(xio) : tmp2.as<T>();
+        launch_gather(perm.as<index_t>(), x.as<T>(), xdst);
+        if (!on_device) HIP_CHECK(hipMemcpyAsync(xio, tmp2.p, vb, hipMemcpyDeviceToHost, s));
+        if (history && res.history_len > 0)
+            HIP_CHECK(hipMemcpyAsync(history, H, sizeof(double) * res.history_len, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        res.solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void profile(int what, int reps, double& avg_us, double& bytes) override
+    {
+        ensure_workspace();
+        hipStream_t s = ctx->stream;
+        const size_t vb = size_t(n) * sizeof(T);
+        const size_t pb = size_t(n) * sizeof(P);
+        // deterministic non-zero operands
+        std::vector<HT> h(n);
+        for (index_t i = 0; i < n; ++i) h[i] = HT(1.0 / (1.0 + (i % 97)));
+        HIP_CHECK(hipMemcpyAsync(p.p, h.data(), vb, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(r.p, h.data(), vb, hipMemcpyHostToDevice, s));
+        auto run = [&] {
+            switch (what) {
+            case SSA_PROFILE_SPMV: launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_a.as<T>(), nullptr); break;
+            case SSA_PROFILE_FORWARD:
+                for (index_t ph = 0; ph < fwd.nphases; ++ph) launch_phase<true>(fwd, ph, r.as<P>(), t.as<P>(), nullptr);
+                break;
+            case SSA_PROFILE_BACKWARD:
+                for (index_t ph = 0; ph < bwd.nphases; ++ph) launch_phase<false>(bwd, ph, t.as<P>(), z.as<P>(), nullptr);
+                break;
+            default: launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr); break;
+            }
+        };
+        run();   // warm-up
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int k = 0; k < reps; ++k) run();
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        avg_us = 1000.0 * double(ms) / std::max(reps, 1);
+        // algorithmic bytes: matrix entries (value + int32 column) once, plus
+        // row pointers, plus every vector operand once
+        const double rowptr = 4.0 * (double(n) + 1.0);
+        const double lbytes = double(nnz_l) * (sizeof(F) + 4.0) + rowptr;
+        const double abytes = double(nnz) * (sizeof(T) + 4.0) + rowptr;
+        switch (what) {
+        case SSA_PROFILE_SPMV: bytes = abytes + 2.0 * double(vb); break;            // read x, write y
+        case SSA_PROFILE_FORWARD: bytes = lbytes + double(vb) + double(pb) + double(n) * sizeof(F); break;
+        case SSA_PROFILE_BACKWARD: bytes = lbytes + 2.0 * double(pb) + double(n) * sizeof(F); break;
+        default: bytes = 2.0 * lbytes + double(vb) + 3.0 * double(pb) + 2.0 * double(n) * sizeof(F); break;
+        }
+    }
+
+    void info(ssa_ic_info& o) const override
+    {
+        o.n = n;
+        o.nnz = nnz;
+        o.nnz_l = nnz_l;
+        o.num_colors = nc;
+        o.num_blocks = nb;
+        o.rows_per_lane = fwd.BR;
+        o.big_blocks = fwd.phase_big.empty() ? 0 : fwd.phase_big.back();
+        o.max_big_rows = std::max(fwd.max_big_rows, bwd.max_big_rows);
+        o.max_levels = std::max(fwd.max_levels, bwd.max_levels);
+        o.shift_trials = shift_trials;
+        o.actual_shift = actual_shift;
+        o.setup_seconds = setup_seconds;
+        o.device_bytes = A_off.bytes + A_val.bytes + A_col.bytes + perm.bytes + rev.bytes +
+                         invd.bytes + fwd.bytes() + bwd.bytes();
+        o.factor_bytes = fwd.stream_bytes;
+        o.spmv_bytes = spmv_bytes;
+    }
+
+    void ordering(int32_t* out) const override { std::copy(h_ordering.begin(), h_ordering.end(), out); }
+    void schedule(int32_t* co, int32_t* bo) const override
+    {
+        if (co) std::copy(h_color_offsets.begin(), h_color_offsets.end(), co);
+        if (bo) std::copy(h_block_offsets.begin(), h_block_offsets.end(), bo);
+    }
+    void factor(int32_t* rp, int32_t* ci, void* vals, void* iv) const override
+    {
+        if (rp) std::copy(h_Lrp.begin(), h_Lrp.end(), rp);
+        if (ci) std::copy(h_Lci.begin(), h_Lci.end(), ci);
+        if (vals) std::memcpy(vals, h_Lv.data(), h_Lv.size() * sizeof(HT));
+        if (iv) std::memcpy(iv, h_invd.data(), h_invd.size() * sizeof(HT));
+    }
+};
+
+}  // namespace
+
+struct ssa_ic {
+    std::unique_ptr<IcBase> impl;
+    int dtype = 0;
+};
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void ssa_config_default(ssa_config* c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->tolerance = 1e-10;
+    c->max_iterations = 1000;
+    c->abs_tolerance = 1e-12;
+    c->norm_type = 0;
+    c->custom_norm = 1.0;
+    c->shift_parameter = 1.05;
+    c->diagonal_scaling = 0;
+    c->auto_shift = 0;
+    c->shift_increment = 0.05;
+    c->max_shift_value = 5.0;
+    c->min_diagonal_threshold = 1e-6;
+    c->zero_diagonal_replacement = 1e-10;
+    c->max_shift_trials = 100;
+    c->divergence_check = 0;
+    c->divergence_threshold = 1000.0;
+    c->divergence_count = 100;
+    c->save_best_result = 1;
+    c->save_residual_history = 0;
+    c->conjugate = 0;
+    c->use_abmc = 0;
+    c->abmc_block_size = 4;
+    c->abmc_num_colors = 4;
+    c->abmc_reorder_spmv = 0;
+    c->abmc_use_rcm = 0;
+    c->precision = SSA_PRECISION_FP64;
+    c->use_graph = 1;
+    c->poll_interval = 0;
+}
+
+const char* ssa_last_error(void) { return g_last_error.c_str(); }
+
+int ssa_context_create(int device, ssa_context** out)
+{
+    return guarded([&] {
+        if (!out) throw SsaError(SSA_ERR_ARGUMENT, "ssa_context_create: null output");
+        int count = 0;
+        HIP_CHECK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_context_create: no such device " + std::to_string(device));
+        HIP_CHECK(hipSetDevice(device));
+        auto* c = new ssa_context();
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw SsaError(SSA_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+        *out = c;
+    });
+}
+
+int ssa_context_destroy(ssa_context* c)
+{
+    return guarded([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+    });
+}
+
+int ssa_ic_create(ssa_context* ctx, int dtype, int n, const int32_t* row_ptr,
+                  const int32_t* col_idx, const void* values, const ssa_config* cfg, ssa_ic** out)
+{
+    return guarded([&] {
+        if (!out || !cfg) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: null argument");
+        if (n < 0 || (n > 0 && (!row_ptr || !col_idx || !values)))
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: bad matrix");
+        if (dtype != SSA_F64 && dtype != SSA_C128)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: dtype must be SSA_F64 or SSA_C128");
+        if (!cfg->use_abmc)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: the GPU path requires use_abmc=1");
+        if (cfg->abmc_use_rcm)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: abmc_use_rcm is not supported");
+        if (cfg->diagonal_scaling)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: diagonal_scaling is not supported");
+        if (n > 0 && row_ptr[0] != 0) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: row_ptr[0] != 0");
+        for (int i = 0; i < n; ++i)
+            if (row_ptr[i + 1] < row_ptr[i]) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: row_ptr not monotone");
+        const int nnz = n > 0 ? row_ptr[n] : 0;
+        for (int k = 0; k < nnz; ++k)
+            if (col_idx[k] < 0 || col_idx[k] >= n) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: column index out of range");
+        if (ctx) HIP_CHECK(hipSetDevice(ctx->device));
+        sparsesolv_amd::IcSettings st;
+        st.shift = cfg->shift_parameter;
+        st.diagonal_scaling = cfg->diagonal_scaling != 0;
+        st.auto_shift = cfg->auto_shift != 0;
+        st.shift_increment = cfg->shift_increment;
+        st.max_shift_value = cfg->max_shift_value;
+        st.min_diagonal_threshold = cfg->min_diagonal_threshold;
+        st.zero_diagonal_replacement = cfg->zero_diagonal_replacement;
+        st.max_shift_trials = cfg->max_shift_trials;
+        st.use_abmc = cfg->use_abmc != 0;
+        st.abmc_block_size = cfg->abmc_block_size;
+        st.abmc_num_colors = cfg->abmc_num_colors;
+        auto h = std::make_unique<ssa_ic>();
+        h->dtype = dtype;
+        const bool mixed = cfg->precision == SSA_PRECISION_MIXED;
+        if (mixed)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: mixed precision is not available yet");
+        if (dtype == SSA_F64) {
+            h->impl.reset(new IcImpl<double, double>(ctx, n, row_ptr, col_idx, static_cast<const double*>(values), st));
+        } else {
+            using cd = std::complex<double>;
+            h->impl.reset(new IcImpl<ssa::c128, ssa::c128>(ctx, n, row_ptr, col_idx, static_cast<const cd*>(values), st));
+        }
+        *out = h.release();
+    });
+}
+
+int ssa_ic_destroy(ssa_ic* ic)
+{
+    return guarded([&] { delete ic; });
+}
+
+int ssa_ic_info_get(const ssa_ic* ic, ssa_ic_info* out)
+{
+    return guarded([&] {
+        if (!ic || !out) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_info_get: null argument");
+        std::memset(out, 0, sizeof(*out));
+        ic->impl->info(*out);
+    });
+}
+
+int ssa_ic_ordering(const ssa_ic* ic, int32_t* ordering)
+{
+    return guarded([&] {
+        if (!ic || !ordering) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_ordering: null argument");
+        ic->impl->ordering(ordering);
+    });
+}
+
+int ssa_ic_schedule(const ssa_ic* ic, int32_t* color_offsets, int32_t* block_offsets)
+{
+    return guarded([&] {
+        if (!ic) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_schedule: null argument");
+        ic->impl->schedule(color_offsets, block_offsets);
+    });
+}
+
+int ssa_ic_factor(const ssa_ic* ic, int32_t* row_ptr, int32_t* col_idx, void* values, void* inv_diag)
+{
+    return guarded([&] {
+        if (!ic) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_factor: null argument");
+        ic->impl->factor(row_ptr, col_idx, values, inv_diag);
+    });
+}
+
+int ssa_ic_apply(ssa_ic* ic, const void* x, void* y, int on_device)
+{
+    return guarded([&] {
+        if (!ic || !x || !y) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_apply: null argument");
+        ic->impl->apply(x, y, on_device != 0);
+    });
+}
+
+int ssa_spmv(ssa_ic* ic, const void* x, void* y, int on_device)
+{
+    return guarded([&] {
+        if (!ic || !x || !y) throw SsaError(SSA_ERR_ARGUMENT, "ssa_spmv: null argument");
+        ic->impl->spmv(x, y, on_device != 0);
+    });
+}
+
+int ssa_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device, const ssa_config* cfg,
+              ssa_result* result, double* history)
+{
+    return guarded([&] {
+        if (!ic || !b || !x || !cfg || !result) throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve: null argument");
+        if (method < SSA_METHOD_ICCG || method > SSA_METHOD_NGS_ICCOCR)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve: unknown method");
+        std::memset(result, 0, sizeof(*result));
+        ic->impl->solve(method, b, x, on_device != 0, *cfg, *result, history);
+    });
+}
+
+int ssa_ic_profile(ssa_ic* ic, int what, int reps, double* avg_us, double* bytes)
+{
+    return guarded([&] {
+        if (!ic || !avg_us || !bytes) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_profile: null argument");
+        if (what < SSA_PROFILE_SPMV || what > SSA_PROFILE_APPLY)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_profile: unknown kernel");
+        ic->impl->profile(what, reps, *avg_us, *bytes);
+    });
+}
+
+}  // extern "C"

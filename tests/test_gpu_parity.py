@@ -1,0 +1,229 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+reference's golden outputs.
+
+Bars (north star): ABMC arrays and real factors bit-exact (checked on the
+host handle in test_host_setup.py); operator / preconditioner applications
+within 1e-12 relative (FMA and summation order differ from the CPU);
+iteration counts within 2% (at least +-1); solutions within 1e-9 relative of
+the oracle's at tol 1e-10.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from sparsesolv_amd import _lib
+from sparsesolv_amd import problems as P
+
+import make_golden as G
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def it_tol(n_ref):
+    return max(1, math.ceil(0.02 * n_ref))
+
+
+def handle(A, **kw):
+    return _lib.IcHandle(A, _lib.default_config(use_abmc=1, **kw))
+
+
+SYSTEMS = {
+    "lap12": lambda: P.poisson(12),
+    "lap20": lambda: P.poisson(20),
+    "cc8": lambda: P.curlcurl(8),
+    "eddy8": lambda: P.eddy(8),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SYSTEMS))
+def test_spmv_matches_scipy(name):
+    s = SYSTEMS[name]()
+    h = handle(s.A)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(s.A.n)
+    if s.A.is_complex:
+        x = x + 1j * rng.standard_normal(s.A.n)
+    y = h.spmv(x)
+    want = s.A.to_scipy() @ x
+    assert rel(y, want) < 1e-13
+
+
+@pytest.mark.parametrize("name,kw", [("lap12", {}), ("lap20", dict(abmc_block_size=8)),
+                                     ("lap20", dict(abmc_block_size=16, abmc_num_colors=16)),
+                                     ("lap12", dict(abmc_block_size=3, abmc_num_colors=5)),
+                                     ("cc8", dict(auto_shift=1)), ("eddy8", dict(abmc_num_colors=8)),
+                                     ("lap20", dict(abmc_block_size=2))])
+def test_ic_apply_matches_oracle(oracle, name, kw):
+    s = SYSTEMS[name]()
+    h = handle(s.A, **kw)
+    ic = oracle.ic(s.A, oracle.config(use_abmc=1, **kw))
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(s.A.n)
+    if s.A.is_complex:
+        x = x + 1j * rng.standard_normal(s.A.n)
+    assert rel(h.apply(x), ic.apply(x)) < 1e-12
+
+
+def test_ic_apply_with_big_block(oracle):
+    # 48^3 Laplacian, B=4: one 366-row straggler block (27 dependency levels)
+    s = P.poisson(48)
+    h = handle(s.A)
+    info = h.info()
+    assert info["big_blocks"] == 1 and info["max_big_rows"] == 366
+    ic = oracle.ic(s.A, oracle.config(use_abmc=1))
+    x = P.uniform(4, s.A.n, -1, 1)
+    assert rel(h.apply(x), ic.apply(x)) < 1e-12
+
+
+@pytest.mark.parametrize("name", [n for n, c in sorted(G.CASES.items())
+                                  if c[1].get("use_abmc", 0) == 1 or c[2].startswith("plain")])
+def test_solver_matches_reference_golden(name):
+    g = load_golden(name)
+    build, kw, solver = G.CASES[name]
+    s = build()
+    kw = dict(kw)
+    kw["use_abmc"] = 1
+    kw.setdefault("abmc_reorder_spmv", 0)
+    cfg = _lib.default_config(save_residual_history=1, **kw)
+    h = _lib.IcHandle(s.A, cfg)
+    method = {"iccg": _lib.METHOD_ICCG, "cocr": _lib.METHOD_ICCOCR,
+              "plain_cg": _lib.METHOD_CG, "plain_cocr": _lib.METHOD_COCR}[solver]
+    x, res, hist = h.solve(method, s.b, None, cfg)
+    n_ref = int(g["iterations"])
+    assert bool(res.converged) == bool(g["converged"])
+    assert abs(res.iterations - n_ref) <= it_tol(n_ref)
+    if g["converged"]:
+        assert rel(x, g["x"]) < 1e-9
+        np.testing.assert_allclose(hist[:5], g["history"][:5], rtol=1e-9)
+    else:
+        # not converged: best iterate restored, residual of the returned x
+        Ax = s.A.to_scipy() @ x
+        r = np.linalg.norm(s.b - Ax) / np.linalg.norm(s.b)
+        assert abs(r - res.final_residual) <= 1e-8 * max(r, 1e-30) + 1e-14
+        if res.iterations == n_ref:
+            np.testing.assert_allclose(res.final_residual, float(g["final_residual"]), rtol=1e-6)
+
+
+@pytest.mark.parametrize("name,kw,method", [
+    ("lap20", dict(), "iccg"),
+    ("lap20", dict(abmc_block_size=8, abmc_num_colors=6), "iccg"),
+    ("cc8", dict(auto_shift=1, max_iterations=5000), "iccg"),
+    ("eddy8", dict(abmc_num_colors=8, max_iterations=5000), "cocr"),
+    ("eddy8", dict(max_iterations=5000), "iccg"),
+])
+def test_solve_matches_oracle(oracle, name, kw, method):
+    s = SYSTEMS[name]()
+    cfg = _lib.default_config(use_abmc=1, save_residual_history=1, **kw)
+    h = _lib.IcHandle(s.A, cfg)
+    code = _lib.METHOD_ICCG if method == "iccg" else _lib.METHOD_ICCOCR
+    x, res, hist = h.solve(code, s.b, None, cfg)
+    fn = oracle.solve_iccg if method == "iccg" else oracle.solve_cocr
+    xo, ro = fn(s.A, s.b, oracle.config(use_abmc=1, save_residual_history=1, **kw))
+    assert res.converged and ro["converged"]
+    assert abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(x, xo) < 1e-9
+    k = min(len(hist), len(ro["residual_history"]), 10)
+    np.testing.assert_allclose(hist[:k], ro["residual_history"][:k], rtol=1e-8)
+
+
+def test_graph_and_eager_agree_bitwise():
+    s = P.poisson(16)
+    base = dict(use_abmc=1, max_iterations=300)
+    xs = []
+    for use_graph in (1, 0, 1):
+        cfg = _lib.default_config(use_graph=use_graph, **base)
+        h = _lib.IcHandle(s.A, cfg)
+        x, res, _ = h.solve(_lib.METHOD_ICCG, s.b, None, cfg)
+        xs.append((x, res.iterations))
+    for x, it in xs[1:]:
+        assert it == xs[0][1]
+        np.testing.assert_array_equal(x, xs[0][0])
+
+
+def test_best_result_and_history_when_not_converged(oracle):
+    s = P.poisson(10)
+    cfg = _lib.default_config(use_abmc=1, max_iterations=6, save_residual_history=1)
+    h = _lib.IcHandle(s.A, cfg)
+    x, res, hist = h.solve(_lib.METHOD_ICCG, s.b, None, cfg)
+    xo, ro = oracle.solve_iccg(s.A, s.b, oracle.config(use_abmc=1, max_iterations=6,
+                                                       save_residual_history=1))
+    assert not res.converged and res.iterations == 6 == ro["iterations"]
+    assert len(hist) == len(ro["residual_history"]) == 7
+    np.testing.assert_allclose(hist, ro["residual_history"], rtol=1e-9)
+    np.testing.assert_allclose(res.final_residual, ro["final_residual"], rtol=1e-9)
+    assert rel(x, xo) < 1e-9
+
+
+def test_initial_guess_and_zero_rhs():
+    s = P.poisson(8)
+    cfg = _lib.default_config(use_abmc=1)
+    h = _lib.IcHandle(s.A, cfg)
+    xe = np.linalg.solve(s.A.to_scipy().toarray(), s.b)
+    x, res, _ = h.solve(_lib.METHOD_ICCG, s.b, xe, cfg)
+    assert res.converged and res.iterations <= 2
+    x0, r0, _ = h.solve(_lib.METHOD_ICCG, np.zeros(s.A.n), None, cfg)
+    assert r0.converged and np.all(x0 == 0)
+
+
+def test_maxiter_zero():
+    s = P.poisson(6)
+    cfg = _lib.default_config(use_abmc=1, max_iterations=0)
+    h = _lib.IcHandle(s.A, cfg)
+    x, res, _ = h.solve(_lib.METHOD_ICCG, s.b, None, cfg)
+    assert not res.converged and res.iterations == 0
+
+
+def test_python_dropin_iccg_and_cocr(oracle):
+    import sparsesolv_amd as ss
+    s = P.poisson(14)
+    solver = ss.SparseSolvSolver(s.A, method="ICCG", use_abmc=True, save_residual_history=True)
+    x = solver * s.b
+    xo, ro = oracle.solve_iccg(s.A, s.b, oracle.config(use_abmc=1))
+    assert solver.last_result.converged and rel(x, xo) < 1e-9
+    assert abs(solver.last_result.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    e = P.eddy(7)
+    pre = ss.ICPreconditioner(e.A, shift=1.05)
+    pre.use_abmc = True
+    pre.abmc_num_colors = 8
+    pre.Update()
+    inv = ss.COCRSolver(e.A, pre, maxiter=500, tol=1e-8)
+    xc = inv * e.b
+    assert 0 < inv.iterations < 500
+    xd = np.linalg.solve(e.A.to_scipy().toarray(), e.b)
+    assert rel(xc, xd) < 1e-6
+    xo, ro = oracle.solve_cocr(e.A, e.b, oracle.config(use_abmc=1, abmc_num_colors=8,
+                                                       tolerance=1e-8, abs_tolerance=0.0))
+    assert abs(inv.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+
+
+def test_python_dropin_freedofs(oracle):
+    import sparsesolv_amd as ss
+    from sparsesolv_amd.solvers import _with_freedofs
+    s = P.poisson(10)
+    free = np.ones(s.A.n, dtype=bool)
+    free[::7] = False
+    solver = ss.SparseSolvSolver(s.A, method="ICCG", freedofs=free, use_abmc=True)
+    sol = np.full(s.A.n, 3.0)
+    res = solver.Solve(s.b, sol)
+    assert res.converged
+    assert np.all(sol[~free] == 3.0)          # constrained entries untouched
+    M = _with_freedofs(s.A, free)
+    xo, _ = oracle.solve_iccg(M, np.where(free, s.b, 0), oracle.config(use_abmc=1),
+                              x0=np.where(free, 3.0, 0.0))
+    assert rel(sol[free], xo[free]) < 1e-9
+
+
+def test_plain_methods_match_oracle(oracle):
+    s = P.poisson(9)
+    cfg = _lib.default_config(use_abmc=1, max_iterations=500)
+    h = _lib.IcHandle(s.A, cfg)
+    x, res, _ = h.solve(_lib.METHOD_CG, s.b, None, cfg)
+    xo, ro = oracle.solve_plain(s.A, s.b, oracle.config(max_iterations=500), "CG")
+    assert res.converged and abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(x, xo) < 1e-9

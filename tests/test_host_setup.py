@@ -1,0 +1,76 @@
+"""Host half of ssa_ic_create (ABMC schedule + IC(0) factor, host C++)
+against the oracle, through the C ABI with a host-only handle (no GPU)."""
+import numpy as np
+import pytest
+
+from sparsesolv_amd import _lib
+from sparsesolv_amd import problems as P
+
+
+CASES = [
+    ("lap9", lambda: P.laplace7(9), dict()),
+    ("lap12_b8", lambda: P.laplace7(12), dict(abmc_block_size=8, abmc_num_colors=6)),
+    ("lap10_b16", lambda: P.laplace7(10), dict(abmc_block_size=16, abmc_num_colors=16)),
+    ("lap8_b3", lambda: P.laplace7(8), dict(abmc_block_size=3, abmc_num_colors=5)),
+    ("cc6_auto", lambda: P.curlcurl(6).A, dict(auto_shift=1)),
+    ("lap8_auto03", lambda: P.laplace7(8), dict(shift_parameter=0.3, auto_shift=1)),
+    ("eddy6", lambda: P.eddy(6).A, dict(abmc_num_colors=8)),
+    ("eddy5_b2", lambda: P.eddy(5).A, dict(abmc_block_size=2)),
+]
+
+
+@pytest.mark.parametrize("name,builder,kw", CASES, ids=[c[0] for c in CASES])
+def test_host_setup_bit_exact(oracle, name, builder, kw):
+    A = builder()
+    h = _lib.IcHandle(A, _lib.default_config(use_abmc=1, **kw), host_only=True)
+    f = oracle.ic(A, oracle.config(use_abmc=1, **kw)).factor()
+    np.testing.assert_array_equal(h.ordering(), f["ordering"])
+    co, bo = h.schedule()
+    np.testing.assert_array_equal(co, f["color_offsets"])
+    np.testing.assert_array_equal(bo, f["block_offsets"])
+    g = h.factor()
+    np.testing.assert_array_equal(g["L_row_ptr"], f["L_row_ptr"])
+    np.testing.assert_array_equal(g["L_col_idx"], f["L_col_idx"])
+    np.testing.assert_array_equal(g["L_values"], f["L_values"])
+    np.testing.assert_array_equal(g["inv_diag"], f["inv_diag"])
+    assert g["actual_shift"] == f["actual_shift"]
+
+
+def test_host_setup_missing_diagonal():
+    import scipy.sparse as sp
+    from sparsesolv_amd.problems import CSR
+    A = sp.csr_matrix(np.array([[2.0, -1.0, 0.0], [-1.0, 0.0, -1.0], [0.0, -1.0, 2.0]]))
+    A.eliminate_zeros()
+    with pytest.raises(_lib.SsaError, match="missing diagonal") as e:
+        _lib.IcHandle(CSR.from_scipy(A), _lib.default_config(use_abmc=1), host_only=True)
+    assert e.value.code == _lib.SSA_ERR_RUNTIME
+
+
+def test_host_setup_rejects_bad_input():
+    from sparsesolv_amd.problems import CSR
+    A = P.laplace7(3)
+    bad = CSR(A.n, A.row_ptr, A.col_idx.copy(), A.values)
+    bad.col_idx[5] = A.n + 3
+    with pytest.raises(_lib.SsaError) as e:
+        _lib.IcHandle(bad, _lib.default_config(use_abmc=1), host_only=True)
+    assert e.value.code == _lib.SSA_ERR_ARGUMENT
+    with pytest.raises(_lib.SsaError) as e:
+        _lib.IcHandle(A, _lib.default_config(use_abmc=1, abmc_use_rcm=1), host_only=True)
+    assert e.value.code == _lib.SSA_ERR_UNSUPPORTED
+
+
+def test_host_only_handle_refuses_device_work():
+    A = P.laplace7(4)
+    h = _lib.IcHandle(A, _lib.default_config(use_abmc=1), host_only=True)
+    with pytest.raises(_lib.SsaError) as e:
+        h.apply(np.ones(A.n))
+    assert e.value.code == _lib.SSA_ERR_ARGUMENT
+
+
+def test_layout_info_reports_big_block():
+    # 48^3 / B=4 leaves one 366-row straggler block with 27 dependency levels
+    A = P.laplace7(48)
+    h = _lib.IcHandle(A, _lib.default_config(use_abmc=1), host_only=True)
+    co, bo = h.schedule()
+    sizes = np.diff(bo)
+    assert len(co) - 1 == 8 and sizes.max() == 366

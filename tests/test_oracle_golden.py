@@ -1,0 +1,90 @@
+"""The C oracle against the reference's own outputs.
+
+tests/golden/*.npz were written by tests/golden/make_golden.py from the
+reference headers compiled unchanged (oracle/_ref).  The oracle restates the
+serial reference, so integer outputs and real factors must agree bit for bit,
+complex ones too (both use libgcc's complex helpers); solver histories are
+compared exactly as well.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from sparsesolv_amd import problems as P
+
+import make_golden as G
+
+
+@pytest.mark.parametrize("name", sorted(G.CASES))
+def test_oracle_matches_reference_golden(oracle, name):
+    g = load_golden(name)
+    build, kw, solver = G.CASES[name]
+    sysm = build()
+    A, b = sysm.A, sysm.b
+    assert A.n == int(g["n"]) and A.nnz == int(g["nnz"])
+    np.testing.assert_allclose(np.sum(A.values), g["values_sum"][0], rtol=1e-14)
+    cfg = oracle.config(save_residual_history=1, **kw)
+    if solver in ("iccg", "cocr"):
+        f = oracle.ic(A, cfg).factor()
+        for k in ("L_row_ptr", "L_col_idx"):
+            np.testing.assert_array_equal(f[k], g[k])
+        np.testing.assert_array_equal(f["L_values"], g["L_values"])
+        np.testing.assert_array_equal(f["inv_diag"], g["inv_diag"])
+        assert f["actual_shift"] == float(g["actual_shift"])
+        if "ordering" in g:
+            for k in ("ordering", "color_offsets", "block_offsets"):
+                np.testing.assert_array_equal(f[k], g[k])
+        y = oracle.ic(A, cfg).apply(g["apply_x"])
+        np.testing.assert_array_equal(y, g["apply_y"])
+    if solver == "iccg":
+        x, r = oracle.solve_iccg(A, b, cfg)
+    elif solver == "cocr":
+        x, r = oracle.solve_cocr(A, b, cfg)
+    elif solver == "plain_cg":
+        x, r = oracle.solve_plain(A, b, cfg, "CG")
+    else:
+        x, r = oracle.solve_plain(A, b, cfg, "COCR")
+    assert r["iterations"] == int(g["iterations"])
+    assert r["converged"] == bool(g["converged"])
+    np.testing.assert_array_equal(r["residual_history"], g["history"])
+    np.testing.assert_array_equal(x, g["x"])
+    assert r["final_residual"] == float(g["final_residual"])
+
+
+@pytest.mark.parametrize("m,B,nc", [(10, 4, 4), (16, 4, 8), (12, 8, 4), (12, 16, 16), (9, 1, 2), (7, 3, 5)])
+def test_oracle_abmc_matches_reference_build(oracle, ref, m, B, nc):
+    A = P.laplace7(m)
+    a = oracle.abmc(A, B, nc)
+    r = ref.abmc(A, B, nc)
+    for k in a:
+        np.testing.assert_array_equal(a[k], r[k])
+
+
+@pytest.mark.parametrize("m,B,nc", [(6, 4, 4), (8, 4, 8), (7, 8, 4)])
+def test_oracle_abmc_curlcurl_matches_reference_build(oracle, ref, m, B, nc):
+    A = P.curlcurl(m).A
+    a = oracle.abmc(A, B, nc)
+    r = ref.abmc(A, B, nc)
+    for k in a:
+        np.testing.assert_array_equal(a[k], r[k])
+
+
+@pytest.mark.parametrize("case", ["lap", "cc", "eddy"])
+def test_oracle_solvers_match_reference_build(oracle, ref, case):
+    if case == "lap":
+        s = P.poisson(11)
+        kw = dict(use_abmc=1, abmc_block_size=4, abmc_num_colors=4)
+        fn = "solve_iccg"
+    elif case == "cc":
+        s = P.curlcurl(7)
+        kw = dict(use_abmc=1, auto_shift=1, max_iterations=3000)
+        fn = "solve_iccg"
+    else:
+        s = P.eddy(7)
+        kw = dict(use_abmc=1, abmc_num_colors=8, max_iterations=3000)
+        fn = "solve_cocr"
+    xo, ro = getattr(oracle, fn)(s.A, s.b, oracle.config(save_residual_history=1, **kw))
+    xr, rr = getattr(ref, fn)(s.A, s.b, ref.config(save_residual_history=1, **kw))
+    assert ro["iterations"] == rr["iterations"]
+    np.testing.assert_array_equal(ro["residual_history"], rr["residual_history"])
+    np.testing.assert_array_equal(xo, xr)
