@@ -106,11 +106,29 @@ struct SweepLayout {
     index_t max_big_rows = 0;
     index_t max_levels = 0;
 
+    // Single-wavefront walk of the big blocks.  Each level is cut into steps
+    // of at most 64 rows; slot t = step*64 + lane.  Per slot: the row, its
+    // position in level order (index of its value in LDS), D^-1, the
+    // external entries (CSR over slots, summed by the whole workgroup
+    // first) and exactly walk_kmax internal entries (value, position;
+    // padding points at the always-zero position max_big_rows).
+    int walk_kmax = 0;                   // 0: walk unavailable (fallback)
+    std::vector<index_t> walk_step_off;  // nbig+1
+    std::vector<index_t> walk_row;       // slots (-1 idle)
+    std::vector<index_t> walk_pos;       // slots (-1 idle)
+    std::vector<index_t> walk_ext_rp;    // slots+1
+    std::vector<index_t> walk_ext_ci;
+    std::vector<F> walk_ext_v;
+    std::vector<F> walk_invd;            // slots
+    std::vector<F> walk_v;               // (step*kmax + k)*64 + lane
+    std::vector<index_t> walk_ix;
+
     index_t nslices() const { return phase_slices.empty() ? 0 : phase_slices.back(); }
 
     // forward: rows of L below the diagonal; backward: rows of U = L^T above.
     template <typename T>
-    void build(const AbmcSchedule& S, const CsrHost<T>& M, bool forward, int block_size)
+    void build(const AbmcSchedule& S, const CsrHost<T>& M, bool forward, int block_size,
+               const std::vector<F>& invd)
     {
         const index_t nc = S.num_colors();
         const index_t nb = S.num_blocks();
@@ -258,12 +276,83 @@ struct SweepLayout {
             }
             phase_big[p + 1] = phase_big[p] + index_t(big.size());
         }
+        build_walk(M, forward, invd);
         if (val.empty()) { val.assign(kWave, F(0)); col.assign(kWave, -1); }
         if (big_ci.empty()) { big_ci.push_back(0); big_v.push_back(F(0)); }
         if (big_rows.empty()) big_rows.push_back(0);
         if (big_lvl.empty()) big_lvl.push_back(0);
         if (lane_row0.empty()) { lane_row0.assign(kWave, -1); lane_nrows.assign(kWave, 0); }
         if (slice_k.empty()) slice_k.assign(size_t(BR > 0 ? BR : 1), 0);
+    }
+
+private:
+    template <typename T>
+    void build_walk(const CsrHost<T>& M, bool forward, const std::vector<F>& invd)
+    {
+        const index_t nbig = index_t(big_pos.size()) - 1;
+        walk_step_off.assign(1, 0);
+        walk_row.clear(); walk_pos.clear(); walk_ext_rp.assign(1, 0); walk_ext_ci.clear();
+        walk_ext_v.clear(); walk_invd.clear(); walk_v.clear(); walk_ix.clear();
+        // internal entries per row: the ones encoded as positions (< 0)
+        int kmax = 0;
+        for (index_t gp = 0; gp + 1 < index_t(big_rp.size()); ++gp) {
+            int k = 0;
+            for (index_t q = big_rp[gp]; q < big_rp[gp + 1]; ++q) k += big_ci[q] < 0;
+            kmax = std::max(kmax, k);
+        }
+        walk_kmax = nbig == 0 ? 0 : (kmax <= 2 ? 2 : kmax <= 4 ? 4 : kmax <= 8 ? 8 : 0);
+        if (walk_kmax == 0) {
+            walk_step_off.assign(size_t(nbig) + 1, 0);
+        } else {
+            const int K = walk_kmax;
+            for (index_t bb = 0; bb < nbig; ++bb) {
+                const index_t p0 = big_pos[bb];
+                const index_t* lv = big_lvl.data() + big_lvl_off[bb];
+                const index_t nlev = big_lvl_off[bb + 1] - big_lvl_off[bb] - 1;
+                index_t nsteps = 0;
+                for (index_t l = 0; l < nlev; ++l) {
+                    for (index_t q0 = lv[l]; q0 < lv[l + 1]; q0 += kWave) {
+                        const index_t q1 = std::min<index_t>(lv[l + 1], q0 + kWave);
+                        const size_t vbase = walk_v.size();
+                        walk_v.resize(vbase + size_t(K) * kWave, F(0));
+                        walk_ix.resize(vbase + size_t(K) * kWave, max_big_rows);
+                        for (int lane = 0; lane < kWave; ++lane) {
+                            const index_t q = q0 + lane;
+                            if (q >= q1) {
+                                walk_row.push_back(-1); walk_pos.push_back(-1);
+                                walk_invd.push_back(F(0));
+                                walk_ext_rp.push_back(index_t(walk_ext_ci.size()));
+                                continue;
+                            }
+                            const index_t gp = p0 + q;
+                            const index_t row = big_rows[gp];
+                            walk_row.push_back(row);
+                            walk_pos.push_back(q);
+                            walk_invd.push_back(invd[row]);
+                            int k = 0;
+                            for (index_t e = big_rp[gp]; e < big_rp[gp + 1]; ++e) {
+                                if (big_ci[e] >= 0) {
+                                    walk_ext_ci.push_back(big_ci[e]);
+                                    walk_ext_v.push_back(big_v[e]);
+                                } else {
+                                    const size_t at = vbase + size_t(k) * kWave + lane;
+                                    walk_v[at] = big_v[e];
+                                    walk_ix[at] = -1 - big_ci[e];
+                                    ++k;
+                                }
+                            }
+                            walk_ext_rp.push_back(index_t(walk_ext_ci.size()));
+                        }
+                        ++nsteps;
+                    }
+                }
+                walk_step_off.push_back(walk_step_off.back() + nsteps);
+            }
+        }
+        if (walk_row.empty()) { walk_row.assign(1, -1); walk_pos.assign(1, -1); walk_invd.assign(1, F(0)); walk_ext_rp.push_back(0); }
+        if (walk_ext_ci.empty()) { walk_ext_ci.push_back(0); walk_ext_v.push_back(F(0)); }
+        if (walk_v.empty()) { walk_v.push_back(F(0)); walk_ix.push_back(0); }
+        (void)M; (void)forward;
     }
 };
 

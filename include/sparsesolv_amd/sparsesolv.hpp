@@ -1,0 +1,344 @@
+// sparsesolv.hpp -- header-only C++17 drop-in for the ICCG / COCR path of
+// ksugahar/ngsolve-sparsesolv (include/sparsesolv/sparsesolv.hpp), running on
+// one MI355X through the C ABI of libsparsesolv_amd.so.
+//
+// The names, members and defaults follow the reference:
+//   SolverConfig        core/solver_config.hpp:33-159
+//   SolverResult        core/types.hpp:29-37
+//   SparseMatrixView    core/sparse_matrix_view.hpp:58-173 (non-owning CSR)
+//   ICPreconditioner    preconditioners/ic_preconditioner.hpp:52-1543
+//   CGSolver, COCRSolver solvers/cg_solver.hpp:59-147, solvers/cocr_solver.hpp:58-187
+//   solve_iccg, solve_cocr sparsesolv.hpp:43-105, :153-169
+// Errors surface as the reference's exception types: std::runtime_error for
+// setup failures (missing diagonal) and apply-before-setup,
+// std::invalid_argument for bad arguments.
+//
+// Define SPARSESOLV_AMD_AS_SPARSESOLV before including to also expose the
+// classes as namespace `sparsesolv`.
+#pragma once
+
+#include <algorithm>
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../sparsesolv_amd.h"
+
+namespace sparsesolv_amd {
+
+using index_t = std::int32_t;
+using complex_t = std::complex<double>;
+
+enum class NormType { RHS, InitialResidual, Custom };
+enum class DivergenceCheck { None, StagnationCount };
+
+struct SolverConfig {
+    double tolerance = 1e-10;
+    int max_iterations = 1000;
+    double abs_tolerance = 1e-12;
+    NormType norm_type = NormType::RHS;
+    double custom_norm = 1.0;
+    double shift_parameter = 1.05;
+    bool diagonal_scaling = false;
+    bool auto_shift = false;
+    double shift_increment = 0.05;
+    double max_shift_value = 5.0;
+    double min_diagonal_threshold = 1e-6;
+    double zero_diagonal_replacement = 1e-10;
+    int max_shift_trials = 100;
+    DivergenceCheck divergence_check = DivergenceCheck::None;
+    double divergence_threshold = 1000.0;
+    int divergence_count = 100;
+    bool save_best_result = true;
+    bool save_residual_history = false;
+    bool conjugate = false;
+    bool use_abmc = false;
+    int abmc_block_size = 4;
+    int abmc_num_colors = 4;
+    bool abmc_reorder_spmv = false;
+    bool abmc_use_rcm = false;
+    // GPU fields
+    int precision = SSA_PRECISION_FP64;
+    bool use_graph = true;
+    int device = 0;
+
+    ssa_config to_c() const
+    {
+        ssa_config c;
+        ssa_config_default(&c);
+        c.tolerance = tolerance;
+        c.max_iterations = max_iterations;
+        c.abs_tolerance = abs_tolerance;
+        c.norm_type = static_cast<int>(norm_type);
+        c.custom_norm = custom_norm;
+        c.shift_parameter = shift_parameter;
+        c.diagonal_scaling = diagonal_scaling;
+        c.auto_shift = auto_shift;
+        c.shift_increment = shift_increment;
+        c.max_shift_value = max_shift_value;
+        c.min_diagonal_threshold = min_diagonal_threshold;
+        c.zero_diagonal_replacement = zero_diagonal_replacement;
+        c.max_shift_trials = max_shift_trials;
+        c.divergence_check = divergence_check == DivergenceCheck::StagnationCount ? 1 : 0;
+        c.divergence_threshold = divergence_threshold;
+        c.divergence_count = divergence_count;
+        c.save_best_result = save_best_result;
+        c.save_residual_history = save_residual_history;
+        c.conjugate = conjugate;
+        c.use_abmc = use_abmc;
+        c.abmc_block_size = abmc_block_size;
+        c.abmc_num_colors = abmc_num_colors;
+        c.abmc_reorder_spmv = abmc_reorder_spmv;
+        c.abmc_use_rcm = abmc_use_rcm;
+        c.precision = precision;
+        c.use_graph = use_graph;
+        return c;
+    }
+};
+
+struct SolverResult {
+    bool converged = false;
+    int iterations = 0;
+    double final_residual = 0.0;
+    std::vector<double> residual_history;
+    explicit operator bool() const { return converged; }
+};
+
+namespace detail {
+
+inline void check(int rc)
+{
+    if (rc == SSA_OK) return;
+    const std::string msg = ssa_last_error();
+    if (rc == SSA_ERR_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+template <typename Scalar>
+constexpr int dtype_of()
+{
+    static_assert(std::is_same_v<Scalar, double> || std::is_same_v<Scalar, complex_t>,
+                  "Scalar must be double or std::complex<double>");
+    return std::is_same_v<Scalar, double> ? SSA_F64 : SSA_C128;
+}
+
+// one context (HIP stream) per device, shared by every object of the process
+inline std::shared_ptr<ssa_context> context(int device)
+{
+    static std::vector<std::weak_ptr<ssa_context>> cache;
+    if (device < 0) throw std::invalid_argument("device must be >= 0");
+    if (cache.size() <= static_cast<size_t>(device)) cache.resize(device + 1);
+    if (auto c = cache[device].lock()) return c;
+    ssa_context* raw = nullptr;
+    check(ssa_context_create(device, &raw));
+    std::shared_ptr<ssa_context> c(raw, [](ssa_context* p) { ssa_context_destroy(p); });
+    cache[device] = c;
+    return c;
+}
+
+}  // namespace detail
+
+template <typename Scalar>
+class SparseMatrixView {
+public:
+    SparseMatrixView(index_t rows, index_t cols, const index_t* row_ptr, const index_t* col_idx,
+                     const Scalar* values)
+        : rows_(rows), cols_(cols), row_ptr_(row_ptr), col_idx_(col_idx), values_(values) {}
+    index_t rows() const { return rows_; }
+    index_t cols() const { return cols_; }
+    index_t nnz() const { return row_ptr_[rows_]; }
+    const index_t* row_ptr() const { return row_ptr_; }
+    const index_t* col_idx() const { return col_idx_; }
+    const Scalar* values() const { return values_; }
+
+private:
+    index_t rows_, cols_;
+    const index_t* row_ptr_;
+    const index_t* col_idx_;
+    const Scalar* values_;
+};
+
+// IC(0) preconditioner whose factor lives on the GPU.  setup() runs the ABMC
+// schedule and factorisation on the host and uploads them; apply() runs the
+// triangular sweeps on the GPU (host vectors in, host vectors out).
+template <typename Scalar = double>
+class ICPreconditioner {
+public:
+    explicit ICPreconditioner(double shift = 1.05) { config_.shift_parameter = shift; }
+
+    void set_config(const SolverConfig& c)
+    {
+        config_ = c;
+    }
+    const SolverConfig& config() const { return config_; }
+    double shift_parameter() const { return config_.shift_parameter; }
+    void set_shift_parameter(double s) { config_.shift_parameter = s; }
+
+    void setup(const SparseMatrixView<Scalar>& A)
+    {
+        ctx_ = detail::context(config_.device);
+        ssa_config c = config_.to_c();
+        ssa_ic* raw = nullptr;
+        detail::check(ssa_ic_create(ctx_.get(), detail::dtype_of<Scalar>(), A.rows(), A.row_ptr(),
+                                    A.col_idx(), A.values(), &c, &raw));
+        ic_.reset(raw, [](ssa_ic* p) { ssa_ic_destroy(p); });
+        n_ = A.rows();
+        a_ = A.values();
+    }
+
+    void apply(const Scalar* x, Scalar* y, index_t size) const
+    {
+        if (!ic_) throw std::runtime_error("ICPreconditioner::apply called before setup");
+        if (size != n_) throw std::invalid_argument("ICPreconditioner::apply: size mismatch");
+        detail::check(ssa_ic_apply(ic_.get(), x, y, 0));
+    }
+
+    std::string name() const { return "IC"; }
+    bool is_ready() const { return static_cast<bool>(ic_); }
+
+    double actual_shift() const
+    {
+        ssa_ic_info info;
+        detail::check(ssa_ic_info_get(ic_.get(), &info));
+        return info.actual_shift;
+    }
+
+    std::vector<index_t> abmc_ordering() const
+    {
+        std::vector<index_t> o(n_);
+        detail::check(ssa_ic_ordering(ic_.get(), o.data()));
+        return o;
+    }
+
+    ssa_ic* handle() const { return ic_.get(); }
+    const Scalar* matrix_values() const { return a_; }
+
+private:
+    SolverConfig config_;
+    std::shared_ptr<ssa_context> ctx_;
+    std::shared_ptr<ssa_ic> ic_;
+    index_t n_ = 0;
+    const Scalar* a_ = nullptr;
+};
+
+namespace detail {
+
+template <typename Scalar>
+SolverResult run(int method, const SparseMatrixView<Scalar>& A, const Scalar* b, Scalar* x,
+                 index_t size, const ICPreconditioner<Scalar>* pc, const SolverConfig& cfg)
+{
+    if (size != A.rows()) throw std::invalid_argument("solve: size mismatch");
+    ICPreconditioner<Scalar> local;
+    const ICPreconditioner<Scalar>* use = pc;
+    if (!use || !use->is_ready() || use->matrix_values() != A.values()) {
+        // operator only (or a preconditioner of another matrix is not usable
+        // by the GPU loop): build the handle on A with the solver config
+        SolverConfig c = cfg;
+        if (!pc) c.use_abmc = true;
+        else c = pc->config();
+        local.set_config(c);
+        local.setup(A);
+        use = &local;
+    }
+    ssa_config c = cfg.to_c();
+    std::vector<double> hist(static_cast<size_t>(std::max(cfg.max_iterations, 0)) + 4);
+    ssa_result r;
+    check(ssa_solve(use->handle(), method, b, x, 0, &c, &r, hist.data()));
+    SolverResult out;
+    out.converged = r.converged != 0;
+    out.iterations = r.iterations;
+    out.final_residual = r.final_residual;
+    if (cfg.save_residual_history) out.residual_history.assign(hist.begin(), hist.begin() + r.history_len);
+    return out;
+}
+
+}  // namespace detail
+
+template <typename Scalar = double>
+class CGSolver {
+public:
+    std::string name() const { return "CG"; }
+    void set_config(const SolverConfig& c) { config_ = c; }
+    const SolverConfig& config() const { return config_; }
+    SolverConfig& config() { return config_; }
+    SolverResult solve(const SparseMatrixView<Scalar>& A, const Scalar* b, Scalar* x, index_t size,
+                       const ICPreconditioner<Scalar>* precond = nullptr)
+    {
+        return detail::run(precond ? SSA_METHOD_ICCG : SSA_METHOD_CG, A, b, x, size, precond, config_);
+    }
+
+private:
+    SolverConfig config_;
+};
+
+template <typename Scalar = double>
+class COCRSolver {
+public:
+    std::string name() const { return "COCR"; }
+    void set_config(const SolverConfig& c) { config_ = c; }
+    const SolverConfig& config() const { return config_; }
+    SolverConfig& config() { return config_; }
+    SolverResult solve(const SparseMatrixView<Scalar>& A, const Scalar* b, Scalar* x, index_t size,
+                       const ICPreconditioner<Scalar>* precond = nullptr)
+    {
+        return detail::run(precond ? SSA_METHOD_ICCOCR : SSA_METHOD_COCR, A, b, x, size, precond,
+                           config_);
+    }
+
+private:
+    SolverConfig config_;
+};
+
+template <typename Scalar = double>
+[[nodiscard]] inline SolverResult solve_iccg(const SparseMatrixView<Scalar>& A, const Scalar* b,
+                                             Scalar* x, index_t size,
+                                             const SolverConfig& config = SolverConfig())
+{
+    ICPreconditioner<Scalar> precond(config.shift_parameter);
+    precond.set_config(config);
+    precond.setup(A);
+    CGSolver<Scalar> solver;
+    solver.set_config(config);
+    return solver.solve(A, b, x, size, &precond);
+}
+
+template <typename Scalar = double>
+[[nodiscard]] inline SolverResult solve_iccg(const SparseMatrixView<Scalar>& A,
+                                             const std::vector<Scalar>& b, std::vector<Scalar>& x,
+                                             const SolverConfig& config = SolverConfig())
+{
+    if (x.size() != b.size()) x.resize(b.size());
+    return solve_iccg(A, b.data(), x.data(), static_cast<index_t>(b.size()), config);
+}
+
+template <typename Scalar = double>
+[[nodiscard]] inline SolverResult solve_cocr(const SparseMatrixView<Scalar>& A, const Scalar* b,
+                                             Scalar* x, index_t size,
+                                             const SolverConfig& config = SolverConfig())
+{
+    ICPreconditioner<Scalar> precond(config.shift_parameter);
+    precond.set_config(config);
+    precond.setup(A);
+    COCRSolver<Scalar> solver;
+    solver.set_config(config);
+    return solver.solve(A, b, x, size, &precond);
+}
+
+template <typename Scalar = double>
+[[nodiscard]] inline SolverResult solve_cocr(const SparseMatrixView<Scalar>& A,
+                                             const std::vector<Scalar>& b, std::vector<Scalar>& x,
+                                             const SolverConfig& config = SolverConfig())
+{
+    if (x.size() != b.size()) x.resize(b.size());
+    return solve_cocr(A, b.data(), x.data(), static_cast<index_t>(b.size()), config);
+}
+
+}  // namespace sparsesolv_amd
+
+#ifdef SPARSESOLV_AMD_AS_SPARSESOLV
+namespace sparsesolv = sparsesolv_amd;
+#endif

@@ -28,10 +28,10 @@ void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_
     U.v.resize(plan.fac.U.v.size());
     for (size_t k = 0; k < L.v.size(); ++k) L.v[k] = narrow<F>(plan.fac.L.v[k]);
     for (size_t k = 0; k < U.v.size(); ++k) U.v[k] = narrow<F>(plan.fac.U.v[k]);
-    plan.fwd.build(plan.fac.sched, L, /*forward=*/true, settings.abmc_block_size);
-    plan.bwd.build(plan.fac.sched, U, /*forward=*/false, settings.abmc_block_size);
     plan.invd.resize(n);
     for (int i = 0; i < n; ++i) plan.invd[i] = narrow<F>(plan.fac.inv_diag[i]);
+    plan.fwd.build(plan.fac.sched, L, /*forward=*/true, settings.abmc_block_size, plan.invd);
+    plan.bwd.build(plan.fac.sched, U, /*forward=*/false, settings.abmc_block_size, plan.invd);
     plan.seconds = std::chrono::duration<double>(clk::now() - t0).count();
 }
 

@@ -70,15 +70,48 @@ __device__ inline P pick(const P (&loc)[BR], int q)
     return v;
 }
 
+// External part of the oversized blocks of this phase (walk slots t0..t1):
+//   FWD: a = in_i - sum_ext L_ik out_k      BWD: a = -sum_ext U_ik out_k
+// written to sa[t] (and in_i to sb[t] for BWD); the walk kernel finishes them.
+struct WalkExt {
+    size_t t0, t1;
+    const index_t* row;
+    const index_t* ext_rp;
+    const index_t* ext_ci;
+    const void* ext_v;
+    void* sa;
+    void* sb;
+};
+
+template <typename P, typename F, bool FWD>
+__device__ inline void walk_external(const WalkExt& w, size_t t, const P* __restrict__ in,
+                                     const P* out)
+{
+    if (t >= w.t1) return;
+    const index_t row = w.row[t];
+    if (row < 0) return;
+    const F* ev = static_cast<const F*>(w.ext_v);
+    P a = FWD ? in[row] : zero<P>();
+    for (index_t k = w.ext_rp[t]; k < w.ext_rp[t + 1]; ++k)
+        a = fms_(cvt<P>(ev[k]), out[w.ext_ci[k]], a);
+    static_cast<P*>(w.sa)[t] = a;
+    if (!FWD) static_cast<P*>(w.sb)[t] = in[row];
+}
+
 template <typename P, typename F, int BR, bool FWD>
 __global__ __launch_bounds__(kBlock) void k_sweep(
     index_t s0, index_t s1, const index_t* __restrict__ lane_row0,
     const index_t* __restrict__ lane_nrows, const index_t* __restrict__ slice_slot,
     const std::uint8_t* __restrict__ slice_k, const F* __restrict__ val,
     const index_t* __restrict__ col, const F* __restrict__ invd,
-    const P* __restrict__ in, P* out, const SolverState* __restrict__ st)
+    const P* __restrict__ in, P* out, WalkExt walk, const SolverState* __restrict__ st)
 {
     if (st && st->status) return;
+    const index_t nreg = (s1 - s0 + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (index_t(blockIdx.x) >= nreg) {
+        walk_external<P, F, FWD>(walk, walk.t0 + size_t(blockIdx.x - nreg) * kBlock + threadIdx.x, in, out);
+        return;
+    }
     const int lane = threadIdx.x & 63;
     const index_t s = s0 + index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
     if (s >= s1) return;
@@ -151,6 +184,82 @@ __global__ __launch_bounds__(1024) void k_sweep_big(
             if (use_lds) lds[q] = res;
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular sweep, oversized blocks, single-wavefront walk (launched with
+// one wavefront per block, after the phase kernel wrote the external parts).
+// Steps run in dependency order; the rows of one step are independent and
+// read the values of earlier steps from LDS (indexed by level position).
+// The operands of step s+D are loaded while step s computes (D register
+// sets, rotated by the unrolled j loop), so the chain of steps pays LDS
+// latency, not memory latency.
+// ---------------------------------------------------------------------------
+template <typename P, typename F, int KMAX>
+struct WalkBuf {
+    P a, b;
+    F d;
+    index_t row, pos;
+    F v[KMAX];
+    index_t ix[KMAX];
+};
+
+template <typename P, typename F, bool FWD, int KMAX, int D>
+__global__ __launch_bounds__(64) void k_sweep_walk(
+    index_t b0, const index_t* __restrict__ step_off, const index_t* __restrict__ wrow,
+    const index_t* __restrict__ wpos, const F* __restrict__ winvd, const F* __restrict__ wv,
+    const index_t* __restrict__ wix, index_t zero_pos, P* out, const P* __restrict__ sa,
+    const P* __restrict__ sb, const SolverState* __restrict__ st)
+{
+    if (st && st->status) return;
+    extern __shared__ unsigned char lds_raw[];
+    P* lds = reinterpret_cast<P*>(lds_raw);
+    const index_t bb = b0 + index_t(blockIdx.x);
+    const index_t s0 = step_off[bb], s1 = step_off[bb + 1];
+    const int lane = threadIdx.x;
+    if (lane == 0) lds[zero_pos] = zero<P>();
+    __syncthreads();
+    WalkBuf<P, F, KMAX> buf[D];
+    auto load = [&](index_t s, WalkBuf<P, F, KMAX>& B) {
+        const size_t t = size_t(s) * 64 + lane;
+        B.a = sa[t];
+        B.b = FWD ? zero<P>() : sb[t];
+        B.d = winvd[t];
+        B.row = wrow[t];
+        B.pos = wpos[t];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            const size_t at = (size_t(s) * KMAX + k) * 64 + lane;
+            B.v[k] = wv[at];
+            B.ix[k] = wix[at];
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        if (s0 + j < s1) load(s0 + j, buf[j]);
+    for (index_t s = s0; s < s1; s += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const index_t sj = s + j;
+            if (sj < s1) {
+                WalkBuf<P, F, KMAX>& B = buf[j];
+                P acc = B.a;
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) acc = fms_(cvt<P>(B.v[k]), lds[B.ix[k]], acc);
+                const P dinv = cvt<P>(B.d);
+                const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, B.b);
+                if (B.pos >= 0) {
+                    lds[B.pos] = res;
+                    out[B.row] = res;
+                }
+                // later steps read these LDS values from other lanes
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (sj + D < s1) load(sj + D, B);
+            }
+        }
     }
 }
 

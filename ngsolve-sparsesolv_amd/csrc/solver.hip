@@ -295,8 +295,12 @@ template <> struct HostOf<ssa::c64> { using type = std::complex<float>; };
 struct DevSweep {
     DevBuf row0, nrows, slot, k, val, col;
     DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
+    DevBuf w_step_off, w_row, w_pos, w_ext_rp, w_ext_ci, w_ext_v, w_invd, w_v, w_ix, w_sa, w_sb;
     std::vector<index_t> phase_slices, phase_big;
     int BR = 1;
+    int walk_kmax = 0;
+    std::vector<index_t> h_step_off;
+    bool walk_checked = false, walk_fits = false;
     index_t nphases = 0, max_big_rows = 0, max_levels = 0;
     size_t stream_bytes = 0;
 
@@ -316,6 +320,19 @@ struct DevSweep {
         big_rp.upload(L.big_rp, s);
         big_ci.upload(L.big_ci, s);
         big_v.upload(L.big_v, s);
+        w_step_off.upload(L.walk_step_off, s);
+        w_row.upload(L.walk_row, s);
+        w_pos.upload(L.walk_pos, s);
+        w_ext_rp.upload(L.walk_ext_rp, s);
+        w_ext_ci.upload(L.walk_ext_ci, s);
+        w_ext_v.upload(L.walk_ext_v, s);
+        w_invd.upload(L.walk_invd, s);
+        w_v.upload(L.walk_v, s);
+        w_ix.upload(L.walk_ix, s);
+        walk_kmax = L.walk_kmax;
+        h_step_off = L.walk_step_off;
+        w_sa.alloc(L.walk_row.size() * sizeof(ssa::c128));
+        w_sb.alloc(L.walk_row.size() * sizeof(ssa::c128));
         phase_slices = L.phase_slices;
         phase_big = L.phase_big;
         BR = L.BR;
@@ -329,7 +346,9 @@ struct DevSweep {
     {
         return row0.bytes + nrows.bytes + slot.bytes + k.bytes + val.bytes + col.bytes +
                big_pos.bytes + big_lvl_off.bytes + big_lvl.bytes + big_rows.bytes +
-               big_rp.bytes + big_ci.bytes + big_v.bytes;
+               big_rp.bytes + big_ci.bytes + big_v.bytes + w_step_off.bytes + w_row.bytes +
+               w_pos.bytes + w_ext_rp.bytes + w_ext_ci.bytes + w_ext_v.bytes + w_invd.bytes +
+               w_v.bytes + w_ix.bytes;
     }
 };
 
@@ -447,17 +466,32 @@ struct IcImpl final : IcBase {
     }
 
     template <bool FWD>
-    void launch_phase(const DevSweep& D, index_t ph, const P* in, P* out, const SolverState* st)
+    void launch_phase(DevSweep& D, index_t ph, const P* in, P* out, const SolverState* st)
     {
         hipStream_t s = ctx->stream;
         const index_t s0 = D.phase_slices[ph], s1 = D.phase_slices[ph + 1];
-        if (s1 > s0) {
-            const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock);
+        const index_t b0 = D.phase_big[ph], b1 = D.phase_big[ph + 1];
+        const bool walk = b1 > b0 && D.walk_kmax > 0 && walk_lds_ok(D);
+        ssa::WalkExt we{};
+        int gw = 0;
+        if (walk) {
+            we.t0 = size_t(D.h_step_off[b0]) * 64;
+            we.t1 = size_t(D.h_step_off[b1]) * 64;
+            we.row = D.w_row.as<index_t>();
+            we.ext_rp = D.w_ext_rp.as<index_t>();
+            we.ext_ci = D.w_ext_ci.as<index_t>();
+            we.ext_v = D.w_ext_v.p;
+            we.sa = D.w_sa.p;
+            we.sb = D.w_sb.p;
+            gw = int((we.t1 - we.t0 + ssa::kBlock - 1) / ssa::kBlock);
+        }
+        if (s1 > s0 || gw > 0) {
+            const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
 #define SSA_SWEEP(BRV)                                                                          \
     ssa::k_sweep<P, F, BRV, FWD><<<g, ssa::kBlock, 0, s>>>(                                    \
         s0, s1, D.row0.as<index_t>(), D.nrows.as<index_t>(), D.slot.as<index_t>(),             \
-        D.k.as<std::uint8_t>(), D.val.as<F>(), D.col.as<index_t>(), invd.as<F>(), in, out, st)
-            switch (D.BR) {
+        D.k.as<std::uint8_t>(), D.val.as<F>(), D.col.as<index_t>(), invd.as<F>(), in, out, we, st)
+            switch (D.BR == 0 ? 1 : D.BR) {
             case 1: SSA_SWEEP(1); break;
             case 2: SSA_SWEEP(2); break;
             case 4: SSA_SWEEP(4); break;
@@ -468,8 +502,21 @@ struct IcImpl final : IcBase {
 #undef SSA_SWEEP
             HIP_CHECK(hipGetLastError());
         }
-        const index_t b0 = D.phase_big[ph], b1 = D.phase_big[ph + 1];
-        if (b1 > b0) {
+        if (walk) {
+            const size_t lds = (size_t(D.max_big_rows) + 1) * sizeof(P);
+#define SSA_WALK(KM, DEPTH)                                                                    \
+    ssa::k_sweep_walk<P, F, FWD, KM, DEPTH><<<int(b1 - b0), 64, lds, s>>>(                    \
+        b0, D.w_step_off.as<index_t>(), D.w_row.as<index_t>(), D.w_pos.as<index_t>(),         \
+        D.w_invd.as<F>(), D.w_v.as<F>(), D.w_ix.as<index_t>(), D.max_big_rows, out,           \
+        D.w_sa.as<P>(), D.w_sb.as<P>(), st)
+            switch (D.walk_kmax) {
+            case 2: SSA_WALK(2, 8); break;
+            case 4: SSA_WALK(4, 8); break;
+            default: SSA_WALK(8, 4); break;
+            }
+#undef SSA_WALK
+            HIP_CHECK(hipGetLastError());
+        } else if (b1 > b0) {
             const size_t lds = size_t(D.max_big_rows) * sizeof(P);
             const int use_lds = lds <= 64 * 1024 ? 1 : 0;
             ssa::k_sweep_big<P, F, FWD><<<int(b1 - b0), 1024, use_lds ? lds : 0, s>>>(
@@ -478,6 +525,30 @@ struct IcImpl final : IcBase {
                 D.big_v.as<F>(), invd.as<F>(), in, out, use_lds, st);
             HIP_CHECK(hipGetLastError());
         }
+    }
+
+    // whether the walk's LDS (one value per big-block row + a zero slot)
+    // fits the workgroup limit; raises the kernels' dynamic LDS cap once
+    bool walk_lds_ok(DevSweep& D)
+    {
+        const size_t need = (size_t(D.max_big_rows) + 1) * sizeof(P);
+        if (D.walk_checked) return D.walk_fits;
+        D.walk_checked = true;
+        int optin = 0;
+        if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device) != hipSuccess ||
+            optin <= 0)
+            HIP_CHECK(hipDeviceGetAttribute(&optin, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+        D.walk_fits = need <= size_t(optin);
+        if (D.walk_fits && need > 64 * 1024) {
+            const int bytes = int(need);
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 2, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 4, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 8, 4>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 2, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 4, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 8, 4>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        }
+        return D.walk_fits;
     }
 
     // out = M^{-1} in (ABMC order); t is the forward result
