@@ -50,7 +50,14 @@ def bytes_model(n, nnz, nnz_l, s_a, s_f, s_v, s_p, cocr):
     return S_A + 2 * S_L + vec
 
 
-def make_workload(name):
+def make_workload(name, budget=None):
+    s, cfg, method = _make_workload(name)
+    if budget:
+        cfg = dict(cfg, max_iterations=int(budget))
+    return s, cfg, method
+
+
+def _make_workload(name):
     from sparsesolv_amd import problems as P
     if name == "curlcurl64":
         s = P.curlcurl(64)
@@ -70,10 +77,10 @@ def make_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_gpu(name, steps, warmup):
+def run_gpu(name, steps, warmup, budget=None, profile_reps=30):
     import torch
     from sparsesolv_amd import _lib
-    s, kw, method = make_workload(name)
+    s, kw, method = make_workload(name, budget)
     A, b = s.A, s.b
     cfg = _lib.default_config(**kw)
     t0 = time.perf_counter()
@@ -114,7 +121,9 @@ def run_gpu(name, steps, warmup):
     kern = {}
     for what, label in ((_lib.PROFILE_SPMV, "spmv"), (_lib.PROFILE_FORWARD, "forward_sweep"),
                         (_lib.PROFILE_BACKWARD, "backward_sweep"), (_lib.PROFILE_APPLY, "precond_apply")):
-        us, by = h.profile(what, 30)
+        if profile_reps <= 0:
+            break
+        us, by = h.profile(what, profile_reps)
         kern[label] = {"us": round(us, 2), "bytes": int(by), "GB_s": round(by / us / 1e3, 1),
                        "frac": round(by / us / 1e3 / PEAK_GBS, 4)}
         log(f"[{name}] {label}: {us:.1f} us  {by / 1e6:.1f} MB  {by / us / 1e3:.0f} GB/s")
@@ -166,13 +175,18 @@ def main():
                     choices=["curlcurl64", "eddy78", "laplace48"])
     ap.add_argument("--all", action="store_true", help="also run the complex eddy78 COCR workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--budget", type=int, default=None,
+                    help="fixed iteration budget per solve (per-iteration timing runs)")
+    ap.add_argument("--profile-reps", type=int, default=30,
+                    help="launches per kernel in the HIP-event kernel timing (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
 
-    main_out, (A, b, kw, method) = run_gpu(args.workload, args.steps, args.warmup)
+    main_out, (A, b, kw, method) = run_gpu(args.workload, args.steps, args.warmup, args.budget,
+                                           args.profile_reps)
     extra = {"primary": main_out}
     if args.all and args.workload != "eddy78":
         extra["eddy78"], _ = run_gpu("eddy78", max(1, args.steps // 2), 1)
@@ -181,7 +195,7 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(A, b, kw, method, main_out["iterations"])
         log(f"cpu baseline: {cpu}")
-    sp = main_out["kernels"]["spmv"]
+    sp = main_out["kernels"].get("spmv", {"GB_s": None, "frac": None})
     line = {
         "metric": METRIC,
         "value": round(main_out["seconds"], 6),

