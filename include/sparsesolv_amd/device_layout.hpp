@@ -47,6 +47,9 @@ constexpr index_t kPadCol = -(1 << 30);
 // steps of the straggler walk prefetched ahead (the step lists are padded to
 // a multiple of it)
 constexpr int kWalkDepth = 8;
+// entries a sweep kernel loads per batch; the entry arrays end with this
+// many padding slots so a batch never needs a bounds guard
+constexpr int kSweepChunk = 16;
 
 template <typename F>
 struct SpmvLayout {
@@ -292,7 +295,8 @@ struct SweepLayout {
             phase_big[p + 1] = phase_big[p] + index_t(big.size());
         }
         build_walk(M, forward, invd);
-        if (val.empty()) { val.assign(kWave, F(0)); col.assign(kWave, kPadCol); }
+        val.resize(val.size() + size_t(kSweepChunk) * kWave, F(0));
+        col.resize(col.size() + size_t(kSweepChunk) * kWave, kPadCol);
         if (big_ci.empty()) { big_ci.push_back(0); big_v.push_back(F(0)); }
         if (big_rows.empty()) big_rows.push_back(0);
         if (big_lvl.empty()) big_lvl.push_back(0);

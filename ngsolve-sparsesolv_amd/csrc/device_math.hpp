@@ -92,6 +92,13 @@ __device__ inline c64 fms_(c64 a, c64 b, c64 acc)
     return r;
 }
 
+// c ? a : b per component (a ternary on a struct can become a select of
+// addresses, which puts arrays of such values in scratch memory)
+__device__ inline double select_(bool c, double a, double b) { return c ? a : b; }
+__device__ inline float select_(bool c, float a, float b) { return c ? a : b; }
+__device__ inline c128 select_(bool c, c128 a, c128 b) { return c128{c ? a.x : b.x, c ? a.y : b.y}; }
+__device__ inline c64 select_(bool c, c64 a, c64 b) { return c64{c ? a.x : b.x, c ? a.y : b.y}; }
+
 __device__ inline double conj_if(double a, bool) { return a; }
 __device__ inline float conj_if(float a, bool) { return a; }
 __device__ inline c128 conj_if(c128 a, bool c) { return c ? c128{a.x, -a.y} : a; }

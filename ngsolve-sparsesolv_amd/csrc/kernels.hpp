@@ -5,16 +5,19 @@
 // captured iteration can be replayed past convergence without effect.
 //
 // Reductions are deterministic: each workgroup writes one partial (wave
-// butterfly + fixed-order sum over the waves) and a control kernel sums the
-// partials in index order.
+// butterfly + fixed-order sum over the waves); the partials are summed in
+// index order by a control kernel or by the producing kernel's last
+// workgroup (control.hpp).
 #pragma once
 
+#include "control.hpp"
 #include "device_math.hpp"
 #include "solver_state.hpp"
 
 namespace ssa {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts)
+static_assert(kBlock == kCtrlThreads, "fused control steps reduce with kBlock threads");
 constexpr index_t kPadColumn = -(1 << 30);   // padding slot of a sweep layout
 constexpr int kWavesPerBlock = kBlock / 64;
 
@@ -27,15 +30,22 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const index_t* __restrict__ slice_off, const T* __restrict__ val,
     const index_t* __restrict__ col, index_t n, index_t nslices,
     const T* __restrict__ x, T* __restrict__ y, T* __restrict__ partials,
-    const SolverState* __restrict__ st)
+    const SolverState* __restrict__ st, Ctrl check)
 {
-    if (st && st->status) return;
+    const int stopped = st ? st->status : 0;
     __shared__ T red[kWavesPerBlock];
     const int lane = threadIdx.x & 63;
     const index_t s = index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const index_t sc = s < nslices ? s : nslices;
+    const index_t o0 = slice_off[sc], o1 = s < nslices ? slice_off[s + 1] : o0;
+    if (stopped) {
+        // consuming the offset keeps its load ahead of the status test
+        // (offsets are >= 0)
+        if (o0 < 0) y[0] = zero<T>();
+        return;
+    }
     T d = zero<T>();
     if (s < nslices) {
-        const index_t o0 = slice_off[s], o1 = slice_off[s + 1];
         T acc = zero<T>();
 #pragma unroll 4
         for (index_t k = o0; k < o1; ++k) {
@@ -51,6 +61,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     if (DOT) {
         T t = block_sum(d, red);
         if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    }
+    if (check.part) {
+        __shared__ double redd[kWavesPerBlock];
+        ctrl_check_wg0(check, redd);
     }
 }
 
@@ -105,57 +119,79 @@ __device__ inline c128 shfl<c128>(c128 v, int src) { return c128{__shfl(v.x, src
 template <>
 __device__ inline c64 shfl<c64>(c64 v, int src) { return c64{__shfl(v.x, src, 64), __shfl(v.y, src, 64)}; }
 
+struct SweepArrays {
+    const index_t* lane_row;
+    const index_t* lane_pos;
+    const index_t* slice_slot;
+    const void* val;
+    const index_t* col;
+    const void* invd;
+};
+
+// One wavefront, one slice.  The slice's offsets, the lane's row and the
+// first batch of entries are requested before `stopped` is tested, so the
+// chain of dependent memory round trips is offsets -> entries -> gathers.
 template <typename P, typename F, int BR, bool FWD, int CH>
-__global__ __launch_bounds__(kBlock) void k_sweep(
-    index_t s0, index_t s1, const index_t* __restrict__ lane_row,
-    const index_t* __restrict__ lane_pos, const index_t* __restrict__ slice_slot,
-    const F* __restrict__ val, const index_t* __restrict__ col, const F* __restrict__ invd,
-    const P* __restrict__ in, P* out, WalkExt walk, const SolverState* __restrict__ st)
+__device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __restrict__ in,
+                                   P* out, int stopped)
 {
-    if (st && st->status) return;
-    const index_t nreg = (s1 - s0 + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (index_t(blockIdx.x) >= nreg) {
-        walk_external<P, F, FWD>(walk, walk.t0 + size_t(blockIdx.x - nreg) * kBlock + threadIdx.x, in, out);
+    const F* __restrict__ val = static_cast<const F*>(A.val);
+    const index_t* __restrict__ col = A.col;
+    const int lane = threadIdx.x & 63;
+    const size_t li = size_t(s) * 64 + lane;
+    const index_t o0 = A.slice_slot[s], o1 = A.slice_slot[s + 1];
+    const index_t row = A.lane_row[li];
+    const int pos = A.lane_pos[li];
+    // entries at or past o1 belong to the next slice or to the padding at
+    // the end of the arrays: loaded, then masked
+    index_t c[CH];
+    F v[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        const size_t at = (size_t(o0) + j) * 64 + lane;
+        c[j] = col[at];
+        v[j] = val[at];
+    }
+    if (stopped) {
+        // consuming the offset and the first entry on this path keeps their
+        // loads ahead of the status test (the compiler would otherwise sink
+        // them into the only branch that uses them); offsets are >= 0
+        if (o0 < 0) out[0] = cvt<P>(v[0]);
         return;
     }
-    const int lane = threadIdx.x & 63;
-    const index_t s = s0 + index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
-    if (s >= s1) return;                       // whole wavefront: no shuffle below
-    const size_t li = size_t(s) * 64 + lane;
-    const index_t row = lane_row[li];
-    const int pos = lane_pos[li];
     const bool live = row >= 0;
-    P acc = (FWD && live) ? in[row] : zero<P>();
-    const P base_in = (!FWD && live) ? in[row] : zero<P>();
-    const P dinv = live ? cvt<P>(invd[row]) : zero<P>();
+    const index_t rr = live ? row : 0;
+    const P xin = in[rr];
+    const P dinv = cvt<P>(static_cast<const F*>(A.invd)[rr]);
+    P acc = FWD ? xin : zero<P>();
     F intv[BR > 1 ? BR - 1 : 1];
 #pragma unroll
     for (int j = 0; j < (BR > 1 ? BR - 1 : 1); ++j) intv[j] = zero<F>();
-    const index_t o0 = slice_slot[s], o1 = slice_slot[s + 1];
-    for (index_t k0 = o0; k0 < o1; k0 += CH) {
-        index_t c[CH];
-        F v[CH];
-#pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            const bool ok = k0 + j < o1;
-            const size_t at = (size_t(k0) + j) * 64 + lane;
-            c[j] = ok ? col[at] : kPadColumn;
-            v[j] = ok ? val[at] : zero<F>();
-        }
+    for (index_t k0 = o0;;) {
+        // gathers from row 0 stand in for non-external entries; their
+        // values are discarded
         P g[CH];
 #pragma unroll
-        for (int j = 0; j < CH; ++j) g[j] = c[j] >= 0 ? out[c[j]] : zero<P>();
+        for (int j = 0; j < CH; ++j) {
+            if (k0 + j >= o1) c[j] = kPadColumn;
+            g[j] = out[c[j] >= 0 ? c[j] : 0];
+        }
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
-            if (c[j] >= 0) {
-                acc = fms_(cvt<P>(v[j]), g[j], acc);
-            } else {
-                // -1 - c is an earlier position of this block; padding slots
-                // carry kPadColumn and match no position
+            const P t = fms_(cvt<P>(v[j]), g[j], acc);
+            acc = select_(c[j] >= 0, t, acc);
+            // -1 - c is an earlier position of this block; padding slots
+            // carry kPadColumn and match no position
 #pragma unroll
-                for (int t = 0; t < BR - 1; ++t)
-                    if (-1 - c[j] == t) intv[t] = v[j];
-            }
+            for (int q = 0; q < BR - 1; ++q) intv[q] = select_(c[j] == -1 - q, v[j], intv[q]);
+        }
+        k0 += CH;
+        if (k0 >= o1) break;
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const size_t at = (size_t(k0) + j) * 64 + lane;
+            c[j] = col[at];
+            v[j] = val[at];
         }
     }
     const int base = lane - pos;
@@ -163,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
 #pragma unroll
     for (int j = 0; j < BR; ++j) {
         if (live && pos == j) {
-            res = FWD ? mul(acc, dinv) : fma_(acc, dinv, base_in);
+            res = FWD ? mul(acc, dinv) : fma_(acc, dinv, xin);
             out[row] = res;
         }
         if (j < BR - 1) {
@@ -172,6 +208,52 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
             const P vj = shfl(res, min(base + j, 63));
             if (pos > j) acc = fms_(cvt<P>(intv[j]), vj, acc);
         }
+    }
+}
+
+// One colour: slices s0..s1, four per workgroup; extra workgroups compute
+// the external part of the colour's oversized blocks.
+template <typename P, typename F, int BR, bool FWD, int CH>
+__global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, SweepArrays A,
+                                                  const P* __restrict__ in, P* out,
+                                                  WalkExt walk,
+                                                  const SolverState* __restrict__ st)
+{
+    const int stopped = st->status;
+    const index_t nreg = (s1 - s0 + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (index_t(blockIdx.x) >= nreg) {
+        if (stopped) return;
+        walk_external<P, F, FWD>(walk, walk.t0 + size_t(blockIdx.x - nreg) * kBlock + threadIdx.x, in, out);
+        return;
+    }
+    const index_t s = s0 + index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    if (s >= s1) return;                       // whole wavefront: no shuffle below
+    sweep_slice<P, F, BR, FWD, CH>(s, A, in, out, stopped);
+}
+
+// Several consecutive small colours in one workgroup of NT threads: the
+// wavefronts share each colour's slices, a workgroup barrier separates the
+// colours (global writes of a workgroup are visible to it after the
+// barrier).  NT = 1024 for real values, 512 for complex (register budget).
+template <typename P> constexpr int tail_threads() { return sizeof(P) > 8 ? 512 : 1024; }
+constexpr int kTailMaxPhases = 16;
+struct TailPhases {
+    int count;
+    index_t slices[kTailMaxPhases + 1];
+};
+
+template <typename P, typename F, int BR, bool FWD, int CH>
+__global__ __launch_bounds__(tail_threads<P>()) void k_sweep_tail(TailPhases T, SweepArrays A,
+                                                                  const P* __restrict__ in, P* out,
+                                                                  const SolverState* __restrict__ st)
+{
+    if (st->status) return;
+    const index_t w = index_t(threadIdx.x >> 6);
+    constexpr index_t kWaves = tail_threads<P>() / 64;
+    for (int ph = 0; ph < T.count; ++ph) {
+        for (index_t s = T.slices[ph] + w; s < T.slices[ph + 1]; s += kWaves)
+            sweep_slice<P, F, BR, FWD, CH>(s, A, in, out, 0);
+        __syncthreads();
     }
 }
 
@@ -300,7 +382,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const T* __restrict__ a,
                                                const T* __restrict__ b, int conj,
                                                T* __restrict__ partials,
-                                               const SolverState* __restrict__ st)
+                                               const SolverState* __restrict__ st, Ctrl check)
 {
     if (st && st->status) return;
     __shared__ T red[kWavesPerBlock];
@@ -310,6 +392,10 @@ __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const T* __restrict__
         d = fma_(conj_if(a[i], conj != 0), b[i], d);
     T t = block_sum(d, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    if (check.part) {
+        __shared__ double redd[kWavesPerBlock];
+        ctrl_check_wg0(check, redd);
+    }
 }
 
 // partial sums of |a_i|^2 and (optionally) |b_i|^2
@@ -365,29 +451,20 @@ __global__ __launch_bounds__(kBlock) void k_copy(index_t n, const T* __restrict_
         dst[i] = src[i];
 }
 
-// best_x <- x when the last convergence check improved the best residual
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_copy_if_improved(index_t n, const T* __restrict__ x,
-                                                            T* __restrict__ best,
-                                                            const SolverState* __restrict__ st)
-{
-    if (!st->improved) return;
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock)
-        best[i] = x[i];
-}
-
 // CG: x += alpha p; r -= alpha Ap; partial |r|^2
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__ x,
                                                      T* __restrict__ r, const T* __restrict__ p,
                                                      const T* __restrict__ Ap,
                                                      double* __restrict__ partials,
-                                                     const SolverState* __restrict__ st)
+                                                     const SolverState* __restrict__ st,
+                                                     Ctrl ctl)
 {
     if (st->status) return;
     __shared__ double red[kWavesPerBlock];
-    const T alpha = from_pair<T>(st->alpha);
+    __shared__ T redt[kWavesPerBlock];
+    T alpha;
+    if (!ctrl_alpha_all(ctl, redt, alpha)) return;
     double s = 0.0;
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock) {
@@ -400,14 +477,31 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// CG: p = z + beta p
+// best_x <- x when the last convergence check improved the best residual
+// (best null: not tracked).  This runs also after a stop: the check that
+// stopped the solve may have improved the best residual.
+template <typename T>
+__device__ inline void copy_if_improved(index_t n, const T* __restrict__ x, T* __restrict__ best,
+                                        const SolverState* __restrict__ st)
+{
+    if (!best || !st->improved) return;
+    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += index_t(gridDim.x) * kBlock)
+        best[i] = x[i];
+}
+
+// CG: p = z + beta p (and the best-x copy of this iteration)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const T* __restrict__ z,
-                                                T* __restrict__ p,
-                                                const SolverState* __restrict__ st)
+                                                T* __restrict__ p, const T* __restrict__ x,
+                                                T* __restrict__ best,
+                                                const SolverState* __restrict__ st, Ctrl ctl)
 {
+    copy_if_improved(n, x, best, st);
     if (st->status) return;
-    const T beta = from_pair<T>(st->beta);
+    __shared__ T redt[kWavesPerBlock];
+    T beta;
+    if (!ctrl_beta_all(ctl, redt, beta)) return;
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock)
         p[i] = fma_(beta, p[i], z[i]);
@@ -421,11 +515,14 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
                                                        const T* __restrict__ q,
                                                        const T* __restrict__ qt,
                                                        double* __restrict__ partials,
-                                                       const SolverState* __restrict__ st)
+                                                       const SolverState* __restrict__ st,
+                                                       Ctrl ctl)
 {
     if (st->status) return;
     __shared__ double red[kWavesPerBlock];
-    const T alpha = from_pair<T>(st->alpha);
+    __shared__ T redt[kWavesPerBlock];
+    T alpha;
+    if (!ctrl_alpha_all(ctl, redt, alpha)) return;
     double s = 0.0;
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock) {
@@ -439,59 +536,24 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// COCR: p = rt + beta p; q = t + beta q
+// COCR: p = rt + beta p; q = t + beta q (and the best-x copy)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restrict__ rt,
                                                    const T* __restrict__ t, T* __restrict__ p,
-                                                   T* __restrict__ q,
-                                                   const SolverState* __restrict__ st)
+                                                   T* __restrict__ q, const T* __restrict__ x,
+                                                   T* __restrict__ best,
+                                                   const SolverState* __restrict__ st, Ctrl ctl)
 {
+    copy_if_improved(n, x, best, st);
     if (st->status) return;
-    const T beta = from_pair<T>(st->beta);
+    __shared__ T redt[kWavesPerBlock];
+    T beta;
+    if (!ctrl_beta_all(ctl, redt, beta)) return;
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock) {
         p[i] = fma_(beta, p[i], rt[i]);
         q[i] = fma_(beta, q[i], t[i]);
     }
 }
-
-// ---------------------------------------------------------------------------
-// Control kernels: one workgroup; sum partials in index order, then thread 0
-// runs the scalar logic of the reference iteration.
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ inline T sum_partials(const T* __restrict__ p, int np, T* red)
-{
-    T s = zero<T>();
-    for (int i = threadIdx.x; i < np; i += blockDim.x) s = add(s, p[i]);
-    return block_sum(s, red);
-}
-
-// After r = b - Ax: normaliser, initial history entry.
-__global__ __launch_bounds__(kBlock) void k_ctrl_init(const double* __restrict__ pb,
-                                                      const double* __restrict__ pr, int np,
-                                                      SolverState* st, SolveParams prm,
-                                                      double* __restrict__ hist);
-
-// rz_old (CG) or rho (COCR) from dot partials; stores the value in slot.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ctrl_store(const T* __restrict__ part, int np,
-                                                      SolverState* st, int slot);
-
-// CG alpha = rz / pAp with breakdown handling; COCR alpha = rho / mu.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ctrl_alpha(const T* __restrict__ part, int np,
-                                                      SolverState* st, SolveParams prm,
-                                                      double* __restrict__ hist);
-
-// convergence check after the residual update
-__global__ __launch_bounds__(kBlock) void k_ctrl_check(const double* __restrict__ part, int np,
-                                                       SolverState* st, SolveParams prm,
-                                                       double* __restrict__ hist);
-
-// beta = new / old (CG: rz; COCR: rho with its breakdown test), iter += 1
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ctrl_beta(const T* __restrict__ part, int np,
-                                                     SolverState* st, SolveParams prm);
 
 }  // namespace ssa

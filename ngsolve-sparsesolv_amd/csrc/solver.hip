@@ -99,7 +99,12 @@ struct DevBuf {
 };
 
 constexpr int kMaxGrid = 1024;  // vector kernels: partial count bound
-constexpr int kSweepChunk = 16; // sweep entries loaded per batch
+// sweep entries per batch: 16 for real, 8 for complex values (register
+// budget); the layouts pad their entry arrays by kSweepChunk
+template <typename P> constexpr int sweep_chunk() { return sizeof(P) > 8 ? 8 : 16; }
+static_assert(sparsesolv_amd::kSweepChunk >= 16, "layout padding shorter than a batch");
+// Krylov iterations per captured graph
+constexpr int kGraphIterations = 8;
 
 inline int vec_grid(index_t n)
 {
@@ -112,174 +117,6 @@ struct ssa_context {
     int device = 0;
     hipStream_t stream = nullptr;
 };
-
-// ===========================================================================
-// control kernels (declared in kernels.hpp)
-// ===========================================================================
-namespace ssa {
-
-__device__ inline double dabs(double a) { return fabs(a); }
-__device__ inline double dabs(c128 a) { return hypot(a.x, a.y); }
-__device__ inline double ddiv(double a, double b) { return a / b; }
-__device__ inline c128 ddiv(c128 a, c128 b)
-{
-    // Smith's algorithm
-    if (fabs(b.y) <= fabs(b.x)) {
-        const double r = b.y / b.x, d = b.x + b.y * r;
-        return c128{(a.x + a.y * r) / d, (a.y - a.x * r) / d};
-    }
-    const double r = b.x / b.y, d = b.y + b.x * r;
-    return c128{(a.x * r + a.y) / d, (a.y * r - a.x) / d};
-}
-
-// IterativeSolver::check_convergence (solvers/iterative_solver.hpp:217-249);
-// with prm.ngs the COCRSolverNGS test res/rhs_norm <= tol
-// (ngsolve/sparsesolv_solvers.hpp:338-347).
-__device__ inline bool check_convergence(SolverState* s, const SolveParams& prm, double nr,
-                                         double* hist)
-{
-    const double rel = nr / s->normalizer;
-    if (prm.save_history) hist[s->hist_len++] = rel;
-    s->improved = 0;
-    if (prm.save_best && rel < s->best_residual) {
-        s->best_residual = rel;
-        s->improved = 1;
-    }
-    if (prm.ngs) return rel <= prm.tolerance;
-    if (rel < prm.tolerance || nr < prm.abs_tolerance) return true;
-    if (prm.divergence_check) {
-        if (rel >= s->best_residual * prm.divergence_threshold) ++s->bad_count;
-        else s->bad_count = 0;
-    }
-    return false;
-}
-
-__device__ inline void stop(SolverState* s, int status, int converged, int iters, double nr)
-{
-    s->status = status;
-    s->converged = converged;
-    s->result_iters = iters;
-    s->final_norm = nr;
-}
-
-// prepare_iteration (iterative_solver.hpp:135-180)
-__global__ __launch_bounds__(kBlock) void k_ctrl_init(const double* __restrict__ pb,
-                                                      const double* __restrict__ pr, int np,
-                                                      SolverState* st, SolveParams prm,
-                                                      double* __restrict__ hist)
-{
-    __shared__ double red[kWavesPerBlock];
-    const double sb = sum_partials(pb, np, red);
-    const double sr = sum_partials(pr, np, red);
-    if (threadIdx.x != 0) return;
-    const double norm_b = sqrt(sb), init = sqrt(sr);
-    double nz = prm.norm_type == 0 ? norm_b : prm.norm_type == 1 ? init : prm.custom_norm;
-    if (prm.ngs) {
-        // COCRSolverNGS: rhs_norm of the projected rhs, early exit below 1e-30
-        nz = norm_b;
-    } else if (nz < 1e-15) {
-        nz = 1.0;
-    }
-    st->normalizer = nz;
-    st->norm_r = init;
-    st->best_residual = DBL_MAX;
-    st->bad_count = 0;
-    st->improved = 0;
-    st->iter = 0;
-    st->hist_len = 0;
-    st->status = kRunning;
-    st->converged = 0;
-    if (prm.save_history) hist[st->hist_len++] = prm.ngs ? 1.0 : init / nz;
-    if (prm.ngs && norm_b < 1e-30) {
-        stop(st, kConverged, 1, 0, 0.0);
-        st->normalizer = 1.0;
-        return;
-    }
-    if (prm.max_iterations <= 0) stop(st, kMaxIter, 0, prm.max_iterations, init);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ctrl_store(const T* __restrict__ part, int np,
-                                                      SolverState* st, int slot)
-{
-    __shared__ T red[kWavesPerBlock];
-    const T v = sum_partials(part, np, red);
-    if (threadIdx.x != 0) return;
-    double pr[2];
-    to_pair(v, pr);
-    double* dst = slot == 0 ? st->rz : st->denom;
-    dst[0] = pr[0];
-    dst[1] = pr[1];
-}
-
-// CG: alpha = rz / pAp (cg_solver.hpp:100-110); COCR: alpha = rho / mu
-// (cocr_solver.hpp:104-116); breakdown below 1e-30.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ctrl_alpha(const T* __restrict__ part, int np,
-                                                      SolverState* st, SolveParams prm,
-                                                      double* __restrict__ hist)
-{
-    if (st->status) return;
-    __shared__ T red[kWavesPerBlock];
-    const T d = sum_partials(part, np, red);
-    if (threadIdx.x != 0) return;
-    to_pair(d, st->denom);
-    if (dabs(d) < 1e-30) {
-        const double nr = st->norm_r;
-        if (prm.ngs) {
-            stop(st, kBreakdown, 0, st->iter, nr);
-            return;
-        }
-        const bool conv = check_convergence(st, prm, nr, hist);
-        stop(st, kBreakdown, conv ? 1 : 0, conv ? st->iter + 1 : st->iter, nr);
-        return;
-    }
-    const T a = ddiv(from_pair<T>(st->rz), d);
-    to_pair(a, st->alpha);
-}
-
-// after the residual update: ||r||, check_convergence, is_diverging
-__global__ __launch_bounds__(kBlock) void k_ctrl_check(const double* __restrict__ part, int np,
-                                                       SolverState* st, SolveParams prm,
-                                                       double* __restrict__ hist)
-{
-    if (st->status) return;
-    __shared__ double red[kWavesPerBlock];
-    const double s = sum_partials(part, np, red);
-    if (threadIdx.x != 0) return;
-    const double nr = sqrt(s);
-    st->norm_r = nr;
-    if (check_convergence(st, prm, nr, hist)) {
-        stop(st, kConverged, 1, st->iter + 1, nr);
-        return;
-    }
-    if (!prm.ngs && prm.divergence_check && st->bad_count >= prm.divergence_count)
-        stop(st, kDiverged, 0, st->iter + 1, nr);
-}
-
-// CG: beta = rz_new / rz_old (cg_solver.hpp:133-135);
-// COCR: rho breakdown test on the old rho, beta = rho_new / rho
-// (cocr_solver.hpp:149-155).  Then the loop index advances.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ctrl_beta(const T* __restrict__ part, int np,
-                                                     SolverState* st, SolveParams prm)
-{
-    if (st->status) return;
-    __shared__ T red[kWavesPerBlock];
-    const T v = sum_partials(part, np, red);
-    if (threadIdx.x != 0) return;
-    const T old = from_pair<T>(st->rz);
-    if (prm.cocr && dabs(old) < 1e-30) {
-        stop(st, kRhoBreakdown, 0, prm.ngs ? st->iter : st->iter + 1, st->norm_r);
-        return;
-    }
-    to_pair(ddiv(v, old), st->beta);
-    to_pair(v, st->rz);
-    st->iter += 1;
-    if (st->iter >= prm.max_iterations) stop(st, kMaxIter, 0, prm.max_iterations, st->norm_r);
-}
-
-}  // namespace ssa
 
 // ===========================================================================
 // IC handle
@@ -390,6 +227,7 @@ struct IcImpl final : IcBase {
     // device: work space
     DevBuf r, z, p, Ap, t, qt, x, best, bvec, tmp, tmp2;
     DevBuf part_a, part_b, part_c, part_d, state, hist;
+    DevBuf idle;   // a state that never stops: sweeps outside a solve
     SolverState* h_state = nullptr;  // pinned
 
     IcImpl(ssa_context* c, index_t n_, const int32_t* rp, const int32_t* ci, const HT* v,
@@ -450,27 +288,88 @@ struct IcImpl final : IcBase {
         const size_t np = size_t(std::max<index_t>(kMaxGrid, (A_nslices + 3) / 4)) * sizeof(ssa::c128);
         for (DevBuf* b : {&part_a, &part_b, &part_c, &part_d}) b->alloc(np);
         state.alloc(sizeof(SolverState));
+        if (!idle.p) {
+            idle.alloc(sizeof(SolverState));
+            HIP_CHECK(hipMemsetAsync(idle.p, 0, sizeof(SolverState), ctx->stream));
+        }
     }
 
     int spmv_grid() const { return int((A_nslices + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock); }
 
     // y = A x (ABMC order), optional fused dot into part
-    void launch_spmv(const T* xin, T* y, int dot, T* part, const SolverState* st)
+    void launch_spmv(const T* xin, T* y, int dot, T* part, const SolverState* st,
+                     const ssa::Ctrl& ctrl = ssa::Ctrl{})
     {
         const int g = std::max(1, spmv_grid());
         hipStream_t s = ctx->stream;
         if (dot == 0)
-            ssa::k_spmv<T, 0><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st);
+            ssa::k_spmv<T, 0><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl);
         else if (dot == 1)
-            ssa::k_spmv<T, 1><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st);
+            ssa::k_spmv<T, 1><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl);
         else
-            ssa::k_spmv<T, 2><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st);
+            ssa::k_spmv<T, 2><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    ssa::SweepArrays sweep_arrays(DevSweep& D)
+    {
+        return ssa::SweepArrays{D.lrow.as<index_t>(), D.lpos.as<index_t>(), D.slot.as<index_t>(),
+                                D.val.p, D.col.as<index_t>(), invd.p};
+    }
+
+    // a colour small enough to share a single-workgroup kernel with its
+    // neighbours: no oversized block, at most two passes of the workgroup
+    bool small_phase(const DevSweep& D, index_t ph) const
+    {
+        return D.phase_big[ph + 1] == D.phase_big[ph] &&
+               D.phase_slices[ph + 1] - D.phase_slices[ph] <= 2 * (ssa::tail_threads<P>() / 64);
+    }
+
+    // all colours of one sweep; runs of small colours go to one workgroup
+    template <bool FWD>
+    void launch_sweep(DevSweep& D, const P* in, P* out, const SolverState* st)
+    {
+        index_t ph = 0;
+        while (ph < D.nphases) {
+            index_t e = ph;
+            while (e < D.nphases && e - ph < ssa::kTailMaxPhases && small_phase(D, e)) ++e;
+            if (e - ph >= 2) {
+                launch_tail<FWD>(D, ph, e, in, out, st);
+                ph = e;
+            } else {
+                launch_phase<FWD>(D, ph, in, out, st);
+                ++ph;
+            }
+        }
+    }
+
+    template <bool FWD>
+    void launch_tail(DevSweep& D, index_t p0, index_t p1, const P* in, P* out, const SolverState* st)
+    {
+        if (!st) st = idle.as<SolverState>();
+        ssa::TailPhases tp{};
+        tp.count = int(p1 - p0);
+        for (index_t q = p0; q <= p1; ++q) tp.slices[q - p0] = D.phase_slices[q];
+        const ssa::SweepArrays A = sweep_arrays(D);
+        hipStream_t s = ctx->stream;
+#define SSA_TAIL(BRV)                                                                           \
+    ssa::k_sweep_tail<P, F, BRV, FWD, sweep_chunk<P>()><<<1, ssa::tail_threads<P>(), 0, s>>>(tp, A, in, out, st)
+        switch (D.BR == 0 ? 1 : D.BR) {
+        case 1: SSA_TAIL(1); break;
+        case 2: SSA_TAIL(2); break;
+        case 4: SSA_TAIL(4); break;
+        case 8: SSA_TAIL(8); break;
+        case 16: SSA_TAIL(16); break;
+        default: throw SsaError(SSA_ERR_UNSUPPORTED, "sweep: unsupported rows per lane");
+        }
+#undef SSA_TAIL
         HIP_CHECK(hipGetLastError());
     }
 
     template <bool FWD>
     void launch_phase(DevSweep& D, index_t ph, const P* in, P* out, const SolverState* st)
     {
+        if (!st) st = idle.as<SolverState>();
         hipStream_t s = ctx->stream;
         const index_t s0 = D.phase_slices[ph], s1 = D.phase_slices[ph + 1];
         const index_t b0 = D.phase_big[ph], b1 = D.phase_big[ph + 1];
@@ -490,10 +389,9 @@ struct IcImpl final : IcBase {
         }
         if (s1 > s0 || gw > 0) {
             const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
+            const ssa::SweepArrays A = sweep_arrays(D);
 #define SSA_SWEEP(BRV)                                                                          \
-    ssa::k_sweep<P, F, BRV, FWD, kSweepChunk><<<g, ssa::kBlock, 0, s>>>(                       \
-        s0, s1, D.lrow.as<index_t>(), D.lpos.as<index_t>(), D.slot.as<index_t>(),             \
-        D.val.as<F>(), D.col.as<index_t>(), invd.as<F>(), in, out, we, st)
+    ssa::k_sweep<P, F, BRV, FWD, sweep_chunk<P>()><<<g, ssa::kBlock, 0, s>>>(s0, s1, A, in, out, we, st)
             switch (D.BR == 0 ? 1 : D.BR) {
             case 1: SSA_SWEEP(1); break;
             case 2: SSA_SWEEP(2); break;
@@ -557,8 +455,8 @@ struct IcImpl final : IcBase {
     // out = M^{-1} in (ABMC order); t is the forward result
     void launch_precond(const P* in, P* out, P* tbuf, const SolverState* st)
     {
-        for (index_t ph = 0; ph < fwd.nphases; ++ph) launch_phase<true>(fwd, ph, in, tbuf, st);
-        for (index_t ph = 0; ph < bwd.nphases; ++ph) launch_phase<false>(bwd, ph, tbuf, out, st);
+        launch_sweep<true>(fwd, in, tbuf, st);
+        launch_sweep<false>(bwd, tbuf, out, st);
     }
 
     void launch_gather(const index_t* map, const T* src, T* dst)
@@ -610,36 +508,34 @@ struct IcImpl final : IcBase {
         hipStream_t s = ctx->stream;
         SolverState* st = state.as<SolverState>();
         const int G = vec_grid(n);
-        const int np_spmv = std::max(1, spmv_grid());
         const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
                          method == SSA_METHOD_NGS_ICCOCR;
         double* H = hist.as<double>();
+        // control steps fused into the kernels that consume their result
+        const int np_spmv = std::max(1, spmv_grid());
+        auto ctl = [&](const void* part, int np) { return ssa::Ctrl{st, H, part, np, 0, prm}; };
+        T* bx = prm.save_best ? best.as<T>() : nullptr;
         if (!prm.cocr) {
-            // CGSolver::do_iterate body (cg_solver.hpp:85-140)
+            // CGSolver::do_iterate body (cg_solver.hpp:85-140): the alpha step
+            // runs in the update kernel, the convergence check in the r.z
+            // kernel (after the preconditioner, which cannot change x or r),
+            // beta and the best-x copy in the p update
             launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
-            ssa::k_ctrl_alpha<T><<<1, ssa::kBlock, 0, s>>>(part_a.as<T>(), np_spmv, st, prm, H);
-            ssa::k_cg_update<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st);
-            ssa::k_ctrl_check<<<1, ssa::kBlock, 0, s>>>(part_b.as<double>(), G, st, prm, H);
-            if (prm.save_best)
-                ssa::k_copy_if_improved<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), best.as<T>(), st);
+            ssa::k_cg_update<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv));
             if (pre) launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), st);
             else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st);
-            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), prm.conjugate, part_c.as<T>(), st);
-            ssa::k_ctrl_beta<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, prm);
-            ssa::k_xpby<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), st);
+            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G));
+            ssa::k_xpby<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G));
         } else {
-            // COCRSolver::do_iterate body (cocr_solver.hpp:96-162); z holds rt, Ap holds q
+            // COCRSolver::do_iterate body (cocr_solver.hpp:96-162); z holds rt,
+            // Ap holds q.  alpha in the update kernel, the check in the
+            // t = A rt kernel, beta and the best-x copy in the p, q update
             if (pre) launch_precond(Ap.as<P>(), qt.as<P>(), t.as<P>(), st);
             else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st);
-            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, qt.as<T>(), Ap.as<T>(), 0, part_a.as<T>(), st);
-            ssa::k_ctrl_alpha<T><<<1, ssa::kBlock, 0, s>>>(part_a.as<T>(), G, st, prm, H);
-            ssa::k_cocr_update<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), qt.as<T>(), part_b.as<double>(), st);
-            ssa::k_ctrl_check<<<1, ssa::kBlock, 0, s>>>(part_b.as<double>(), G, st, prm, H);
-            if (prm.save_best)
-                ssa::k_copy_if_improved<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), best.as<T>(), st);
-            launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st);
-            ssa::k_ctrl_beta<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), np_spmv, st, prm);
-            ssa::k_cocr_pq<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), st);
+            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, qt.as<T>(), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{});
+            ssa::k_cocr_update<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), qt.as<T>(), part_b.as<double>(), st, ctl(part_a.p, G));
+            launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G));
+            ssa::k_cocr_pq<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv));
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -702,7 +598,7 @@ struct IcImpl final : IcBase {
             // rz_old = (r, M^{-1} r); p = z (cg_solver.hpp:73-77)
             if (pre) launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr);
             else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
-            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), prm.conjugate, part_c.as<T>(), nullptr);
+            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), prm.conjugate, part_c.as<T>(), nullptr, ssa::Ctrl{});
             ssa::k_ctrl_store<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, 0);
             ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr);
         } else {
@@ -715,27 +611,32 @@ struct IcImpl final : IcBase {
         }
         HIP_CHECK(hipGetLastError());
 
-        // iterations: replay a captured iteration, poll the stop status
-        int poll = cfg.poll_interval > 0 ? cfg.poll_interval : 8;
+        // iterations: replay a captured group of iterations, poll the stop
+        // status between groups.  The control kernels stop the solve at
+        // max_iterations themselves and every kernel returns at once after a
+        // stop, so iterations enqueued past the stop cost only launches.
+        const int maxit = std::max(cfg.max_iterations, 0);
+        const int unroll = cfg.use_graph ? std::min(kGraphIterations, maxit + 1) : 1;
+        int poll = std::max(cfg.poll_interval > 0 ? cfg.poll_interval : 8, unroll);
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         if (cfg.use_graph) {
             HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            enqueue_iteration(method, prm);
+            for (int u = 0; u < unroll; ++u) enqueue_iteration(method, prm);
             HIP_CHECK(hipStreamEndCapture(s, &graph));
             HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
         int done = 0;
-        const int maxit = std::max(cfg.max_iterations, 0);
         HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         while (h_state->status == ssa::kRunning && done <= maxit) {
             const int k = std::min(poll, maxit + 1 - done);
-            for (int i = 0; i < k; ++i) {
+            const int groups = (k + unroll - 1) / unroll;
+            for (int i = 0; i < groups; ++i) {
                 if (exec) HIP_CHECK(hipGraphLaunch(exec, s));
                 else enqueue_iteration(method, prm);
             }
-            done += k;
+            done += groups * unroll;
             HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
             if (poll < 64) poll *= 2;
@@ -779,10 +680,10 @@ struct IcImpl final : IcBase {
             switch (what) {
             case SSA_PROFILE_SPMV: launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_a.as<T>(), nullptr); break;
             case SSA_PROFILE_FORWARD:
-                for (index_t ph = 0; ph < fwd.nphases; ++ph) launch_phase<true>(fwd, ph, r.as<P>(), t.as<P>(), nullptr);
+                launch_sweep<true>(fwd, r.as<P>(), t.as<P>(), nullptr);
                 break;
             case SSA_PROFILE_BACKWARD:
-                for (index_t ph = 0; ph < bwd.nphases; ++ph) launch_phase<false>(bwd, ph, t.as<P>(), z.as<P>(), nullptr);
+                launch_sweep<false>(bwd, t.as<P>(), z.as<P>(), nullptr);
                 break;
             default: launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr); break;
             }

@@ -3,8 +3,9 @@
 // The state mirrors the members of sparsesolv::IterativeSolver that its
 // iteration updates (solvers/iterative_solver.hpp:324-350): normaliser,
 // best residual, divergence counter, residual history length, plus the
-// recurrence scalars of CGSolver / COCRSolver.  It is written only by the
-// single-workgroup control kernels and read by every other kernel.
+// recurrence scalars of CGSolver / COCRSolver.  It is written only by
+// the set-up control kernels and by workgroup 0 of the kernels that fuse a
+// control step.
 #pragma once
 
 namespace ssa {
@@ -20,6 +21,7 @@ enum SolveStatus : int {
 
 struct SolverState {
     double rz[2];          // CG: (r, z); COCR: rho
+    double rz_prev[2];     // rz of the previous iteration (read by beta)
     double alpha[2];
     double beta[2];
     double denom[2];       // last pAp / mu
