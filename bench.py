@@ -1,24 +1,41 @@
 #!/usr/bin/env python3
-"""Time-to-tolerance benchmark of the ABMC-IC(0) Krylov path on one MI355X.
+"""Benchmark of the ABMC-IC(0) Krylov path on one MI355X.
 
-Workload (default): the real curl-curl system on a 64^3 hex grid with
-lowest-order edge unknowns, boundary edges eliminated (n = 762,048,
-nnz = 9,762,984), A = C^T diag(nu_f) C + 1e-6 diag(nu_e), nu = 1e-3 in a
-central iron cube; RHS = C^T g with Gaussian face currents (seed 1).
-Auto-shift ICCG in FP64, shift 1.05, ABMC block 4 / 4 colours, tol 1e-10.
+Primary workload (the `value` line): the real curl-curl system on a 64^3 hex
+grid, lowest-order edge unknowns, boundary edges eliminated (n = 762,048,
+nnz = 9,762,984): A = C^T diag(nu_f) C + 1e-6 diag(nu_e), nu = 1e-3 in a
+central iron cube, RHS = C^T g with Gaussian face currents (seed 1).
+Auto-shift ABMC-ICCG in FP64: shift 1.05, block 4, 4 colours requested,
+tol 1e-10, max_iterations 1000 (SparseSolvSolver's default).  IC(0)-CG does
+not reach 1e-10 on this near-singular operator within 1000 iterations (the
+reference does not either): a step is the solve exactly as the reference
+runs it, and the line reports the residual it reached.
 
-One step = one full solve from x = 0 to ||r||/||b|| < 1e-10 with b, x and
-the factor resident in HBM.  `value` = seconds per solve (time to 1e-10).
+One step = one solve from x = 0 with b, x, the operator and the factor
+resident in HBM (setup excluded; it is reported separately).  `value` =
+seconds per solve.
 
-The JSON line also carries:
-  roofline      the SpMV kernel (largest single kernel): algorithmic bytes per
-                launch / mean launch time, timed with HIP events on the
-                library's stream, against the 8 TB/s HBM peak;
-  cpu_baseline  the reference (its own headers, OpenMP build) timed on this
-                host on a bounded sample of the same workload;
-  extra         iterations, iterations/s, setup time, the sweep kernels' rates,
-                the effective per-iteration bandwidth and (with --all) the
-                complex eddy-current COCR workload.
+Extra workloads (`extra`, skipped with --quick):
+  laplace48  7-point Laplacian on a 48^3 interior grid (n = 110,592), ABMC-ICCG
+             shift 1.05, block 4, 4 colours, tol 1e-10: converges, so this is
+             a true time-to-1e-10, with the error against the reference's
+             solution;
+  eddy78     complex-symmetric eddy-current system on 78^3 (n = 1,387,386),
+             A = K + j*10*M_e in a conductor box, ABMC-IC(0)-COCR in
+             complex<double>, block 4, 8 colours, tol 1e-10, 1000 iterations.
+
+The JSON line also carries
+  roofline      the dominant kernel by time, the ABMC colour kernel k_sweep:
+                algorithmic bytes per launch / mean launch time.  Every colour
+                launched on its own runs `--profile-reps` times back to back
+                between two HIP events on the library's stream; the figures
+                are means over those colours, against the 8 TB/s HBM peak.
+                `traffic` is the HBM bytes per launch that rocprofv3's
+                FETCH_SIZE + WRITE_SIZE counters read for the same launches
+                (recorded beside the trace summary; null here);
+  cpu_baseline  the reference (its own headers, OpenMP build) on this host's
+                cores, on the same system: the full solve when it fits the
+                time budget, else a bounded sample extrapolated.
 """
 from __future__ import annotations
 
@@ -36,73 +53,86 @@ sys.path.insert(0, os.path.join(ROOT, "ngsolve-sparsesolv_amd"))
 PEAK_GBS = 8000.0
 METRIC = "time-to-1e-10 (s) & HBM GB/s: FP64 vs FP32-precond batched ABMC-ICCG/COCR"
 
+WORKLOADS = {
+    "curlcurl64": dict(
+        desc=("real curl-curl, 64^3 hex grid, interior edges (n=762,048, nnz=9,762,984), "
+              "auto-shift ABMC-ICCG fp64, shift 1.05, block 4, 4 colours, tol 1e-10, "
+              "max 1000 iterations"),
+        method="iccg",
+        cfg=dict(use_abmc=1, auto_shift=1, shift_parameter=1.05, abmc_block_size=4,
+                 abmc_num_colors=4, tolerance=1e-10, max_iterations=1000)),
+    "laplace48": dict(
+        desc=("7-point Laplacian, 48^3 interior grid (n=110,592, nnz=760,320), ABMC-ICCG fp64, "
+              "shift 1.05, block 4, 4 colours, tol 1e-10"),
+        method="iccg",
+        cfg=dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=4,
+                 tolerance=1e-10, max_iterations=1000)),
+    "eddy78": dict(
+        desc=("complex-symmetric eddy current, 78^3 hex grid (n=1,387,386, nnz=17,821,650), "
+              "ABMC-IC(0)-COCR complex<double>, shift 1.05, block 4, 8 colours, tol 1e-10, "
+              "max 1000 iterations"),
+        method="cocr",
+        cfg=dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=8,
+                 tolerance=1e-10, max_iterations=1000)),
+}
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def bytes_model(n, nnz, nnz_l, s_a, s_f, s_v, s_p, cocr):
-    """Compulsory bytes of one Krylov iteration in ABMC order: every operand
-    touched once, gathers served from cache."""
-    S_A = nnz * (s_a + 4) + 4 * (n + 1)
-    S_L = nnz_l * (s_f + 4) + 4 * (n + 1)
-    vec = n * ((20 if cocr else 14) * s_v + 2 * s_p + s_f)
-    return S_A + 2 * S_L + vec
-
-
-def make_workload(name, budget=None):
-    s, cfg, method = _make_workload(name)
-    if budget:
-        cfg = dict(cfg, max_iterations=int(budget))
-    return s, cfg, method
-
-
-def _make_workload(name):
+def make_system(name):
     from sparsesolv_amd import problems as P
     if name == "curlcurl64":
-        s = P.curlcurl(64)
-        cfg = dict(use_abmc=1, auto_shift=1, shift_parameter=1.05, abmc_block_size=4,
-                   abmc_num_colors=4, tolerance=1e-10, max_iterations=20000)
-        return s, cfg, "iccg"
-    if name == "eddy78":
-        s = P.eddy(78)
-        cfg = dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=8,
-                   tolerance=1e-10, max_iterations=20000)
-        return s, cfg, "cocr"
+        return P.curlcurl(64)
     if name == "laplace48":
-        s = P.poisson(48)
-        cfg = dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=4,
-                   tolerance=1e-10, max_iterations=20000)
-        return s, cfg, "iccg"
+        return P.poisson(48)
+    if name == "eddy78":
+        return P.eddy(78)
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_gpu(name, steps, warmup, budget=None, profile_reps=30):
+def bytes_per_iteration(n, nnz, nnz_l, s, cocr):
+    """Compulsory bytes of one Krylov iteration in ABMC order: every operand
+    touched once, gathers served from cache (s = scalar size of the matrix,
+    the factor and every vector; nnz_l counts the diagonal)."""
+    S_A = nnz * (s + 4) + 4 * (n + 1)
+    S_L = nnz_l * (s + 4) + 4 * (n + 1)
+    vec = n * ((20 if cocr else 14) * s + 2 * s + s)
+    return S_A + 2 * S_L + vec
+
+
+def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
     import torch
     from sparsesolv_amd import _lib
-    s, kw, method = make_workload(name, budget)
-    A, b = s.A, s.b
+    w = WORKLOADS[name]
+    kw = dict(w["cfg"])
+    if budget:
+        kw["max_iterations"] = int(budget)
+    t0 = time.perf_counter()
+    sysm = make_system(name)
+    A, b = sysm.A, sysm.b
+    build_s = time.perf_counter() - t0
     cfg = _lib.default_config(**kw)
     t0 = time.perf_counter()
     h = _lib.IcHandle(A, cfg)
     setup_wall = time.perf_counter() - t0
     info = h.info()
     log(f"[{name}] n={A.n} nnz={A.nnz} colours={info['num_colors']} blocks={info['num_blocks']} "
-        f"big={info['big_blocks']}x{info['max_big_rows']} levels={info['max_levels']} "
-        f"shift={info['actual_shift']} setup={setup_wall:.2f}s")
+        f"oversized={info['big_blocks']}x{info['max_big_rows']} levels={info['max_levels']} "
+        f"shift={info['actual_shift']} setup={setup_wall:.2f}s (matrix build {build_s:.1f}s)")
     dt = torch.complex128 if A.is_complex else torch.float64
     dev = torch.device("cuda:0")
     b_d = torch.from_numpy(np.ascontiguousarray(b)).to(dev)
     x_d = torch.zeros(A.n, dtype=dt, device=dev)
-    code = _lib.METHOD_ICCG if method == "iccg" else _lib.METHOD_ICCOCR
+    code = _lib.METHOD_ICCG if w["method"] == "iccg" else _lib.METHOD_ICCOCR
     res = None
     for _ in range(warmup):
         x_d.zero_()
         torch.cuda.synchronize()
         res = h.solve_device(code, b_d.data_ptr(), x_d.data_ptr(), cfg)
     torch.cuda.synchronize()
-    times = []
-    iters = []
+    times, iters = [], []
     for k in range(steps):
         x_d.zero_()
         torch.cuda.synchronize()
@@ -116,54 +146,82 @@ def run_gpu(name, steps, warmup, budget=None, profile_reps=30):
     x = x_d.cpu().numpy()
     true_rel = float(np.linalg.norm(b - A.to_scipy() @ x) / np.linalg.norm(b))
     sec = float(np.mean(times))
-    s_v = 16 if A.is_complex else 8
-    model = bytes_model(A.n, A.nnz, info["nnz_l"], s_v, s_v, s_v, s_v, method == "cocr")
+    it = int(np.median(iters))
+    s = 16 if A.is_complex else 8
+    model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], s, w["method"] == "cocr")
     kern = {}
     for what, label in ((_lib.PROFILE_SPMV, "spmv"), (_lib.PROFILE_FORWARD, "forward_sweep"),
-                        (_lib.PROFILE_BACKWARD, "backward_sweep"), (_lib.PROFILE_APPLY, "precond_apply")):
+                        (_lib.PROFILE_BACKWARD, "backward_sweep"),
+                        (_lib.PROFILE_APPLY, "precond_apply"),
+                        (_lib.PROFILE_COLOUR_KERNELS, "colour_kernel")):
         if profile_reps <= 0:
             break
         us, by = h.profile(what, profile_reps)
+        if us <= 0:
+            continue   # no colour launched on its own (all in small-colour kernels)
         kern[label] = {"us": round(us, 2), "bytes": int(by), "GB_s": round(by / us / 1e3, 1),
                        "frac": round(by / us / 1e3 / PEAK_GBS, 4)}
-        log(f"[{name}] {label}: {us:.1f} us  {by / 1e6:.1f} MB  {by / us / 1e3:.0f} GB/s")
+        log(f"[{name}] {label}: {us:.2f} us  {by / 1e6:.2f} MB  {by / us / 1e3:.0f} GB/s")
     out = {
-        "workload": name, "method": method, "n": A.n, "nnz": A.nnz, "nnz_l": info["nnz_l"],
-        "colours": info["num_colors"], "blocks": info["num_blocks"],
-        "big_block_rows": info["max_big_rows"], "big_block_levels": info["max_levels"],
-        "actual_shift": info["actual_shift"], "seconds": sec, "iterations": int(np.median(iters)),
-        "converged": bool(res.converged), "final_residual": res.final_residual,
-        "true_rel_residual": true_rel, "setup_s": round(setup_wall, 3),
-        "host_setup_s": round(info["setup_seconds"], 3),
-        "iterations_per_s": round(np.median(iters) / sec, 1),
-        "us_per_iteration": round(1e6 * sec / max(np.median(iters), 1), 2),
+        "workload": name, "desc": w["desc"], "method": w["method"], "n": A.n, "nnz": A.nnz,
+        "nnz_l": info["nnz_l"], "colours": info["num_colors"],
+        "blocks": info["num_blocks"], "oversized_block_rows": info["max_big_rows"],
+        "oversized_block_levels": info["max_levels"], "actual_shift": info["actual_shift"],
+        "seconds": sec, "iterations": it, "converged": bool(res.converged),
+        "final_residual": res.final_residual, "true_rel_residual": true_rel,
+        "setup_s": round(setup_wall, 3), "host_setup_s": round(info["setup_seconds"], 3),
+        "setup_plus_solve_s": round(setup_wall + sec, 4),
+        "iterations_per_s": round(it / sec, 1) if sec > 0 else None,
+        "us_per_iteration": round(1e6 * sec / max(it, 1), 2),
         "bytes_per_iteration_model": int(model),
-        "effective_GB_s": round(model * np.median(iters) / sec / 1e9, 1),
+        "effective_GB_s": round(model * it / sec / 1e9, 1) if sec > 0 else None,
         "kernels": kern, "times": [round(t, 5) for t in times],
     }
-    return out, (A, b, kw, method)
+    return out, (A, b, kw, w["method"], x)
 
 
-def cpu_baseline(A, b, kw, method, gpu_iters, budget_s=20.0):
-    """The reference's own ICCG / COCR (headers compiled with OpenMP) on this
-    host, on a bounded number of iterations of the same system."""
+def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0):
+    """The reference's own ICCG / COCR (its headers compiled with OpenMP) on
+    this host.  When it fits `budget_s`: one full solve (solution compared
+    with the GPU's), then the same number of iterations timed three times
+    (setup and iterations apart, the fastest run kept).  Otherwise setup + a
+    bounded number of iterations, extrapolated to the GPU's count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     kind = "reference" if pyoracle.available("ref_omp") else "port"
     orc = pyoracle.Oracle("ref_omp" if kind == "reference" else "oracle")
-    cfg = orc.config(**{k: v for k, v in kw.items() if k in dict(orc.config()._fields_)})
+    fields = dict(orc.config()._fields_)
+    cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
+    mname = "COCR" if method == "cocr" else "CG"
     probe = 5
-    t_probe, setup = orc.time_iterations(A, b, cfg, probe, "COCR" if method == "cocr" else "CG")
+    t_probe, setup = orc.time_iterations(A, b, cfg, probe, mname)
     per_it = max(t_probe / probe, 1e-6)
+    out = {"unit": "s", "cores": orc.num_threads(), "kind": kind}
+    if 2 * setup + 4 * per_it * gpu_iters <= budget_s:
+        x, r = (orc.solve_cocr if method == "cocr" else orc.solve_iccg)(A, b, cfg)
+        its = max(int(r["iterations"]), 1)
+        runs = [orc.time_iterations(A, b, cfg, its, mname) for _ in range(3)]
+        t_it = min(t for t, _ in runs)
+        setup = min(su for _, su in runs)
+        out.update(value=round(t_it, 4), setup_s=round(setup, 3), iterations=its,
+                   sample=(f"the reference's full solve of the same system ({its} iterations, "
+                           f"converged={r['converged']}, residual {r['final_residual']:.3e}); "
+                           f"value = its {its} iterations timed alone, fastest of 3 runs, "
+                           f"setup apart"),
+                   ms_per_iteration=round(1e3 * t_it / its, 3))
+        if x_gpu is not None:
+            out["gpu_vs_cpu_rel_diff"] = float(np.linalg.norm(x_gpu - x) /
+                                               max(np.linalg.norm(x), 1e-300))
+        return out
     iters = int(max(probe, min(gpu_iters, (budget_s - setup) / per_it)))
-    t_it, setup = orc.time_iterations(A, b, cfg, iters, "COCR" if method == "cocr" else "CG")
+    t_it, setup = orc.time_iterations(A, b, cfg, iters, mname)
     per_it = t_it / iters
-    est = setup + per_it * gpu_iters
-    return {"value": round(est, 3), "unit": "s", "cores": orc.num_threads(), "kind": kind,
-            "sample": (f"setup + {iters} {method.upper()} iterations of the same system timed "
-                       f"({setup:.2f}s + {t_it:.2f}s); time-to-1e-10 extrapolated at the GPU's "
-                       f"{gpu_iters} iterations ({per_it * 1e3:.2f} ms/iteration)"),
-            "ms_per_iteration": round(per_it * 1e3, 3), "setup_s": round(setup, 3)}
+    out.update(value=round(per_it * gpu_iters, 4), setup_s=round(setup, 3),
+               sample=(f"setup + {iters} {mname} iterations of the same system timed "
+                       f"({setup:.2f}s + {t_it:.2f}s), extrapolated to the GPU's {gpu_iters} "
+                       f"iterations; value excludes setup"),
+               ms_per_iteration=round(per_it * 1e3, 3))
+    return out
 
 
 def main():
@@ -171,31 +229,51 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="curlcurl64",
-                    choices=["curlcurl64", "eddy78", "laplace48"])
-    ap.add_argument("--all", action="store_true", help="also run the complex eddy78 COCR workload")
+    ap.add_argument("--workload", default="curlcurl64", choices=sorted(WORKLOADS))
+    ap.add_argument("--quick", action="store_true", help="primary workload only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--budget", type=int, default=None,
                     help="fixed iteration budget per solve (per-iteration timing runs)")
-    ap.add_argument("--profile-reps", type=int, default=30,
-                    help="launches per kernel in the HIP-event kernel timing (0: skip)")
+    ap.add_argument("--profile-reps", type=int, default=20,
+                    help="repetitions in the HIP-event kernel timing (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB between two device buffers (a copy kernel of "
+                         "known size for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path "
+              f"(got --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
-    main_out, (A, b, kw, method) = run_gpu(args.workload, args.steps, args.warmup, args.budget,
-                                           args.profile_reps)
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")   # 1 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1073741824 bytes read, 1073741824 bytes written")
+        del src, dst
+    main_out, (A, b, kw, method, x_gpu) = run_gpu(args.workload, args.steps, args.warmup,
+                                                 args.budget, args.profile_reps)
     extra = {"primary": main_out}
-    if args.all and args.workload != "eddy78":
-        extra["eddy78"], _ = run_gpu("eddy78", max(1, args.steps // 2), 1)
+    if not args.quick:
+        for other in ("laplace48", "eddy78"):
+            if other == args.workload:
+                continue
+            o, (A2, b2, kw2, m2, x2) = run_gpu(other, max(1, args.steps), 1, args.budget,
+                                               args.profile_reps)
+            if not args.no_cpu and other == "laplace48":
+                log("cpu baseline (laplace48) ...")
+                o["cpu_baseline"] = cpu_baseline(A2, b2, kw2, m2, o["iterations"], x2)
+                log(f"cpu baseline (laplace48): {o['cpu_baseline']}")
+            extra[other] = o
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(A, b, kw, method, main_out["iterations"])
+        cpu = cpu_baseline(A, b, kw, method, main_out["iterations"], x_gpu)
         log(f"cpu baseline: {cpu}")
-    sp = main_out["kernels"].get("spmv", {"GB_s": None, "frac": None})
+    ck = main_out["kernels"].get("colour_kernel")
     line = {
         "metric": METRIC,
         "value": round(main_out["seconds"], 6),
@@ -209,13 +287,15 @@ def main():
         "vs_baseline": None,
         "dtype": "c128" if A.is_complex else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}-{method}-fp64", "n": A.n, "nnz": A.nnz,
-                   "abmc_block_size": kw["abmc_block_size"], "abmc_num_colors": kw["abmc_num_colors"],
-                   "auto_shift": bool(kw.get("auto_shift", 0)), "tol": kw["tolerance"],
+        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]['desc']}",
+                   "n": A.n, "nnz": A.nnz, "iterations": main_out["iterations"],
+                   "converged": main_out["converged"],
+                   "final_residual": main_out["final_residual"],
                    "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv (SELL-64 SpMV + fused dot)",
-                     "achieved": sp["GB_s"], "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": sp["frac"], "traffic": None},
+        "roofline": None if ck is None else {
+            "bound": "hbm", "kernel": "k_sweep (ABMC colour kernel, forward and backward)",
+            "achieved": ck["GB_s"], "peak": PEAK_GBS, "unit": "GB/s", "frac": ck["frac"],
+            "traffic": None, "bytes_per_launch": ck["bytes"], "us_per_launch": ck["us"]},
         "cpu_baseline": cpu,
         "extra": extra,
     }

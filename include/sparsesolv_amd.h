@@ -98,7 +98,7 @@ typedef struct ssa_result {
 typedef struct ssa_ic_info {
     int n;
     int nnz;
-    int nnz_l;
+    int nnz_l;                /* entries of the factor L, diagonal included */
     int num_colors;
     int num_blocks;
     int rows_per_lane;        /* regular blocks per lane walk (power of two) */
@@ -162,13 +162,21 @@ int ssa_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device,
  *   SSA_PROFILE_SPMV    one SpMV + fused dot over the Krylov vectors;
  *   SSA_PROFILE_FORWARD the forward sweep (all colour phases);
  *   SSA_PROFILE_BACKWARD the backward sweep (all colour phases);
- *   SSA_PROFILE_APPLY   the whole preconditioner apply.
- * avg_us = mean wall time of one call over `reps` calls; bytes = algorithmic
- * bytes one call must move (every operand once). */
+ *   SSA_PROFILE_APPLY   the whole preconditioner apply;
+ *   SSA_PROFILE_COLOUR_KERNELS the colour kernel: every colour launched on
+ *                       its own (not sharing a small-colour kernel, no
+ *                       oversized-block work) runs `reps` times back to back
+ *                       between two events; both figures are means over
+ *                       those colours, 0 when none qualifies.
+ * avg_us = mean wall time of one call (colour kernels: of one launch);
+ * bytes = algorithmic bytes one call (launch) must move: every operand once,
+ * factor entries with the diagonal at value + int32 index, one int32 row
+ * pointer and the input and output entry per row. */
 #define SSA_PROFILE_SPMV 0
 #define SSA_PROFILE_FORWARD 1
 #define SSA_PROFILE_BACKWARD 2
 #define SSA_PROFILE_APPLY 3
+#define SSA_PROFILE_COLOUR_KERNELS 4
 int ssa_ic_profile(ssa_ic* ic, int what, int reps, double* avg_us, double* bytes);
 
 #ifdef __cplusplus

@@ -102,6 +102,10 @@ struct SweepLayout {
     std::vector<index_t> slice_slot;    // nslices+1: entry-slot offsets
     std::vector<F> val;                 // slots*64
     std::vector<index_t> col;
+    // per colour, rows of the regular slices and their factor entries
+    // counting the diagonal (the algorithmic size of one colour kernel)
+    std::vector<index_t> phase_rows;
+    std::vector<int64_t> phase_entries;
 
     // big blocks (level scheduled), each owned by one workgroup
     std::vector<index_t> phase_big;     // nphases+1: big-block ranges
@@ -294,6 +298,16 @@ struct SweepLayout {
             }
             phase_big[p + 1] = phase_big[p] + index_t(big.size());
         }
+        phase_rows.assign(size_t(nphases), 0);
+        phase_entries.assign(size_t(nphases), 0);
+        for (index_t p = 0; p < nphases; ++p)
+            for (index_t sl = phase_slices[p]; sl < phase_slices[p + 1]; ++sl)
+                for (int lane = 0; lane < kWave; ++lane) {
+                    const index_t row = lane_row[size_t(sl) * kWave + lane];
+                    if (row < 0) continue;
+                    ++phase_rows[p];
+                    phase_entries[p] += M.rp[row + 1] - M.rp[row];   // diagonal included
+                }
         build_walk(M, forward, invd);
         val.resize(val.size() + size_t(kSweepChunk) * kWave, F(0));
         col.resize(col.size() + size_t(kSweepChunk) * kWave, kPadCol);

@@ -134,7 +134,8 @@ struct DevSweep {
     DevBuf lrow, lpos, slot, val, col;
     DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
     DevBuf w_step_off, w_row, w_pos, w_ext_rp, w_ext_ci, w_ext_v, w_invd, w_v, w_ix, w_sa, w_sb;
-    std::vector<index_t> phase_slices, phase_big;
+    std::vector<index_t> phase_slices, phase_big, phase_rows;
+    std::vector<int64_t> phase_entries;
     int BR = 1;
     int walk_kmax = 0;
     std::vector<index_t> h_step_off;
@@ -175,6 +176,8 @@ struct DevSweep {
         HIP_CHECK(hipMemsetAsync(w_sb.p, 0, w_sb.bytes, s));
         phase_slices = L.phase_slices;
         phase_big = L.phase_big;
+        phase_rows = L.phase_rows;
+        phase_entries = L.phase_entries;
         BR = L.BR;
         nphases = L.nphases;
         max_big_rows = L.max_big_rows;
@@ -228,6 +231,7 @@ struct IcImpl final : IcBase {
     DevBuf r, z, p, Ap, t, qt, x, best, bvec, tmp, tmp2;
     DevBuf part_a, part_b, part_c, part_d, state, hist;
     DevBuf idle;   // a state that never stops: sweeps outside a solve
+
     SolverState* h_state = nullptr;  // pinned
 
     IcImpl(ssa_context* c, index_t n_, const int32_t* rp, const int32_t* ci, const HT* v,
@@ -325,15 +329,37 @@ struct IcImpl final : IcBase {
                D.phase_slices[ph + 1] - D.phase_slices[ph] <= 2 * (ssa::tail_threads<P>() / 64);
     }
 
+    // end of the run of small colours that starts at ph (ph itself when
+    // fewer than two colours would share the tail kernel)
+    index_t tail_end(const DevSweep& D, index_t ph) const
+    {
+        index_t e = ph;
+        while (e < D.nphases && e - ph < ssa::kTailMaxPhases && small_phase(D, e)) ++e;
+        return e - ph >= 2 ? e : ph;
+    }
+
+    bool in_tail_group(const DevSweep& D, index_t ph) const
+    {
+        for (index_t q = 0; q < D.nphases;) {
+            const index_t e = tail_end(D, q);
+            if (e > q) {
+                if (ph >= q && ph < e) return true;
+                q = e;
+            } else {
+                ++q;
+            }
+        }
+        return false;
+    }
+
     // all colours of one sweep; runs of small colours go to one workgroup
     template <bool FWD>
     void launch_sweep(DevSweep& D, const P* in, P* out, const SolverState* st)
     {
         index_t ph = 0;
         while (ph < D.nphases) {
-            index_t e = ph;
-            while (e < D.nphases && e - ph < ssa::kTailMaxPhases && small_phase(D, e)) ++e;
-            if (e - ph >= 2) {
+            const index_t e = tail_end(D, ph);
+            if (e > ph) {
                 launch_tail<FWD>(D, ph, e, in, out, st);
                 ph = e;
             } else {
@@ -665,6 +691,50 @@ struct IcImpl final : IcBase {
         res.solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
+    // Each colour launched on its own (no tail group, no oversized-block
+    // work) is run `reps` times back to back between two events, after the
+    // whole sweep has run once; avg_us is the mean over those colours of one
+    // launch's share, bytes the mean of their algorithmic bytes.  Zero when
+    // no colour qualifies.
+    void profile_colour_kernels(int reps, double& avg_us, double& bytes)
+    {
+        hipStream_t s = ctx->stream;
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        double total_us = 0.0, total_bytes = 0.0;
+        int count = 0;
+        for (int dir = 0; dir < 2; ++dir) {
+            DevSweep& D = dir == 0 ? fwd : bwd;
+            P* in = dir == 0 ? r.as<P>() : t.as<P>();
+            P* out = dir == 0 ? t.as<P>() : z.as<P>();
+            if (dir == 0) launch_sweep<true>(D, in, out, nullptr);
+            else launch_sweep<false>(D, in, out, nullptr);
+            for (index_t ph = 0; ph < D.nphases; ++ph) {
+                if (in_tail_group(D, ph) || D.phase_big[ph + 1] > D.phase_big[ph] ||
+                    D.phase_slices[ph + 1] == D.phase_slices[ph])
+                    continue;
+                HIP_CHECK(hipEventRecord(e0, s));
+                for (int k = 0; k < reps; ++k) {
+                    if (dir == 0) launch_phase<true>(D, ph, in, out, nullptr);
+                    else launch_phase<false>(D, ph, in, out, nullptr);
+                }
+                HIP_CHECK(hipEventRecord(e1, s));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                total_us += 1000.0 * double(ms) / reps;
+                total_bytes += double(D.phase_entries[ph]) * (sizeof(F) + 4.0) +
+                               double(D.phase_rows[ph]) * (4.0 + 2.0 * sizeof(P));
+                ++count;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        avg_us = count ? total_us / count : 0.0;
+        bytes = count ? total_bytes / count : 0.0;
+    }
+
     void profile(int what, int reps, double& avg_us, double& bytes) override
     {
         ensure_workspace();
@@ -689,6 +759,10 @@ struct IcImpl final : IcBase {
             }
         };
         run();   // warm-up
+        if (what == SSA_PROFILE_COLOUR_KERNELS) {
+            profile_colour_kernels(std::max(reps, 1), avg_us, bytes);
+            return;
+        }
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
         HIP_CHECK(hipEventCreate(&e1));
@@ -944,7 +1018,7 @@ int ssa_ic_profile(ssa_ic* ic, int what, int reps, double* avg_us, double* bytes
 {
     return guarded([&] {
         if (!ic || !avg_us || !bytes) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_profile: null argument");
-        if (what < SSA_PROFILE_SPMV || what > SSA_PROFILE_APPLY)
+        if (what < SSA_PROFILE_SPMV || what > SSA_PROFILE_COLOUR_KERNELS)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_profile: unknown kernel");
         ic->impl->profile(what, reps, *avg_us, *bytes);
     });

@@ -43,6 +43,7 @@ PROFILE_SPMV = 0
 PROFILE_FORWARD = 1
 PROFILE_BACKWARD = 2
 PROFILE_APPLY = 3
+PROFILE_COLOUR_KERNELS = 4
 
 
 class Config(C.Structure):

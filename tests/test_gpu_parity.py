@@ -308,3 +308,28 @@ def test_plain_methods_match_oracle(oracle):
     xo, ro = oracle.solve_plain(s.A, s.b, oracle.config(max_iterations=500), "CG")
     assert res.converged and abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
     assert rel(x, xo) < 1e-9
+
+
+@pytest.mark.parametrize("m,colour_kernels", [(32, True), (8, None)])
+def test_profile_kinds(m, colour_kernels):
+    # small grids may leave no timed colour kernel (tail kernels, colours
+    # with oversized-block work): then both figures are zero
+    s = P.poisson(m)
+    h = handle(s.A, abmc_block_size=4, abmc_num_colors=4)
+    for what in (_lib.PROFILE_SPMV, _lib.PROFILE_FORWARD, _lib.PROFILE_BACKWARD,
+                 _lib.PROFILE_APPLY):
+        us, by = h.profile(what, 2)
+        assert us > 0 and by > 0, what
+    us, by = h.profile(_lib.PROFILE_COLOUR_KERNELS, 2)
+    assert (us > 0 and by > 0) or (colour_kernels is None and us == 0 and by == 0)
+    # the colour kernels' bytes are a share of one forward + backward sweep
+    _, by_fwd = h.profile(_lib.PROFILE_FORWARD, 1)
+    _, by_launch = h.profile(_lib.PROFILE_COLOUR_KERNELS, 1)
+    assert by_launch < 2 * by_fwd
+    with pytest.raises(_lib.SsaError):
+        h.profile(99, 1)
+    # the sweeps after profiling still give the preconditioner's result
+    x = np.linspace(-1.0, 1.0, s.A.n).astype(s.A.values.dtype)
+    y1 = h.apply(x)
+    h.profile(_lib.PROFILE_COLOUR_KERNELS, 1)
+    assert np.array_equal(h.apply(x), y1)
