@@ -22,7 +22,10 @@ Extra workloads (`extra`, skipped with --quick):
              solution;
   eddy78     complex-symmetric eddy-current system on 78^3 (n = 1,387,386),
              A = K + j*10*M_e in a conductor box, ABMC-IC(0)-COCR in
-             complex<double>, block 4, 8 colours, tol 1e-10, 1000 iterations.
+             complex<double>, block 4, 8 colours, tol 1e-10, 1000 iterations;
+  *-mixed    each of the three with the IC factor and both sweeps in float /
+             complex<float> inside the FP64 Krylov loop (SpMV, vectors, dots
+             and the final true-residual check in FP64).
 
 The JSON line also carries
   roofline      the dominant kernel by time, the ABMC colour kernel k_sweep:
@@ -81,24 +84,42 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+for _base in list(WORKLOADS):
+    _w = WORKLOADS[_base]
+    WORKLOADS[_base + "-mixed"] = dict(
+        desc=_w["desc"].replace("fp64", "fp64 outer").replace(
+            "complex<double>", "complex<double> outer") +
+        "; IC factor and sweeps in float / complex<float> (mixed precision)",
+        method=_w["method"], cfg=dict(_w["cfg"], precision=1))
+
+_SYSTEMS = {}
+
+
 def make_system(name):
     from sparsesolv_amd import problems as P
-    if name == "curlcurl64":
-        return P.curlcurl(64)
-    if name == "laplace48":
-        return P.poisson(48)
-    if name == "eddy78":
-        return P.eddy(78)
-    raise SystemExit(f"unknown workload {name}")
+    base = name.replace("-mixed", "")
+    if base not in _SYSTEMS:
+        if base == "curlcurl64":
+            _SYSTEMS[base] = P.curlcurl(64)
+        elif base == "laplace48":
+            _SYSTEMS[base] = P.poisson(48)
+        elif base == "eddy78":
+            _SYSTEMS[base] = P.eddy(78)
+        else:
+            raise SystemExit(f"unknown workload {name}")
+    return _SYSTEMS[base]
 
 
-def bytes_per_iteration(n, nnz, nnz_l, s, cocr):
+def bytes_per_iteration(n, nnz, nnz_l, s, cocr, s_low=None):
     """Compulsory bytes of one Krylov iteration in ABMC order: every operand
-    touched once, gathers served from cache (s = scalar size of the matrix,
-    the factor and every vector; nnz_l counts the diagonal)."""
+    touched once, gathers served from cache.  s = scalar size of the matrix
+    and the Krylov vectors; s_low = scalar size of the factor and of the
+    preconditioner's internal vectors (s unless mixed); nnz_l counts the
+    diagonal."""
+    s_f = s_p = s if s_low is None else s_low
     S_A = nnz * (s + 4) + 4 * (n + 1)
-    S_L = nnz_l * (s + 4) + 4 * (n + 1)
-    vec = n * ((20 if cocr else 14) * s + 2 * s + s)
+    S_L = nnz_l * (s_f + 4) + 4 * (n + 1)
+    vec = n * ((20 if cocr else 14) * s + 2 * s_p + s_f)
     return S_A + 2 * S_L + vec
 
 
@@ -148,7 +169,9 @@ def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
     sec = float(np.mean(times))
     it = int(np.median(iters))
     s = 16 if A.is_complex else 8
-    model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], s, w["method"] == "cocr")
+    mixed = kw.get("precision", 0) == 1
+    model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], s, w["method"] == "cocr",
+                                s // 2 if mixed else None)
     kern = {}
     for what, label in ((_lib.PROFILE_SPMV, "spmv"), (_lib.PROFILE_FORWARD, "forward_sweep"),
                         (_lib.PROFILE_BACKWARD, "backward_sweep"),
@@ -258,7 +281,8 @@ def main():
                                                  args.budget, args.profile_reps)
     extra = {"primary": main_out}
     if not args.quick:
-        for other in ("laplace48", "eddy78"):
+        for other in ("curlcurl64-mixed", "laplace48", "laplace48-mixed", "eddy78",
+                      "eddy78-mixed"):
             if other == args.workload:
                 continue
             o, (A2, b2, kw2, m2, x2) = run_gpu(other, max(1, args.steps), 1, args.budget,

@@ -377,10 +377,10 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
 // Vector kernels (grid-stride; one partial per workgroup)
 // ---------------------------------------------------------------------------
 
-// partial sums of conj?(a_i) b_i
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot(index_t n, const T* __restrict__ a,
-                                               const T* __restrict__ b, int conj,
+// partial sums of conj?(a_i) b_i, accumulated in T (operands converted)
+template <typename T, typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_dot(index_t n, const TA* __restrict__ a,
+                                               const TB* __restrict__ b, int conj,
                                                T* __restrict__ partials,
                                                const SolverState* __restrict__ st, Ctrl check)
 {
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const T* __restrict__
     T d = zero<T>();
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock)
-        d = fma_(conj_if(a[i], conj != 0), b[i], d);
+        d = fma_(conj_if(cvt<T>(a[i]), conj != 0), cvt<T>(b[i]), d);
     T t = block_sum(d, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
     if (check.part) {
@@ -440,25 +440,27 @@ __global__ __launch_bounds__(kBlock) void k_gather(index_t n, const index_t* __r
         dst[i] = src[map[i]];
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_copy(index_t n, const T* __restrict__ src,
-                                                T* __restrict__ dst,
+// dst = src, converted to the destination's precision
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kBlock) void k_copy(index_t n, const TS* __restrict__ src,
+                                                TD* __restrict__ dst,
                                                 const SolverState* __restrict__ st)
 {
     if (st && st->status) return;
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock)
-        dst[i] = src[i];
+        dst[i] = cvt<TD>(src[i]);
 }
 
-// CG: x += alpha p; r -= alpha Ap; partial |r|^2
-template <typename T>
+// CG: x += alpha p; r -= alpha Ap; partial |r|^2; r also in the
+// preconditioner's precision when r_low is given
+template <typename T, typename P>
 __global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__ x,
                                                      T* __restrict__ r, const T* __restrict__ p,
                                                      const T* __restrict__ Ap,
                                                      double* __restrict__ partials,
                                                      const SolverState* __restrict__ st,
-                                                     Ctrl ctl)
+                                                     Ctrl ctl, P* __restrict__ r_low)
 {
     if (st->status) return;
     __shared__ double red[kWavesPerBlock];
@@ -471,6 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__
         x[i] = fma_(alpha, p[i], x[i]);
         const T ri = fms_(alpha, Ap[i], r[i]);
         r[i] = ri;
+        if (r_low) r_low[i] = cvt<P>(ri);
         s += norm2(ri);
     }
     double t = block_sum(s, red);
@@ -490,9 +493,10 @@ __device__ inline void copy_if_improved(index_t n, const T* __restrict__ x, T* _
         best[i] = x[i];
 }
 
-// CG: p = z + beta p (and the best-x copy of this iteration)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const T* __restrict__ z,
+// CG: p = z + beta p (and the best-x copy of this iteration); z may be in
+// the preconditioner's precision
+template <typename T, typename TZ>
+__global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const TZ* __restrict__ z,
                                                 T* __restrict__ p, const T* __restrict__ x,
                                                 T* __restrict__ best,
                                                 const SolverState* __restrict__ st, Ctrl ctl)
@@ -504,16 +508,17 @@ __global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const T* __restrict_
     if (!ctrl_beta_all(ctl, redt, beta)) return;
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock)
-        p[i] = fma_(beta, p[i], z[i]);
+        p[i] = fma_(beta, p[i], cvt<T>(z[i]));
 }
 
-// COCR: x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2
-template <typename T>
+// COCR: x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt may
+// be in the preconditioner's precision)
+template <typename T, typename TQ>
 __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict__ x,
                                                        T* __restrict__ r, T* __restrict__ rt,
                                                        const T* __restrict__ p,
                                                        const T* __restrict__ q,
-                                                       const T* __restrict__ qt,
+                                                       const TQ* __restrict__ qt,
                                                        double* __restrict__ partials,
                                                        const SolverState* __restrict__ st,
                                                        Ctrl ctl)
@@ -529,20 +534,22 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
         x[i] = fma_(alpha, p[i], x[i]);
         const T ri = fms_(alpha, q[i], r[i]);
         r[i] = ri;
-        rt[i] = fms_(alpha, qt[i], rt[i]);
+        rt[i] = fms_(alpha, cvt<T>(qt[i]), rt[i]);
         s += norm2(ri);
     }
     double t = block_sum(s, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// COCR: p = rt + beta p; q = t + beta q (and the best-x copy)
-template <typename T>
+// COCR: p = rt + beta p; q = t + beta q (and the best-x copy); q also in
+// the preconditioner's precision when q_low is given
+template <typename T, typename P>
 __global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restrict__ rt,
                                                    const T* __restrict__ t, T* __restrict__ p,
                                                    T* __restrict__ q, const T* __restrict__ x,
                                                    T* __restrict__ best,
-                                                   const SolverState* __restrict__ st, Ctrl ctl)
+                                                   const SolverState* __restrict__ st, Ctrl ctl,
+                                                   P* __restrict__ q_low)
 {
     copy_if_improved(n, x, best, st);
     if (st->status) return;
@@ -552,7 +559,9 @@ __global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restri
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock) {
         p[i] = fma_(beta, p[i], rt[i]);
-        q[i] = fma_(beta, q[i], t[i]);
+        const T qi = fma_(beta, q[i], t[i]);
+        q[i] = qi;
+        if (q_low) q_low[i] = cvt<P>(qi);
     }
 }
 

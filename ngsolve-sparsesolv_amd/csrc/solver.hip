@@ -105,6 +105,10 @@ template <typename P> constexpr int sweep_chunk() { return sizeof(P) > 8 ? 8 : 1
 static_assert(sparsesolv_amd::kSweepChunk >= 16, "layout padding shorter than a batch");
 // Krylov iterations per captured graph
 constexpr int kGraphIterations = 8;
+// mixed precision: refinement passes per solve, and the residual reduction
+// a COCR pass aims for before the next restart
+constexpr int kRefinePasses = 8;
+constexpr double kInnerReduction = 1e-5;
 
 inline int vec_grid(index_t n)
 {
@@ -231,6 +235,10 @@ struct IcImpl final : IcBase {
     DevBuf r, z, p, Ap, t, qt, x, best, bvec, tmp, tmp2;
     DevBuf part_a, part_b, part_c, part_d, state, hist;
     DevBuf idle;   // a state that never stops: sweeps outside a solve
+    // Mixed precision (P narrower than T): the preconditioner's input, output
+    // and intermediate vectors in P.  In FP64 they alias the Krylov vectors.
+    static constexpr bool kMixed = !std::is_same<T, P>::value;
+    DevBuf lo_a, lo_b, lo_t;
 
     SolverState* h_state = nullptr;  // pinned
 
@@ -292,6 +300,8 @@ struct IcImpl final : IcBase {
         const size_t np = size_t(std::max<index_t>(kMaxGrid, (A_nslices + 3) / 4)) * sizeof(ssa::c128);
         for (DevBuf* b : {&part_a, &part_b, &part_c, &part_d}) b->alloc(np);
         state.alloc(sizeof(SolverState));
+        if (kMixed)
+            for (DevBuf* b : {&lo_a, &lo_b, &lo_t}) b->alloc(size_t(n) * sizeof(P));
         if (!idle.p) {
             idle.alloc(sizeof(SolverState));
             HIP_CHECK(hipMemsetAsync(idle.p, 0, sizeof(SolverState), ctx->stream));
@@ -491,6 +501,21 @@ struct IcImpl final : IcBase {
         HIP_CHECK(hipGetLastError());
     }
 
+    // preconditioner operands: CG applies M^{-1} to r (z out), COCR to q = Ap
+    // (qt out); in mixed precision both go through the P copies lo_a, lo_b
+    P* pre_in(bool cocr) { return kMixed ? lo_a.as<P>() : (cocr ? Ap.as<P>() : r.as<P>()); }
+    P* pre_out(bool cocr) { return kMixed ? lo_b.as<P>() : (cocr ? qt.as<P>() : z.as<P>()); }
+    P* pre_tmp() { return kMixed ? lo_t.as<P>() : t.as<P>(); }
+    // the P copy a mixed-precision kernel also writes (none in FP64)
+    P* low_copy() { return kMixed ? lo_a.as<P>() : nullptr; }
+
+    template <typename S, typename D>
+    void convert(const S* src, D* dst, const SolverState* st)
+    {
+        ssa::k_copy<S, D><<<vec_grid(n), ssa::kBlock, 0, ctx->stream>>>(n, src, dst, st);
+        HIP_CHECK(hipGetLastError());
+    }
+
     // ---- public operations -------------------------------------------------
     void apply(const void* xin, void* yout, bool on_device) override
     {
@@ -503,7 +528,9 @@ struct IcImpl final : IcBase {
             src = tmp.as<T>();
         }
         launch_gather(rev.as<index_t>(), src, r.as<T>());
-        launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr);
+        if (kMixed) convert(r.as<T>(), pre_in(false), nullptr);
+        launch_precond(pre_in(false), pre_out(false), pre_tmp(), nullptr);
+        if (kMixed) convert(pre_out(false), z.as<T>(), nullptr);
         T* dst = on_device ? static_cast<T*>(yout) : tmp2.as<T>();
         launch_gather(perm.as<index_t>(), z.as<T>(), dst);
         if (!on_device) HIP_CHECK(hipMemcpyAsync(yout, tmp2.p, vb, hipMemcpyDeviceToHost, s));
@@ -541,77 +568,76 @@ struct IcImpl final : IcBase {
         const int np_spmv = std::max(1, spmv_grid());
         auto ctl = [&](const void* part, int np) { return ssa::Ctrl{st, H, part, np, 0, prm}; };
         T* bx = prm.save_best ? best.as<T>() : nullptr;
-        if (!prm.cocr) {
+        const bool c = prm.cocr != 0;
+        if (!c) {
             // CGSolver::do_iterate body (cg_solver.hpp:85-140): the alpha step
             // runs in the update kernel, the convergence check in the r.z
             // kernel (after the preconditioner, which cannot change x or r),
             // beta and the best-x copy in the p update
             launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
-            ssa::k_cg_update<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv));
-            if (pre) launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), st);
-            else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st);
-            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G));
-            ssa::k_xpby<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G));
+            ssa::k_cg_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv), low_copy());
+            if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
+            else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st);
+            ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G));
+            ssa::k_xpby<T, P><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G));
         } else {
             // COCRSolver::do_iterate body (cocr_solver.hpp:96-162); z holds rt,
             // Ap holds q.  alpha in the update kernel, the check in the
             // t = A rt kernel, beta and the best-x copy in the p, q update
-            if (pre) launch_precond(Ap.as<P>(), qt.as<P>(), t.as<P>(), st);
-            else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st);
-            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, qt.as<T>(), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{});
-            ssa::k_cocr_update<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), qt.as<T>(), part_b.as<double>(), st, ctl(part_a.p, G));
+            if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
+            else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st);
+            if (kMixed)   // (q, qt) with the narrow operand second
+                ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>(), st, ssa::Ctrl{});
+            else
+                ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{});
+            ssa::k_cocr_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G));
             launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G));
-            ssa::k_cocr_pq<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv));
+            ssa::k_cocr_pq<T, P><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv), low_copy());
         }
         HIP_CHECK(hipGetLastError());
     }
 
-    void solve(int method, const void* bin, void* xio, bool on_device, const ssa_config& cfg,
-               ssa_result& res, double* history) override
+    // ||b|| in FP64 (partials summed on the host in index order)
+    double rhs_norm()
     {
-        ensure_workspace();
         hipStream_t s = ctx->stream;
-        const size_t vb = size_t(n) * sizeof(T);
         const int G = vec_grid(n);
-        SolveParams prm{};
-        prm.tolerance = cfg.tolerance;
-        prm.abs_tolerance = cfg.abs_tolerance;
-        prm.custom_norm = cfg.custom_norm;
-        prm.divergence_threshold = cfg.divergence_threshold;
-        prm.max_iterations = cfg.max_iterations;
-        prm.norm_type = cfg.norm_type;
-        prm.save_best = cfg.save_best_result ? 1 : 0;
-        prm.save_history = cfg.save_residual_history ? 1 : 0;
-        prm.divergence_check = cfg.divergence_check ? 1 : 0;
-        prm.divergence_count = cfg.divergence_count;
-        prm.conjugate = cfg.conjugate ? 1 : 0;
-        prm.cocr = (method == SSA_METHOD_ICCOCR || method == SSA_METHOD_COCR ||
-                    method == SSA_METHOD_NGS_ICCOCR) ? 1 : 0;
-        prm.ngs = method == SSA_METHOD_NGS_ICCOCR ? 1 : 0;
-        if (prm.ngs) {
-            prm.save_best = 0;
-            prm.divergence_check = 0;
-        }
-        const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
-                         method == SSA_METHOD_NGS_ICCOCR;
-        hist.alloc(sizeof(double) * size_t(std::max(cfg.max_iterations, 0) + 4));
+        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), nullptr, part_d.as<double>(), nullptr);
+        HIP_CHECK(hipGetLastError());
+        return std::sqrt(host_sum(part_d.as<double>(), G));
+    }
+
+    double host_sum(const double* dev, int count)
+    {
+        std::vector<double> h(size_t(count), 0.0);
+        HIP_CHECK(hipMemcpyAsync(h.data(), dev, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        double sum = 0.0;
+        for (double v : h) sum += v;
+        return sum;
+    }
+
+    // ||b - A x|| in FP64 (uses Ap and r; partials summed on the host in
+    // index order)
+    double true_residual()
+    {
+        hipStream_t s = ctx->stream;
+        const int G = vec_grid(n);
+        launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
+        ssa::k_residual<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), Ap.as<T>(), r.as<T>());
+        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), nullptr, part_d.as<double>(), nullptr);
+        HIP_CHECK(hipGetLastError());
+        return std::sqrt(host_sum(part_d.as<double>(), G));
+    }
+
+    // prepare_iteration and the iteration loop from the current x; leaves
+    // the final state in h_state
+    void run_pass(int method, const SolveParams& prm, bool pre, const ssa_config& cfg)
+    {
+        hipStream_t s = ctx->stream;
+        const int G = vec_grid(n);
         SolverState* st = state.as<SolverState>();
         double* H = hist.as<double>();
-
-        const auto t0 = std::chrono::steady_clock::now();
-        // inputs into ABMC order: b -> bvec, x0 -> x
-        const T* bsrc = static_cast<const T*>(bin);
-        const T* xsrc = static_cast<const T*>(xio);
-        if (!on_device) {
-            HIP_CHECK(hipMemcpyAsync(tmp.p, bin, vb, hipMemcpyHostToDevice, s));
-            HIP_CHECK(hipMemcpyAsync(tmp2.p, xio, vb, hipMemcpyHostToDevice, s));
-            bsrc = tmp.as<T>();
-            xsrc = tmp2.as<T>();
-        }
-        launch_gather(rev.as<index_t>(), bsrc, bvec.as<T>());
-        if (prm.ngs) HIP_CHECK(hipMemsetAsync(x.p, 0, vb, s));
-        else launch_gather(rev.as<index_t>(), xsrc, x.as<T>());
-        HIP_CHECK(hipMemsetAsync(best.p, 0, vb, s));
         HIP_CHECK(hipMemsetAsync(state.p, 0, sizeof(SolverState), s));
 
         // prepare_iteration: r = b - A x, norms, normaliser
@@ -620,20 +646,32 @@ struct IcImpl final : IcBase {
         ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), r.as<T>(), part_b.as<double>(), part_d.as<double>());
         ssa::k_ctrl_init<<<1, ssa::kBlock, 0, s>>>(part_b.as<double>(), part_d.as<double>(), G, st, prm, H);
         HIP_CHECK(hipGetLastError());
-        if (!prm.cocr) {
+        const bool c = prm.cocr != 0;
+        if (!c) {
             // rz_old = (r, M^{-1} r); p = z (cg_solver.hpp:73-77)
-            if (pre) launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr);
-            else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
-            ssa::k_dot<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), prm.conjugate, part_c.as<T>(), nullptr, ssa::Ctrl{});
+            if (pre) {
+                if (kMixed) convert(r.as<T>(), pre_in(c), nullptr);
+                launch_precond(pre_in(c), pre_out(c), pre_tmp(), nullptr);
+            } else {
+                ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
+            }
+            ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), nullptr, ssa::Ctrl{});
             ssa::k_ctrl_store<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, 0);
-            ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr);
+            convert(pre_out(c), p.as<T>(), nullptr);
         } else {
             // rt = M^{-1} r; p = rt; q = A p; rho = rt^T q (cocr_solver.hpp:78-88)
-            if (pre) launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr);
-            else ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
-            ssa::k_copy<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr);
+            if (pre) {
+                if (kMixed) convert(r.as<T>(), pre_in(c), nullptr);
+                launch_precond(kMixed ? pre_in(c) : r.as<P>(), kMixed ? pre_out(c) : z.as<P>(),
+                               pre_tmp(), nullptr);
+                if (kMixed) convert(pre_out(c), z.as<T>(), nullptr);
+            } else {
+                ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
+            }
+            ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr);
             launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_c.as<T>(), nullptr);
             ssa::k_ctrl_store<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), std::max(1, spmv_grid()), st, 0);
+            if (kMixed) convert(Ap.as<T>(), pre_in(c), nullptr);
         }
         HIP_CHECK(hipGetLastError());
 
@@ -641,7 +679,7 @@ struct IcImpl final : IcBase {
         // status between groups.  The control kernels stop the solve at
         // max_iterations themselves and every kernel returns at once after a
         // stop, so iterations enqueued past the stop cost only launches.
-        const int maxit = std::max(cfg.max_iterations, 0);
+        const int maxit = std::max(prm.max_iterations, 0);
         const int unroll = cfg.use_graph ? std::min(kGraphIterations, maxit + 1) : 1;
         int poll = std::max(cfg.poll_interval > 0 ? cfg.poll_interval : 8, unroll);
         hipGraph_t graph = nullptr;
@@ -669,16 +707,111 @@ struct IcImpl final : IcBase {
         }
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+    }
+
+    void solve(int method, const void* bin, void* xio, bool on_device, const ssa_config& cfg,
+               ssa_result& res, double* history) override
+    {
+        ensure_workspace();
+        hipStream_t s = ctx->stream;
+        const size_t vb = size_t(n) * sizeof(T);
+        SolveParams prm{};
+        prm.tolerance = cfg.tolerance;
+        prm.abs_tolerance = cfg.abs_tolerance;
+        prm.custom_norm = cfg.custom_norm;
+        prm.divergence_threshold = cfg.divergence_threshold;
+        prm.max_iterations = cfg.max_iterations;
+        prm.norm_type = cfg.norm_type;
+        prm.save_best = cfg.save_best_result ? 1 : 0;
+        prm.save_history = cfg.save_residual_history ? 1 : 0;
+        prm.divergence_check = cfg.divergence_check ? 1 : 0;
+        prm.divergence_count = cfg.divergence_count;
+        prm.conjugate = cfg.conjugate ? 1 : 0;
+        prm.cocr = (method == SSA_METHOD_ICCOCR || method == SSA_METHOD_COCR ||
+                    method == SSA_METHOD_NGS_ICCOCR) ? 1 : 0;
+        prm.ngs = method == SSA_METHOD_NGS_ICCOCR ? 1 : 0;
+        if (prm.ngs) {
+            prm.save_best = 0;
+            prm.divergence_check = 0;
+        }
+        const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
+                         method == SSA_METHOD_NGS_ICCOCR;
+        hist.alloc(sizeof(double) * size_t(std::max(cfg.max_iterations, 0) + 4));
+        double* H = hist.as<double>();
+
+        if (kMixed && !pre)
+            throw SsaError(SSA_ERR_UNSUPPORTED,
+                           "ssa_solve: mixed precision needs the IC preconditioner");
+        const auto t0 = std::chrono::steady_clock::now();
+        // inputs into ABMC order: b -> bvec, x0 -> x
+        const T* bsrc = static_cast<const T*>(bin);
+        const T* xsrc = static_cast<const T*>(xio);
+        if (!on_device) {
+            HIP_CHECK(hipMemcpyAsync(tmp.p, bin, vb, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(tmp2.p, xio, vb, hipMemcpyHostToDevice, s));
+            bsrc = tmp.as<T>();
+            xsrc = tmp2.as<T>();
+        }
+        launch_gather(rev.as<index_t>(), bsrc, bvec.as<T>());
+        if (prm.ngs) HIP_CHECK(hipMemsetAsync(x.p, 0, vb, s));
+        else launch_gather(rev.as<index_t>(), xsrc, x.as<T>());
+        HIP_CHECK(hipMemsetAsync(best.p, 0, vb, s));
+
+        int total_iters = 0;
+        double true_rel = -1.0;
+        SolverState S{};
+        if (!kMixed || prm.ngs) {
+            run_pass(method, prm, pre, cfg);
+            S = *h_state;
+            total_iters = S.result_iters;
+        } else {
+            // Mixed precision, iterative refinement: the residual is computed
+            // in FP64 from x, each pass runs the preconditioned Krylov method
+            // from x until its residual falls below the tolerance or (COCR)
+            // below kInnerReduction times the residual the pass started from
+            // -- COCR carries M^{-1} r by a recurrence that, with a float
+            // preconditioner, stalls near float precision -- and the next
+            // pass restarts from the new x.  A pass that ends unconverged
+            // leaves its best iterate (when save_best_result); a pass that
+            // does not lower the true residual is undone, so x and the
+            // reported residual always match.
+            const double bnorm = rhs_norm();
+            const double scale = bnorm > 0.0 ? bnorm : 1.0;
+            double rel = true_residual() / scale;
+            for (int pass = 0; pass < kRefinePasses; ++pass) {
+                if (rel < cfg.tolerance || total_iters >= cfg.max_iterations) break;
+                HIP_CHECK(hipMemcpyAsync(tmp.p, x.p, vb, hipMemcpyDeviceToDevice, s));
+                prm.tolerance = prm.cocr ? std::max(cfg.tolerance, kInnerReduction * rel)
+                                         : cfg.tolerance;
+                prm.max_iterations = cfg.max_iterations - total_iters;
+                run_pass(method, prm, pre, cfg);
+                S = *h_state;
+                total_iters += S.result_iters;
+                if (!S.converged && prm.save_best)
+                    HIP_CHECK(hipMemcpyAsync(x.p, best.p, vb, hipMemcpyDeviceToDevice, s));
+                const double next = true_residual() / scale;
+                if (!(next < rel)) {
+                    HIP_CHECK(hipMemcpyAsync(x.p, tmp.p, vb, hipMemcpyDeviceToDevice, s));
+                    break;
+                }
+                const bool progress = next < 0.5 * rel;
+                rel = next;
+                if (!progress) break;
+            }
+            true_rel = rel;
+            S.converged = rel < cfg.tolerance;
+            if (S.converged) S.status = ssa::kConverged;
+            else if (S.status == ssa::kRunning) S.status = ssa::kMaxIter;
+        }
 
         // build_result (iterative_solver.hpp:262-279)
-        const SolverState S = *h_state;
         res.converged = S.converged;
-        res.iterations = S.result_iters;
+        res.iterations = total_iters;
         res.status = S.status;
-        res.final_residual = S.final_norm / S.normalizer;
+        res.final_residual = true_rel >= 0.0 ? true_rel : S.final_norm / S.normalizer;
         res.history_len = prm.save_history ? S.hist_len : 0;
         res.actual_shift = pre ? actual_shift : 0.0;
-        if (!S.converged && prm.save_best) {
+        if (true_rel < 0.0 && !S.converged && prm.save_best) {
             HIP_CHECK(hipMemcpyAsync(x.p, best.p, vb, hipMemcpyDeviceToDevice, s));
             res.final_residual = S.best_residual;
         }
@@ -935,14 +1068,20 @@ int ssa_ic_create(ssa_context* ctx, int dtype, int n, const int32_t* row_ptr,
         st.abmc_num_colors = cfg->abmc_num_colors;
         auto h = std::make_unique<ssa_ic>();
         h->dtype = dtype;
+        if (cfg->precision != SSA_PRECISION_FP64 && cfg->precision != SSA_PRECISION_MIXED)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: unknown precision");
+        // mixed: the factor (computed in FP64) is stored and applied in
+        // float / complex<float>; SpMV, vectors and dots stay FP64
         const bool mixed = cfg->precision == SSA_PRECISION_MIXED;
-        if (mixed)
-            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: mixed precision is not available yet");
+        using cd = std::complex<double>;
         if (dtype == SSA_F64) {
-            h->impl.reset(new IcImpl<double, double>(ctx, n, row_ptr, col_idx, static_cast<const double*>(values), st));
+            const double* v = static_cast<const double*>(values);
+            if (mixed) h->impl.reset(new IcImpl<double, float>(ctx, n, row_ptr, col_idx, v, st));
+            else h->impl.reset(new IcImpl<double, double>(ctx, n, row_ptr, col_idx, v, st));
         } else {
-            using cd = std::complex<double>;
-            h->impl.reset(new IcImpl<ssa::c128, ssa::c128>(ctx, n, row_ptr, col_idx, static_cast<const cd*>(values), st));
+            const cd* v = static_cast<const cd*>(values);
+            if (mixed) h->impl.reset(new IcImpl<ssa::c128, ssa::c64>(ctx, n, row_ptr, col_idx, v, st));
+            else h->impl.reset(new IcImpl<ssa::c128, ssa::c128>(ctx, n, row_ptr, col_idx, v, st));
         }
         *out = h.release();
     });

@@ -207,12 +207,22 @@ class _SparseSolvSolver:
         self.abmc_num_colors = 4
         self.abmc_reorder_spmv = False
         self.abmc_use_rcm = False
-        # GPU-only knobs
+        # GPU-only knobs (no reference counterpart)
         self.precision = _lib.PRECISION_FP64
         self.use_graph = True
         self._last_result = SparseSolvResult()
         self._cache_key = None
         self._handle = None
+
+    @property
+    def mixed_precision(self) -> bool:
+        """IC factor and sweeps in float / complex<float> inside the FP64
+        Krylov loop (GPU extension; FP64 is the reference behaviour)."""
+        return self.precision == _lib.PRECISION_MIXED
+
+    @mixed_precision.setter
+    def mixed_precision(self, on: bool):
+        self.precision = _lib.PRECISION_MIXED if on else _lib.PRECISION_FP64
 
     @property
     def last_result(self) -> SparseSolvResult:

@@ -26,3 +26,6 @@ def test_bytes_per_iteration_model():
     # eddy 78^3, complex<double> COCR
     n, nnz = 1387386, 17821650
     assert bench.bytes_per_iteration(n, nnz, (nnz + n) // 2, 16, True) == 1_267_820_412
+    # the same system with a complex<float> factor and sweeps
+    assert bench.bytes_per_iteration(n, nnz, (nnz + n) // 2, 16, True, 8) == \
+        1_267_820_412 - 2 * ((nnz + n) // 2) * 8 - n * (2 * 8 + 8)

@@ -6,6 +6,7 @@ import pytest
 import scipy.sparse as sp
 
 import sparsesolv_amd as ss
+from sparsesolv_amd import _lib
 from sparsesolv_amd import problems as P
 from sparsesolv_amd.solvers import _with_freedofs
 
@@ -89,3 +90,18 @@ def test_freedofs_identity_rows():
                 assert D[i, j] == 0.0
             else:
                 assert D[i, j] == Ad[i, j]
+
+
+def test_mixed_precision_switch_and_host_setup():
+    s = P.poisson(8)
+    solver = ss.SparseSolvSolver(s.A, method="ICCG")
+    assert not solver.mixed_precision
+    solver.mixed_precision = True
+    assert solver.precision == _lib.PRECISION_MIXED
+    # a host-only mixed handle runs the same FP64 setup as an FP64 handle
+    f = []
+    for prec in (_lib.PRECISION_FP64, _lib.PRECISION_MIXED):
+        h = _lib.IcHandle(s.A, _lib.default_config(use_abmc=1, precision=prec), host_only=True)
+        f.append(h.factor())
+    for k in f[0]:
+        assert np.array_equal(f[0][k], f[1][k]), k
