@@ -34,8 +34,9 @@ The JSON line also carries
                 between two HIP events on the library's stream; the figures
                 are means over those colours, against the 8 TB/s HBM peak.
                 `traffic` is the HBM bytes per launch that rocprofv3's
-                FETCH_SIZE + WRITE_SIZE counters read for the same launches
-                (recorded beside the trace summary; null here);
+                FETCH_SIZE + WRITE_SIZE counters read for the same launches,
+                taken from the counter summary under profiles/ (null when
+                there is none for the workload);
   cpu_baseline  the reference (its own headers, OpenMP build) on this host's
                 cores, on the same system: the full solve when it fits the
                 time budget, else a bounded sample extrapolated.
@@ -166,6 +167,10 @@ def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
             f"converged={res.converged} residual={res.final_residual:.3e}")
     x = x_d.cpu().numpy()
     true_rel = float(np.linalg.norm(b - A.to_scipy() @ x) / np.linalg.norm(b))
+    # the same solve through host buffers (b in, x out over PCIe)
+    t = time.perf_counter()
+    h.solve(code, b, None, cfg)
+    host_s = time.perf_counter() - t
     sec = float(np.mean(times))
     it = int(np.median(iters))
     s = 16 if A.is_complex else 8
@@ -193,6 +198,7 @@ def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
         "seconds": sec, "iterations": it, "converged": bool(res.converged),
         "final_residual": res.final_residual, "true_rel_residual": true_rel,
         "setup_s": round(setup_wall, 3), "host_setup_s": round(info["setup_seconds"], 3),
+        "host_buffers_solve_s": round(host_s, 5),
         "setup_plus_solve_s": round(setup_wall + sec, 4),
         "iterations_per_s": round(it / sec, 1) if sec > 0 else None,
         "us_per_iteration": round(1e6 * sec / max(it, 1), 2),
@@ -203,12 +209,26 @@ def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
     return out, (A, b, kw, w["method"], x)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0):
     """The reference's own ICCG / COCR (its headers compiled with OpenMP) on
     this host.  When it fits `budget_s`: one full solve (solution compared
     with the GPU's), then the same number of iterations timed three times
     (setup and iterations apart, the fastest run kept).  Otherwise setup + a
     bounded number of iterations, extrapolated to the GPU's count."""
+    # thread placement for the OpenMP runtime, set before the library loads
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     kind = "reference" if pyoracle.available("ref_omp") else "port"
@@ -219,7 +239,7 @@ def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0):
     probe = 5
     t_probe, setup = orc.time_iterations(A, b, cfg, probe, mname)
     per_it = max(t_probe / probe, 1e-6)
-    out = {"unit": "s", "cores": orc.num_threads(), "kind": kind}
+    out = {"unit": "s", "cores": orc.num_threads(), "kind": kind, "cpu": cpu_model()}
     if 2 * setup + 4 * per_it * gpu_iters <= budget_s:
         x, r = (orc.solve_cocr if method == "cocr" else orc.solve_iccg)(A, b, cfg)
         its = max(int(r["iterations"]), 1)
@@ -245,6 +265,19 @@ def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0):
                        f"iterations; value excludes setup"),
                ms_per_iteration=round(per_it * 1e3, 3))
     return out
+
+
+def recorded_traffic(workload):
+    """HBM bytes per colour-kernel launch from the rocprofv3 counter summary
+    kept beside this script (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", f"colour_kernel_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def main():
@@ -298,6 +331,7 @@ def main():
         cpu = cpu_baseline(A, b, kw, method, main_out["iterations"], x_gpu)
         log(f"cpu baseline: {cpu}")
     ck = main_out["kernels"].get("colour_kernel")
+    traffic, traffic_src = recorded_traffic(args.workload)
     line = {
         "metric": METRIC,
         "value": round(main_out["seconds"], 6),
@@ -319,7 +353,8 @@ def main():
         "roofline": None if ck is None else {
             "bound": "hbm", "kernel": "k_sweep (ABMC colour kernel, forward and backward)",
             "achieved": ck["GB_s"], "peak": PEAK_GBS, "unit": "GB/s", "frac": ck["frac"],
-            "traffic": None, "bytes_per_launch": ck["bytes"], "us_per_launch": ck["us"]},
+            "traffic": traffic, "traffic_source": traffic_src,
+            "bytes_per_launch": ck["bytes"], "us_per_launch": ck["us"]},
         "cpu_baseline": cpu,
         "extra": extra,
     }
