@@ -25,7 +25,10 @@ Extra workloads (`extra`, skipped with --quick):
              complex<double>, block 4, 8 colours, tol 1e-10, 1000 iterations;
   *-mixed    each of the three with the IC factor and both sweeps in float /
              complex<float> inside the FP64 Krylov loop (SpMV, vectors, dots
-             and the final true-residual check in FP64).
+             and the final true-residual check in FP64);
+  *-level    curlcurl64 and laplace48 on the level-scheduled path
+             (use_abmc=False, the factories' default), with the reference's
+             own level path as the CPU baseline.
 
 The JSON line also carries
   roofline      the dominant kernel by time, the ABMC colour kernel k_sweep:
@@ -93,12 +96,19 @@ for _base in list(WORKLOADS):
         "; IC factor and sweeps in float / complex<float> (mixed precision)",
         method=_w["method"], cfg=dict(_w["cfg"], precision=1))
 
+for _base in ("curlcurl64", "laplace48"):
+    _w = WORKLOADS[_base]
+    WORKLOADS[_base + "-level"] = dict(
+        desc=_w["desc"].replace("ABMC-ICCG", "level-scheduled ICCG (use_abmc=False, the "
+                                             "factories' default)"),
+        method=_w["method"], cfg=dict(_w["cfg"], use_abmc=0))
+
 _SYSTEMS = {}
 
 
 def make_system(name):
     from sparsesolv_amd import problems as P
-    base = name.replace("-mixed", "")
+    base = name.replace("-mixed", "").replace("-level", "")
     if base not in _SYSTEMS:
         if base == "curlcurl64":
             _SYSTEMS[base] = P.curlcurl(64)
@@ -314,16 +324,16 @@ def main():
                                                  args.budget, args.profile_reps)
     extra = {"primary": main_out}
     if not args.quick:
-        for other in ("curlcurl64-mixed", "laplace48", "laplace48-mixed", "eddy78",
-                      "eddy78-mixed"):
+        for other in ("curlcurl64-mixed", "curlcurl64-level", "laplace48", "laplace48-mixed",
+                      "laplace48-level", "eddy78", "eddy78-mixed"):
             if other == args.workload:
                 continue
             o, (A2, b2, kw2, m2, x2) = run_gpu(other, max(1, args.steps), 1, args.budget,
                                                args.profile_reps)
-            if not args.no_cpu and other == "laplace48":
-                log("cpu baseline (laplace48) ...")
+            if not args.no_cpu and other in ("laplace48", "laplace48-level", "curlcurl64-level"):
+                log(f"cpu baseline ({other}) ...")
                 o["cpu_baseline"] = cpu_baseline(A2, b2, kw2, m2, o["iterations"], x2)
-                log(f"cpu baseline (laplace48): {o['cpu_baseline']}")
+                log(f"cpu baseline ({other}): {o['cpu_baseline']}")
             extra[other] = o
     cpu = None
     if not args.no_cpu:

@@ -1,5 +1,6 @@
 // host_setup.cpp -- host half of ssa_ic_create: ABMC schedule, IC(0)
 // factor and HBM layouts (compiled by g++ with -ffp-contract=off).
+#include <algorithm>
 #include <chrono>
 
 #include "host_plan.hpp"
@@ -11,6 +12,78 @@ static F narrow(const T& v) { return static_cast<F>(v); }
 template <>
 cf narrow<cf, cd>(const cd& v) { return cf(float(v.real()), float(v.imag())); }
 
+// Rows moved to ord[row], columns renamed to ord[col], each row's entries
+// kept in their original order (so the diagonal stays where the sweep
+// expects it and every row sums its terms in the reference's order).
+template <typename T>
+static sparsesolv_amd::CsrHost<T> relabel(const sparsesolv_amd::CsrHost<T>& M,
+                                          const std::vector<int>& ord)
+{
+    const int n = M.n;
+    std::vector<int> inv(n);
+    for (int i = 0; i < n; ++i) inv[ord[i]] = i;
+    sparsesolv_amd::CsrHost<T> R;
+    R.n = n;
+    R.rp.assign(size_t(n) + 1, 0);
+    for (int r = 0; r < n; ++r) R.rp[r + 1] = R.rp[r] + (M.rp[inv[r] + 1] - M.rp[inv[r]]);
+    R.ci.resize(M.ci.size());
+    R.v.resize(M.v.size());
+    for (int r = 0; r < n; ++r) {
+        int pos = R.rp[r];
+        for (int k = M.rp[inv[r]]; k < M.rp[inv[r] + 1]; ++k, ++pos) {
+            R.ci[pos] = ord[M.ci[k]];
+            R.v[pos] = M.v[k];
+        }
+    }
+    return R;
+}
+
+// Level schedule of the natural-order factor (core/level_schedule.hpp:58-89):
+// level(i) = 1 + max level of the columns j < i of row i of L.  Rows are
+// renumbered level by level (natural order inside a level); every level is a
+// "colour" of one-row blocks, so the colour sweeps run it unchanged.  The
+// backward sweep takes the same levels in reverse: every row that row i of
+// U = L^T depends on has a higher forward level.
+static void level_schedule(const sparsesolv_amd::CsrHost<double>* Lr,
+                           const sparsesolv_amd::CsrHost<cd>* Lc, int n,
+                           sparsesolv_amd::AbmcSchedule& S)
+{
+    const std::vector<int>& rp = Lr ? Lr->rp : Lc->rp;
+    const std::vector<int>& ci = Lr ? Lr->ci : Lc->ci;
+    std::vector<int> level(n, 0);
+    int nlev = 0;
+    for (int i = 0; i < n; ++i) {
+        int m = -1;
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] < i) m = std::max(m, level[ci[k]]);
+        level[i] = m + 1;
+        nlev = std::max(nlev, level[i] + 1);
+    }
+    std::vector<int> count(size_t(nlev) + 1, 0);
+    for (int i = 0; i < n; ++i) ++count[level[i] + 1];
+    for (int l = 0; l < nlev; ++l) count[l + 1] += count[l];
+    S.color_offsets = count;                  // blocks are single rows
+    S.block_offsets.resize(size_t(n) + 1);
+    for (int i = 0; i <= n; ++i) S.block_offsets[i] = i;
+    S.ordering.assign(n, 0);
+    S.reverse_ordering.assign(n, 0);
+    std::vector<int> next(count.begin(), count.end() - 1);
+    for (int i = 0; i < n; ++i) {
+        const int r = next[level[i]]++;
+        S.ordering[i] = r;
+        S.reverse_ordering[r] = i;
+    }
+}
+
+template <typename T>
+static const sparsesolv_amd::CsrHost<double>* as_real(const sparsesolv_amd::CsrHost<T>&) { return nullptr; }
+template <>
+const sparsesolv_amd::CsrHost<double>* as_real<double>(const sparsesolv_amd::CsrHost<double>& M) { return &M; }
+template <typename T>
+static const sparsesolv_amd::CsrHost<cd>* as_complex(const sparsesolv_amd::CsrHost<T>&) { return nullptr; }
+template <>
+const sparsesolv_amd::CsrHost<cd>* as_complex<cd>(const sparsesolv_amd::CsrHost<cd>& M) { return &M; }
+
 template <typename T, typename F>
 void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_idx,
                 const T* values, const sparsesolv_amd::IcSettings& settings)
@@ -18,20 +91,38 @@ void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     plan.fac.build(n, row_ptr, col_idx, values, settings, /*keep_permuted_matrix=*/true);
+    // the device works in the schedule's row order: ABMC as built, or the
+    // level order of the natural-order factor
+    const sparsesolv_amd::CsrHost<T>* Ldev = &plan.fac.L;
+    const sparsesolv_amd::CsrHost<T>* Udev = &plan.fac.U;
+    sparsesolv_amd::CsrHost<T> Lp, Up;
+    std::vector<T> invd_dev = plan.fac.inv_diag;
+    int block_size = settings.abmc_block_size;
+    if (!settings.use_abmc) {
+        level_schedule(as_real(plan.fac.L), as_complex(plan.fac.L), n, plan.fac.sched);
+        const std::vector<int>& ord = plan.fac.sched.ordering;
+        Lp = relabel(plan.fac.L, ord);
+        Up = relabel(plan.fac.U, ord);
+        plan.fac.A_perm = relabel(plan.fac.A_perm, ord);
+        for (int i = 0; i < n; ++i) invd_dev[ord[i]] = plan.fac.inv_diag[i];
+        Ldev = &Lp;
+        Udev = &Up;
+        block_size = 1;
+    }
     plan.spmv.build(plan.fac.A_perm);
     // the sweep layouts are built from the factor converted to F
     sparsesolv_amd::CsrHost<F> L, U;
     L.n = U.n = n;
-    L.rp = plan.fac.L.rp; L.ci = plan.fac.L.ci;
-    U.rp = plan.fac.U.rp; U.ci = plan.fac.U.ci;
-    L.v.resize(plan.fac.L.v.size());
-    U.v.resize(plan.fac.U.v.size());
-    for (size_t k = 0; k < L.v.size(); ++k) L.v[k] = narrow<F>(plan.fac.L.v[k]);
-    for (size_t k = 0; k < U.v.size(); ++k) U.v[k] = narrow<F>(plan.fac.U.v[k]);
+    L.rp = Ldev->rp; L.ci = Ldev->ci;
+    U.rp = Udev->rp; U.ci = Udev->ci;
+    L.v.resize(Ldev->v.size());
+    U.v.resize(Udev->v.size());
+    for (size_t k = 0; k < L.v.size(); ++k) L.v[k] = narrow<F>(Ldev->v[k]);
+    for (size_t k = 0; k < U.v.size(); ++k) U.v[k] = narrow<F>(Udev->v[k]);
     plan.invd.resize(n);
-    for (int i = 0; i < n; ++i) plan.invd[i] = narrow<F>(plan.fac.inv_diag[i]);
-    plan.fwd.build(plan.fac.sched, L, /*forward=*/true, settings.abmc_block_size, plan.invd);
-    plan.bwd.build(plan.fac.sched, U, /*forward=*/false, settings.abmc_block_size, plan.invd);
+    for (int i = 0; i < n; ++i) plan.invd[i] = narrow<F>(invd_dev[i]);
+    plan.fwd.build(plan.fac.sched, L, /*forward=*/true, block_size, plan.invd);
+    plan.bwd.build(plan.fac.sched, U, /*forward=*/false, block_size, plan.invd);
     plan.seconds = std::chrono::duration<double>(clk::now() - t0).count();
 }
 

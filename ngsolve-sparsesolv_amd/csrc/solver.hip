@@ -1041,9 +1041,7 @@ int ssa_ic_create(ssa_context* ctx, int dtype, int n, const int32_t* row_ptr,
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: bad matrix");
         if (dtype != SSA_F64 && dtype != SSA_C128)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: dtype must be SSA_F64 or SSA_C128");
-        if (!cfg->use_abmc)
-            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: the GPU path requires use_abmc=1");
-        if (cfg->abmc_use_rcm)
+        if (cfg->use_abmc && cfg->abmc_use_rcm)
             throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: abmc_use_rcm is not supported");
         if (cfg->diagonal_scaling)
             throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: diagonal_scaling is not supported");

@@ -333,3 +333,54 @@ def test_profile_kinds(m, colour_kernels):
     y1 = h.apply(x)
     h.profile(_lib.PROFILE_COLOUR_KERNELS, 1)
     assert np.array_equal(h.apply(x), y1)
+
+
+# ---------------------------------------------------------------------------
+# Level-schedule path (use_abmc = false, the factories' default)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name,kw", [("lap12", {}), ("cc8", dict(auto_shift=1)),
+                                     ("eddy8", {})])
+def test_level_apply_matches_oracle(oracle, name, kw):
+    A = {"lap12": lambda: P.poisson(12).A, "cc8": lambda: P.curlcurl(8).A,
+         "eddy8": lambda: P.eddy(8).A}[name]()
+    h = _lib.IcHandle(A, _lib.default_config(use_abmc=0, **kw))
+    x = P.uniform(3, A.n, -1.0, 1.0)
+    if A.is_complex:
+        x = x + 1j * P.uniform(4, A.n, -1.0, 1.0)
+    y = h.apply(x)
+    yo = oracle.ic(A, oracle.config(use_abmc=0, **kw)).apply(x)
+    assert rel(y, yo) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["lap20", "varcoef16", "zlap12"])
+def test_level_solve_matches_oracle(oracle, case):
+    if case == "lap20":
+        s = P.poisson(20)
+        A, b, method = s.A, s.b, "iccg"
+    elif case == "varcoef16":
+        coef = 10.0 ** P.uniform(11, 16 ** 3, -1.0, 1.0)
+        A, b, method = P.laplace7_varcoef(16, coef), P.uniform(12, 16 ** 3, -1.0, 1.0), "iccg"
+    else:
+        A, b = complex_shifted_laplace(12)
+        method = "cocr"
+    cfg = _lib.default_config(use_abmc=0, max_iterations=2000)
+    h = _lib.IcHandle(A, cfg)
+    code = _lib.METHOD_ICCG if method == "iccg" else _lib.METHOD_ICCOCR
+    x, res, _ = h.solve(code, b, None, cfg)
+    fn = oracle.solve_iccg if method == "iccg" else oracle.solve_cocr
+    xo, ro = fn(A, b, oracle.config(use_abmc=0, max_iterations=2000))
+    assert res.converged and ro["converged"]
+    assert abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(x, xo) < 1e-9
+
+
+def test_level_path_is_the_dropin_default(oracle):
+    import sparsesolv_amd as ss
+    s = P.poisson(14)
+    solver = ss.SparseSolvSolver(s.A, method="ICCG")          # use_abmc defaults to False
+    assert solver.use_abmc is False
+    sol = np.zeros(s.A.n)
+    res = solver.Solve(s.b, sol)
+    xo, ro = oracle.solve_iccg(s.A, s.b, oracle.config())
+    assert res.converged and abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(sol, xo) < 1e-9

@@ -74,3 +74,38 @@ def test_layout_info_reports_big_block():
     co, bo = h.schedule()
     sizes = np.diff(bo)
     assert len(co) - 1 == 8 and sizes.max() == 366
+
+
+LEVEL_CASES = [
+    ("lap9", lambda: P.laplace7(9), dict()),
+    ("cc6_auto", lambda: P.curlcurl(6).A, dict(auto_shift=1)),
+    ("lap8_auto03", lambda: P.laplace7(8), dict(shift_parameter=0.3, auto_shift=1)),
+    ("eddy6", lambda: P.eddy(6).A, dict()),
+]
+
+
+@pytest.mark.parametrize("name,builder,kw", LEVEL_CASES, ids=[c[0] for c in LEVEL_CASES])
+def test_level_path_host_setup(oracle, name, builder, kw):
+    # use_abmc = false: the factor is the natural-order one, bit for bit
+    A = builder()
+    h = _lib.IcHandle(A, _lib.default_config(use_abmc=0, **kw), host_only=True)
+    f = oracle.ic(A, oracle.config(use_abmc=0, **kw)).factor()
+    g = h.factor()
+    for k in ("L_row_ptr", "L_col_idx", "L_values", "inv_diag"):
+        np.testing.assert_array_equal(g[k], f[k], err_msg=k)
+    assert g["actual_shift"] == f["actual_shift"]
+    # the device order is the level order of core/level_schedule.hpp:58-89:
+    # level(i) = 1 + max level(j) over the columns j < i of row i of L,
+    # rows grouped by level, natural order inside a level
+    rp, ci = f["L_row_ptr"], f["L_col_idx"]
+    level = np.zeros(A.n, dtype=np.int64)
+    for i in range(A.n):
+        js = ci[rp[i]:rp[i + 1]]
+        js = js[js < i]
+        level[i] = level[js].max() + 1 if len(js) else 0
+    expect = np.lexsort((np.arange(A.n), level))       # new -> old
+    ordering = h.ordering()
+    np.testing.assert_array_equal(np.argsort(ordering), expect)
+    co, bo = h.schedule()
+    np.testing.assert_array_equal(np.diff(co), np.bincount(level))
+    np.testing.assert_array_equal(bo, np.arange(A.n + 1))
