@@ -157,6 +157,26 @@ int ssa_spmv(ssa_ic* ic, const void* x, void* y, int on_device);
 int ssa_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device,
               const ssa_config* cfg, ssa_result* result, double* history);
 
+/* Several independent systems in one handle.  No reference counterpart:
+ * the reference builds one ICPreconditioner per system; this is that setup
+ * for `count` systems at once, for callers that solve many systems
+ * (parameter studies, frequency sweeps).  System s is n[s], row_ptr[s],
+ * col_idx[s], values[s]; all share cfg (FP64 only).  Vectors passed to
+ * ssa_ic_apply, ssa_spmv and ssa_batch_solve hold the systems back to back
+ * in their own orderings (sum of n[s] entries).  The ordering / schedule /
+ * factor queries are for one-system handles only. */
+int ssa_batch_create(ssa_context* ctx, int dtype, int count, const int32_t* n,
+                     const int32_t* const* row_ptr, const int32_t* const* col_idx,
+                     const void* const* values, const ssa_config* cfg, ssa_ic** out);
+/* solve_iccg / solve_cocr (or unpreconditioned CG / COCR) for every system
+ * of a handle at once: one sequence of launches covers all systems, each
+ * with its own scalars and stop status, and results[s] (count entries) is
+ * what ssa_solve gives for system s alone, bit for bit. */
+int ssa_batch_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device,
+                    const ssa_config* cfg, ssa_result* results);
+/* Number of systems of a handle (1 unless made by ssa_batch_create). */
+int ssa_ic_systems(const ssa_ic* ic, int* count);
+
 /* Kernel timing with HIP events on the handle's stream (no reference
  * counterpart: measurement hook for the roofline report).  what:
  *   SSA_PROFILE_SPMV    one SpMV + fused dot over the Krylov vectors;

@@ -91,6 +91,41 @@ struct SpmvLayout {
     }
 };
 
+// Several systems in one SELL-64 layout: system s occupies the rows
+// [row0[s], row0[s+1]); each system's slice count is padded to a multiple of
+// `align` slices (empty slices) so no group of `align` slices spans two
+// systems.  Columns are shifted by the system's first row.
+template <typename F>
+SpmvLayout<F> combine_spmv(const std::vector<const SpmvLayout<F>*>& parts, index_t align,
+                           std::vector<index_t>& row0)
+{
+    SpmvLayout<F> out;
+    row0.assign(1, 0);
+    out.slice_off.assign(1, 0);
+    for (const SpmvLayout<F>* P : parts) {
+        const index_t r0 = row0.back();
+        for (index_t s = 0; s < P->nslices; ++s) {
+            const index_t K = P->slice_off[s + 1] - P->slice_off[s];
+            const size_t src = size_t(P->slice_off[s]) * kWave;
+            const size_t dst = size_t(out.slice_off.back()) * kWave;
+            out.val.insert(out.val.end(), P->val.begin() + src, P->val.begin() + src + size_t(K) * kWave);
+            out.col.resize(dst + size_t(K) * kWave);
+            for (size_t q = 0; q < size_t(K) * kWave; ++q) out.col[dst + q] = P->col[src + q] + r0;
+            out.slice_off.push_back(out.slice_off.back() + K);
+        }
+        index_t ns = P->nslices;
+        while (ns % align) {
+            out.slice_off.push_back(out.slice_off.back());
+            ++ns;
+        }
+        row0.push_back(r0 + ns * kWave);
+    }
+    out.nslices = index_t(out.slice_off.size()) - 1;
+    out.n = row0.back();
+    if (out.val.empty()) { out.val.assign(kWave, F(0)); out.col.assign(kWave, 0); }
+    return out;
+}
+
 // One triangular sweep (forward over L or backward over L^T).
 template <typename F>
 struct SweepLayout {
@@ -406,6 +441,125 @@ private:
         if (walk_ext_ci.empty()) { walk_ext_ci.push_back(0); walk_ext_v.push_back(F(0)); }
         if (walk_v.empty()) { walk_v.push_back(F(0)); walk_ix.push_back(0); }
         (void)M; (void)forward;
+    }
+
+public:
+    // Several systems' sweeps in one layout: phase p holds phase p of every
+    // system that has one (systems are independent, so their colours can
+    // run side by side); rows and external columns are shifted by the
+    // system's first row (row0[s]).  slice_sys / walk_sys name the system of
+    // every slice / oversized block.  Oversized blocks must use the walk
+    // (walk_kmax > 0); the level-kernel fallback arrays are left empty.
+    static SweepLayout combine(const std::vector<const SweepLayout*>& parts,
+                               const std::vector<index_t>& row0, std::vector<int>& slice_sys,
+                               std::vector<int>& walk_sys)
+    {
+        SweepLayout out;
+        slice_sys.clear();
+        walk_sys.clear();
+        bool any_big = false;
+        for (const SweepLayout* P : parts) {
+            if (P->BR == 0) throw std::runtime_error("batch: blocks wider than 16 rows are not supported");
+            out.BR = std::max(out.BR, P->BR);
+            out.nphases = std::max(out.nphases, P->nphases);
+            out.max_big_rows = std::max(out.max_big_rows, P->max_big_rows);
+            out.max_levels = std::max(out.max_levels, P->max_levels);
+            if (!P->phase_big.empty() && P->phase_big.back() > 0) {
+                if (P->walk_kmax == 0)
+                    throw std::runtime_error("batch: an oversized block needs more than 8 in-block entries per row");
+                any_big = true;
+                out.walk_kmax = std::max(out.walk_kmax, P->walk_kmax);
+            }
+        }
+        const int K = out.walk_kmax;
+        out.phase_slices.assign(1, 0);
+        out.phase_big.assign(1, 0);
+        out.slice_slot.assign(1, 0);
+        out.walk_step_off.assign(1, 0);
+        out.walk_ext_rp.assign(1, 0);
+        out.phase_rows.assign(size_t(out.nphases), 0);
+        out.phase_entries.assign(size_t(out.nphases), 0);
+        for (index_t p = 0; p < out.nphases; ++p) {
+            for (size_t si = 0; si < parts.size(); ++si) {
+                const SweepLayout& P = *parts[si];
+                if (p >= P.nphases) continue;
+                const index_t r0 = row0[si];
+                out.phase_rows[p] += P.phase_rows[p];
+                out.phase_entries[p] += P.phase_entries[p];
+                for (index_t sl = P.phase_slices[p]; sl < P.phase_slices[p + 1]; ++sl) {
+                    const index_t Ks = P.slice_slot[sl + 1] - P.slice_slot[sl];
+                    const size_t src = size_t(P.slice_slot[sl]) * kWave;
+                    const size_t dst = size_t(out.slice_slot.back()) * kWave;
+                    out.slice_slot.push_back(out.slice_slot.back() + Ks);
+                    out.val.insert(out.val.end(), P.val.begin() + src, P.val.begin() + src + size_t(Ks) * kWave);
+                    out.col.resize(dst + size_t(Ks) * kWave);
+                    for (size_t q = 0; q < size_t(Ks) * kWave; ++q) {
+                        const index_t c = P.col[src + q];
+                        out.col[dst + q] = c >= 0 ? c + r0 : c;
+                    }
+                    for (int l = 0; l < kWave; ++l) {
+                        const index_t r = P.lane_row[size_t(sl) * kWave + l];
+                        out.lane_row.push_back(r >= 0 ? r + r0 : r);
+                        out.lane_pos.push_back(P.lane_pos[size_t(sl) * kWave + l]);
+                    }
+                    slice_sys.push_back(int(si));
+                }
+            }
+            out.phase_slices.push_back(index_t(slice_sys.size()));
+            for (size_t si = 0; si < parts.size(); ++si) {
+                const SweepLayout& P = *parts[si];
+                if (p >= P.nphases) continue;
+                const index_t r0 = row0[si];
+                for (index_t bb = P.phase_big[p]; bb < P.phase_big[p + 1]; ++bb) {
+                    for (index_t st = P.walk_step_off[bb]; st < P.walk_step_off[bb + 1]; ++st) {
+                        for (int l = 0; l < kWave; ++l) {
+                            const size_t t = size_t(st) * kWave + l;
+                            const index_t r = P.walk_row[t];
+                            out.walk_row.push_back(r >= 0 ? r + r0 : r);
+                            out.walk_pos.push_back(P.walk_pos[t]);
+                            out.walk_invd.push_back(P.walk_invd[t]);
+                            for (index_t e = P.walk_ext_rp[t]; e < P.walk_ext_rp[t + 1]; ++e) {
+                                out.walk_ext_ci.push_back(P.walk_ext_ci[e] + r0);
+                                out.walk_ext_v.push_back(P.walk_ext_v[e]);
+                            }
+                            out.walk_ext_rp.push_back(index_t(out.walk_ext_ci.size()));
+                        }
+                        const size_t vbase = out.walk_v.size();
+                        out.walk_v.resize(vbase + size_t(K) * kWave, F(0));
+                        out.walk_ix.resize(vbase + size_t(K) * kWave, out.max_big_rows);
+                        for (int k = 0; k < P.walk_kmax; ++k)
+                            for (int l = 0; l < kWave; ++l) {
+                                const size_t a = (size_t(st) * P.walk_kmax + k) * kWave + l;
+                                const index_t ix = P.walk_ix[a];
+                                out.walk_v[vbase + size_t(k) * kWave + l] = P.walk_v[a];
+                                // the system's zero slot becomes the combined one
+                                out.walk_ix[vbase + size_t(k) * kWave + l] =
+                                    ix == P.max_big_rows ? out.max_big_rows : ix;
+                            }
+                    }
+                    out.walk_step_off.push_back(out.walk_step_off.back() +
+                                                (P.walk_step_off[bb + 1] - P.walk_step_off[bb]));
+                    walk_sys.push_back(int(si));
+                }
+            }
+            out.phase_big.push_back(index_t(walk_sys.size()));
+        }
+        if (any_big)
+            for (int k = 0; k < kWalkDepth; ++k) out.push_idle_step();
+        out.val.resize(out.val.size() + size_t(kSweepChunk) * kWave, F(0));
+        out.col.resize(out.col.size() + size_t(kSweepChunk) * kWave, kPadCol);
+        out.big_pos.assign(size_t(walk_sys.size()) + 1, 0);
+        out.big_lvl_off.assign(size_t(walk_sys.size()) + 1, 0);
+        out.big_lvl.assign(1, 0);
+        out.big_rows.assign(1, 0);
+        out.big_rp.assign(1, 0);
+        out.big_ci.assign(1, 0);
+        out.big_v.assign(1, F(0));
+        if (out.lane_row.empty()) { out.lane_row.assign(kWave, -1); out.lane_pos.assign(kWave, 0); }
+        if (out.walk_row.empty()) { out.walk_row.assign(1, -1); out.walk_pos.assign(1, -1); out.walk_invd.assign(1, F(0)); out.walk_ext_rp.push_back(0); }
+        if (out.walk_ext_ci.empty()) { out.walk_ext_ci.push_back(0); out.walk_ext_v.push_back(F(0)); }
+        if (out.walk_v.empty()) { out.walk_v.push_back(F(0)); out.walk_ix.push_back(0); }
+        return out;
     }
 };
 

@@ -133,9 +133,58 @@ struct Ctrl {
     double* hist;
     const void* part;      // the producing kernel's partials
     int np;                // their count
-    int pad_;
+    int src_spmv;          // batches: the producer ran on the SpMV grid
     SolveParams prm;
+    int part_off;          // batches: first partial of the workgroup's system
+    int first;             // batches: the workgroup is its system's first
 };
+
+// Several independent systems in one launch (null pointers: one system).
+// System s owns the rows [row0[s], row0[s] + nrows[s]); on the vector grid
+// its workgroups are [vec_g0[s], vec_g0[s+1]) and cover its rows exactly
+// as a one-system launch of that many workgroups would (same partials,
+// same bits); on the SpMV grid its workgroups are [spmv_g0[s], ...).
+struct Systems {
+    const int* vec_sys;
+    const int* vec_g0;
+    const int* spmv_sys;
+    const int* spmv_g0;
+    const int* row0;
+    const int* nrows;
+    int hist_stride;
+};
+
+// The rows a vector-grid workgroup owns: i = local*kBlock + lane, stepping
+// by G*kBlock while i < n, at device row row0 + i.
+struct Span {
+    index_t row0, n, local, G;
+    int sys;
+};
+
+__device__ inline Span vec_span(const Systems& y, index_t n)
+{
+    if (!y.vec_sys) return Span{0, n, index_t(blockIdx.x), index_t(gridDim.x), 0};
+    const int s = y.vec_sys[blockIdx.x];
+    const int g0 = y.vec_g0[s];
+    return Span{y.row0[s], y.nrows[s], index_t(blockIdx.x) - g0, index_t(y.vec_g0[s + 1] - g0), s};
+}
+
+// the control step as seen from a workgroup of system `sys` whose index
+// among the system's workgroups is `local`
+__device__ inline Ctrl ctrl_for(Ctrl c, const Systems& y, int sys, index_t local)
+{
+    c.first = local == 0;
+    if (!y.vec_sys) {
+        c.part_off = 0;
+        return c;
+    }
+    const int* g0 = c.src_spmv ? y.spmv_g0 : y.vec_g0;
+    c.st += sys;
+    if (c.hist) c.hist += size_t(sys) * size_t(y.hist_stride);
+    c.part_off = g0[sys];
+    c.np = g0[sys + 1] - g0[sys];
+    return c;
+}
 
 // sum of np partials, valid in every thread
 template <typename T>
@@ -154,19 +203,20 @@ __device__ inline T sum_partials_all(const T* __restrict__ p, int np, T* red)
 template <typename T>
 __device__ inline bool ctrl_alpha_all(const Ctrl& c, T* red, T& alpha)
 {
-    const T d = sum_partials_all(static_cast<const T*>(c.part), c.np, red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl_alpha_op(c.st, c.prm, c.hist, d);
+    const T d = sum_partials_all(static_cast<const T*>(c.part) + c.part_off, c.np, red);
+    if (c.first && threadIdx.x == 0) ctrl_alpha_op(c.st, c.prm, c.hist, d);
     if (dabs(d) < 1e-30) return false;
     alpha = ddiv(from_pair<T>(c.st->rz), d);
     return true;
 }
 
-// convergence check from the |r|^2 partials, in workgroup 0 only; every
-// thread of the kernel must call it (it synchronises workgroup 0)
+// convergence check from the |r|^2 partials, in the system's first
+// workgroup only; every thread of the kernel must call it (it synchronises
+// that workgroup)
 __device__ inline void ctrl_check_wg0(const Ctrl& c, double* red)
 {
-    if (blockIdx.x != 0) return;
-    const double s = sum_partials(static_cast<const double*>(c.part), c.np, red);
+    if (!c.first) return;
+    const double s = sum_partials(static_cast<const double*>(c.part) + c.part_off, c.np, red);
     if (threadIdx.x != 0) return;
     c.st->rz_prev[0] = c.st->rz[0];
     c.st->rz_prev[1] = c.st->rz[1];
@@ -177,9 +227,9 @@ __device__ inline void ctrl_check_wg0(const Ctrl& c, double* red)
 template <typename T>
 __device__ inline bool ctrl_beta_all(const Ctrl& c, T* red, T& beta)
 {
-    const T v = sum_partials_all(static_cast<const T*>(c.part), c.np, red);
+    const T v = sum_partials_all(static_cast<const T*>(c.part) + c.part_off, c.np, red);
     const T old = from_pair<T>(c.st->rz_prev);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl_beta_op(c.st, c.prm, v, old);
+    if (c.first && threadIdx.x == 0) ctrl_beta_op(c.st, c.prm, v, old);
     if (c.prm.cocr && dabs(old) < 1e-30) return false;
     beta = ddiv(v, old);
     return true;
@@ -195,8 +245,18 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_init(const double* __rest
                                                             const double* __restrict__ pr,
                                                             int np, SolverState* st,
                                                             SolveParams prm,
-                                                            double* __restrict__ hist)
+                                                            double* __restrict__ hist,
+                                                            Systems y)
 {
+    // one workgroup per system; its partials are those of its vector grid
+    if (y.vec_sys) {
+        const int s = blockIdx.x, g0 = y.vec_g0[s];
+        pb += g0;
+        pr += g0;
+        np = y.vec_g0[s + 1] - g0;
+        st += s;
+        if (hist) hist += size_t(s) * size_t(y.hist_stride);
+    }
     __shared__ double red[kCtrlThreads / 64];
     const double sb = sum_partials(pb, np, red);
     const double sr = sum_partials(pr, np, red);
@@ -230,8 +290,16 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_init(const double* __rest
 // rz_old (CG) or rho (COCR) from dot partials into slot 0 (rz) or 1 (denom)
 template <typename T>
 __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_store(const T* __restrict__ part, int np,
-                                                            SolverState* st, int slot)
+                                                            SolverState* st, int slot,
+                                                            Systems y, int src_spmv)
 {
+    if (y.vec_sys) {
+        const int s = blockIdx.x;
+        const int* g0 = src_spmv ? y.spmv_g0 : y.vec_g0;
+        part += g0[s];
+        np = g0[s + 1] - g0[s];
+        st += s;
+    }
     __shared__ T red[kCtrlThreads / 64];
     const T v = sum_partials(part, np, red);
     if (threadIdx.x != 0) return;

@@ -1,7 +1,11 @@
 // host_setup.cpp -- host half of ssa_ic_create: ABMC schedule, IC(0)
 // factor and HBM layouts (compiled by g++ with -ffp-contract=off).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <stdexcept>
+#include <string>
+#include <thread>
 
 #include "host_plan.hpp"
 
@@ -126,6 +130,88 @@ void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_
     plan.seconds = std::chrono::duration<double>(clk::now() - t0).count();
 }
 
+template <typename T, typename F>
+void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const* row_ptr,
+                 const int* const* col_idx, const T* const* values,
+                 const sparsesolv_amd::IcSettings& settings)
+{
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    if (count < 1) throw std::invalid_argument("batch: count must be >= 1");
+    bp = BatchPlan<T, F>{};
+    bp.count = count;
+    std::vector<HostPlan<T, F>> plans(static_cast<size_t>(count));
+    // independent systems: planned on host threads, errors re-raised in order
+    std::vector<std::string> errors(static_cast<size_t>(count));
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int s; (s = next.fetch_add(1)) < count;) {
+            try {
+                build_plan<T, F>(plans[s], n[s], row_ptr[s], col_idx[s], values[s], settings);
+            } catch (const std::exception& e) {
+                errors[s] = e.what();
+            }
+        }
+    };
+    const int nthreads = std::max(1, std::min<int>(count, int(std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int k = 1; k < nthreads; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    for (int s = 0; s < count; ++s)
+        if (!errors[s].empty()) throw std::runtime_error(errors[s]);
+
+    for (const auto& P : plans) {
+        bp.actual_shift.push_back(P.fac.actual_shift);
+        bp.shift_trials.push_back(P.fac.shift_trials);
+        bp.seconds.push_back(P.seconds);
+        bp.num_colors.push_back(P.fac.sched.num_colors());
+        bp.num_blocks.push_back(P.fac.sched.num_blocks());
+        bp.nnz.push_back(int(P.fac.A_perm.ci.size()));
+        bp.nnz_l.push_back(int(P.fac.L.v.size()));
+    }
+    bp.in_off.assign(1, 0);
+    for (int s = 0; s < count; ++s) {
+        bp.nrows.push_back(n[s]);
+        bp.in_off.push_back(bp.in_off.back() + n[s]);
+    }
+    if (count == 1) {
+        HostPlan<T, F>& P = plans[0];
+        bp.row0 = {0, n[0]};
+        bp.gather_in = P.fac.sched.reverse_ordering;
+        bp.gather_out = P.fac.sched.ordering;
+        bp.spmv = P.spmv;
+        bp.fwd = P.fwd;
+        bp.bwd = P.bwd;
+        bp.invd = P.invd;
+        bp.single = std::move(P);
+    } else {
+        std::vector<const sparsesolv_amd::SpmvLayout<T>*> sp;
+        std::vector<const sparsesolv_amd::SweepLayout<F>*> lf, lb;
+        for (const auto& P : plans) {
+            sp.push_back(&P.spmv);
+            lf.push_back(&P.fwd);
+            lb.push_back(&P.bwd);
+        }
+        bp.spmv = sparsesolv_amd::combine_spmv(sp, 4, bp.row0);
+        bp.fwd = sparsesolv_amd::SweepLayout<F>::combine(lf, bp.row0, bp.slice_sys_f, bp.walk_sys_f);
+        bp.bwd = sparsesolv_amd::SweepLayout<F>::combine(lb, bp.row0, bp.slice_sys_b, bp.walk_sys_b);
+        const int N = bp.row0.back();
+        bp.invd.assign(size_t(N), F(0));
+        bp.gather_in.assign(size_t(N), -1);
+        bp.gather_out.assign(size_t(bp.in_off.back()), 0);
+        for (int s = 0; s < count; ++s) {
+            const auto& S = plans[s].fac.sched;
+            for (int i = 0; i < n[s]; ++i) {
+                bp.invd[bp.row0[s] + i] = plans[s].invd[i];
+                bp.gather_in[bp.row0[s] + i] = bp.in_off[s] + S.reverse_ordering[i];
+                bp.gather_out[bp.in_off[s] + i] = bp.row0[s] + S.ordering[i];
+            }
+        }
+    }
+    bp.wall_seconds = std::chrono::duration<double>(clk::now() - t0).count();
+}
+
 template void build_plan<double, double>(HostPlan<double, double>&, int, const int*,
                                          const int*, const double*,
                                          const sparsesolv_amd::IcSettings&);
@@ -136,5 +222,15 @@ template void build_plan<cd, cd>(HostPlan<cd, cd>&, int, const int*, const int*,
                                  const sparsesolv_amd::IcSettings&);
 template void build_plan<cd, cf>(HostPlan<cd, cf>&, int, const int*, const int*, const cd*,
                                  const sparsesolv_amd::IcSettings&);
+
+#define SSA_DEFINE_BATCH(T, F)                                                                 \
+    template void build_batch<T, F>(BatchPlan<T, F>&, int, const int*, const int* const*,      \
+                                    const int* const*, const T* const*,                        \
+                                    const sparsesolv_amd::IcSettings&);
+SSA_DEFINE_BATCH(double, double)
+SSA_DEFINE_BATCH(double, float)
+SSA_DEFINE_BATCH(cd, cd)
+SSA_DEFINE_BATCH(cd, cf)
+#undef SSA_DEFINE_BATCH
 
 }  // namespace ssa_host

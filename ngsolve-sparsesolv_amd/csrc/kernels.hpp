@@ -30,9 +30,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const index_t* __restrict__ slice_off, const T* __restrict__ val,
     const index_t* __restrict__ col, index_t n, index_t nslices,
     const T* __restrict__ x, T* __restrict__ y, T* __restrict__ partials,
-    const SolverState* __restrict__ st, Ctrl check)
+    const SolverState* __restrict__ st, Ctrl check, Systems sy)
 {
-    const int stopped = st ? st->status : 0;
+    const int sys = sy.spmv_sys ? sy.spmv_sys[blockIdx.x] : 0;
+    const int stopped = st ? st[sys].status : 0;
     __shared__ T red[kWavesPerBlock];
     const int lane = threadIdx.x & 63;
     const index_t s = index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
@@ -64,7 +65,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     }
     if (check.part) {
         __shared__ double redd[kWavesPerBlock];
-        ctrl_check_wg0(check, redd);
+        const index_t local = index_t(blockIdx.x) - (sy.spmv_g0 ? sy.spmv_g0[sys] : 0);
+        ctrl_check_wg0(ctrl_for(check, sy, sys, local), redd);
     }
 }
 
@@ -126,6 +128,7 @@ struct SweepArrays {
     const void* val;
     const index_t* col;
     const void* invd;
+    const int* slice_sys;   // batches: system of every slice (null: one system)
 };
 
 // One wavefront, one slice.  The slice's offsets, the lane's row and the
@@ -219,23 +222,26 @@ __global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, SweepA
                                                   WalkExt walk,
                                                   const SolverState* __restrict__ st)
 {
-    const int stopped = st->status;
     const index_t nreg = (s1 - s0 + kWavesPerBlock - 1) / kWavesPerBlock;
     if (index_t(blockIdx.x) >= nreg) {
-        if (stopped) return;
+        // batches: computed for every system (the results of a stopped
+        // system's sweep are never read)
+        if (!A.slice_sys && st->status) return;
         walk_external<P, F, FWD>(walk, walk.t0 + size_t(blockIdx.x - nreg) * kBlock + threadIdx.x, in, out);
         return;
     }
     const index_t s = s0 + index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
     if (s >= s1) return;                       // whole wavefront: no shuffle below
+    const int stopped = st[A.slice_sys ? A.slice_sys[s] : 0].status;
     sweep_slice<P, F, BR, FWD, CH>(s, A, in, out, stopped);
 }
 
 // Several consecutive small colours in one workgroup of NT threads: the
 // wavefronts share each colour's slices, a workgroup barrier separates the
 // colours (global writes of a workgroup are visible to it after the
-// barrier).  NT = 1024 for real values, 512 for complex (register budget).
-template <typename P> constexpr int tail_threads() { return sizeof(P) > 8 ? 512 : 1024; }
+// barrier).  NT = 1024 for real values, 512 for complex values or 16-row
+// blocks (register budget).
+template <typename P, int BR> constexpr int tail_threads() { return (sizeof(P) > 8 || BR >= 16) ? 512 : 1024; }
 constexpr int kTailMaxPhases = 16;
 struct TailPhases {
     int count;
@@ -243,16 +249,16 @@ struct TailPhases {
 };
 
 template <typename P, typename F, int BR, bool FWD, int CH>
-__global__ __launch_bounds__(tail_threads<P>()) void k_sweep_tail(TailPhases T, SweepArrays A,
+__global__ __launch_bounds__((tail_threads<P, BR>())) void k_sweep_tail(TailPhases T, SweepArrays A,
                                                                   const P* __restrict__ in, P* out,
                                                                   const SolverState* __restrict__ st)
 {
-    if (st->status) return;
+    if (!A.slice_sys && st->status) return;
     const index_t w = index_t(threadIdx.x >> 6);
-    constexpr index_t kWaves = tail_threads<P>() / 64;
+    constexpr index_t kWaves = tail_threads<P, BR>() / 64;
     for (int ph = 0; ph < T.count; ++ph) {
         for (index_t s = T.slices[ph] + w; s < T.slices[ph + 1]; s += kWaves)
-            sweep_slice<P, F, BR, FWD, CH>(s, A, in, out, 0);
+            sweep_slice<P, F, BR, FWD, CH>(s, A, in, out, A.slice_sys ? st[A.slice_sys[s]].status : 0);
         __syncthreads();
     }
 }
@@ -323,12 +329,13 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
     index_t b0, const index_t* __restrict__ step_off, const index_t* __restrict__ wrow,
     const index_t* __restrict__ wpos, const F* __restrict__ winvd, const F* __restrict__ wv,
     const index_t* __restrict__ wix, index_t zero_pos, P* out, const P* __restrict__ sa,
-    const P* __restrict__ sb, const SolverState* __restrict__ st)
+    const P* __restrict__ sb, const SolverState* __restrict__ st,
+    const int* __restrict__ walk_sys)
 {
-    if (st && st->status) return;
+    const index_t bb = b0 + index_t(blockIdx.x);
+    if (st && st[walk_sys ? walk_sys[bb] : 0].status) return;
     extern __shared__ unsigned char lds_raw[];
     P* lds = reinterpret_cast<P*>(lds_raw);   // [0, zero_pos): values; zero_pos: 0; +1: idle lanes
-    const index_t bb = b0 + index_t(blockIdx.x);
     const index_t s0 = step_off[bb], s1 = step_off[bb + 1];   // s1 - s0 is a multiple of D
     const int lane = threadIdx.x;
     const index_t trash = zero_pos + 1;
@@ -382,19 +389,20 @@ template <typename T, typename TA, typename TB>
 __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const TA* __restrict__ a,
                                                const TB* __restrict__ b, int conj,
                                                T* __restrict__ partials,
-                                               const SolverState* __restrict__ st, Ctrl check)
+                                               const SolverState* __restrict__ st, Ctrl check,
+                                               Systems sy)
 {
-    if (st && st->status) return;
+    const Span sp = vec_span(sy, n);
+    if (st && st[sp.sys].status) return;
     __shared__ T red[kWavesPerBlock];
     T d = zero<T>();
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock)
-        d = fma_(conj_if(cvt<T>(a[i]), conj != 0), cvt<T>(b[i]), d);
+    for (index_t i = sp.local * kBlock + threadIdx.x; i < sp.n; i += sp.G * kBlock)
+        d = fma_(conj_if(cvt<T>(a[sp.row0 + i]), conj != 0), cvt<T>(b[sp.row0 + i]), d);
     T t = block_sum(d, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
     if (check.part) {
         __shared__ double redd[kWavesPerBlock];
-        ctrl_check_wg0(check, redd);
+        ctrl_check_wg0(ctrl_for(check, sy, sp.sys, sp.local), redd);
     }
 }
 
@@ -403,14 +411,14 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_norms(index_t n, const T* __restrict__ a,
                                                  const T* __restrict__ b,
                                                  double* __restrict__ pa,
-                                                 double* __restrict__ pb)
+                                                 double* __restrict__ pb, Systems sy)
 {
+    const Span sp = vec_span(sy, n);
     __shared__ double red[kWavesPerBlock];
     double sa = 0.0, sb = 0.0;
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock) {
-        sa += norm2(a[i]);
-        if (b) sb += norm2(b[i]);
+    for (index_t i = sp.local * kBlock + threadIdx.x; i < sp.n; i += sp.G * kBlock) {
+        sa += norm2(a[sp.row0 + i]);
+        if (b) sb += norm2(b[sp.row0 + i]);
     }
     double t = block_sum(sa, red);
     if (threadIdx.x == 0) pa[blockIdx.x] = t;
@@ -436,20 +444,22 @@ __global__ __launch_bounds__(kBlock) void k_gather(index_t n, const index_t* __r
                                                   const T* __restrict__ src, T* __restrict__ dst)
 {
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock)
-        dst[i] = src[map[i]];
+         i += index_t(gridDim.x) * kBlock) {
+        const index_t m = map[i];
+        dst[i] = m >= 0 ? src[m] : zero<T>();    // -1: a padding row of a batch
+    }
 }
 
 // dst = src, converted to the destination's precision
 template <typename TS, typename TD>
 __global__ __launch_bounds__(kBlock) void k_copy(index_t n, const TS* __restrict__ src,
                                                 TD* __restrict__ dst,
-                                                const SolverState* __restrict__ st)
+                                                const SolverState* __restrict__ st, Systems sy)
 {
-    if (st && st->status) return;
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock)
-        dst[i] = cvt<TD>(src[i]);
+    const Span sp = vec_span(sy, n);
+    if (st && st[sp.sys].status) return;
+    for (index_t i = sp.local * kBlock + threadIdx.x; i < sp.n; i += sp.G * kBlock)
+        dst[sp.row0 + i] = cvt<TD>(src[sp.row0 + i]);
 }
 
 // CG: x += alpha p; r -= alpha Ap; partial |r|^2; r also in the
@@ -460,16 +470,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__
                                                      const T* __restrict__ Ap,
                                                      double* __restrict__ partials,
                                                      const SolverState* __restrict__ st,
-                                                     Ctrl ctl, P* __restrict__ r_low)
+                                                     Ctrl ctl, P* __restrict__ r_low, Systems sy)
 {
-    if (st->status) return;
+    const Span sp = vec_span(sy, n);
+    if (st[sp.sys].status) return;
     __shared__ double red[kWavesPerBlock];
     __shared__ T redt[kWavesPerBlock];
     T alpha;
-    if (!ctrl_alpha_all(ctl, redt, alpha)) return;
+    if (!ctrl_alpha_all(ctrl_for(ctl, sy, sp.sys, sp.local), redt, alpha)) return;
     double s = 0.0;
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock) {
+    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock) {
+        const index_t i = sp.row0 + j;
         x[i] = fma_(alpha, p[i], x[i]);
         const T ri = fms_(alpha, Ap[i], r[i]);
         r[i] = ri;
@@ -484,13 +495,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__
 // (best null: not tracked).  This runs also after a stop: the check that
 // stopped the solve may have improved the best residual.
 template <typename T>
-__device__ inline void copy_if_improved(index_t n, const T* __restrict__ x, T* __restrict__ best,
-                                        const SolverState* __restrict__ st)
+__device__ inline void copy_if_improved(const Span& sp, const T* __restrict__ x,
+                                        T* __restrict__ best, const SolverState* __restrict__ st)
 {
-    if (!best || !st->improved) return;
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock)
-        best[i] = x[i];
+    if (!best || !st[sp.sys].improved) return;
+    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock)
+        best[sp.row0 + j] = x[sp.row0 + j];
 }
 
 // CG: p = z + beta p (and the best-x copy of this iteration); z may be in
@@ -499,16 +509,19 @@ template <typename T, typename TZ>
 __global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const TZ* __restrict__ z,
                                                 T* __restrict__ p, const T* __restrict__ x,
                                                 T* __restrict__ best,
-                                                const SolverState* __restrict__ st, Ctrl ctl)
+                                                const SolverState* __restrict__ st, Ctrl ctl,
+                                                Systems sy)
 {
-    copy_if_improved(n, x, best, st);
-    if (st->status) return;
+    const Span sp = vec_span(sy, n);
+    copy_if_improved(sp, x, best, st);
+    if (st[sp.sys].status) return;
     __shared__ T redt[kWavesPerBlock];
     T beta;
-    if (!ctrl_beta_all(ctl, redt, beta)) return;
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock)
+    if (!ctrl_beta_all(ctrl_for(ctl, sy, sp.sys, sp.local), redt, beta)) return;
+    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock) {
+        const index_t i = sp.row0 + j;
         p[i] = fma_(beta, p[i], cvt<T>(z[i]));
+    }
 }
 
 // COCR: x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt may
@@ -521,16 +534,17 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
                                                        const TQ* __restrict__ qt,
                                                        double* __restrict__ partials,
                                                        const SolverState* __restrict__ st,
-                                                       Ctrl ctl)
+                                                       Ctrl ctl, Systems sy)
 {
-    if (st->status) return;
+    const Span sp = vec_span(sy, n);
+    if (st[sp.sys].status) return;
     __shared__ double red[kWavesPerBlock];
     __shared__ T redt[kWavesPerBlock];
     T alpha;
-    if (!ctrl_alpha_all(ctl, redt, alpha)) return;
+    if (!ctrl_alpha_all(ctrl_for(ctl, sy, sp.sys, sp.local), redt, alpha)) return;
     double s = 0.0;
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock) {
+    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock) {
+        const index_t i = sp.row0 + j;
         x[i] = fma_(alpha, p[i], x[i]);
         const T ri = fms_(alpha, q[i], r[i]);
         r[i] = ri;
@@ -549,15 +563,16 @@ __global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restri
                                                    T* __restrict__ q, const T* __restrict__ x,
                                                    T* __restrict__ best,
                                                    const SolverState* __restrict__ st, Ctrl ctl,
-                                                   P* __restrict__ q_low)
+                                                   P* __restrict__ q_low, Systems sy)
 {
-    copy_if_improved(n, x, best, st);
-    if (st->status) return;
+    const Span sp = vec_span(sy, n);
+    copy_if_improved(sp, x, best, st);
+    if (st[sp.sys].status) return;
     __shared__ T redt[kWavesPerBlock];
     T beta;
-    if (!ctrl_beta_all(ctl, redt, beta)) return;
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock) {
+    if (!ctrl_beta_all(ctrl_for(ctl, sy, sp.sys, sp.local), redt, beta)) return;
+    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock) {
+        const index_t i = sp.row0 + j;
         p[i] = fma_(beta, p[i], rt[i]);
         const T qi = fma_(beta, q[i], t[i]);
         q[i] = qi;

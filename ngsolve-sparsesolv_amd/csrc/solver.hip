@@ -81,13 +81,15 @@ struct DevBuf {
         p = nullptr;
         bytes = 0;
     }
-    void alloc(size_t b)
+    // true when new memory was allocated
+    bool alloc(size_t b)
     {
-        if (b == bytes && p) return;
+        if (b == bytes && p) return false;
         release();
         if (b == 0) b = 16;
         HIP_CHECK(hipMalloc(&p, b));
         bytes = b;
+        return true;
     }
     template <typename V>
     void upload(const std::vector<V>& v, hipStream_t s)
@@ -99,8 +101,9 @@ struct DevBuf {
 };
 
 constexpr int kMaxGrid = 1024;  // vector kernels: partial count bound
-// sweep entries per batch: 16 for real, 8 for complex values (register
-// budget); the layouts pad their entry arrays by kSweepChunk
+// sweep entries per batch: 16 for real, 8 for complex values, 8 in the
+// small-colour kernel (register budget of its large workgroup); the layouts
+// pad their entry arrays by kSweepChunk
 template <typename P> constexpr int sweep_chunk() { return sizeof(P) > 8 ? 8 : 16; }
 static_assert(sparsesolv_amd::kSweepChunk >= 16, "layout padding shorter than a batch");
 // Krylov iterations per captured graph
@@ -136,6 +139,8 @@ template <> struct HostOf<ssa::c64> { using type = std::complex<float>; };
 
 struct DevSweep {
     DevBuf lrow, lpos, slot, val, col;
+    DevBuf slice_sys, walk_sys;   // batches: system of every slice / oversized block
+    bool batched = false;
     DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
     DevBuf w_step_off, w_row, w_pos, w_ext_rp, w_ext_ci, w_ext_v, w_invd, w_v, w_ix, w_sa, w_sb;
     std::vector<index_t> phase_slices, phase_big, phase_rows;
@@ -204,7 +209,8 @@ struct IcBase {
     virtual void apply(const void* x, void* y, bool on_device) = 0;
     virtual void spmv(const void* x, void* y, bool on_device) = 0;
     virtual void solve(int method, const void* b, void* x, bool on_device, const ssa_config& cfg,
-                       ssa_result& res, double* history) = 0;
+                       ssa_result* results, double* history) = 0;
+    virtual int system_count() const = 0;
     virtual void info(ssa_ic_info& out) const = 0;
     virtual void ordering(int32_t* out) const = 0;
     virtual void schedule(int32_t* co, int32_t* bo) const = 0;
@@ -219,12 +225,20 @@ struct IcImpl final : IcBase {
     using P = F;  // preconditioner-internal vector type
 
     ssa_context* ctx;
-    index_t n = 0, nnz = 0, nnz_l = 0, nc = 0, nb = 0;
+    // count systems; n = device rows (the systems' rows, each padded to a
+    // multiple of 256 when count > 1); tot = rows of the caller's vectors
+    int count = 1;
+    index_t n = 0, tot = 0, nnz = 0, nnz_l = 0, nc = 0, nb = 0;
     double actual_shift = 0.0, setup_seconds = 0.0;
     int shift_trials = 0;
-    // host copies kept for queries
+    // host copies kept for queries (one system)
     std::vector<index_t> h_ordering, h_color_offsets, h_block_offsets, h_Lrp, h_Lci;
     std::vector<HT> h_Lv, h_invd;
+    // batches: per system row ranges and grid ranges
+    std::vector<int> h_row0, h_nrows, h_vec_g0, h_spmv_g0;
+    DevBuf d_vec_sys, d_vec_g0, d_spmv_sys, d_spmv_g0, d_row0, d_nrows;
+    int vec_total = 0;
+    int hist_stride = 0;
 
     // device: operator, permutations, factor
     DevBuf A_off, A_val, A_col, perm, rev, invd;
@@ -240,29 +254,39 @@ struct IcImpl final : IcBase {
     static constexpr bool kMixed = !std::is_same<T, P>::value;
     DevBuf lo_a, lo_b, lo_t;
 
-    SolverState* h_state = nullptr;  // pinned
+    SolverState* h_state = nullptr;  // pinned, one per system
 
-    IcImpl(ssa_context* c, index_t n_, const int32_t* rp, const int32_t* ci, const HT* v,
-           const sparsesolv_amd::IcSettings& st)
-        : ctx(c)
+    IcImpl(ssa_context* c, int count_, const int* ns, const int32_t* const* rp,
+           const int32_t* const* ci, const HT* const* v, const sparsesolv_amd::IcSettings& st)
+        : ctx(c), count(count_)
     {
-        ssa_host::HostPlan<HT, HF> plan;
-        ssa_host::build_plan<HT, HF>(plan, n_, rp, ci, v, st);
-        n = n_;
-        nnz = rp[n_];
-        nnz_l = index_t(plan.fac.L.v.size());
-        nc = plan.fac.sched.num_colors();
-        nb = plan.fac.sched.num_blocks();
-        actual_shift = plan.fac.actual_shift;
-        shift_trials = plan.fac.shift_trials;
-        setup_seconds = plan.seconds;
-        h_ordering = plan.fac.sched.ordering;
-        h_color_offsets = plan.fac.sched.color_offsets;
-        h_block_offsets = plan.fac.sched.block_offsets;
-        h_Lrp = plan.fac.L.rp;
-        h_Lci = plan.fac.L.ci;
-        h_Lv = plan.fac.L.v;
-        h_invd = plan.fac.inv_diag;
+        if (count > 1 && kMixed)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "batched solves run in FP64 only");
+        ssa_host::BatchPlan<HT, HF> plan;
+        ssa_host::build_batch<HT, HF>(plan, count, ns, rp, ci, v, st);
+        n = plan.row0.back();
+        tot = plan.in_off.back();
+        setup_seconds = plan.wall_seconds;
+        for (int s = 0; s < count; ++s) {
+            nnz += plan.nnz[s];
+            nnz_l += plan.nnz_l[s];
+            nc = std::max<index_t>(nc, plan.num_colors[s]);
+            nb += plan.num_blocks[s];
+            actual_shift = std::max(actual_shift, plan.actual_shift[s]);
+            shift_trials = std::max(shift_trials, plan.shift_trials[s]);
+        }
+        h_row0 = plan.row0;
+        h_nrows = plan.nrows;
+        if (count == 1) {
+            const auto& f = plan.single.fac;
+            h_ordering = f.sched.ordering;
+            h_color_offsets = f.sched.color_offsets;
+            h_block_offsets = f.sched.block_offsets;
+            h_Lrp = f.L.rp;
+            h_Lci = f.L.ci;
+            h_Lv = f.L.v;
+            h_invd = f.inv_diag;
+        }
 
         if (!ctx) return;   // host-only handle: schedule and factor queries only
         hipStream_t s = ctx->stream;
@@ -271,13 +295,57 @@ struct IcImpl final : IcBase {
         A_col.upload(plan.spmv.col, s);
         A_nslices = plan.spmv.nslices;
         spmv_bytes = plan.spmv.val.size() * sizeof(HT) + plan.spmv.col.size() * 4;
-        perm.upload(plan.fac.sched.ordering, s);
-        rev.upload(plan.fac.sched.reverse_ordering, s);
+        perm.upload(plan.gather_out, s);
+        rev.upload(plan.gather_in, s);
         invd.upload(plan.invd, s);
         fwd.upload(plan.fwd, s);
         bwd.upload(plan.bwd, s);
+        if (count > 1) {
+            // each system keeps the grid a one-system solve would use, so its
+            // reductions give the same bits
+            std::vector<int> vsys, ssys;
+            h_vec_g0.assign(1, 0);
+            h_spmv_g0.assign(1, 0);
+            for (int q = 0; q < count; ++q) {
+                const int g = vec_grid(plan.nrows[q]);
+                for (int k = 0; k < g; ++k) vsys.push_back(q);
+                h_vec_g0.push_back(h_vec_g0.back() + g);
+                const int gs = (plan.row0[q + 1] - plan.row0[q]) / (64 * ssa::kWavesPerBlock);
+                for (int k = 0; k < gs; ++k) ssys.push_back(q);
+                h_spmv_g0.push_back(h_spmv_g0.back() + gs);
+            }
+            vec_total = h_vec_g0.back();
+            d_vec_sys.upload(vsys, s);
+            d_vec_g0.upload(h_vec_g0, s);
+            d_spmv_sys.upload(ssys, s);
+            d_spmv_g0.upload(h_spmv_g0, s);
+            d_row0.upload(plan.row0, s);
+            d_nrows.upload(plan.nrows, s);
+            fwd.slice_sys.upload(plan.slice_sys_f, s);
+            bwd.slice_sys.upload(plan.slice_sys_b, s);
+            fwd.walk_sys.upload(plan.walk_sys_f, s);
+            bwd.walk_sys.upload(plan.walk_sys_b, s);
+            fwd.batched = bwd.batched = true;
+            // oversized blocks must take the walk: the level kernel's arrays
+            // are not combined
+            for (DevSweep* D : {&fwd, &bwd})
+                if (D->phase_big.back() > 0 && !walk_lds_ok(*D))
+                    throw SsaError(SSA_ERR_UNSUPPORTED,
+                                   "batch: an oversized block does not fit the walk's LDS");
+        }
         HIP_CHECK(hipStreamSynchronize(s));   // host plan is freed on return
     }
+
+    // the systems descriptor kernels take (all null for one system)
+    ssa::Systems systems() const
+    {
+        if (count == 1) return ssa::Systems{};
+        return ssa::Systems{d_vec_sys.as<int>(), d_vec_g0.as<int>(), d_spmv_sys.as<int>(),
+                            d_spmv_g0.as<int>(), d_row0.as<int>(), d_nrows.as<int>(), hist_stride};
+    }
+    // vector-grid workgroups
+    int vgrid() const { return count == 1 ? vec_grid(n) : vec_total; }
+    int system_count() const override { return count; }
 
     void need_device() const
     {
@@ -294,17 +362,25 @@ struct IcImpl final : IcBase {
     void ensure_workspace()
     {
         need_device();
-        if (!h_state) HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_state), sizeof(SolverState)));
+        if (!h_state)
+            HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_state), sizeof(SolverState) * count));
         const size_t vb = size_t(n) * sizeof(T);
-        for (DevBuf* b : {&r, &z, &p, &Ap, &t, &qt, &x, &best, &bvec, &tmp, &tmp2}) b->alloc(vb);
-        const size_t np = size_t(std::max<index_t>(kMaxGrid, (A_nslices + 3) / 4)) * sizeof(ssa::c128);
+        // zeroed once: the padding rows between the systems of a batch are
+        // never written and must stay zero (the SpMV's fused dot multiplies
+        // them by zero entries)
+        for (DevBuf* b : {&r, &z, &p, &Ap, &t, &qt, &x, &best, &bvec, &tmp, &tmp2})
+            if (b->alloc(vb)) HIP_CHECK(hipMemsetAsync(b->p, 0, vb, ctx->stream));
+        const size_t np = size_t(std::max<index_t>(std::max(kMaxGrid, vec_total), (A_nslices + 3) / 4)) *
+                          sizeof(ssa::c128);
         for (DevBuf* b : {&part_a, &part_b, &part_c, &part_d}) b->alloc(np);
-        state.alloc(sizeof(SolverState));
+        state.alloc(sizeof(SolverState) * count);
         if (kMixed)
-            for (DevBuf* b : {&lo_a, &lo_b, &lo_t}) b->alloc(size_t(n) * sizeof(P));
+            for (DevBuf* b : {&lo_a, &lo_b, &lo_t})
+                if (b->alloc(size_t(n) * sizeof(P)))
+                    HIP_CHECK(hipMemsetAsync(b->p, 0, size_t(n) * sizeof(P), ctx->stream));
         if (!idle.p) {
-            idle.alloc(sizeof(SolverState));
-            HIP_CHECK(hipMemsetAsync(idle.p, 0, sizeof(SolverState), ctx->stream));
+            idle.alloc(sizeof(SolverState) * count);
+            HIP_CHECK(hipMemsetAsync(idle.p, 0, sizeof(SolverState) * count, ctx->stream));
         }
     }
 
@@ -317,26 +393,32 @@ struct IcImpl final : IcBase {
         const int g = std::max(1, spmv_grid());
         hipStream_t s = ctx->stream;
         if (dot == 0)
-            ssa::k_spmv<T, 0><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl);
+            ssa::k_spmv<T, 0><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl, systems());
         else if (dot == 1)
-            ssa::k_spmv<T, 1><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl);
+            ssa::k_spmv<T, 1><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl, systems());
         else
-            ssa::k_spmv<T, 2><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl);
+            ssa::k_spmv<T, 2><<<g, ssa::kBlock, 0, s>>>(A_off.as<index_t>(), A_val.as<T>(), A_col.as<index_t>(), n, A_nslices, xin, y, part, st, ctrl, systems());
         HIP_CHECK(hipGetLastError());
     }
 
     ssa::SweepArrays sweep_arrays(DevSweep& D)
     {
         return ssa::SweepArrays{D.lrow.as<index_t>(), D.lpos.as<index_t>(), D.slot.as<index_t>(),
-                                D.val.p, D.col.as<index_t>(), invd.p};
+                                D.val.p, D.col.as<index_t>(), invd.p,
+                                D.batched ? D.slice_sys.as<int>() : nullptr};
     }
 
     // a colour small enough to share a single-workgroup kernel with its
     // neighbours: no oversized block, at most two passes of the workgroup
+    static int tail_threads(int BR)
+    {
+        return BR >= 16 ? ssa::tail_threads<P, 16>() : ssa::tail_threads<P, 1>();
+    }
+
     bool small_phase(const DevSweep& D, index_t ph) const
     {
         return D.phase_big[ph + 1] == D.phase_big[ph] &&
-               D.phase_slices[ph + 1] - D.phase_slices[ph] <= 2 * (ssa::tail_threads<P>() / 64);
+               D.phase_slices[ph + 1] - D.phase_slices[ph] <= 2 * (tail_threads(D.BR) / 64);
     }
 
     // end of the run of small colours that starts at ph (ph itself when
@@ -389,7 +471,7 @@ struct IcImpl final : IcBase {
         const ssa::SweepArrays A = sweep_arrays(D);
         hipStream_t s = ctx->stream;
 #define SSA_TAIL(BRV)                                                                           \
-    ssa::k_sweep_tail<P, F, BRV, FWD, sweep_chunk<P>()><<<1, ssa::tail_threads<P>(), 0, s>>>(tp, A, in, out, st)
+    ssa::k_sweep_tail<P, F, BRV, FWD, 8><<<1, ssa::tail_threads<P, BRV>(), 0, s>>>(tp, A, in, out, st)
         switch (D.BR == 0 ? 1 : D.BR) {
         case 1: SSA_TAIL(1); break;
         case 2: SSA_TAIL(2); break;
@@ -445,7 +527,7 @@ struct IcImpl final : IcBase {
     ssa::k_sweep_walk<P, F, FWD, KM, DEPTH><<<int(b1 - b0), 64, lds, s>>>(                    \
         b0, D.w_step_off.as<index_t>(), D.w_row.as<index_t>(), D.w_pos.as<index_t>(),         \
         D.w_invd.as<F>(), D.w_v.as<F>(), D.w_ix.as<index_t>(), D.max_big_rows, out,           \
-        D.w_sa.as<P>(), D.w_sb.as<P>(), st)
+        D.w_sa.as<P>(), D.w_sb.as<P>(), st, D.batched ? D.walk_sys.as<int>() : nullptr)
             switch (D.walk_kmax) {
             case 2: SSA_WALK(2, 8); break;
             case 4: SSA_WALK(4, 8); break;
@@ -495,9 +577,10 @@ struct IcImpl final : IcBase {
         launch_sweep<false>(bwd, tbuf, out, st);
     }
 
-    void launch_gather(const index_t* map, const T* src, T* dst)
+    // dst[i] = src[map[i]] for i < len (map -1: zero)
+    void launch_gather(const index_t* map, const T* src, T* dst, index_t len)
     {
-        ssa::k_gather<T><<<vec_grid(n), ssa::kBlock, 0, ctx->stream>>>(n, map, src, dst);
+        ssa::k_gather<T><<<vec_grid(len), ssa::kBlock, 0, ctx->stream>>>(len, map, src, dst);
         HIP_CHECK(hipGetLastError());
     }
 
@@ -512,7 +595,7 @@ struct IcImpl final : IcBase {
     template <typename S, typename D>
     void convert(const S* src, D* dst, const SolverState* st)
     {
-        ssa::k_copy<S, D><<<vec_grid(n), ssa::kBlock, 0, ctx->stream>>>(n, src, dst, st);
+        ssa::k_copy<S, D><<<vgrid(), ssa::kBlock, 0, ctx->stream>>>(n, src, dst, st, systems());
         HIP_CHECK(hipGetLastError());
     }
 
@@ -521,18 +604,18 @@ struct IcImpl final : IcBase {
     {
         ensure_workspace();
         hipStream_t s = ctx->stream;
-        const size_t vb = size_t(n) * sizeof(T);
+        const size_t vb = size_t(tot) * sizeof(T);   // the caller's vectors
         const T* src = static_cast<const T*>(xin);
         if (!on_device) {
             HIP_CHECK(hipMemcpyAsync(tmp.p, xin, vb, hipMemcpyHostToDevice, s));
             src = tmp.as<T>();
         }
-        launch_gather(rev.as<index_t>(), src, r.as<T>());
+        launch_gather(rev.as<index_t>(), src, r.as<T>(), n);
         if (kMixed) convert(r.as<T>(), pre_in(false), nullptr);
         launch_precond(pre_in(false), pre_out(false), pre_tmp(), nullptr);
         if (kMixed) convert(pre_out(false), z.as<T>(), nullptr);
         T* dst = on_device ? static_cast<T*>(yout) : tmp2.as<T>();
-        launch_gather(perm.as<index_t>(), z.as<T>(), dst);
+        launch_gather(perm.as<index_t>(), z.as<T>(), dst, tot);
         if (!on_device) HIP_CHECK(hipMemcpyAsync(yout, tmp2.p, vb, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
@@ -541,16 +624,16 @@ struct IcImpl final : IcBase {
     {
         ensure_workspace();
         hipStream_t s = ctx->stream;
-        const size_t vb = size_t(n) * sizeof(T);
+        const size_t vb = size_t(tot) * sizeof(T);   // the caller's vectors
         const T* src = static_cast<const T*>(xin);
         if (!on_device) {
             HIP_CHECK(hipMemcpyAsync(tmp.p, xin, vb, hipMemcpyHostToDevice, s));
             src = tmp.as<T>();
         }
-        launch_gather(rev.as<index_t>(), src, r.as<T>());
+        launch_gather(rev.as<index_t>(), src, r.as<T>(), n);
         launch_spmv(r.as<T>(), z.as<T>(), 0, nullptr, nullptr);
         T* dst = on_device ? static_cast<T*>(yout) : tmp2.as<T>();
-        launch_gather(perm.as<index_t>(), z.as<T>(), dst);
+        launch_gather(perm.as<index_t>(), z.as<T>(), dst, tot);
         if (!on_device) HIP_CHECK(hipMemcpyAsync(yout, tmp2.p, vb, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
@@ -560,13 +643,17 @@ struct IcImpl final : IcBase {
     {
         hipStream_t s = ctx->stream;
         SolverState* st = state.as<SolverState>();
-        const int G = vec_grid(n);
+        const int G = vgrid();
+        const ssa::Systems y = systems();
         const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
                          method == SSA_METHOD_NGS_ICCOCR;
         double* H = hist.as<double>();
-        // control steps fused into the kernels that consume their result
+        // control steps fused into the kernels that consume their result;
+        // `spmv`: the partials come from the SpMV grid
         const int np_spmv = std::max(1, spmv_grid());
-        auto ctl = [&](const void* part, int np) { return ssa::Ctrl{st, H, part, np, 0, prm}; };
+        auto ctl = [&](const void* part, int np, bool spmv) {
+            return ssa::Ctrl{st, H, part, np, spmv ? 1 : 0, prm, 0, 0};
+        };
         T* bx = prm.save_best ? best.as<T>() : nullptr;
         const bool c = prm.cocr != 0;
         if (!c) {
@@ -575,24 +662,24 @@ struct IcImpl final : IcBase {
             // kernel (after the preconditioner, which cannot change x or r),
             // beta and the best-x copy in the p update
             launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
-            ssa::k_cg_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv), low_copy());
+            ssa::k_cg_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv, true), low_copy(), y);
             if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
-            else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st);
-            ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G));
-            ssa::k_xpby<T, P><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G));
+            else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st, y);
+            ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G, false), y);
+            ssa::k_xpby<T, P><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G, false), y);
         } else {
             // COCRSolver::do_iterate body (cocr_solver.hpp:96-162); z holds rt,
             // Ap holds q.  alpha in the update kernel, the check in the
             // t = A rt kernel, beta and the best-x copy in the p, q update
             if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
-            else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st);
+            else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st, y);
             if (kMixed)   // (q, qt) with the narrow operand second
-                ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>(), st, ssa::Ctrl{});
+                ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
             else
-                ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{});
-            ssa::k_cocr_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G));
-            launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G));
-            ssa::k_cocr_pq<T, P><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv), low_copy());
+                ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
+            ssa::k_cocr_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
+            launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G, false));
+            ssa::k_cocr_pq<T, P><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -602,7 +689,7 @@ struct IcImpl final : IcBase {
     {
         hipStream_t s = ctx->stream;
         const int G = vec_grid(n);
-        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), nullptr, part_d.as<double>(), nullptr);
+        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), nullptr, part_d.as<double>(), nullptr, systems());
         HIP_CHECK(hipGetLastError());
         return std::sqrt(host_sum(part_d.as<double>(), G));
     }
@@ -625,7 +712,7 @@ struct IcImpl final : IcBase {
         const int G = vec_grid(n);
         launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
         ssa::k_residual<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), Ap.as<T>(), r.as<T>());
-        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), nullptr, part_d.as<double>(), nullptr);
+        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), nullptr, part_d.as<double>(), nullptr, systems());
         HIP_CHECK(hipGetLastError());
         return std::sqrt(host_sum(part_d.as<double>(), G));
     }
@@ -635,16 +722,18 @@ struct IcImpl final : IcBase {
     void run_pass(int method, const SolveParams& prm, bool pre, const ssa_config& cfg)
     {
         hipStream_t s = ctx->stream;
-        const int G = vec_grid(n);
+        const int G = vgrid();
+        const ssa::Systems y = systems();
         SolverState* st = state.as<SolverState>();
         double* H = hist.as<double>();
-        HIP_CHECK(hipMemsetAsync(state.p, 0, sizeof(SolverState), s));
+        HIP_CHECK(hipMemsetAsync(state.p, 0, sizeof(SolverState) * count, s));
 
-        // prepare_iteration: r = b - A x, norms, normaliser
+        // prepare_iteration: r = b - A x, norms, normaliser (one control
+        // workgroup per system)
         launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
-        ssa::k_residual<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), Ap.as<T>(), r.as<T>());
-        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), r.as<T>(), part_b.as<double>(), part_d.as<double>());
-        ssa::k_ctrl_init<<<1, ssa::kBlock, 0, s>>>(part_b.as<double>(), part_d.as<double>(), G, st, prm, H);
+        ssa::k_residual<T><<<vec_grid(n), ssa::kBlock, 0, s>>>(n, bvec.as<T>(), Ap.as<T>(), r.as<T>());
+        ssa::k_norms<T><<<G, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), r.as<T>(), part_b.as<double>(), part_d.as<double>(), y);
+        ssa::k_ctrl_init<<<count, ssa::kBlock, 0, s>>>(part_b.as<double>(), part_d.as<double>(), G, st, prm, H, y);
         HIP_CHECK(hipGetLastError());
         const bool c = prm.cocr != 0;
         if (!c) {
@@ -653,10 +742,10 @@ struct IcImpl final : IcBase {
                 if (kMixed) convert(r.as<T>(), pre_in(c), nullptr);
                 launch_precond(pre_in(c), pre_out(c), pre_tmp(), nullptr);
             } else {
-                ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
+                ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr, y);
             }
-            ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), nullptr, ssa::Ctrl{});
-            ssa::k_ctrl_store<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, 0);
+            ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), nullptr, ssa::Ctrl{}, y);
+            ssa::k_ctrl_store<T><<<count, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, 0, y, 0);
             convert(pre_out(c), p.as<T>(), nullptr);
         } else {
             // rt = M^{-1} r; p = rt; q = A p; rho = rt^T q (cocr_solver.hpp:78-88)
@@ -666,11 +755,11 @@ struct IcImpl final : IcBase {
                                pre_tmp(), nullptr);
                 if (kMixed) convert(pre_out(c), z.as<T>(), nullptr);
             } else {
-                ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr);
+                ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr, y);
             }
-            ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr);
+            ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr, y);
             launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_c.as<T>(), nullptr);
-            ssa::k_ctrl_store<T><<<1, ssa::kBlock, 0, s>>>(part_c.as<T>(), std::max(1, spmv_grid()), st, 0);
+            ssa::k_ctrl_store<T><<<count, ssa::kBlock, 0, s>>>(part_c.as<T>(), std::max(1, spmv_grid()), st, 0, y, 1);
             if (kMixed) convert(Ap.as<T>(), pre_in(c), nullptr);
         }
         HIP_CHECK(hipGetLastError());
@@ -691,9 +780,16 @@ struct IcImpl final : IcBase {
             HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
         int done = 0;
-        HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        while (h_state->status == ssa::kRunning && done <= maxit) {
+        // runs while any system runs (a stopped system's kernels return at once)
+        auto running = [&] {
+            HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState) * count, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            for (int q = 0; q < count; ++q)
+                if (h_state[q].status == ssa::kRunning) return true;
+            return false;
+        };
+        bool more = running();
+        while (more && done <= maxit) {
             const int k = std::min(poll, maxit + 1 - done);
             const int groups = (k + unroll - 1) / unroll;
             for (int i = 0; i < groups; ++i) {
@@ -701,8 +797,7 @@ struct IcImpl final : IcBase {
                 else enqueue_iteration(method, prm);
             }
             done += groups * unroll;
-            HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipStreamSynchronize(s));
+            more = running();
             if (poll < 64) poll *= 2;
         }
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -710,11 +805,12 @@ struct IcImpl final : IcBase {
     }
 
     void solve(int method, const void* bin, void* xio, bool on_device, const ssa_config& cfg,
-               ssa_result& res, double* history) override
+               ssa_result* results, double* history) override
     {
         ensure_workspace();
         hipStream_t s = ctx->stream;
-        const size_t vb = size_t(n) * sizeof(T);
+        const size_t vb = size_t(tot) * sizeof(T);   // the caller's vectors
+        const size_t db = size_t(n) * sizeof(T);     // device rows
         SolveParams prm{};
         prm.tolerance = cfg.tolerance;
         prm.abs_tolerance = cfg.abs_tolerance;
@@ -736,9 +832,12 @@ struct IcImpl final : IcBase {
         }
         const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
                          method == SSA_METHOD_NGS_ICCOCR;
-        hist.alloc(sizeof(double) * size_t(std::max(cfg.max_iterations, 0) + 4));
+        hist_stride = std::max(cfg.max_iterations, 0) + 4;
+        hist.alloc(sizeof(double) * size_t(hist_stride) * size_t(count));
         double* H = hist.as<double>();
 
+        if (count > 1 && prm.ngs)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "batched solves: COCRSolverNGS runs one system");
         if (kMixed && !pre)
             throw SsaError(SSA_ERR_UNSUPPORTED,
                            "ssa_solve: mixed precision needs the IC preconditioner");
@@ -752,10 +851,10 @@ struct IcImpl final : IcBase {
             bsrc = tmp.as<T>();
             xsrc = tmp2.as<T>();
         }
-        launch_gather(rev.as<index_t>(), bsrc, bvec.as<T>());
-        if (prm.ngs) HIP_CHECK(hipMemsetAsync(x.p, 0, vb, s));
-        else launch_gather(rev.as<index_t>(), xsrc, x.as<T>());
-        HIP_CHECK(hipMemsetAsync(best.p, 0, vb, s));
+        launch_gather(rev.as<index_t>(), bsrc, bvec.as<T>(), n);
+        if (prm.ngs) HIP_CHECK(hipMemsetAsync(x.p, 0, db, s));
+        else launch_gather(rev.as<index_t>(), xsrc, x.as<T>(), n);
+        HIP_CHECK(hipMemsetAsync(best.p, 0, db, s));
 
         int total_iters = 0;
         double true_rel = -1.0;
@@ -780,7 +879,7 @@ struct IcImpl final : IcBase {
             double rel = true_residual() / scale;
             for (int pass = 0; pass < kRefinePasses; ++pass) {
                 if (rel < cfg.tolerance || total_iters >= cfg.max_iterations) break;
-                HIP_CHECK(hipMemcpyAsync(tmp.p, x.p, vb, hipMemcpyDeviceToDevice, s));
+                HIP_CHECK(hipMemcpyAsync(tmp.p, x.p, db, hipMemcpyDeviceToDevice, s));
                 prm.tolerance = prm.cocr ? std::max(cfg.tolerance, kInnerReduction * rel)
                                          : cfg.tolerance;
                 prm.max_iterations = cfg.max_iterations - total_iters;
@@ -788,10 +887,10 @@ struct IcImpl final : IcBase {
                 S = *h_state;
                 total_iters += S.result_iters;
                 if (!S.converged && prm.save_best)
-                    HIP_CHECK(hipMemcpyAsync(x.p, best.p, vb, hipMemcpyDeviceToDevice, s));
+                    HIP_CHECK(hipMemcpyAsync(x.p, best.p, db, hipMemcpyDeviceToDevice, s));
                 const double next = true_residual() / scale;
                 if (!(next < rel)) {
-                    HIP_CHECK(hipMemcpyAsync(x.p, tmp.p, vb, hipMemcpyDeviceToDevice, s));
+                    HIP_CHECK(hipMemcpyAsync(x.p, tmp.p, db, hipMemcpyDeviceToDevice, s));
                     break;
                 }
                 const bool progress = next < 0.5 * rel;
@@ -804,24 +903,31 @@ struct IcImpl final : IcBase {
             else if (S.status == ssa::kRunning) S.status = ssa::kMaxIter;
         }
 
-        // build_result (iterative_solver.hpp:262-279)
-        res.converged = S.converged;
-        res.iterations = total_iters;
-        res.status = S.status;
-        res.final_residual = true_rel >= 0.0 ? true_rel : S.final_norm / S.normalizer;
-        res.history_len = prm.save_history ? S.hist_len : 0;
-        res.actual_shift = pre ? actual_shift : 0.0;
-        if (true_rel < 0.0 && !S.converged && prm.save_best) {
-            HIP_CHECK(hipMemcpyAsync(x.p, best.p, vb, hipMemcpyDeviceToDevice, s));
-            res.final_residual = S.best_residual;
+        // build_result (iterative_solver.hpp:262-279), per system
+        for (int q = 0; q < count; ++q) {
+            const SolverState& Sq = count == 1 ? S : h_state[q];
+            ssa_result& res = results[q];
+            res.converged = Sq.converged;
+            res.iterations = count == 1 ? total_iters : Sq.result_iters;
+            res.status = Sq.status;
+            res.final_residual = true_rel >= 0.0 ? true_rel : Sq.final_norm / Sq.normalizer;
+            res.history_len = prm.save_history && count == 1 ? Sq.hist_len : 0;
+            res.actual_shift = pre ? actual_shift : 0.0;
+            if (true_rel < 0.0 && !Sq.converged && prm.save_best) {
+                const size_t off = size_t(h_row0[q]) * sizeof(T);
+                HIP_CHECK(hipMemcpyAsync(static_cast<char*>(x.p) + off, static_cast<char*>(best.p) + off,
+                                         size_t(h_nrows[q]) * sizeof(T), hipMemcpyDeviceToDevice, s));
+                res.final_residual = Sq.best_residual;
+            }
         }
         T* xdst = on_device ? static_cast<T*>(xio) : tmp2.as<T>();
-        launch_gather(perm.as<index_t>(), x.as<T>(), xdst);
+        launch_gather(perm.as<index_t>(), x.as<T>(), xdst, tot);
         if (!on_device) HIP_CHECK(hipMemcpyAsync(xio, tmp2.p, vb, hipMemcpyDeviceToHost, s));
-        if (history && res.history_len > 0)
-            HIP_CHECK(hipMemcpyAsync(history, H, sizeof(double) * res.history_len, hipMemcpyDeviceToHost, s));
+        if (history && count == 1 && results[0].history_len > 0)
+            HIP_CHECK(hipMemcpyAsync(history, H, sizeof(double) * results[0].history_len, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
-        res.solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int q = 0; q < count; ++q) results[q].solve_seconds = secs;
     }
 
     // Each colour launched on its own (no tail group, no oversized-block
@@ -923,7 +1029,7 @@ struct IcImpl final : IcBase {
 
     void info(ssa_ic_info& o) const override
     {
-        o.n = n;
+        o.n = tot;   // batches: all systems' rows (colours: the most of any system)
         o.nnz = nnz;
         o.nnz_l = nnz_l;
         o.num_colors = nc;
@@ -941,14 +1047,25 @@ struct IcImpl final : IcBase {
         o.spmv_bytes = spmv_bytes;
     }
 
-    void ordering(int32_t* out) const override { std::copy(h_ordering.begin(), h_ordering.end(), out); }
+    void one_system(const char* who) const
+    {
+        if (count != 1)
+            throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": not available for a batched handle");
+    }
+    void ordering(int32_t* out) const override
+    {
+        one_system("ssa_ic_ordering");
+        std::copy(h_ordering.begin(), h_ordering.end(), out);
+    }
     void schedule(int32_t* co, int32_t* bo) const override
     {
+        one_system("ssa_ic_schedule");
         if (co) std::copy(h_color_offsets.begin(), h_color_offsets.end(), co);
         if (bo) std::copy(h_block_offsets.begin(), h_block_offsets.end(), bo);
     }
     void factor(int32_t* rp, int32_t* ci, void* vals, void* iv) const override
     {
+        one_system("ssa_ic_factor");
         if (rp) std::copy(h_Lrp.begin(), h_Lrp.end(), rp);
         if (ci) std::copy(h_Lci.begin(), h_Lci.end(), ci);
         if (vals) std::memcpy(vals, h_Lv.data(), h_Lv.size() * sizeof(HT));
@@ -1032,56 +1149,95 @@ int ssa_context_destroy(ssa_context* c)
     });
 }
 
+static void check_matrix(const char* who, int n, const int32_t* row_ptr, const int32_t* col_idx,
+                         const void* values)
+{
+    if (n < 0 || (n > 0 && (!row_ptr || !col_idx || !values)))
+        throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": bad matrix");
+    if (n > 0 && row_ptr[0] != 0) throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": row_ptr[0] != 0");
+    for (int i = 0; i < n; ++i)
+        if (row_ptr[i + 1] < row_ptr[i])
+            throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": row_ptr not monotone");
+    const int nnz = n > 0 ? row_ptr[n] : 0;
+    for (int k = 0; k < nnz; ++k)
+        if (col_idx[k] < 0 || col_idx[k] >= n)
+            throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": column index out of range");
+}
+
+static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int count,
+                             const int32_t* n, const int32_t* const* row_ptr,
+                             const int32_t* const* col_idx, const void* const* values,
+                             const ssa_config* cfg)
+{
+    if (!cfg) throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": null argument");
+    if (dtype != SSA_F64 && dtype != SSA_C128)
+        throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": dtype must be SSA_F64 or SSA_C128");
+    if (cfg->use_abmc && cfg->abmc_use_rcm)
+        throw SsaError(SSA_ERR_UNSUPPORTED, std::string(who) + ": abmc_use_rcm is not supported");
+    if (cfg->diagonal_scaling)
+        throw SsaError(SSA_ERR_UNSUPPORTED, std::string(who) + ": diagonal_scaling is not supported");
+    if (cfg->precision != SSA_PRECISION_FP64 && cfg->precision != SSA_PRECISION_MIXED)
+        throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": unknown precision");
+    for (int q = 0; q < count; ++q) check_matrix(who, n[q], row_ptr[q], col_idx[q], values[q]);
+    if (ctx) HIP_CHECK(hipSetDevice(ctx->device));
+    sparsesolv_amd::IcSettings st;
+    st.shift = cfg->shift_parameter;
+    st.diagonal_scaling = cfg->diagonal_scaling != 0;
+    st.auto_shift = cfg->auto_shift != 0;
+    st.shift_increment = cfg->shift_increment;
+    st.max_shift_value = cfg->max_shift_value;
+    st.min_diagonal_threshold = cfg->min_diagonal_threshold;
+    st.zero_diagonal_replacement = cfg->zero_diagonal_replacement;
+    st.max_shift_trials = cfg->max_shift_trials;
+    st.use_abmc = cfg->use_abmc != 0;
+    st.abmc_block_size = cfg->abmc_block_size;
+    st.abmc_num_colors = cfg->abmc_num_colors;
+    auto h = std::make_unique<ssa_ic>();
+    h->dtype = dtype;
+    // mixed: the factor (computed in FP64) is stored and applied in
+    // float / complex<float>; SpMV, vectors and dots stay FP64
+    const bool mixed = cfg->precision == SSA_PRECISION_MIXED;
+    using cd = std::complex<double>;
+    if (dtype == SSA_F64) {
+        std::vector<const double*> v(static_cast<size_t>(count));
+        for (int q = 0; q < count; ++q) v[q] = static_cast<const double*>(values[q]);
+        if (mixed) h->impl.reset(new IcImpl<double, float>(ctx, count, n, row_ptr, col_idx, v.data(), st));
+        else h->impl.reset(new IcImpl<double, double>(ctx, count, n, row_ptr, col_idx, v.data(), st));
+    } else {
+        std::vector<const cd*> v(static_cast<size_t>(count));
+        for (int q = 0; q < count; ++q) v[q] = static_cast<const cd*>(values[q]);
+        if (mixed) h->impl.reset(new IcImpl<ssa::c128, ssa::c64>(ctx, count, n, row_ptr, col_idx, v.data(), st));
+        else h->impl.reset(new IcImpl<ssa::c128, ssa::c128>(ctx, count, n, row_ptr, col_idx, v.data(), st));
+    }
+    return h.release();
+}
+
 int ssa_ic_create(ssa_context* ctx, int dtype, int n, const int32_t* row_ptr,
                   const int32_t* col_idx, const void* values, const ssa_config* cfg, ssa_ic** out)
 {
     return guarded([&] {
-        if (!out || !cfg) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: null argument");
-        if (n < 0 || (n > 0 && (!row_ptr || !col_idx || !values)))
-            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: bad matrix");
-        if (dtype != SSA_F64 && dtype != SSA_C128)
-            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: dtype must be SSA_F64 or SSA_C128");
-        if (cfg->use_abmc && cfg->abmc_use_rcm)
-            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: abmc_use_rcm is not supported");
-        if (cfg->diagonal_scaling)
-            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_create: diagonal_scaling is not supported");
-        if (n > 0 && row_ptr[0] != 0) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: row_ptr[0] != 0");
-        for (int i = 0; i < n; ++i)
-            if (row_ptr[i + 1] < row_ptr[i]) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: row_ptr not monotone");
-        const int nnz = n > 0 ? row_ptr[n] : 0;
-        for (int k = 0; k < nnz; ++k)
-            if (col_idx[k] < 0 || col_idx[k] >= n) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: column index out of range");
-        if (ctx) HIP_CHECK(hipSetDevice(ctx->device));
-        sparsesolv_amd::IcSettings st;
-        st.shift = cfg->shift_parameter;
-        st.diagonal_scaling = cfg->diagonal_scaling != 0;
-        st.auto_shift = cfg->auto_shift != 0;
-        st.shift_increment = cfg->shift_increment;
-        st.max_shift_value = cfg->max_shift_value;
-        st.min_diagonal_threshold = cfg->min_diagonal_threshold;
-        st.zero_diagonal_replacement = cfg->zero_diagonal_replacement;
-        st.max_shift_trials = cfg->max_shift_trials;
-        st.use_abmc = cfg->use_abmc != 0;
-        st.abmc_block_size = cfg->abmc_block_size;
-        st.abmc_num_colors = cfg->abmc_num_colors;
-        auto h = std::make_unique<ssa_ic>();
-        h->dtype = dtype;
-        if (cfg->precision != SSA_PRECISION_FP64 && cfg->precision != SSA_PRECISION_MIXED)
-            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: unknown precision");
-        // mixed: the factor (computed in FP64) is stored and applied in
-        // float / complex<float>; SpMV, vectors and dots stay FP64
-        const bool mixed = cfg->precision == SSA_PRECISION_MIXED;
-        using cd = std::complex<double>;
-        if (dtype == SSA_F64) {
-            const double* v = static_cast<const double*>(values);
-            if (mixed) h->impl.reset(new IcImpl<double, float>(ctx, n, row_ptr, col_idx, v, st));
-            else h->impl.reset(new IcImpl<double, double>(ctx, n, row_ptr, col_idx, v, st));
-        } else {
-            const cd* v = static_cast<const cd*>(values);
-            if (mixed) h->impl.reset(new IcImpl<ssa::c128, ssa::c64>(ctx, n, row_ptr, col_idx, v, st));
-            else h->impl.reset(new IcImpl<ssa::c128, ssa::c128>(ctx, n, row_ptr, col_idx, v, st));
-        }
-        *out = h.release();
+        if (!out) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_create: null argument");
+        const int32_t nn = n;
+        *out = create_handle("ssa_ic_create", ctx, dtype, 1, &nn, &row_ptr, &col_idx, &values, cfg);
+    });
+}
+
+int ssa_batch_create(ssa_context* ctx, int dtype, int count, const int32_t* n,
+                     const int32_t* const* row_ptr, const int32_t* const* col_idx,
+                     const void* const* values, const ssa_config* cfg, ssa_ic** out)
+{
+    return guarded([&] {
+        if (!out || !n || !row_ptr || !col_idx || !values || count < 1)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_batch_create: bad argument");
+        *out = create_handle("ssa_batch_create", ctx, dtype, count, n, row_ptr, col_idx, values, cfg);
+    });
+}
+
+int ssa_ic_systems(const ssa_ic* ic, int* count)
+{
+    return guarded([&] {
+        if (!ic || !count) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_systems: null argument");
+        *count = ic->impl->system_count();
     });
 }
 
@@ -1146,8 +1302,23 @@ int ssa_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device, con
         if (!ic || !b || !x || !cfg || !result) throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve: null argument");
         if (method < SSA_METHOD_ICCG || method > SSA_METHOD_NGS_ICCOCR)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve: unknown method");
+        if (ic->impl->system_count() != 1)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve: a batched handle is solved with ssa_batch_solve");
         std::memset(result, 0, sizeof(*result));
-        ic->impl->solve(method, b, x, on_device != 0, *cfg, *result, history);
+        ic->impl->solve(method, b, x, on_device != 0, *cfg, result, history);
+    });
+}
+
+int ssa_batch_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device,
+                    const ssa_config* cfg, ssa_result* results)
+{
+    return guarded([&] {
+        if (!ic || !b || !x || !cfg || !results) throw SsaError(SSA_ERR_ARGUMENT, "ssa_batch_solve: null argument");
+        if (method < SSA_METHOD_ICCG || method > SSA_METHOD_COCR)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_batch_solve: unknown method");
+        const int count = ic->impl->system_count();
+        std::memset(results, 0, sizeof(ssa_result) * size_t(count));
+        ic->impl->solve(method, b, x, on_device != 0, *cfg, results, nullptr);
     });
 }
 

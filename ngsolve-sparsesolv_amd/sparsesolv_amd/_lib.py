@@ -36,7 +36,7 @@ EXPORTED = (
     "ssa_config_default", "ssa_last_error", "ssa_context_create", "ssa_context_destroy",
     "ssa_ic_create", "ssa_ic_destroy", "ssa_ic_info_get", "ssa_ic_ordering",
     "ssa_ic_schedule", "ssa_ic_factor", "ssa_ic_apply", "ssa_spmv", "ssa_solve",
-    "ssa_ic_profile",
+    "ssa_ic_profile", "ssa_batch_create", "ssa_batch_solve", "ssa_ic_systems",
 )
 
 PROFILE_SPMV = 0
@@ -122,6 +122,11 @@ def load() -> C.CDLL:
                                 C.POINTER(Result), vp]
         L.ssa_ic_profile.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
+        L.ssa_batch_create.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(Config),
+                                       C.POINTER(vp)]
+        L.ssa_batch_solve.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.POINTER(Config),
+                                      C.POINTER(Result)]
+        L.ssa_ic_systems.argtypes = [vp, C.POINTER(C.c_int)]
         for name in EXPORTED:
             getattr(L, name).restype = C.c_char_p if name == "ssa_last_error" else (
                 None if name == "ssa_config_default" else C.c_int)
@@ -283,3 +288,66 @@ class IcHandle:
         check(self.lib.ssa_solve(self.h, method, C.c_void_p(b_ptr), C.c_void_p(x_ptr), 1,
                                  C.byref(cfg), C.byref(res), hp))
         return res
+
+
+class BatchHandle(IcHandle):
+    """Several independent systems in one handle (ssa_batch_create).  Vectors
+    hold the systems back to back, each in its own ordering."""
+
+    def __init__(self, systems, cfg: Config, ctx: Context | None = None, host_only: bool = False):
+        self.ctx = None if host_only else (ctx or context(0))
+        self.lib = load()
+        mats = list(systems)
+        if not mats:
+            raise ValueError("no systems")
+        self.is_complex = bool(mats[0].is_complex)
+        if any(bool(A.is_complex) != self.is_complex for A in mats):
+            raise TypeError("all systems must have the same scalar type")
+        self.dtype = np.complex128 if self.is_complex else np.float64
+        self.sizes = [int(A.n) for A in mats]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n = int(self.offsets[-1])
+        self.count = len(mats)
+        self._keep = []
+        rps, cis, vals = [], [], []
+        for A in mats:
+            rp = np.ascontiguousarray(A.row_ptr, dtype=np.int32)
+            ci = np.ascontiguousarray(A.col_idx, dtype=np.int32)
+            v = np.ascontiguousarray(A.values, dtype=self.dtype)
+            self._keep += [rp, ci, v]
+            rps.append(rp.ctypes.data)
+            cis.append(ci.ctypes.data)
+            vals.append(v.ctypes.data)
+        ns = np.array(self.sizes, dtype=np.int32)
+        arr = C.c_void_p * self.count
+        h = C.c_void_p()
+        check(self.lib.ssa_batch_create(self.ctx.h if self.ctx else None,
+                                        SSA_C128 if self.is_complex else SSA_F64, self.count,
+                                        ns.ctypes.data, arr(*rps), arr(*cis), arr(*vals),
+                                        C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def split(self, v):
+        """Per-system views of a concatenated vector."""
+        return [v[self.offsets[k]:self.offsets[k + 1]] for k in range(self.count)]
+
+    def solve_batch(self, method: int, b, x0, cfg: Config):
+        """(x, [Result per system]) for concatenated b / x0 (x0 None: zeros)."""
+        b = self._vec(b)
+        x = np.array(self._vec(np.zeros(self.n) if x0 is None else x0), dtype=self.dtype)
+        res = (Result * self.count)()
+        check(self.lib.ssa_batch_solve(self.h, method, b.ctypes.data, x.ctypes.data, 0,
+                                       C.byref(cfg), res))
+        return x, list(res)
+
+    def solve_batch_device(self, method: int, b_ptr: int, x_ptr: int, cfg: Config):
+        res = (Result * self.count)()
+        check(self.lib.ssa_batch_solve(self.h, method, C.c_void_p(b_ptr), C.c_void_p(x_ptr), 1,
+                                       C.byref(cfg), res))
+        return list(res)
+
+    def systems(self) -> int:
+        c = C.c_int()
+        check(self.lib.ssa_ic_systems(self.h, C.byref(c)))
+        return c.value

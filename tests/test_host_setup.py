@@ -109,3 +109,17 @@ def test_level_path_host_setup(oracle, name, builder, kw):
     co, bo = h.schedule()
     np.testing.assert_array_equal(np.diff(co), np.bincount(level))
     np.testing.assert_array_equal(bo, np.arange(A.n + 1))
+
+
+def test_batch_host_handle():
+    # host-only batched handle: the setup of every system runs, and the
+    # one-system queries are refused
+    mats = [P.laplace7(6), P.laplace7(4), P.curlcurl(4).A]
+    h = _lib.BatchHandle(mats, _lib.default_config(use_abmc=1, auto_shift=1), host_only=True)
+    assert h.systems() == 3
+    info = h.info()
+    assert info["n"] == sum(A.n for A in mats)
+    with pytest.raises(_lib.SsaError):
+        h.factor()
+    with pytest.raises(TypeError):
+        _lib.BatchHandle([mats[0], P.eddy(4).A], _lib.default_config(use_abmc=1), host_only=True)
