@@ -28,7 +28,10 @@ Extra workloads (`extra`, skipped with --quick):
              and the final true-residual check in FP64);
   *-level    curlcurl64 and laplace48 on the level-scheduled path
              (use_abmc=False, the factories' default), with the reference's
-             own level path as the CPU baseline.
+             own level path as the CPU baseline;
+  batch64    64 independent variable-coefficient Laplacians (16.2M DOFs)
+             in one batched solve, against the same systems solved one by
+             one on the GPU and the reference on a sample of them.
 
 The JSON line also carries
   roofline      the dominant kernel by time, the ABMC colour kernel k_sweep:
@@ -219,6 +222,136 @@ def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
     return out, (A, b, kw, w["method"], x)
 
 
+BATCH_DESC = ("64 independent 7-point variable-coefficient Laplacians, m ~ U{16..96} (seed 3), "
+              "16,184,264 DOFs in total, cell coefficients log-uniform in [1e-2, 1e2], one uniform "
+              "RHS each; ABMC-ICCG fp64, shift 1.05, block 4, 4 colours, tol 1e-10; all systems "
+              "in one batched solve with per-system stop status")
+
+
+def run_batch(steps, warmup, cpu=True, budget_s=25.0, profile_reps=10, tune=True):
+    """The 64-system batch: one batched solve per step (device-resident
+    vectors), then the same systems solved one after another on the GPU, and
+    the reference on the host on a bounded sample of the systems."""
+    import torch
+    from sparsesolv_amd import _lib
+    from sparsesolv_amd import problems as P
+    t0 = time.perf_counter()
+    systems = P.varcoef_batch(64)
+    build_s = time.perf_counter() - t0
+    kw = dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=4,
+              tolerance=1e-10, max_iterations=2000)
+    cfg = _lib.default_config(**kw)
+    t0 = time.perf_counter()
+    hb = _lib.BatchHandle([s.A for s in systems], cfg)
+    setup_s = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    b_d = torch.from_numpy(np.concatenate([s.b for s in systems])).to(dev)
+    x_d = torch.zeros(hb.n, dtype=torch.float64, device=dev)
+    log(f"[batch64] {len(systems)} systems, {hb.n} DOFs, setup {setup_s:.2f}s "
+        f"(matrices {build_s:.1f}s)")
+    res = None
+    for _ in range(warmup):
+        x_d.zero_()
+        torch.cuda.synchronize()
+        res = hb.solve_batch_device(_lib.METHOD_ICCG, b_d.data_ptr(), x_d.data_ptr(), cfg)
+    times = []
+    for k in range(steps):
+        x_d.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = hb.solve_batch_device(_lib.METHOD_ICCG, b_d.data_ptr(), x_d.data_ptr(), cfg)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        log(f"[batch64] step {k}: {times[-1]:.4f}s max iterations={max(r.iterations for r in res)} "
+            f"all converged={all(r.converged for r in res)}")
+    x = x_d.cpu().numpy()
+    worst = 0.0
+    for k, s in enumerate(systems):
+        xk = hb.split(x)[k]
+        worst = max(worst, float(np.linalg.norm(s.b - s.A.to_scipy() @ xk) / np.linalg.norm(s.b)))
+    iters = [int(r.iterations) for r in res]
+    # the same systems, one solve each (handles built beforehand, untimed)
+    singles = [_lib.IcHandle(s.A, cfg) for s in systems]
+    bufs = [(torch.from_numpy(s.b).to(dev), torch.zeros(s.A.n, dtype=torch.float64, device=dev))
+            for s in systems]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for h, (bb, xx) in zip(singles, bufs):
+        h.solve_device(_lib.METHOD_ICCG, bb.data_ptr(), xx.data_ptr(), cfg)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t
+    del singles, bufs
+    us, by = hb.profile(_lib.PROFILE_COLOUR_KERNELS, profile_reps) if profile_reps > 0 else (0.0, 0.0)
+    out = {
+        "workload": "batch64", "desc": BATCH_DESC, "systems": len(systems), "dofs": hb.n,
+        "seconds": float(np.mean(times)), "times": [round(t, 5) for t in times],
+        "iterations_max": max(iters), "iterations_mean": round(float(np.mean(iters)), 1),
+        "all_converged": bool(all(r.converged for r in res)),
+        "worst_true_rel_residual": worst, "setup_s": round(setup_s, 3),
+        "sequential_single_solves_s": round(seq_s, 4),
+        "batching_speedup": round(seq_s / float(np.mean(times)), 2),
+        "colour_kernel": None if us <= 0 else {"us": round(us, 2), "bytes": int(by),
+                                                "GB_s": round(by / us / 1e3, 1)},
+    }
+    if cpu:
+        out["cpu_baseline"] = batch_cpu_baseline(systems, kw, iters, budget_s)
+    if tune:
+        # autotuned ABMC parameters, chosen on the largest system
+        from sparsesolv_amd import autotune as at
+        big = max(systems, key=lambda q: q.A.n)
+        r = at.autotune(big.A, big.b, tolerance=1e-10, max_iterations=2000, shift_parameter=1.05)
+        tkw = r.config(**kw)
+        tcfg = _lib.default_config(**tkw)
+        del hb
+        ht = _lib.BatchHandle([q.A for q in systems], tcfg)
+        ht.solve_batch_device(_lib.METHOD_ICCG, b_d.data_ptr(), x_d.data_ptr(), tcfg)
+        tt = []
+        for _ in range(max(1, steps)):
+            x_d.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            tres = ht.solve_batch_device(_lib.METHOD_ICCG, b_d.data_ptr(), x_d.data_ptr(), tcfg)
+            torch.cuda.synchronize()
+            tt.append(time.perf_counter() - t)
+        out["tuned"] = {
+            "abmc_block_size": r.best.block_size, "abmc_num_colors": r.best.num_colors,
+            "tuned_on": f"the largest system (n={big.A.n})", "seconds": float(np.mean(tt)),
+            "iterations_max": max(int(q.iterations) for q in tres),
+            "all_converged": bool(all(q.converged for q in tres)),
+            "trials": [[t.block_size, t.num_colors, round(t.seconds, 4), t.iterations, t.converged]
+                       for t in r.trials]}
+        log(f"[batch64] tuned: {out['tuned']}")
+    return out
+
+
+def batch_cpu_baseline(systems, kw, iters, budget_s):
+    """The reference solving the systems one after another on this host:
+    systems taken in order until the time budget is spent, each timed for the
+    GPU's iteration count of that system; the total is extrapolated by
+    DOFs x iterations."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    kind = "reference" if pyoracle.available("ref_omp") else "port"
+    orc = pyoracle.Oracle("ref_omp" if kind == "reference" else "oracle")
+    fields = dict(orc.config()._fields_)
+    cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
+    spent, work_done, k = 0.0, 0.0, 0
+    while k < len(systems) and spent < budget_s:
+        s = systems[k]
+        t_it, _ = orc.time_iterations(s.A, s.b, cfg, max(iters[k], 1), "CG")
+        spent += t_it
+        work_done += s.A.n * iters[k]
+        k += 1
+    work_all = float(sum(s.A.n * it for s, it in zip(systems, iters)))
+    return {"value": round(spent * work_all / max(work_done, 1.0), 3), "unit": "s",
+            "cores": orc.num_threads(), "kind": kind, "cpu": cpu_model(),
+            "sample": (f"the first {k} of {len(systems)} systems solved by the reference "
+                       f"({spent:.2f}s, each for the GPU's iteration count, setup apart), "
+                       f"extrapolated by DOFs x iterations")}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -335,6 +468,10 @@ def main():
                 o["cpu_baseline"] = cpu_baseline(A2, b2, kw2, m2, o["iterations"], x2)
                 log(f"cpu baseline ({other}): {o['cpu_baseline']}")
             extra[other] = o
+    if not args.quick:
+        extra["batch64"] = run_batch(max(1, args.steps), 1, cpu=not args.no_cpu,
+                                     profile_reps=args.profile_reps)
+        log(f"[batch64] {json.dumps({k: v for k, v in extra['batch64'].items() if k != 'desc'})}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -116,3 +116,14 @@ def test_batch_handle_rejects_single_system_calls():
     with pytest.raises(_lib.SsaError):
         _lib.BatchHandle([A for A, _ in systems],
                          _lib.default_config(use_abmc=1, precision=_lib.PRECISION_MIXED))
+
+
+def test_autotune_on_gpu():
+    from sparsesolv_amd import autotune as at
+    A, b = varcoef(12, 40)
+    r = at.autotune(A, b, block_sizes=(2, 4), colours=(4, 8), reps=1, max_iterations=1000)
+    assert len(r.trials) == 4 and all(t.converged for t in r.trials)
+    assert r.best.seconds == min(t.seconds for t in r.trials)
+    cfg = _lib.default_config(**r.config(max_iterations=1000))
+    x, res, _ = _lib.IcHandle(A, cfg).solve(_lib.METHOD_ICCG, b, None, cfg)
+    assert res.converged and np.linalg.norm(b - A.to_scipy() @ x) < 1e-9 * np.linalg.norm(b)
