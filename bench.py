@@ -31,7 +31,13 @@ Extra workloads (`extra`, skipped with --quick):
              own level path as the CPU baseline;
   batch64    64 independent variable-coefficient Laplacians (16.2M DOFs)
              in one batched solve, against the same systems solved one by
-             one on the GPU and the reference on a sample of them.
+             one on the GPU and the reference on a sample of them;
+  eddy78-multi16  eddy78 with a block of 16 seeded complex-Gaussian
+             right-hand sides in one mixed-precision block solve (the SpMV
+             and the colour sweeps load each matrix and factor entry once for
+             all 16), with each right-hand side's error against its FP64
+             solution, beside the FP64 block solve and the one-vector FP64
+             and mixed solves.
 
 The JSON line also carries
   roofline      the dominant kernel by time, the ABMC colour kernel k_sweep:
@@ -124,17 +130,19 @@ def make_system(name):
     return _SYSTEMS[base]
 
 
-def bytes_per_iteration(n, nnz, nnz_l, s, cocr, s_low=None):
+def bytes_per_iteration(n, nnz, nnz_l, s, cocr, s_low=None, nrhs=1):
     """Compulsory bytes of one Krylov iteration in ABMC order: every operand
     touched once, gathers served from cache.  s = scalar size of the matrix
     and the Krylov vectors; s_low = scalar size of the factor and of the
     preconditioner's internal vectors (s unless mixed); nnz_l counts the
-    diagonal."""
+    diagonal.  A block of nrhs right-hand sides reads the matrix and the
+    factor once and every vector operand once per right-hand side."""
     s_f = s_p = s if s_low is None else s_low
     S_A = nnz * (s + 4) + 4 * (n + 1)
     S_L = nnz_l * (s_f + 4) + 4 * (n + 1)
-    vec = n * ((20 if cocr else 14) * s + 2 * s_p + s_f)
-    return S_A + 2 * S_L + vec
+    vec = n * ((20 if cocr else 14) * s + 2 * s_p) + n * s_f
+    vec_rhs = n * ((20 if cocr else 14) * s + 2 * s_p)
+    return S_A + 2 * S_L + (vec if nrhs == 1 else n * s_f + nrhs * vec_rhs)
 
 
 def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
@@ -324,6 +332,145 @@ def run_batch(steps, warmup, cpu=True, budget_s=25.0, profile_reps=10, tune=True
     return out
 
 
+MULTI_DESC = ("complex-symmetric eddy current, 78^3 hex grid (n=1,387,386, nnz=17,821,650), "
+              "16 seeded complex-Gaussian right-hand sides (seed 2; the first is eddy78's); "
+              "ABMC-IC(0)-COCR, shift 1.05, block 4, 8 colours, tol 1e-10, max 1000 "
+              "iterations; IC factor and sweeps in complex<float> inside the complex<double> "
+              "COCR with iterative refinement, all 16 in one block solve")
+
+
+def run_multi(steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_s=40.0):
+    """eddy78 with a block of nrhs right-hand sides: the mixed-precision block
+    solve per step (device-resident vectors), the FP64 block solve (each
+    right-hand side's FP64 solution, for its error), and the one-vector FP64
+    and mixed solves of the first right-hand side for the time per vector."""
+    import torch
+    from sparsesolv_amd import _lib
+    from sparsesolv_amd import problems as P
+    w = WORKLOADS["eddy78"]
+    sysm = make_system("eddy78")
+    A = sysm.A
+    B = P.complex_gaussian_block(A.n, nrhs, 2)
+    kw = dict(w["cfg"])
+    cfg64 = _lib.default_config(**kw)
+    cfgm = _lib.default_config(**dict(kw, precision=1))
+    t0 = time.perf_counter()
+    hm = _lib.IcHandle(A, cfgm)
+    setup_s = time.perf_counter() - t0
+    h64 = _lib.IcHandle(A, cfg64)
+    info = hm.info()
+    dev = torch.device("cuda:0")
+    B_d = torch.from_numpy(B).to(dev)
+    X_d = torch.zeros_like(B_d)
+    code = _lib.METHOD_ICCOCR
+
+    def block(h, cfg):
+        X_d.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = h.solve_multi_device(code, nrhs, B_d.data_ptr(), X_d.data_ptr(), cfg)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, res
+
+    def single(h, cfg):
+        b0 = B_d[0].contiguous()
+        x0 = torch.zeros_like(b0)
+        h.solve_device(code, b0.data_ptr(), x0.data_ptr(), cfg)    # warm-up
+        x0.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = h.solve_device(code, b0.data_ptr(), x0.data_ptr(), cfg)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, r
+
+    for _ in range(warmup):
+        block(hm, cfgm)
+    times, res = [], None
+    for k in range(steps):
+        t, res = block(hm, cfgm)
+        times.append(t)
+        log(f"[multi16] step {k}: {t:.4f}s iterations max={max(r.iterations for r in res)} "
+            f"all converged={all(r.converged for r in res)}")
+    Xm = X_d.cpu().numpy()
+    block(h64, cfg64)
+    t64, res64 = block(h64, cfg64)
+    X64 = X_d.cpu().numpy()
+    log(f"[multi16] fp64 block: {t64:.4f}s iterations max={max(r.iterations for r in res64)}")
+    t1_64, r1_64 = single(h64, cfg64)
+    t1_m, r1_m = single(hm, cfgm)
+    log(f"[multi16] one vector: fp64 {t1_64:.4f}s ({r1_64.iterations} it), "
+        f"mixed {t1_m:.4f}s ({r1_m.iterations} it)")
+    As = A.to_scipy()
+    err, trel = [], []
+    for k in range(nrhs):
+        err.append(float(np.linalg.norm(Xm[k] - X64[k]) / np.linalg.norm(X64[k])))
+        trel.append(float(np.linalg.norm(B[k] - As @ Xm[k]) / np.linalg.norm(B[k])))
+    sec = float(np.mean(times))
+    its = [int(r.iterations) for r in res]
+    kern = {}
+    for what, label in ((_lib.PROFILE_SPMV, "spmv"), (_lib.PROFILE_FORWARD, "forward_sweep"),
+                        (_lib.PROFILE_BACKWARD, "backward_sweep"),
+                        (_lib.PROFILE_APPLY, "precond_apply"),
+                        (_lib.PROFILE_COLOUR_KERNELS, "colour_kernel")):
+        if profile_reps <= 0:
+            break
+        us, by = hm.profile_multi(what, nrhs, profile_reps)
+        if us <= 0:
+            continue
+        kern[label] = {"us": round(us, 2), "bytes": int(by), "GB_s": round(by / us / 1e3, 1),
+                       "frac": round(by / us / 1e3 / PEAK_GBS, 4)}
+        log(f"[multi16] {label} (x{nrhs}): {us:.2f} us  {by / 1e6:.2f} MB  {by / us / 1e3:.0f} GB/s")
+    model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], 16, True, 8, nrhs)
+    it_mean = float(np.mean(its))
+    out = {
+        "workload": "eddy78-multi16", "desc": MULTI_DESC, "nrhs": nrhs, "n": A.n,
+        "seconds": sec, "times": [round(t, 5) for t in times],
+        "seconds_per_rhs": round(sec / nrhs, 5),
+        "iterations": its, "all_converged": bool(all(r.converged for r in res)),
+        "max_rel_error_vs_fp64": max(err), "rel_error_vs_fp64": err,
+        "max_true_rel_residual": max(trel),
+        "fp64_block": {"seconds": round(t64, 5),
+                       "iterations": [int(r.iterations) for r in res64],
+                       "all_converged": bool(all(r.converged for r in res64))},
+        "fp64_one_vector": {"seconds": round(t1_64, 5), "iterations": int(r1_64.iterations),
+                            "converged": bool(r1_64.converged)},
+        "mixed_one_vector": {"seconds": round(t1_m, 5), "iterations": int(r1_m.iterations)},
+        "speedup_vs_fp64_one_vector_each": round(nrhs * t1_64 / sec, 2),
+        "speedup_vs_fp64_block": round(t64 / sec, 2),
+        "setup_s": round(setup_s, 3),
+        "bytes_per_iteration_model": int(model),
+        "effective_GB_s": round(model * it_mean / sec / 1e9, 1),
+        "kernels": kern,
+    }
+    if cpu:
+        out["cpu_reference_rhs0"] = multi_cpu_check(A, B[0], kw, Xm[0], X64[0], budget_s)
+    return out
+
+
+def multi_cpu_check(A, b, kw, xm, x64, budget_s):
+    """The reference's FP64 COCR on the first right-hand side, when a full
+    solve fits the budget (estimated from 5 timed iterations): the errors of
+    the GPU's mixed and FP64 solutions against it."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    kind = "reference" if pyoracle.available("ref_omp") else "port"
+    orc = pyoracle.Oracle("ref_omp" if kind == "reference" else "oracle")
+    fields = dict(orc.config()._fields_)
+    cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
+    t_probe, setup = orc.time_iterations(A, b, cfg, 5, "COCR")
+    est = setup + t_probe / 5 * kw.get("max_iterations", 1000)
+    if est > budget_s:
+        return {"skipped": f"estimated {est:.0f}s for a full reference solve > {budget_s:.0f}s"}
+    t = time.perf_counter()
+    x, r = orc.solve_cocr(A, b, cfg)
+    return {"kind": kind, "cores": orc.num_threads(), "seconds": round(time.perf_counter() - t, 3),
+            "iterations": int(r["iterations"]), "converged": bool(r["converged"]),
+            "mixed_rel_error": float(np.linalg.norm(xm - x) / np.linalg.norm(x)),
+            "fp64_rel_error": float(np.linalg.norm(x64 - x) / np.linalg.norm(x))}
+
+
 def batch_cpu_baseline(systems, kw, iters, budget_s):
     """The reference solving the systems one after another on this host:
     systems taken in order until the time budget is spent, each timed for the
@@ -435,6 +582,8 @@ def main():
                     help="fixed iteration budget per solve (per-iteration timing runs)")
     ap.add_argument("--profile-reps", type=int, default=20,
                     help="repetitions in the HIP-event kernel timing (0: skip)")
+    ap.add_argument("--extras", default=None,
+                    help="comma-separated extra workloads to run (default: all of them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB between two device buffers (a copy kernel of "
                          "known size for scaling hardware byte counters)")
@@ -456,10 +605,12 @@ def main():
     main_out, (A, b, kw, method, x_gpu) = run_gpu(args.workload, args.steps, args.warmup,
                                                  args.budget, args.profile_reps)
     extra = {"primary": main_out}
+    names = ("curlcurl64-mixed", "curlcurl64-level", "laplace48", "laplace48-mixed",
+             "laplace48-level", "eddy78", "eddy78-mixed", "batch64", "eddy78-multi16")
+    wanted = set(names if args.extras is None else args.extras.split(","))
     if not args.quick:
-        for other in ("curlcurl64-mixed", "curlcurl64-level", "laplace48", "laplace48-mixed",
-                      "laplace48-level", "eddy78", "eddy78-mixed"):
-            if other == args.workload:
+        for other in names[:7]:
+            if other == args.workload or other not in wanted:
                 continue
             o, (A2, b2, kw2, m2, x2) = run_gpu(other, max(1, args.steps), 1, args.budget,
                                                args.profile_reps)
@@ -468,10 +619,14 @@ def main():
                 o["cpu_baseline"] = cpu_baseline(A2, b2, kw2, m2, o["iterations"], x2)
                 log(f"cpu baseline ({other}): {o['cpu_baseline']}")
             extra[other] = o
-    if not args.quick:
+    if not args.quick and "batch64" in wanted:
         extra["batch64"] = run_batch(max(1, args.steps), 1, cpu=not args.no_cpu,
                                      profile_reps=args.profile_reps)
         log(f"[batch64] {json.dumps({k: v for k, v in extra['batch64'].items() if k != 'desc'})}")
+    if not args.quick and "eddy78-multi16" in wanted:
+        extra["eddy78-multi16"] = run_multi(max(1, args.steps), 1, cpu=not args.no_cpu,
+                                            profile_reps=args.profile_reps)
+        log(f"[multi16] {json.dumps({k: v for k, v in extra['eddy78-multi16'].items() if k != 'desc'})}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")

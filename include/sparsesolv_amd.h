@@ -161,7 +161,7 @@ int ssa_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device,
  * the reference builds one ICPreconditioner per system; this is that setup
  * for `count` systems at once, for callers that solve many systems
  * (parameter studies, frequency sweeps).  System s is n[s], row_ptr[s],
- * col_idx[s], values[s]; all share cfg (FP64 only).  Vectors passed to
+ * col_idx[s], values[s]; all share cfg.  Vectors passed to
  * ssa_ic_apply, ssa_spmv and ssa_batch_solve hold the systems back to back
  * in their own orderings (sum of n[s] entries).  The ordering / schedule /
  * factor queries are for one-system handles only. */
@@ -176,6 +176,21 @@ int ssa_batch_solve(ssa_ic* ic, int method, const void* b, void* x, int on_devic
                     const ssa_config* cfg, ssa_result* results);
 /* Number of systems of a handle (1 unless made by ssa_batch_create). */
 int ssa_ic_systems(const ssa_ic* ic, int* count);
+
+/* Several right-hand sides on one matrix.  No reference counterpart: the
+ * reference solves one vector per call; these run nrhs of them on a
+ * one-system handle in blocks of up to 16 (1, 4, 8 or 16 per launch, a
+ * partial block padded), the SpMV and the colour sweeps loading each matrix
+ * and factor entry once for the whole block.  b, x, and the vectors of
+ * ssa_ic_apply_multi / ssa_spmv_multi hold nrhs vectors of n values back to
+ * back (vector k at offset k*n).  ssa_solve_multi fills results[nrhs]; each
+ * right-hand side has its own scalars and stop status, and its result is
+ * what ssa_solve gives for it alone, bit for bit.  Methods as ssa_batch_solve
+ * (not SSA_METHOD_NGS_ICCOCR). */
+int ssa_solve_multi(ssa_ic* ic, int method, int nrhs, const void* b, void* x, int on_device,
+                    const ssa_config* cfg, ssa_result* results);
+int ssa_ic_apply_multi(ssa_ic* ic, int nrhs, const void* x, void* y, int on_device);
+int ssa_spmv_multi(ssa_ic* ic, int nrhs, const void* x, void* y, int on_device);
 
 /* Kernel timing with HIP events on the handle's stream (no reference
  * counterpart: measurement hook for the roofline report).  what:
@@ -198,6 +213,10 @@ int ssa_ic_systems(const ssa_ic* ic, int* count);
 #define SSA_PROFILE_APPLY 3
 #define SSA_PROFILE_COLOUR_KERNELS 4
 int ssa_ic_profile(ssa_ic* ic, int what, int reps, double* avg_us, double* bytes);
+/* The same timing for a block of nrhs right-hand sides (1..16, padded to the
+ * block width); the bytes count every vector operand once per right-hand
+ * side and the matrix and factor entries once. */
+int ssa_ic_profile_multi(ssa_ic* ic, int what, int nrhs, int reps, double* avg_us, double* bytes);
 
 #ifdef __cplusplus
 }

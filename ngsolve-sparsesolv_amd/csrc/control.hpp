@@ -44,8 +44,8 @@ __device__ inline bool check_convergence(SolverState* s, const SolveParams& prm,
         s->best_residual = rel;
         s->improved = 1;
     }
-    if (prm.ngs) return rel <= prm.tolerance;
-    if (rel < prm.tolerance || nr < prm.abs_tolerance) return true;
+    if (prm.ngs) return rel <= s->tol;
+    if (rel < s->tol || nr < prm.abs_tolerance) return true;
     if (prm.divergence_check) {
         if (rel >= s->best_residual * prm.divergence_threshold) ++s->bad_count;
         else s->bad_count = 0;
@@ -108,7 +108,7 @@ __device__ inline void ctrl_beta_op(SolverState* st, const SolveParams& prm, T v
     to_pair(ddiv(v, old), st->beta);
     to_pair(v, st->rz);
     st->iter += 1;
-    if (st->iter >= prm.max_iterations) stop(st, kMaxIter, 0, prm.max_iterations, st->norm_r);
+    if (st->iter >= st->maxit) stop(st, kMaxIter, 0, st->maxit, st->norm_r);
 }
 
 template <typename T>
@@ -132,11 +132,11 @@ struct Ctrl {
     SolverState* st;
     double* hist;
     const void* part;      // the producing kernel's partials
-    int np;                // their count
+    int np;                // their count (per right-hand side)
     int src_spmv;          // batches: the producer ran on the SpMV grid
     SolveParams prm;
-    int part_off;          // batches: first partial of the workgroup's system
-    int first;             // batches: the workgroup is its system's first
+    int part_off;          // first partial of the workgroup's state
+    int first;             // the workgroup is its state's first
 };
 
 // Several independent systems in one launch (null pointers: one system).
@@ -144,6 +144,11 @@ struct Ctrl {
 // its workgroups are [vec_g0[s], vec_g0[s+1]) and cover its rows exactly
 // as a one-system launch of that many workgroups would (same partials,
 // same bits); on the SpMV grid its workgroups are [spmv_g0[s], ...).
+//
+// Several right-hand sides of one system: right-hand side r owns the rows
+// [r*vstride, (r+1)*vstride) of every vector, state r, history r and the
+// partials [r*np, (r+1)*np) of a producer with np workgroups per
+// right-hand side.  The vector kernels run a (G, W) grid, blockIdx.y = r.
 struct Systems {
     const int* vec_sys;
     const int* vec_g0;
@@ -152,35 +157,42 @@ struct Systems {
     const int* row0;
     const int* nrows;
     int hist_stride;
+    index_t vstride;
 };
 
 // The rows a vector-grid workgroup owns: i = local*kBlock + lane, stepping
-// by G*kBlock while i < n, at device row row0 + i.
+// by G*kBlock while i < n, at device row row0 + i; `sys` is its state and
+// `slot` the index of its partial.
 struct Span {
     index_t row0, n, local, G;
-    int sys;
+    int sys, slot;
 };
 
 __device__ inline Span vec_span(const Systems& y, index_t n)
 {
-    if (!y.vec_sys) return Span{0, n, index_t(blockIdx.x), index_t(gridDim.x), 0};
+    if (!y.vec_sys) {
+        const int r = int(blockIdx.y);
+        return Span{index_t(r) * y.vstride, n, index_t(blockIdx.x), index_t(gridDim.x), r,
+                    r * int(gridDim.x) + int(blockIdx.x)};
+    }
     const int s = y.vec_sys[blockIdx.x];
     const int g0 = y.vec_g0[s];
-    return Span{y.row0[s], y.nrows[s], index_t(blockIdx.x) - g0, index_t(y.vec_g0[s + 1] - g0), s};
+    return Span{y.row0[s], y.nrows[s], index_t(blockIdx.x) - g0, index_t(y.vec_g0[s + 1] - g0), s,
+                int(blockIdx.x)};
 }
 
-// the control step as seen from a workgroup of system `sys` whose index
-// among the system's workgroups is `local`
+// the control step as seen from a workgroup of state `sys` (batch system
+// or right-hand side) whose index among that state's workgroups is `local`
 __device__ inline Ctrl ctrl_for(Ctrl c, const Systems& y, int sys, index_t local)
 {
     c.first = local == 0;
+    c.st += sys;
+    if (c.hist) c.hist += size_t(sys) * size_t(y.hist_stride);
     if (!y.vec_sys) {
-        c.part_off = 0;
+        c.part_off = sys * c.np;
         return c;
     }
     const int* g0 = c.src_spmv ? y.spmv_g0 : y.vec_g0;
-    c.st += sys;
-    if (c.hist) c.hist += size_t(sys) * size_t(y.hist_stride);
     c.part_off = g0[sys];
     c.np = g0[sys + 1] - g0[sys];
     return c;
@@ -240,23 +252,30 @@ __device__ inline bool ctrl_beta_all(const Ctrl& c, T* red, T& beta)
 // ---------------------------------------------------------------------------
 
 // prepare_iteration (iterative_solver.hpp:135-180): normaliser, initial
-// history entry after r = b - Ax.
-__global__ __launch_bounds__(kCtrlThreads) void k_ctrl_init(const double* __restrict__ pb,
+// history entry after r = b - Ax.  One workgroup per state; pp (optional)
+// holds each state's tolerance, iteration budget and whether it runs.
+// (static: every translation unit that launches it has its own copy)
+static __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_init(const double* __restrict__ pb,
                                                             const double* __restrict__ pr,
                                                             int np, SolverState* st,
                                                             SolveParams prm,
                                                             double* __restrict__ hist,
-                                                            Systems y)
+                                                            Systems y,
+                                                            const PassParams* __restrict__ pp)
 {
-    // one workgroup per system; its partials are those of its vector grid
+    // the state's partials are those of its vector grid
+    const int s = blockIdx.x;
     if (y.vec_sys) {
-        const int s = blockIdx.x, g0 = y.vec_g0[s];
+        const int g0 = y.vec_g0[s];
         pb += g0;
         pr += g0;
         np = y.vec_g0[s + 1] - g0;
-        st += s;
-        if (hist) hist += size_t(s) * size_t(y.hist_stride);
+    } else {
+        pb += size_t(s) * size_t(np);
+        pr += size_t(s) * size_t(np);
     }
+    st += s;
+    if (hist) hist += size_t(s) * size_t(y.hist_stride);
     __shared__ double red[kCtrlThreads / 64];
     const double sb = sum_partials(pb, np, red);
     const double sr = sum_partials(pr, np, red);
@@ -278,28 +297,37 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_init(const double* __rest
     st->hist_len = 0;
     st->status = kRunning;
     st->converged = 0;
+    st->tol = pp ? pp[s].tolerance : prm.tolerance;
+    st->maxit = pp ? pp[s].max_iterations : prm.max_iterations;
+    if (pp && !pp[s].active) {
+        stop(st, kConverged, 1, 0, init);
+        return;
+    }
     if (prm.save_history) hist[st->hist_len++] = prm.ngs ? 1.0 : init / nz;
     if (prm.ngs && norm_b < 1e-30) {
         stop(st, kConverged, 1, 0, 0.0);
         st->normalizer = 1.0;
         return;
     }
-    if (prm.max_iterations <= 0) stop(st, kMaxIter, 0, prm.max_iterations, init);
+    if (st->maxit <= 0) stop(st, kMaxIter, 0, st->maxit, init);
 }
 
-// rz_old (CG) or rho (COCR) from dot partials into slot 0 (rz) or 1 (denom)
+// rz_old (CG) or rho (COCR) from dot partials into slot 0 (rz) or 1 (denom);
+// one workgroup per state
 template <typename T>
 __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_store(const T* __restrict__ part, int np,
                                                             SolverState* st, int slot,
                                                             Systems y, int src_spmv)
 {
+    const int s = blockIdx.x;
     if (y.vec_sys) {
-        const int s = blockIdx.x;
         const int* g0 = src_spmv ? y.spmv_g0 : y.vec_g0;
         part += g0[s];
         np = g0[s + 1] - g0[s];
-        st += s;
+    } else {
+        part += size_t(s) * size_t(np);
     }
+    st += s;
     __shared__ T red[kCtrlThreads / 64];
     const T v = sum_partials(part, np, red);
     if (threadIdx.x != 0) return;

@@ -1,0 +1,13 @@
+// ic_f64_mixed.hip -- the handle for real systems with a float factor.
+#include "ic_impl.hpp"
+
+namespace ssa_impl {
+
+IcBase* make_f64_mixed(ssa_context* ctx, int count, const int32_t* n, const int32_t* const* rp,
+                      const int32_t* const* ci, const void* const* values,
+                      const sparsesolv_amd::IcSettings& st)
+{
+    return make_impl<double, float, double>(ctx, count, n, rp, ci, values, st);
+}
+
+}  // namespace ssa_impl

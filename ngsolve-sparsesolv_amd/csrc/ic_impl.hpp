@@ -1,0 +1,1229 @@
+// ic_impl.hpp -- the handle behind the C ABI: upload, IC apply, SpMV and the
+// ICCG / COCR drivers on one MI355X.  Each ic_*.hip file instantiates it for
+// one precision pair.
+//
+// Design (one GPU, one HIP stream per context):
+//  * the host (host_setup.cpp) computes the ABMC schedule and the IC(0)
+//    factor; this file uploads A (ABMC order, SELL-64), the forward and
+//    backward sweep layouts and D^-1 once per ssa_ic_create;
+//  * the Krylov loop runs entirely in ABMC order (b is gathered in, x is
+//    gathered out), so the preconditioner applies no permutation;
+//  * scalars never leave the device: control kernels reduce the
+//    per-workgroup partials in index order and update SolverState; the host
+//    only polls the stop status every few iterations;
+//  * one iteration is captured into a hipGraph and replayed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "host_plan.hpp"
+#include "kernels.hpp"
+#include "ic_handle.hpp"
+#include "sparsesolv_amd.h"
+
+using ssa::index_t;
+
+namespace ssa_impl {
+
+// owning device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // true when new memory was allocated
+    bool alloc(size_t b)
+    {
+        if (b == bytes && p) return false;
+        release();
+        if (b == 0) b = 16;
+        HIP_CHECK(hipMalloc(&p, b));
+        bytes = b;
+        return true;
+    }
+    // at least b bytes (kept when large enough); true when new memory was
+    // allocated
+    bool reserve(size_t b)
+    {
+        if (p && bytes >= b) return false;
+        return alloc(b);
+    }
+    template <typename V>
+    void upload(const std::vector<V>& v, hipStream_t s)
+    {
+        alloc(v.size() * sizeof(V));
+        if (!v.empty()) HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, s));
+    }
+    template <typename V> V* as() const { return static_cast<V*>(p); }
+};
+
+constexpr int kMaxGrid = 1024;  // vector kernels: partial count bound
+// sweep entries per batch: 16 for real, 8 for complex values, 8 in the
+// small-colour kernel (register budget of its large workgroup); with W
+// right-hand sides per lane at most 8, with at most 64 gathers (a
+// wavefront's limit of outstanding loads) and 512 bytes of gathered values
+// per lane in flight.  The layouts pad their entry arrays by kSweepChunk.
+template <typename P, int W> constexpr int sweep_chunk()
+{
+    if (W == 1) return sizeof(P) > 8 ? 8 : 16;
+    constexpr int by_count = 64 / W, by_bytes = int(512 / (W * sizeof(P)));
+    constexpr int c = by_count < by_bytes ? by_count : by_bytes;
+    return c > 8 ? 8 : c < 1 ? 1 : c;
+}
+// right-hand sides one launch carries (larger blocks are split)
+constexpr int kMaxRhs = 16;
+// the block width that carries nrhs right-hand sides: 1, 4, 8 or 16
+inline int rhs_width(int nrhs) { return nrhs <= 1 ? 1 : nrhs <= 4 ? 4 : nrhs <= 8 ? 8 : 16; }
+static_assert(sparsesolv_amd::kSweepChunk >= 16, "layout padding shorter than a batch");
+// Krylov iterations per captured graph
+constexpr int kGraphIterations = 8;
+// mixed precision: refinement passes per solve, and the residual reduction
+// a COCR pass aims for before the next restart
+constexpr int kRefinePasses = 8;
+constexpr double kInnerReduction = 1e-5;
+
+inline int vec_grid(index_t n)
+{
+    return int(std::max<index_t>(1, std::min<index_t>((n + ssa::kBlock - 1) / ssa::kBlock, kMaxGrid)));
+}
+
+
+// ===========================================================================
+// IC handle
+// ===========================================================================
+
+using ssa::SolverState;
+using ssa::SolveParams;
+
+template <typename T> struct HostOf { using type = T; };
+template <> struct HostOf<ssa::c128> { using type = std::complex<double>; };
+template <> struct HostOf<ssa::c64> { using type = std::complex<float>; };
+
+struct DevSweep {
+    DevBuf lrow, lpos, slot, val, col;
+    DevBuf slice_sys, walk_sys;   // batches: system of every slice / oversized block
+    bool batched = false;
+    DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
+    DevBuf w_step_off, w_row, w_pos, w_ext_rp, w_ext_ci, w_ext_v, w_invd, w_v, w_ix, w_sa, w_sb;
+    std::vector<index_t> phase_slices, phase_big, phase_rows;
+    std::vector<int64_t> phase_entries;
+    int BR = 1;
+    int walk_kmax = 0;
+    std::vector<index_t> h_step_off;
+    size_t walk_slots = 0;   // walk slots of one right-hand side
+    bool walk_checked = false, walk_fits = false;
+    index_t nphases = 0, max_big_rows = 0, max_levels = 0;
+    size_t stream_bytes = 0;
+
+    template <typename F>
+    void upload(const sparsesolv_amd::SweepLayout<F>& L, hipStream_t s)
+    {
+        lrow.upload(L.lane_row, s);
+        lpos.upload(L.lane_pos, s);
+        slot.upload(L.slice_slot, s);
+        val.upload(L.val, s);
+        col.upload(L.col, s);
+        big_pos.upload(L.big_pos, s);
+        big_lvl_off.upload(L.big_lvl_off, s);
+        big_lvl.upload(L.big_lvl, s);
+        big_rows.upload(L.big_rows, s);
+        big_rp.upload(L.big_rp, s);
+        big_ci.upload(L.big_ci, s);
+        big_v.upload(L.big_v, s);
+        w_step_off.upload(L.walk_step_off, s);
+        w_row.upload(L.walk_row, s);
+        w_pos.upload(L.walk_pos, s);
+        w_ext_rp.upload(L.walk_ext_rp, s);
+        w_ext_ci.upload(L.walk_ext_ci, s);
+        w_ext_v.upload(L.walk_ext_v, s);
+        w_invd.upload(L.walk_invd, s);
+        w_v.upload(L.walk_v, s);
+        w_ix.upload(L.walk_ix, s);
+        walk_kmax = L.walk_kmax;
+        h_step_off = L.walk_step_off;
+        walk_slots = L.walk_row.size();
+        walk_scratch(1, s);
+        phase_slices = L.phase_slices;
+        phase_big = L.phase_big;
+        phase_rows = L.phase_rows;
+        phase_entries = L.phase_entries;
+        BR = L.BR;
+        nphases = L.nphases;
+        max_big_rows = L.max_big_rows;
+        max_levels = L.max_levels;
+        stream_bytes = L.val.size() * sizeof(F) + L.col.size() * 4 + L.big_v.size() * sizeof(F) +
+                       L.big_ci.size() * 4;
+    }
+    // the walk's per-slot scratch for w right-hand sides
+    void walk_scratch(int w, hipStream_t s)
+    {
+        const size_t b = size_t(w) * walk_slots * sizeof(ssa::c128);
+        // idle walk lanes read these slots and never have them written
+        if (w_sa.reserve(b)) HIP_CHECK(hipMemsetAsync(w_sa.p, 0, w_sa.bytes, s));
+        if (w_sb.reserve(b)) HIP_CHECK(hipMemsetAsync(w_sb.p, 0, w_sb.bytes, s));
+    }
+    size_t bytes() const
+    {
+        return lrow.bytes + lpos.bytes + slot.bytes + val.bytes + col.bytes +
+               big_pos.bytes + big_lvl_off.bytes + big_lvl.bytes + big_rows.bytes +
+               big_rp.bytes + big_ci.bytes + big_v.bytes + w_step_off.bytes + w_row.bytes +
+               w_pos.bytes + w_ext_rp.bytes + w_ext_ci.bytes + w_ext_v.bytes + w_invd.bytes +
+               w_v.bytes + w_ix.bytes;
+    }
+};
+
+
+template <typename T, typename F>
+struct IcImpl final : IcBase {
+    using HT = typename HostOf<T>::type;
+    using HF = typename HostOf<F>::type;
+    using P = F;  // preconditioner-internal vector type
+
+    ssa_context* ctx;
+    // count systems; n = device rows (the systems' rows, each padded to a
+    // multiple of 256 when count > 1); tot = rows of the caller's vectors
+    int count = 1;
+    index_t n = 0, tot = 0, nnz = 0, nnz_l = 0, nc = 0, nb = 0;
+    double actual_shift = 0.0, setup_seconds = 0.0;
+    int shift_trials = 0;
+    // host copies kept for queries (one system)
+    std::vector<index_t> h_ordering, h_color_offsets, h_block_offsets, h_Lrp, h_Lci;
+    std::vector<HT> h_Lv, h_invd;
+    // batches: per system row ranges and grid ranges
+    std::vector<int> h_row0, h_nrows, h_vec_g0, h_spmv_g0;
+    DevBuf d_vec_sys, d_vec_g0, d_spmv_sys, d_spmv_g0, d_row0, d_nrows;
+    int vec_total = 0;
+    int hist_stride = 0;
+
+    // device: operator, permutations, factor
+    DevBuf A_off, A_val, A_col, perm, rev, invd;
+    index_t A_nslices = 0;
+    size_t spmv_bytes = 0;
+    DevSweep fwd, bwd;
+    // device: work space
+    DevBuf r, z, p, Ap, t, qt, x, best, bvec, tmp, tmp2;
+    DevBuf part_a, part_b, part_c, part_d, state, hist;
+    DevBuf idle;   // states that never stop: sweeps outside a solve
+    // right-hand sides of the current operation (every vector holds W blocks
+    // of n rows; W > 1 for one system only) and the width the work space
+    // was sized for
+    int W = 1, ws_width = 0;
+    DevBuf pass;                          // PassParams per state
+    ssa::PassParams* h_pass = nullptr;    // pinned staging for `pass`
+    int h_cap = 0;                        // states h_state / h_pass hold
+    // Mixed precision (P narrower than T): the preconditioner's input, output
+    // and intermediate vectors in P.  In FP64 they alias the Krylov vectors.
+    static constexpr bool kMixed = !std::is_same<T, P>::value;
+    DevBuf lo_a, lo_b, lo_t;
+
+    SolverState* h_state = nullptr;  // pinned, one per state
+
+    IcImpl(ssa_context* c, int count_, const int* ns, const int32_t* const* rp,
+           const int32_t* const* ci, const HT* const* v, const sparsesolv_amd::IcSettings& st)
+        : ctx(c), count(count_)
+    {
+        ssa_host::BatchPlan<HT, HF> plan;
+        ssa_host::build_batch<HT, HF>(plan, count, ns, rp, ci, v, st);
+        n = plan.row0.back();
+        tot = plan.in_off.back();
+        setup_seconds = plan.wall_seconds;
+        for (int s = 0; s < count; ++s) {
+            nnz += plan.nnz[s];
+            nnz_l += plan.nnz_l[s];
+            nc = std::max<index_t>(nc, plan.num_colors[s]);
+            nb += plan.num_blocks[s];
+            actual_shift = std::max(actual_shift, plan.actual_shift[s]);
+            shift_trials = std::max(shift_trials, plan.shift_trials[s]);
+        }
+        h_row0 = plan.row0;
+        h_nrows = plan.nrows;
+        if (count == 1) {
+            const auto& f = plan.single.fac;
+            h_ordering = f.sched.ordering;
+            h_color_offsets = f.sched.color_offsets;
+            h_block_offsets = f.sched.block_offsets;
+            h_Lrp = f.L.rp;
+            h_Lci = f.L.ci;
+            h_Lv = f.L.v;
+            h_invd = f.inv_diag;
+        }
+
+        if (!ctx) return;   // host-only handle: schedule and factor queries only
+        hipStream_t s = ctx->stream;
+        A_off.upload(plan.spmv.slice_off, s);
+        A_val.upload(plan.spmv.val, s);
+        A_col.upload(plan.spmv.col, s);
+        A_nslices = plan.spmv.nslices;
+        spmv_bytes = plan.spmv.val.size() * sizeof(HT) + plan.spmv.col.size() * 4;
+        perm.upload(plan.gather_out, s);
+        rev.upload(plan.gather_in, s);
+        invd.upload(plan.invd, s);
+        fwd.upload(plan.fwd, s);
+        bwd.upload(plan.bwd, s);
+        if (count > 1) {
+            // each system keeps the grid a one-system solve would use, so its
+            // reductions give the same bits
+            std::vector<int> vsys, ssys;
+            h_vec_g0.assign(1, 0);
+            h_spmv_g0.assign(1, 0);
+            for (int q = 0; q < count; ++q) {
+                const int g = vec_grid(plan.nrows[q]);
+                for (int k = 0; k < g; ++k) vsys.push_back(q);
+                h_vec_g0.push_back(h_vec_g0.back() + g);
+                const int gs = (plan.row0[q + 1] - plan.row0[q]) / (64 * ssa::kWavesPerBlock);
+                for (int k = 0; k < gs; ++k) ssys.push_back(q);
+                h_spmv_g0.push_back(h_spmv_g0.back() + gs);
+            }
+            vec_total = h_vec_g0.back();
+            d_vec_sys.upload(vsys, s);
+            d_vec_g0.upload(h_vec_g0, s);
+            d_spmv_sys.upload(ssys, s);
+            d_spmv_g0.upload(h_spmv_g0, s);
+            d_row0.upload(plan.row0, s);
+            d_nrows.upload(plan.nrows, s);
+            fwd.slice_sys.upload(plan.slice_sys_f, s);
+            bwd.slice_sys.upload(plan.slice_sys_b, s);
+            fwd.walk_sys.upload(plan.walk_sys_f, s);
+            bwd.walk_sys.upload(plan.walk_sys_b, s);
+            fwd.batched = bwd.batched = true;
+            // oversized blocks must take the walk: the level kernel's arrays
+            // are not combined
+            for (DevSweep* D : {&fwd, &bwd})
+                if (D->phase_big.back() > 0 && !walk_lds_ok(*D))
+                    throw SsaError(SSA_ERR_UNSUPPORTED,
+                                   "batch: an oversized block does not fit the walk's LDS");
+        }
+        HIP_CHECK(hipStreamSynchronize(s));   // host plan is freed on return
+    }
+
+    // the systems descriptor kernels take (null tables for one system)
+    ssa::Systems systems() const
+    {
+        if (count == 1)
+            return ssa::Systems{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, hist_stride, n};
+        return ssa::Systems{d_vec_sys.as<int>(), d_vec_g0.as<int>(), d_spmv_sys.as<int>(),
+                            d_spmv_g0.as<int>(), d_row0.as<int>(), d_nrows.as<int>(), hist_stride, 0};
+    }
+    // vector-grid workgroups per right-hand side, and the launch grid
+    int vgrid() const { return count == 1 ? vec_grid(n) : vec_total; }
+    dim3 vgrid2() const { return dim3(unsigned(vgrid()), unsigned(W)); }
+    // solver states: one per system of a batch, one per right-hand side
+    int nstates() const { return count > 1 ? count : W; }
+    int system_count() const override { return count; }
+
+    void need_device() const
+    {
+        if (!ctx)
+            throw SsaError(SSA_ERR_ARGUMENT, "operation needs a handle created with a device context");
+    }
+
+    ~IcImpl() override
+    {
+        if (h_state) (void)hipHostFree(h_state);
+        if (h_pass) (void)hipHostFree(h_pass);
+    }
+
+    // ---- building blocks ---------------------------------------------------
+    // work space for `width` right-hand sides; sets W
+    void ensure_workspace(int width = 1)
+    {
+        need_device();
+        if (width > 1 && count > 1)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "several right-hand sides need a one-system handle");
+        W = std::max(width, 1);
+        if (W == ws_width) return;
+        hipStream_t s = ctx->stream;
+        const int nst = nstates();
+        if (nst > h_cap) {
+            if (h_state) (void)hipHostFree(h_state);
+            if (h_pass) (void)hipHostFree(h_pass);
+            h_state = nullptr;
+            h_pass = nullptr;
+            h_cap = 0;
+            HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_state), sizeof(SolverState) * nst));
+            HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_pass), sizeof(ssa::PassParams) * nst));
+            h_cap = nst;
+        }
+        const size_t vb = size_t(n) * size_t(W) * sizeof(T);
+        // zeroed once: the padding rows between the systems of a batch are
+        // never written and must stay zero (the SpMV's fused dot multiplies
+        // them by zero entries)
+        for (DevBuf* b : {&r, &z, &p, &Ap, &t, &qt, &x, &best, &bvec, &tmp, &tmp2})
+            if (b->reserve(vb)) HIP_CHECK(hipMemsetAsync(b->p, 0, vb, s));
+        const size_t np = size_t(std::max<index_t>(std::max(kMaxGrid, vec_total), (A_nslices + 3) / 4)) *
+                          size_t(W) * sizeof(ssa::c128);
+        for (DevBuf* b : {&part_a, &part_b, &part_c, &part_d}) b->reserve(np);
+        state.reserve(sizeof(SolverState) * nst);
+        pass.reserve(sizeof(ssa::PassParams) * nst);
+        if (kMixed)
+            for (DevBuf* b : {&lo_a, &lo_b, &lo_t})
+                if (b->reserve(size_t(n) * size_t(W) * sizeof(P)))
+                    HIP_CHECK(hipMemsetAsync(b->p, 0, b->bytes, s));
+        if (idle.reserve(sizeof(SolverState) * nst)) HIP_CHECK(hipMemsetAsync(idle.p, 0, idle.bytes, s));
+        fwd.walk_scratch(W, s);
+        bwd.walk_scratch(W, s);
+        ws_width = W;
+    }
+
+    int spmv_grid() const { return int((A_nslices + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock); }
+
+    // y = A x (ABMC order) for the W right-hand sides, optional fused dot
+    // into part
+    void launch_spmv(const T* xin, T* y, int dot, T* part, const SolverState* st,
+                     const ssa::Ctrl& ctrl = ssa::Ctrl{})
+    {
+        switch (W) {
+        case 1: spmv_w<1>(xin, y, dot, part, st, ctrl); break;
+        case 4: spmv_w<4>(xin, y, dot, part, st, ctrl); break;
+        case 8: spmv_w<8>(xin, y, dot, part, st, ctrl); break;
+        case 16: spmv_w<16>(xin, y, dot, part, st, ctrl); break;
+        default: throw SsaError(SSA_ERR_ARGUMENT, "unsupported block width");
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    template <int WV>
+    void spmv_w(const T* xin, T* y, int dot, T* part, const SolverState* st, const ssa::Ctrl& ctrl)
+    {
+        const int g = std::max(1, spmv_grid());
+        hipStream_t s = ctx->stream;
+        const ssa::Systems sy = systems();
+        const index_t* off = A_off.as<index_t>();
+        const index_t* col = A_col.as<index_t>();
+        const T* val = A_val.as<T>();
+        if (dot == 0)
+            ssa::k_spmv<T, 0, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, n, A_nslices, xin, y, part, st, ctrl, sy);
+        else if (dot == 1)
+            ssa::k_spmv<T, 1, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, n, A_nslices, xin, y, part, st, ctrl, sy);
+        else
+            ssa::k_spmv<T, 2, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, n, A_nslices, xin, y, part, st, ctrl, sy);
+    }
+
+    ssa::SweepArrays sweep_arrays(DevSweep& D)
+    {
+        return ssa::SweepArrays{D.lrow.as<index_t>(), D.lpos.as<index_t>(), D.slot.as<index_t>(),
+                                D.val.p, D.col.as<index_t>(), invd.p,
+                                D.batched ? D.slice_sys.as<int>() : nullptr};
+    }
+
+    // a colour small enough to share a single-workgroup kernel with its
+    // neighbours: no oversized block, at most two passes of the workgroup
+    static int tail_threads(int BR)
+    {
+        return BR >= 16 ? ssa::tail_threads<P, 16>() : ssa::tail_threads<P, 1>();
+    }
+
+    bool small_phase(const DevSweep& D, index_t ph) const
+    {
+        return D.phase_big[ph + 1] == D.phase_big[ph] &&
+               D.phase_slices[ph + 1] - D.phase_slices[ph] <= 2 * (tail_threads(D.BR) / 64);
+    }
+
+    // end of the run of small colours that starts at ph (ph itself when
+    // fewer than two colours would share the tail kernel)
+    index_t tail_end(const DevSweep& D, index_t ph) const
+    {
+        index_t e = ph;
+        while (e < D.nphases && e - ph < ssa::kTailMaxPhases && small_phase(D, e)) ++e;
+        return e - ph >= 2 ? e : ph;
+    }
+
+    bool in_tail_group(const DevSweep& D, index_t ph) const
+    {
+        for (index_t q = 0; q < D.nphases;) {
+            const index_t e = tail_end(D, q);
+            if (e > q) {
+                if (ph >= q && ph < e) return true;
+                q = e;
+            } else {
+                ++q;
+            }
+        }
+        return false;
+    }
+
+    // all colours of one sweep; runs of small colours go to one workgroup
+    template <bool FWD>
+    void launch_sweep(DevSweep& D, const P* in, P* out, const SolverState* st)
+    {
+        index_t ph = 0;
+        while (ph < D.nphases) {
+            const index_t e = tail_end(D, ph);
+            if (e > ph) {
+                launch_tail<FWD>(D, ph, e, in, out, st);
+                ph = e;
+            } else {
+                launch_phase<FWD>(D, ph, in, out, st);
+                ++ph;
+            }
+        }
+    }
+
+    template <bool FWD>
+    void launch_tail(DevSweep& D, index_t p0, index_t p1, const P* in, P* out, const SolverState* st)
+    {
+        if (!st) st = idle.as<SolverState>();
+        ssa::TailPhases tp{};
+        tp.count = int(p1 - p0);
+        for (index_t q = p0; q <= p1; ++q) tp.slices[q - p0] = D.phase_slices[q];
+        const ssa::SweepArrays A = sweep_arrays(D);
+        hipStream_t s = ctx->stream;
+        // one workgroup per right-hand side
+#define SSA_TAIL(BRV)                                                                           \
+    ssa::k_sweep_tail<P, F, BRV, FWD, 8><<<W, ssa::tail_threads<P, BRV>(), 0, s>>>(tp, A, in, out, n, st)
+        switch (D.BR == 0 ? 1 : D.BR) {
+        case 1: SSA_TAIL(1); break;
+        case 2: SSA_TAIL(2); break;
+        case 4: SSA_TAIL(4); break;
+        case 8: SSA_TAIL(8); break;
+        case 16: SSA_TAIL(16); break;
+        default: throw SsaError(SSA_ERR_UNSUPPORTED, "sweep: unsupported rows per lane");
+        }
+#undef SSA_TAIL
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // one colour's regular slices (and the external part of its oversized
+    // blocks) with WV right-hand sides per lane
+    template <bool FWD, int BRV, int WV>
+    void sweep_w(int g, index_t s0, index_t s1, const ssa::SweepArrays& A, const P* in, P* out,
+                 const ssa::WalkExt& we, const SolverState* st)
+    {
+        ssa::k_sweep<P, F, BRV, FWD, sweep_chunk<P, WV>(), WV><<<g, ssa::kBlock, 0, ctx->stream>>>(
+            s0, s1, A, in, out, n, we, st);
+    }
+
+    template <bool FWD, int BRV>
+    void sweep_br(int g, index_t s0, index_t s1, const ssa::SweepArrays& A, const P* in, P* out,
+                  const ssa::WalkExt& we, const SolverState* st)
+    {
+        switch (W) {
+        case 1: sweep_w<FWD, BRV, 1>(g, s0, s1, A, in, out, we, st); break;
+        case 4: sweep_w<FWD, BRV, 4>(g, s0, s1, A, in, out, we, st); break;
+        case 8: sweep_w<FWD, BRV, 8>(g, s0, s1, A, in, out, we, st); break;
+        case 16: sweep_w<FWD, BRV, 16>(g, s0, s1, A, in, out, we, st); break;
+        default: throw SsaError(SSA_ERR_ARGUMENT, "unsupported block width");
+        }
+    }
+
+    template <bool FWD>
+    void launch_phase(DevSweep& D, index_t ph, const P* in, P* out, const SolverState* st)
+    {
+        if (!st) st = idle.as<SolverState>();
+        hipStream_t s = ctx->stream;
+        const index_t s0 = D.phase_slices[ph], s1 = D.phase_slices[ph + 1];
+        const index_t b0 = D.phase_big[ph], b1 = D.phase_big[ph + 1];
+        const bool walk = b1 > b0 && D.walk_kmax > 0 && walk_lds_ok(D);
+        ssa::WalkExt we{};
+        int gw = 0;
+        if (walk) {
+            we.t0 = size_t(D.h_step_off[b0]) * 64;
+            we.t1 = size_t(D.h_step_off[b1]) * 64;
+            we.row = D.w_row.as<index_t>();
+            we.ext_rp = D.w_ext_rp.as<index_t>();
+            we.ext_ci = D.w_ext_ci.as<index_t>();
+            we.ext_v = D.w_ext_v.p;
+            we.sa = D.w_sa.p;
+            we.sb = D.w_sb.p;
+            we.ws = D.walk_slots;
+            gw = int((we.t1 - we.t0 + ssa::kBlock - 1) / ssa::kBlock);
+        }
+        if (s1 > s0 || gw > 0) {
+            const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
+            const ssa::SweepArrays A = sweep_arrays(D);
+#define SSA_SWEEP(BRV) sweep_br<FWD, BRV>(g, s0, s1, A, in, out, we, st)
+            switch (D.BR == 0 ? 1 : D.BR) {
+            case 1: SSA_SWEEP(1); break;
+            case 2: SSA_SWEEP(2); break;
+            case 4: SSA_SWEEP(4); break;
+            case 8: SSA_SWEEP(8); break;
+            case 16: SSA_SWEEP(16); break;
+            default: throw SsaError(SSA_ERR_UNSUPPORTED, "sweep: unsupported rows per lane");
+            }
+#undef SSA_SWEEP
+            HIP_CHECK(hipGetLastError());
+        }
+        if (walk) {
+            const size_t lds = (size_t(D.max_big_rows) + 2) * sizeof(P);
+#define SSA_WALK(KM, DEPTH)                                                                    \
+    ssa::k_sweep_walk<P, F, FWD, KM, DEPTH><<<dim3(unsigned(b1 - b0), unsigned(W)), 64, lds, s>>>( \
+        b0, D.w_step_off.as<index_t>(), D.w_row.as<index_t>(), D.w_pos.as<index_t>(),         \
+        D.w_invd.as<F>(), D.w_v.as<F>(), D.w_ix.as<index_t>(), D.max_big_rows, out,           \
+        D.w_sa.as<P>(), D.w_sb.as<P>(), n, D.walk_slots, st,                                  \
+        D.batched ? D.walk_sys.as<int>() : nullptr)
+            switch (D.walk_kmax) {
+            case 2: SSA_WALK(2, 8); break;
+            case 4: SSA_WALK(4, 8); break;
+            default: SSA_WALK(8, 8); break;
+            }
+#undef SSA_WALK
+            HIP_CHECK(hipGetLastError());
+        } else if (b1 > b0) {
+            const size_t lds = size_t(D.max_big_rows) * sizeof(P);
+            const int use_lds = lds <= 64 * 1024 ? 1 : 0;
+            ssa::k_sweep_big<P, F, FWD><<<dim3(unsigned(b1 - b0), unsigned(W)), 1024, use_lds ? lds : 0, s>>>(
+                b0, D.big_pos.as<index_t>(), D.big_lvl_off.as<index_t>(), D.big_lvl.as<index_t>(),
+                D.big_rows.as<index_t>(), D.big_rp.as<index_t>(), D.big_ci.as<index_t>(),
+                D.big_v.as<F>(), invd.as<F>(), in, out, use_lds, n, st);
+            HIP_CHECK(hipGetLastError());
+        }
+    }
+
+    // whether the walk's LDS (one value per big-block row + a zero slot)
+    // fits the workgroup limit; raises the kernels' dynamic LDS cap once
+    bool walk_lds_ok(DevSweep& D)
+    {
+        const size_t need = (size_t(D.max_big_rows) + 2) * sizeof(P);
+        if (D.walk_checked) return D.walk_fits;
+        D.walk_checked = true;
+        int optin = 0;
+        if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device) != hipSuccess ||
+            optin <= 0)
+            HIP_CHECK(hipDeviceGetAttribute(&optin, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+        D.walk_fits = need <= size_t(optin);
+        if (D.walk_fits && need > 64 * 1024) {
+            const int bytes = int(need);
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 2, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 4, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 8, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 2, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 4, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 8, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        }
+        return D.walk_fits;
+    }
+
+    // out = M^{-1} in (ABMC order); t is the forward result
+    void launch_precond(const P* in, P* out, P* tbuf, const SolverState* st)
+    {
+        launch_sweep<true>(fwd, in, tbuf, st);
+        launch_sweep<false>(bwd, tbuf, out, st);
+    }
+
+    // dst[i] = src[map[i]] for i < len (map -1: zero), for the first m
+    // right-hand sides (vector r of src at r*sstride, of dst at r*dstride)
+    void launch_gather(const index_t* map, const T* src, index_t sstride, T* dst, index_t dstride,
+                       index_t len, int m = 1)
+    {
+        ssa::k_gather<T><<<dim3(unsigned(vec_grid(len)), unsigned(m)), ssa::kBlock, 0, ctx->stream>>>(
+            len, map, src, sstride, dst, dstride);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // preconditioner operands: CG applies M^{-1} to r (z out), COCR to q = Ap
+    // (qt out); in mixed precision both go through the P copies lo_a, lo_b
+    P* pre_in(bool cocr) { return kMixed ? lo_a.as<P>() : (cocr ? Ap.as<P>() : r.as<P>()); }
+    P* pre_out(bool cocr) { return kMixed ? lo_b.as<P>() : (cocr ? qt.as<P>() : z.as<P>()); }
+    P* pre_tmp() { return kMixed ? lo_t.as<P>() : t.as<P>(); }
+    // the P copy a mixed-precision kernel also writes (none in FP64)
+    P* low_copy() { return kMixed ? lo_a.as<P>() : nullptr; }
+
+    template <typename S, typename D>
+    void convert(const S* src, D* dst, const SolverState* st)
+    {
+        ssa::k_copy<S, D><<<vgrid2(), ssa::kBlock, 0, ctx->stream>>>(n, src, dst, st, systems());
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // zero the vectors of right-hand sides m..W-1 (block padding)
+    void zero_tail(DevBuf& b, int m)
+    {
+        if (m < W)
+            HIP_CHECK(hipMemsetAsync(b.as<T>() + size_t(m) * size_t(n), 0,
+                                     size_t(W - m) * size_t(n) * sizeof(T), ctx->stream));
+    }
+
+    // device rows [first, first + count) of state q's vector
+    std::pair<size_t, size_t> rows_of(int q) const
+    {
+        if (count > 1) return {size_t(h_row0[q]), size_t(h_nrows[q])};
+        return {size_t(q) * size_t(n), size_t(n)};
+    }
+    void copy_rows(DevBuf& dst, const DevBuf& src, int q)
+    {
+        const auto rr = rows_of(q);
+        HIP_CHECK(hipMemcpyAsync(dst.as<T>() + rr.first, src.as<T>() + rr.first, rr.second * sizeof(T),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+    }
+
+    // the caller's nrhs vectors (tot values each), in blocks of at most
+    // kMaxRhs right-hand sides
+    template <typename Fn>
+    void in_blocks(int nrhs, Fn&& fn)
+    {
+        if (nrhs < 1) throw SsaError(SSA_ERR_ARGUMENT, "nrhs must be at least 1");
+        if (nrhs > 1 && count > 1)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "several right-hand sides need a one-system handle");
+        for (int r0 = 0; r0 < nrhs; r0 += kMaxRhs) fn(r0, std::min(kMaxRhs, nrhs - r0));
+    }
+
+    // ---- public operations -------------------------------------------------
+    void apply(const void* xin, void* yout, bool on_device, int nrhs) override
+    {
+        in_blocks(nrhs, [&](int r0, int m) {
+            const size_t off = size_t(r0) * size_t(tot);
+            operate(static_cast<const T*>(xin) + off, static_cast<T*>(yout) + off, on_device, m, true);
+        });
+    }
+
+    void spmv(const void* xin, void* yout, bool on_device, int nrhs) override
+    {
+        in_blocks(nrhs, [&](int r0, int m) {
+            const size_t off = size_t(r0) * size_t(tot);
+            operate(static_cast<const T*>(xin) + off, static_cast<T*>(yout) + off, on_device, m, false);
+        });
+    }
+
+    // y = M^{-1} x (precond) or y = A x for m right-hand sides
+    void operate(const T* xin, T* yout, bool on_device, int m, bool precond)
+    {
+        ensure_workspace(rhs_width(m));
+        hipStream_t s = ctx->stream;
+        const size_t vb = size_t(tot) * size_t(m) * sizeof(T);   // the caller's vectors
+        const T* src = xin;
+        if (!on_device) {
+            HIP_CHECK(hipMemcpyAsync(tmp.p, xin, vb, hipMemcpyHostToDevice, s));
+            src = tmp.as<T>();
+        }
+        launch_gather(rev.as<index_t>(), src, tot, r.as<T>(), n, n, m);
+        zero_tail(r, m);
+        if (precond) {
+            if (kMixed) convert(r.as<T>(), pre_in(false), nullptr);
+            launch_precond(pre_in(false), pre_out(false), pre_tmp(), nullptr);
+            if (kMixed) convert(pre_out(false), z.as<T>(), nullptr);
+        } else {
+            launch_spmv(r.as<T>(), z.as<T>(), 0, nullptr, nullptr);
+        }
+        T* dst = on_device ? yout : tmp2.as<T>();
+        launch_gather(perm.as<index_t>(), z.as<T>(), n, dst, tot, tot, m);
+        if (!on_device) HIP_CHECK(hipMemcpyAsync(yout, tmp2.p, vb, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // one Krylov iteration (kernels only; no host synchronisation)
+    void enqueue_iteration(int method, const SolveParams& prm)
+    {
+        hipStream_t s = ctx->stream;
+        SolverState* st = state.as<SolverState>();
+        const int G = vgrid();
+        const dim3 G2 = vgrid2();
+        const ssa::Systems y = systems();
+        const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
+                         method == SSA_METHOD_NGS_ICCOCR;
+        double* H = hist.as<double>();
+        // control steps fused into the kernels that consume their result;
+        // `spmv`: the partials come from the SpMV grid; np counts the
+        // producer's workgroups per right-hand side
+        const int np_spmv = std::max(1, spmv_grid());
+        auto ctl = [&](const void* part, int np, bool spmv) {
+            return ssa::Ctrl{st, H, part, np, spmv ? 1 : 0, prm, 0, 0};
+        };
+        T* bx = prm.save_best ? best.as<T>() : nullptr;
+        const bool c = prm.cocr != 0;
+        if (!c) {
+            // CGSolver::do_iterate body (cg_solver.hpp:85-140): the alpha step
+            // runs in the update kernel, the convergence check in the r.z
+            // kernel (after the preconditioner, which cannot change x or r),
+            // beta and the best-x copy in the p update
+            launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
+            ssa::k_cg_update<T, P><<<G2, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv, true), low_copy(), y);
+            if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
+            else ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st, y);
+            ssa::k_dot<T, T, P><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G, false), y);
+            ssa::k_xpby<T, P><<<G2, ssa::kBlock, 0, s>>>(n, pre_out(c), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G, false), y);
+        } else {
+            // COCRSolver::do_iterate body (cocr_solver.hpp:96-162); z holds rt,
+            // Ap holds q.  alpha in the update kernel, the check in the
+            // t = A rt kernel, beta and the best-x copy in the p, q update
+            if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
+            else ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st, y);
+            if (kMixed)   // (q, qt) with the narrow operand second
+                ssa::k_dot<T, T, P><<<G2, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
+            else
+                ssa::k_dot<T, P, T><<<G2, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
+            ssa::k_cocr_update<T, P><<<G2, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
+            launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G, false));
+            ssa::k_cocr_pq<T, P><<<G2, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // ||v|| of every state in FP64 (partials summed on the host in index
+    // order, as a one-system solve of that system sums them)
+    std::vector<double> state_norms(const T* v)
+    {
+        hipStream_t s = ctx->stream;
+        ssa::k_norms<T><<<vgrid2(), ssa::kBlock, 0, s>>>(n, v, nullptr, part_d.as<double>(), nullptr, systems());
+        HIP_CHECK(hipGetLastError());
+        const int G = vgrid();
+        std::vector<double> h(size_t(G) * size_t(W), 0.0);
+        HIP_CHECK(hipMemcpyAsync(h.data(), part_d.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        const int S = nstates();
+        std::vector<double> out(size_t(S), 0.0);
+        for (int q = 0; q < S; ++q) {
+            const int a = count > 1 ? h_vec_g0[q] : q * G, b = count > 1 ? h_vec_g0[q + 1] : (q + 1) * G;
+            double sum = 0.0;
+            for (int i = a; i < b; ++i) sum += h[size_t(i)];
+            out[size_t(q)] = std::sqrt(sum);
+        }
+        return out;
+    }
+
+    // ||b - A x|| of every state in FP64 (uses Ap and r)
+    std::vector<double> true_residuals()
+    {
+        const index_t rows = n * index_t(W);
+        launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
+        ssa::k_residual<T><<<vec_grid(rows), ssa::kBlock, 0, ctx->stream>>>(rows, bvec.as<T>(), Ap.as<T>(), r.as<T>());
+        HIP_CHECK(hipGetLastError());
+        return state_norms(r.as<T>());
+    }
+
+    // prepare_iteration and the iteration loop from the current x, with the
+    // per-state settings in h_pass; leaves the final states in h_state
+    void run_pass(int method, const SolveParams& prm, bool pre, const ssa_config& cfg)
+    {
+        hipStream_t s = ctx->stream;
+        const int G = vgrid();
+        const dim3 G2 = vgrid2();
+        const int S = nstates();
+        const ssa::Systems y = systems();
+        SolverState* st = state.as<SolverState>();
+        double* H = hist.as<double>();
+        HIP_CHECK(hipMemsetAsync(state.p, 0, sizeof(SolverState) * S, s));
+        HIP_CHECK(hipMemcpyAsync(pass.p, h_pass, sizeof(ssa::PassParams) * S, hipMemcpyHostToDevice, s));
+        int maxit = 0;
+        for (int q = 0; q < S; ++q)
+            if (h_pass[q].active) maxit = std::max(maxit, h_pass[q].max_iterations);
+
+        // prepare_iteration: r = b - A x, norms, normaliser (one control
+        // workgroup per state)
+        const index_t rows = n * index_t(W);
+        launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
+        ssa::k_residual<T><<<vec_grid(rows), ssa::kBlock, 0, s>>>(rows, bvec.as<T>(), Ap.as<T>(), r.as<T>());
+        ssa::k_norms<T><<<G2, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), r.as<T>(), part_b.as<double>(), part_d.as<double>(), y);
+        ssa::k_ctrl_init<<<S, ssa::kBlock, 0, s>>>(part_b.as<double>(), part_d.as<double>(), G, st, prm, H, y,
+                                                   pass.as<ssa::PassParams>());
+        HIP_CHECK(hipGetLastError());
+        const bool c = prm.cocr != 0;
+        if (!c) {
+            // rz_old = (r, M^{-1} r); p = z (cg_solver.hpp:73-77)
+            if (pre) {
+                if (kMixed) convert(r.as<T>(), pre_in(c), nullptr);
+                launch_precond(pre_in(c), pre_out(c), pre_tmp(), nullptr);
+            } else {
+                ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr, y);
+            }
+            ssa::k_dot<T, T, P><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), nullptr, ssa::Ctrl{}, y);
+            ssa::k_ctrl_store<T><<<S, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, 0, y, 0);
+            convert(pre_out(c), p.as<T>(), nullptr);
+        } else {
+            // rt = M^{-1} r; p = rt; q = A p; rho = rt^T q (cocr_solver.hpp:78-88)
+            if (pre) {
+                if (kMixed) convert(r.as<T>(), pre_in(c), nullptr);
+                launch_precond(kMixed ? pre_in(c) : r.as<P>(), kMixed ? pre_out(c) : z.as<P>(),
+                               pre_tmp(), nullptr);
+                if (kMixed) convert(pre_out(c), z.as<T>(), nullptr);
+            } else {
+                ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr, y);
+            }
+            ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr, y);
+            launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_c.as<T>(), nullptr);
+            ssa::k_ctrl_store<T><<<S, ssa::kBlock, 0, s>>>(part_c.as<T>(), std::max(1, spmv_grid()), st, 0, y, 1);
+            if (kMixed) convert(Ap.as<T>(), pre_in(c), nullptr);
+        }
+        HIP_CHECK(hipGetLastError());
+
+        // iterations: replay a captured group of iterations, poll the stop
+        // status between groups.  The control kernels stop each state at its
+        // iteration budget themselves and every kernel returns at once after
+        // a stop, so iterations enqueued past the stop cost only launches.
+        const int unroll = cfg.use_graph ? std::min(kGraphIterations, maxit + 1) : 1;
+        int poll = std::max(cfg.poll_interval > 0 ? cfg.poll_interval : 8, unroll);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (cfg.use_graph) {
+            HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int u = 0; u < unroll; ++u) enqueue_iteration(method, prm);
+            HIP_CHECK(hipStreamEndCapture(s, &graph));
+            HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        int done = 0;
+        // runs while any state runs (a stopped state's kernels return at once)
+        auto running = [&] {
+            HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState) * S, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            for (int q = 0; q < S; ++q)
+                if (h_state[q].status == ssa::kRunning) return true;
+            return false;
+        };
+        bool more = running();
+        while (more && done <= maxit) {
+            const int k = std::min(poll, maxit + 1 - done);
+            const int groups = (k + unroll - 1) / unroll;
+            for (int i = 0; i < groups; ++i) {
+                if (exec) HIP_CHECK(hipGraphLaunch(exec, s));
+                else enqueue_iteration(method, prm);
+            }
+            done += groups * unroll;
+            more = running();
+            if (poll < 64) poll *= 2;
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+
+    void solve(int method, const void* bin, void* xio, bool on_device, const ssa_config& cfg,
+               ssa_result* results, double* history, int nrhs) override
+    {
+        in_blocks(nrhs, [&](int r0, int m) {
+            const size_t off = size_t(r0) * size_t(tot);
+            solve_block(method, static_cast<const T*>(bin) + off, static_cast<T*>(xio) + off, on_device,
+                        cfg, results + r0, nrhs == 1 ? history : nullptr, m);
+        });
+    }
+
+    // one block of m right-hand sides (m = 1 for a batch: its systems)
+    void solve_block(int method, const T* bin, T* xio, bool on_device, const ssa_config& cfg,
+                     ssa_result* results, double* history, int m)
+    {
+        ensure_workspace(rhs_width(m));
+        hipStream_t s = ctx->stream;
+        const int S = nstates();
+        const int nres = count > 1 ? count : m;      // results the caller receives
+        const size_t vb = size_t(tot) * size_t(m) * sizeof(T);   // the caller's vectors
+        const size_t db = size_t(n) * size_t(W) * sizeof(T);     // device rows
+        SolveParams prm{};
+        prm.tolerance = cfg.tolerance;
+        prm.abs_tolerance = cfg.abs_tolerance;
+        prm.custom_norm = cfg.custom_norm;
+        prm.divergence_threshold = cfg.divergence_threshold;
+        prm.max_iterations = cfg.max_iterations;
+        prm.norm_type = cfg.norm_type;
+        prm.save_best = cfg.save_best_result ? 1 : 0;
+        prm.save_history = cfg.save_residual_history ? 1 : 0;
+        prm.divergence_check = cfg.divergence_check ? 1 : 0;
+        prm.divergence_count = cfg.divergence_count;
+        prm.conjugate = cfg.conjugate ? 1 : 0;
+        prm.cocr = (method == SSA_METHOD_ICCOCR || method == SSA_METHOD_COCR ||
+                    method == SSA_METHOD_NGS_ICCOCR) ? 1 : 0;
+        prm.ngs = method == SSA_METHOD_NGS_ICCOCR ? 1 : 0;
+        if (prm.ngs) {
+            prm.save_best = 0;
+            prm.divergence_check = 0;
+        }
+        const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
+                         method == SSA_METHOD_NGS_ICCOCR;
+        hist_stride = std::max(cfg.max_iterations, 0) + 4;
+        hist.reserve(sizeof(double) * size_t(hist_stride) * size_t(S));
+        double* H = hist.as<double>();
+
+        if (S > 1 && prm.ngs)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "COCRSolverNGS runs one system and one right-hand side");
+        if (kMixed && !pre)
+            throw SsaError(SSA_ERR_UNSUPPORTED,
+                           "ssa_solve: mixed precision needs the IC preconditioner");
+        const auto t0 = std::chrono::steady_clock::now();
+        // inputs into ABMC order: b -> bvec, x0 -> x
+        const T* bsrc = bin;
+        const T* xsrc = xio;
+        if (!on_device) {
+            HIP_CHECK(hipMemcpyAsync(tmp.p, bin, vb, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(tmp2.p, xio, vb, hipMemcpyHostToDevice, s));
+            bsrc = tmp.as<T>();
+            xsrc = tmp2.as<T>();
+        }
+        launch_gather(rev.as<index_t>(), bsrc, tot, bvec.as<T>(), n, n, m);
+        zero_tail(bvec, m);
+        if (prm.ngs) HIP_CHECK(hipMemsetAsync(x.p, 0, db, s));
+        else launch_gather(rev.as<index_t>(), xsrc, tot, x.as<T>(), n, n, m);
+        zero_tail(x, m);
+        HIP_CHECK(hipMemsetAsync(best.p, 0, db, s));
+
+        // per state: whether it is solved at all (block padding is not),
+        // iterations, the FP64 true residual (mixed) and the final state
+        std::vector<char> active(size_t(S), 1);
+        for (int q = nres; q < S; ++q) active[size_t(q)] = 0;
+        std::vector<int> iters(size_t(S), 0);
+        std::vector<double> true_rel(size_t(S), -1.0);
+        std::vector<SolverState> fin(size_t(S), SolverState{});
+        if (!kMixed || prm.ngs) {
+            for (int q = 0; q < S; ++q)
+                h_pass[q] = ssa::PassParams{cfg.tolerance, cfg.max_iterations, active[size_t(q)]};
+            run_pass(method, prm, pre, cfg);
+            for (int q = 0; q < S; ++q) {
+                fin[size_t(q)] = h_state[q];
+                iters[size_t(q)] = h_state[q].result_iters;
+            }
+        } else {
+            // Mixed precision, iterative refinement, per state: the residual
+            // is computed in FP64 from x, each pass runs the preconditioned
+            // Krylov method from x until its residual falls below the
+            // tolerance or (COCR) below kInnerReduction times the residual the
+            // pass started from -- COCR carries M^{-1} r by a recurrence that,
+            // with a float preconditioner, stalls near float precision -- and
+            // the next pass restarts from the new x.  A pass that ends
+            // unconverged leaves its best iterate (when save_best_result); a
+            // pass that does not lower the true residual is undone, so x and
+            // the reported residual always match.  A state leaves the loop
+            // when it converges, runs out of iterations, or stops improving.
+            const std::vector<double> bnorm = state_norms(bvec.as<T>());
+            std::vector<double> scale(static_cast<size_t>(S)), rel(static_cast<size_t>(S));
+            std::vector<double> res0 = true_residuals();
+            for (int q = 0; q < S; ++q) {
+                scale[size_t(q)] = bnorm[size_t(q)] > 0.0 ? bnorm[size_t(q)] : 1.0;
+                rel[size_t(q)] = res0[size_t(q)] / scale[size_t(q)];
+            }
+            std::vector<char> done(size_t(S), 0), run(size_t(S), 0);
+            for (int pass = 0; pass < kRefinePasses; ++pass) {
+                bool any = false;
+                for (int q = 0; q < S; ++q) {
+                    const size_t u = size_t(q);
+                    run[u] = active[u] && !done[u] && !(rel[u] < cfg.tolerance) &&
+                             iters[u] < cfg.max_iterations;
+                    any = any || run[u];
+                    const double tol = prm.cocr ? std::max(cfg.tolerance, kInnerReduction * rel[u])
+                                                : cfg.tolerance;
+                    h_pass[q] = ssa::PassParams{tol, cfg.max_iterations - iters[u], run[u] ? 1 : 0};
+                }
+                if (!any) break;
+                HIP_CHECK(hipMemcpyAsync(tmp.p, x.p, db, hipMemcpyDeviceToDevice, s));
+                run_pass(method, prm, pre, cfg);
+                for (int q = 0; q < S; ++q) {
+                    const size_t u = size_t(q);
+                    if (!run[u]) continue;
+                    fin[u] = h_state[q];
+                    iters[u] += h_state[q].result_iters;
+                    if (!h_state[q].converged && prm.save_best) copy_rows(x, best, q);
+                }
+                const std::vector<double> res = true_residuals();
+                for (int q = 0; q < S; ++q) {
+                    const size_t u = size_t(q);
+                    if (!run[u]) continue;
+                    const double next = res[u] / scale[u];
+                    if (!(next < rel[u])) {
+                        copy_rows(x, tmp, q);
+                        done[u] = 1;
+                        continue;
+                    }
+                    if (!(next < 0.5 * rel[u])) done[u] = 1;
+                    rel[u] = next;
+                }
+            }
+            for (int q = 0; q < S; ++q) {
+                const size_t u = size_t(q);
+                SolverState& Sq = fin[u];
+                true_rel[u] = rel[u];
+                Sq.converged = rel[u] < cfg.tolerance;
+                if (Sq.converged) Sq.status = ssa::kConverged;
+                else if (Sq.status == ssa::kRunning) Sq.status = ssa::kMaxIter;
+            }
+        }
+
+        // build_result (iterative_solver.hpp:262-279), per state
+        for (int q = 0; q < nres; ++q) {
+            const size_t u = size_t(q);
+            const SolverState& Sq = fin[u];
+            ssa_result& res = results[q];
+            res.converged = Sq.converged;
+            res.iterations = iters[u];
+            res.status = Sq.status;
+            res.final_residual = true_rel[u] >= 0.0 ? true_rel[u] : Sq.final_norm / Sq.normalizer;
+            res.history_len = prm.save_history && S == 1 ? Sq.hist_len : 0;
+            res.actual_shift = pre ? actual_shift : 0.0;
+            if (true_rel[u] < 0.0 && !Sq.converged && prm.save_best) {
+                copy_rows(x, best, q);
+                res.final_residual = Sq.best_residual;
+            }
+        }
+        T* xdst = on_device ? xio : tmp2.as<T>();
+        launch_gather(perm.as<index_t>(), x.as<T>(), n, xdst, tot, tot, m);
+        if (!on_device) HIP_CHECK(hipMemcpyAsync(xio, tmp2.p, vb, hipMemcpyDeviceToHost, s));
+        if (history && S == 1 && results[0].history_len > 0)
+            HIP_CHECK(hipMemcpyAsync(history, H, sizeof(double) * results[0].history_len, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int q = 0; q < nres; ++q) results[q].solve_seconds = secs;
+    }
+
+    // Each colour launched on its own (no tail group, no oversized-block
+    // work) is run `reps` times back to back between two events, after the
+    // whole sweep has run once; avg_us is the mean over those colours of one
+    // launch's share, bytes the mean of their algorithmic bytes.  Zero when
+    // no colour qualifies.
+    void profile_colour_kernels(int reps, double& avg_us, double& bytes)
+    {
+        hipStream_t s = ctx->stream;
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        double total_us = 0.0, total_bytes = 0.0;
+        int count = 0;
+        for (int dir = 0; dir < 2; ++dir) {
+            DevSweep& D = dir == 0 ? fwd : bwd;
+            P* in = dir == 0 ? r.as<P>() : t.as<P>();
+            P* out = dir == 0 ? t.as<P>() : z.as<P>();
+            if (dir == 0) launch_sweep<true>(D, in, out, nullptr);
+            else launch_sweep<false>(D, in, out, nullptr);
+            for (index_t ph = 0; ph < D.nphases; ++ph) {
+                if (in_tail_group(D, ph) || D.phase_big[ph + 1] > D.phase_big[ph] ||
+                    D.phase_slices[ph + 1] == D.phase_slices[ph])
+                    continue;
+                HIP_CHECK(hipEventRecord(e0, s));
+                for (int k = 0; k < reps; ++k) {
+                    if (dir == 0) launch_phase<true>(D, ph, in, out, nullptr);
+                    else launch_phase<false>(D, ph, in, out, nullptr);
+                }
+                HIP_CHECK(hipEventRecord(e1, s));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                total_us += 1000.0 * double(ms) / reps;
+                total_bytes += double(D.phase_entries[ph]) * (sizeof(F) + 4.0) +
+                               double(D.phase_rows[ph]) * (4.0 + 2.0 * W * sizeof(P));
+                ++count;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        avg_us = count ? total_us / count : 0.0;
+        bytes = count ? total_bytes / count : 0.0;
+    }
+
+    void profile(int what, int reps, int nrhs, double& avg_us, double& bytes) override
+    {
+        if (nrhs < 1 || nrhs > kMaxRhs)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_profile: nrhs must be 1.." + std::to_string(kMaxRhs));
+        ensure_workspace(rhs_width(nrhs));
+        hipStream_t s = ctx->stream;
+        // operands of all W right-hand sides (a block pads nrhs up to W)
+        const size_t rows = size_t(n) * size_t(W);
+        const size_t vb = rows * sizeof(T);
+        const size_t pb = rows * sizeof(P);
+        // deterministic non-zero operands
+        std::vector<HT> h(rows);
+        for (size_t i = 0; i < rows; ++i) h[i] = HT(1.0 / (1.0 + double(i % 97)));
+        HIP_CHECK(hipMemcpyAsync(p.p, h.data(), vb, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(r.p, h.data(), vb, hipMemcpyHostToDevice, s));
+        auto run = [&] {
+            switch (what) {
+            case SSA_PROFILE_SPMV: launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_a.as<T>(), nullptr); break;
+            case SSA_PROFILE_FORWARD:
+                launch_sweep<true>(fwd, r.as<P>(), t.as<P>(), nullptr);
+                break;
+            case SSA_PROFILE_BACKWARD:
+                launch_sweep<false>(bwd, t.as<P>(), z.as<P>(), nullptr);
+                break;
+            default: launch_precond(r.as<P>(), z.as<P>(), t.as<P>(), nullptr); break;
+            }
+        };
+        run();   // warm-up
+        if (what == SSA_PROFILE_COLOUR_KERNELS) {
+            profile_colour_kernels(std::max(reps, 1), avg_us, bytes);
+            return;
+        }
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int k = 0; k < reps; ++k) run();
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        avg_us = 1000.0 * double(ms) / std::max(reps, 1);
+        // algorithmic bytes: matrix entries (value + int32 column) once, plus
+        // row pointers, plus every vector operand once
+        const double rowptr = 4.0 * (double(n) + 1.0);
+        const double lbytes = double(nnz_l) * (sizeof(F) + 4.0) + rowptr;
+        const double abytes = double(nnz) * (sizeof(T) + 4.0) + rowptr;
+        switch (what) {
+        case SSA_PROFILE_SPMV: bytes = abytes + 2.0 * double(vb); break;            // read x, write y
+        case SSA_PROFILE_FORWARD: bytes = lbytes + 2.0 * double(pb) + double(n) * sizeof(F); break;
+        case SSA_PROFILE_BACKWARD: bytes = lbytes + 2.0 * double(pb) + double(n) * sizeof(F); break;
+        default: bytes = 2.0 * lbytes + 4.0 * double(pb) + 2.0 * double(n) * sizeof(F); break;
+        }
+    }
+
+    void info(ssa_ic_info& o) const override
+    {
+        o.n = tot;   // batches: all systems' rows (colours: the most of any system)
+        o.nnz = nnz;
+        o.nnz_l = nnz_l;
+        o.num_colors = nc;
+        o.num_blocks = nb;
+        o.rows_per_lane = fwd.BR;
+        o.big_blocks = fwd.phase_big.empty() ? 0 : fwd.phase_big.back();
+        o.max_big_rows = std::max(fwd.max_big_rows, bwd.max_big_rows);
+        o.max_levels = std::max(fwd.max_levels, bwd.max_levels);
+        o.shift_trials = shift_trials;
+        o.actual_shift = actual_shift;
+        o.setup_seconds = setup_seconds;
+        o.device_bytes = A_off.bytes + A_val.bytes + A_col.bytes + perm.bytes + rev.bytes +
+                         invd.bytes + fwd.bytes() + bwd.bytes();
+        o.factor_bytes = fwd.stream_bytes;
+        o.spmv_bytes = spmv_bytes;
+    }
+
+    void one_system(const char* who) const
+    {
+        if (count != 1)
+            throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": not available for a batched handle");
+    }
+    void ordering(int32_t* out) const override
+    {
+        one_system("ssa_ic_ordering");
+        std::copy(h_ordering.begin(), h_ordering.end(), out);
+    }
+    void schedule(int32_t* co, int32_t* bo) const override
+    {
+        one_system("ssa_ic_schedule");
+        if (co) std::copy(h_color_offsets.begin(), h_color_offsets.end(), co);
+        if (bo) std::copy(h_block_offsets.begin(), h_block_offsets.end(), bo);
+    }
+    void factor(int32_t* rp, int32_t* ci, void* vals, void* iv) const override
+    {
+        one_system("ssa_ic_factor");
+        if (rp) std::copy(h_Lrp.begin(), h_Lrp.end(), rp);
+        if (ci) std::copy(h_Lci.begin(), h_Lci.end(), ci);
+        if (vals) std::memcpy(vals, h_Lv.data(), h_Lv.size() * sizeof(HT));
+        if (iv) std::memcpy(iv, h_invd.data(), h_invd.size() * sizeof(HT));
+    }
+};
+
+
+// the factory of one precision pair: T vectors, F factor, HT host scalar
+template <typename T, typename F, typename HT>
+IcBase* make_impl(ssa_context* ctx, int count, const int32_t* n, const int32_t* const* rp,
+                  const int32_t* const* ci, const void* const* values,
+                  const sparsesolv_amd::IcSettings& st)
+{
+    std::vector<const HT*> v(static_cast<size_t>(count));
+    for (int q = 0; q < count; ++q) v[size_t(q)] = static_cast<const HT*>(values[q]);
+    return new IcImpl<T, F>(ctx, count, n, rp, ci, v.data(), st);
+}
+
+}  // namespace ssa_impl

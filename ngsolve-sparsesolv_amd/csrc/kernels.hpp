@@ -24,8 +24,13 @@ constexpr int kWavesPerBlock = kBlock / 64;
 // ---------------------------------------------------------------------------
 // SpMV over SELL-64 (one wavefront per 64-row slice), optional fused dot
 // DOT: 0 none, 1 sum x_i y_i, 2 sum conj(x_i) y_i
+//
+// W right-hand sides (vector r at offset r*sy.vstride): each lane loads a
+// matrix entry once and applies it to all W vectors; the partials of
+// right-hand side r are [r*gridDim.x, (r+1)*gridDim.x).  Right-hand sides
+// that have stopped are still multiplied (their results are not read).
 // ---------------------------------------------------------------------------
-template <typename T, int DOT>
+template <typename T, int DOT, int W>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const index_t* __restrict__ slice_off, const T* __restrict__ val,
     const index_t* __restrict__ col, index_t n, index_t nslices,
@@ -33,40 +38,60 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const SolverState* __restrict__ st, Ctrl check, Systems sy)
 {
     const int sys = sy.spmv_sys ? sy.spmv_sys[blockIdx.x] : 0;
-    const int stopped = st ? st[sys].status : 0;
+    unsigned live = 0;   // bit r: right-hand side r (or the batch system) runs
+#pragma unroll
+    for (int r = 0; r < W; ++r)
+        if (!st || !st[sys + r].status) live |= 1u << r;
     __shared__ T red[kWavesPerBlock];
     const int lane = threadIdx.x & 63;
     const index_t s = index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
     const index_t sc = s < nslices ? s : nslices;
     const index_t o0 = slice_off[sc], o1 = s < nslices ? slice_off[s + 1] : o0;
-    if (stopped) {
+    if (!live) {
         // consuming the offset keeps its load ahead of the status test
         // (offsets are >= 0)
         if (o0 < 0) y[0] = zero<T>();
         return;
     }
-    T d = zero<T>();
+    const size_t vs = size_t(sy.vstride);
+    T d[W];
+#pragma unroll
+    for (int r = 0; r < W; ++r) d[r] = zero<T>();
     if (s < nslices) {
-        T acc = zero<T>();
-#pragma unroll 4
+        T acc[W];
+#pragma unroll
+        for (int r = 0; r < W; ++r) acc[r] = zero<T>();
+        constexpr int kUnroll = W >= 8 ? 1 : W >= 4 ? 2 : 4;
+#pragma unroll kUnroll
         for (index_t k = o0; k < o1; ++k) {
             const size_t at = size_t(k) * 64 + lane;
-            acc = fma_(val[at], x[col[at]], acc);
+            const T v = val[at];
+            const index_t c = col[at];
+#pragma unroll
+            for (int r = 0; r < W; ++r) acc[r] = fma_(v, x[r * vs + c], acc[r]);
         }
         const index_t row = s * 64 + lane;
         if (row < n) {
-            y[row] = acc;
-            if (DOT) d = mul(conj_if(x[row], DOT == 2), acc);
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                y[r * vs + row] = acc[r];
+                if (DOT) d[r] = mul(conj_if(x[r * vs + row], DOT == 2), acc[r]);
+            }
         }
     }
     if (DOT) {
-        T t = block_sum(d, red);
-        if (threadIdx.x == 0) partials[blockIdx.x] = t;
+#pragma unroll
+        for (int r = 0; r < W; ++r) {
+            T t = block_sum(d[r], red);
+            if (threadIdx.x == 0) partials[size_t(r) * gridDim.x + blockIdx.x] = t;
+        }
     }
     if (check.part) {
         __shared__ double redd[kWavesPerBlock];
         const index_t local = index_t(blockIdx.x) - (sy.spmv_g0 ? sy.spmv_g0[sys] : 0);
-        ctrl_check_wg0(ctrl_for(check, sy, sys, local), redd);
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+            if (live & (1u << r)) ctrl_check_wg0(ctrl_for(check, sy, sys + r, local), redd);
     }
 }
 
@@ -85,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 // External part of the oversized blocks of this phase (walk slots t0..t1):
 //   FWD: a = in_i - sum_ext L_ik out_k      BWD: a = -sum_ext U_ik out_k
 // written to sa[t] (and in_i to sb[t] for BWD); the walk kernel finishes them.
+// Right-hand side r uses the slots [r*ws, (r+1)*ws).
 struct WalkExt {
     size_t t0, t1;
     const index_t* row;
@@ -93,21 +119,31 @@ struct WalkExt {
     const void* ext_v;
     void* sa;
     void* sb;
+    size_t ws;
 };
 
-template <typename P, typename F, bool FWD>
+template <typename P, typename F, bool FWD, int W>
 __device__ inline void walk_external(const WalkExt& w, size_t t, const P* __restrict__ in,
-                                     const P* out)
+                                     const P* out, size_t vs)
 {
     if (t >= w.t1) return;
     const index_t row = w.row[t];
     if (row < 0) return;
     const F* ev = static_cast<const F*>(w.ext_v);
-    P a = FWD ? in[row] : zero<P>();
-    for (index_t k = w.ext_rp[t]; k < w.ext_rp[t + 1]; ++k)
-        a = fms_(cvt<P>(ev[k]), out[w.ext_ci[k]], a);
-    static_cast<P*>(w.sa)[t] = a;
-    if (!FWD) static_cast<P*>(w.sb)[t] = in[row];
+    P a[W];
+#pragma unroll
+    for (int r = 0; r < W; ++r) a[r] = FWD ? in[r * vs + row] : zero<P>();
+    for (index_t k = w.ext_rp[t]; k < w.ext_rp[t + 1]; ++k) {
+        const P v = cvt<P>(ev[k]);
+        const index_t c = w.ext_ci[k];
+#pragma unroll
+        for (int r = 0; r < W; ++r) a[r] = fms_(v, out[r * vs + c], a[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < W; ++r) {
+        static_cast<P*>(w.sa)[r * w.ws + t] = a[r];
+        if (!FWD) static_cast<P*>(w.sb)[r * w.ws + t] = in[r * vs + row];
+    }
 }
 
 template <typename P>
@@ -134,9 +170,11 @@ struct SweepArrays {
 // One wavefront, one slice.  The slice's offsets, the lane's row and the
 // first batch of entries are requested before `stopped` is tested, so the
 // chain of dependent memory round trips is offsets -> entries -> gathers.
-template <typename P, typename F, int BR, bool FWD, int CH>
+// W right-hand sides (vector r at offset r*vs): each entry is loaded once
+// and applied to all of them; CH entries per batch.
+template <typename P, typename F, int BR, bool FWD, int CH, int W>
 __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __restrict__ in,
-                                   P* out, int stopped)
+                                   P* out, size_t vs, int stopped)
 {
     const F* __restrict__ val = static_cast<const F*>(A.val);
     const index_t* __restrict__ col = A.col;
@@ -164,25 +202,38 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
     }
     const bool live = row >= 0;
     const index_t rr = live ? row : 0;
-    const P xin = in[rr];
+    // the backward sweep needs in_i only at the end: with several
+    // right-hand sides it is loaded there (registers), with one up front
+    constexpr bool kLateIn = !FWD && W > 1;
+    P xin[W], acc[W];
+#pragma unroll
+    for (int r = 0; r < W; ++r) {
+        xin[r] = kLateIn ? zero<P>() : in[r * vs + rr];
+        acc[r] = FWD ? xin[r] : zero<P>();
+    }
     const P dinv = cvt<P>(static_cast<const F*>(A.invd)[rr]);
-    P acc = FWD ? xin : zero<P>();
     F intv[BR > 1 ? BR - 1 : 1];
 #pragma unroll
     for (int j = 0; j < (BR > 1 ? BR - 1 : 1); ++j) intv[j] = zero<F>();
     for (index_t k0 = o0;;) {
         // gathers from row 0 stand in for non-external entries; their
         // values are discarded
-        P g[CH];
+        P g[CH][W];
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
             if (k0 + j >= o1) c[j] = kPadColumn;
-            g[j] = out[c[j] >= 0 ? c[j] : 0];
+            const index_t cj = c[j] >= 0 ? c[j] : 0;
+#pragma unroll
+            for (int r = 0; r < W; ++r) g[j][r] = out[r * vs + cj];
         }
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
-            const P t = fms_(cvt<P>(v[j]), g[j], acc);
-            acc = select_(c[j] >= 0, t, acc);
+            const P vj = cvt<P>(v[j]);
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                const P t = fms_(vj, g[j][r], acc[r]);
+                acc[r] = select_(c[j] >= 0, t, acc[r]);
+            }
             // -1 - c is an earlier position of this block; padding slots
             // carry kPadColumn and match no position
 #pragma unroll
@@ -197,43 +248,68 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
             v[j] = val[at];
         }
     }
+    if (kLateIn) {
+#pragma unroll
+        for (int r = 0; r < W; ++r) xin[r] = in[r * vs + rr];
+    }
     const int base = lane - pos;
-    P res = zero<P>();
+    P res[W];
+#pragma unroll
+    for (int r = 0; r < W; ++r) res[r] = zero<P>();
 #pragma unroll
     for (int j = 0; j < BR; ++j) {
         if (live && pos == j) {
-            res = FWD ? mul(acc, dinv) : fma_(acc, dinv, xin);
-            out[row] = res;
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                res[r] = FWD ? mul(acc[r], dinv) : fma_(acc[r], dinv, xin[r]);
+                out[r * vs + row] = res[r];
+            }
         }
         if (j < BR - 1) {
             // position j of this lane's block (clamped for idle lanes and
             // short blocks at the end of a slice; those lanes do not use it)
-            const P vj = shfl(res, min(base + j, 63));
-            if (pos > j) acc = fms_(cvt<P>(intv[j]), vj, acc);
+            const int src = min(base + j, 63);
+            const P cj = cvt<P>(intv[j]);
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                const P vj = shfl(res[r], src);
+                if (pos > j) acc[r] = fms_(cj, vj, acc[r]);
+            }
         }
     }
 }
 
 // One colour: slices s0..s1, four per workgroup; extra workgroups compute
-// the external part of the colour's oversized blocks.
-template <typename P, typename F, int BR, bool FWD, int CH>
+// the external part of the colour's oversized blocks.  W right-hand sides
+// per lane; the colour is skipped once all of them have stopped.
+template <int W>
+__device__ inline int all_stopped(const SolverState* __restrict__ st)
+{
+    int stopped = 1;
+#pragma unroll
+    for (int r = 0; r < W; ++r) stopped &= st[r].status != 0;
+    return stopped;
+}
+
+template <typename P, typename F, int BR, bool FWD, int CH, int W>
 __global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, SweepArrays A,
                                                   const P* __restrict__ in, P* out,
-                                                  WalkExt walk,
+                                                  index_t vstride, WalkExt walk,
                                                   const SolverState* __restrict__ st)
 {
+    const size_t vs = size_t(vstride);
     const index_t nreg = (s1 - s0 + kWavesPerBlock - 1) / kWavesPerBlock;
     if (index_t(blockIdx.x) >= nreg) {
         // batches: computed for every system (the results of a stopped
         // system's sweep are never read)
-        if (!A.slice_sys && st->status) return;
-        walk_external<P, F, FWD>(walk, walk.t0 + size_t(blockIdx.x - nreg) * kBlock + threadIdx.x, in, out);
+        if (!A.slice_sys && all_stopped<W>(st)) return;
+        walk_external<P, F, FWD, W>(walk, walk.t0 + size_t(blockIdx.x - nreg) * kBlock + threadIdx.x, in, out, vs);
         return;
     }
     const index_t s = s0 + index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
     if (s >= s1) return;                       // whole wavefront: no shuffle below
-    const int stopped = st[A.slice_sys ? A.slice_sys[s] : 0].status;
-    sweep_slice<P, F, BR, FWD, CH>(s, A, in, out, stopped);
+    const int stopped = all_stopped<W>(st + (A.slice_sys ? A.slice_sys[s] : 0));
+    sweep_slice<P, F, BR, FWD, CH, W>(s, A, in, out, vs, stopped);
 }
 
 // Several consecutive small colours in one workgroup of NT threads: the
@@ -248,17 +324,25 @@ struct TailPhases {
     index_t slices[kTailMaxPhases + 1];
 };
 
+// Several right-hand sides: one workgroup each (blockIdx.x = r, vector r
+// at offset r*vs); the workgroups run side by side, so the chain of
+// colours costs what it costs for one right-hand side.
 template <typename P, typename F, int BR, bool FWD, int CH>
 __global__ __launch_bounds__((tail_threads<P, BR>())) void k_sweep_tail(TailPhases T, SweepArrays A,
                                                                   const P* __restrict__ in, P* out,
+                                                                  index_t vstride,
                                                                   const SolverState* __restrict__ st)
 {
+    const int r = int(blockIdx.x);
+    in += size_t(r) * size_t(vstride);
+    out += size_t(r) * size_t(vstride);
+    st += r;
     if (!A.slice_sys && st->status) return;
     const index_t w = index_t(threadIdx.x >> 6);
     constexpr index_t kWaves = tail_threads<P, BR>() / 64;
     for (int ph = 0; ph < T.count; ++ph) {
         for (index_t s = T.slices[ph] + w; s < T.slices[ph + 1]; s += kWaves)
-            sweep_slice<P, F, BR, FWD, CH>(s, A, in, out, A.slice_sys ? st[A.slice_sys[s]].status : 0);
+            sweep_slice<P, F, BR, FWD, CH, 1>(s, A, in, out, 0, A.slice_sys ? st[A.slice_sys[s]].status : 0);
         __syncthreads();
     }
 }
@@ -275,9 +359,12 @@ __global__ __launch_bounds__(1024) void k_sweep_big(
     const index_t* __restrict__ big_lvl, const index_t* __restrict__ big_rows,
     const index_t* __restrict__ big_rp, const index_t* __restrict__ big_ci,
     const F* __restrict__ big_v, const F* __restrict__ invd, const P* __restrict__ in,
-    P* out, int use_lds, const SolverState* __restrict__ st)
+    P* out, int use_lds, index_t vstride, const SolverState* __restrict__ st)
 {
-    if (st && st->status) return;
+    // right-hand side blockIdx.y
+    in += size_t(blockIdx.y) * size_t(vstride);
+    out += size_t(blockIdx.y) * size_t(vstride);
+    if (st && st[blockIdx.y].status) return;
     extern __shared__ unsigned char lds_raw[];
     P* lds = reinterpret_cast<P*>(lds_raw);
     const index_t bb = b0 + index_t(blockIdx.x);
@@ -313,7 +400,9 @@ __global__ __launch_bounds__(1024) void k_sweep_big(
 // read the values of earlier steps from LDS (indexed by level position).
 // The operands of step s+D are loaded while step s computes (D register
 // sets, rotated by the unrolled j loop), so the chain of steps pays LDS
-// latency, not memory latency.
+// latency, not memory latency.  Several right-hand sides: one wavefront
+// per (block, right-hand side), blockIdx.y = r; walk slots of r start at
+// r*ws, its vector at r*vs.
 // ---------------------------------------------------------------------------
 template <typename P, typename F, int KMAX>
 struct WalkBuf {
@@ -329,11 +418,15 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
     index_t b0, const index_t* __restrict__ step_off, const index_t* __restrict__ wrow,
     const index_t* __restrict__ wpos, const F* __restrict__ winvd, const F* __restrict__ wv,
     const index_t* __restrict__ wix, index_t zero_pos, P* out, const P* __restrict__ sa,
-    const P* __restrict__ sb, const SolverState* __restrict__ st,
+    const P* __restrict__ sb, index_t vstride, size_t ws, const SolverState* __restrict__ st,
     const int* __restrict__ walk_sys)
 {
     const index_t bb = b0 + index_t(blockIdx.x);
-    if (st && st[walk_sys ? walk_sys[bb] : 0].status) return;
+    const int r = int(blockIdx.y);
+    out += size_t(r) * size_t(vstride);
+    sa += size_t(r) * ws;
+    sb += size_t(r) * ws;
+    if (st && st[(walk_sys ? walk_sys[bb] : 0) + r].status) return;
     extern __shared__ unsigned char lds_raw[];
     P* lds = reinterpret_cast<P*>(lds_raw);   // [0, zero_pos): values; zero_pos: 0; +1: idle lanes
     const index_t s0 = step_off[bb], s1 = step_off[bb + 1];   // s1 - s0 is a multiple of D
@@ -399,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const TA* __restrict_
     for (index_t i = sp.local * kBlock + threadIdx.x; i < sp.n; i += sp.G * kBlock)
         d = fma_(conj_if(cvt<T>(a[sp.row0 + i]), conj != 0), cvt<T>(b[sp.row0 + i]), d);
     T t = block_sum(d, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    if (threadIdx.x == 0) partials[sp.slot] = t;
     if (check.part) {
         __shared__ double redd[kWavesPerBlock];
         ctrl_check_wg0(ctrl_for(check, sy, sp.sys, sp.local), redd);
@@ -421,10 +514,10 @@ __global__ __launch_bounds__(kBlock) void k_norms(index_t n, const T* __restrict
         if (b) sb += norm2(b[sp.row0 + i]);
     }
     double t = block_sum(sa, red);
-    if (threadIdx.x == 0) pa[blockIdx.x] = t;
+    if (threadIdx.x == 0) pa[sp.slot] = t;
     if (b) {
         t = block_sum(sb, red);
-        if (threadIdx.x == 0) pb[blockIdx.x] = t;
+        if (threadIdx.x == 0) pb[sp.slot] = t;
     }
 }
 
@@ -438,11 +531,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(index_t n, const T* __restr
         r[i] = sub(b[i], Ap[i]);
 }
 
-// dst[i] = src[map[i]]
+// dst[i] = src[map[i]]; vector blockIdx.y of several at the given strides
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gather(index_t n, const index_t* __restrict__ map,
-                                                  const T* __restrict__ src, T* __restrict__ dst)
+                                                  const T* __restrict__ src, index_t src_stride,
+                                                  T* __restrict__ dst, index_t dst_stride)
 {
+    src += size_t(blockIdx.y) * size_t(src_stride);
+    dst += size_t(blockIdx.y) * size_t(dst_stride);
     for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += index_t(gridDim.x) * kBlock) {
         const index_t m = map[i];
@@ -488,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__
         s += norm2(ri);
     }
     double t = block_sum(s, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    if (threadIdx.x == 0) partials[sp.slot] = t;
 }
 
 // best_x <- x when the last convergence check improved the best residual
@@ -552,7 +648,7 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
         s += norm2(ri);
     }
     double t = block_sum(s, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    if (threadIdx.x == 0) partials[sp.slot] = t;
 }
 
 // COCR: p = rt + beta p; q = t + beta q (and the best-x copy); q also in

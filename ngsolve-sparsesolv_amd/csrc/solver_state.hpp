@@ -29,6 +29,7 @@ struct SolverState {
     double normalizer;
     double best_residual;
     double final_norm;
+    double tol;            // relative tolerance of this pass
     int iter;              // reference loop index of the current iteration
     int status;
     int converged;
@@ -36,7 +37,16 @@ struct SolverState {
     int improved;          // last check improved best_residual
     int bad_count;
     int hist_len;
-    int pad_;
+    int maxit;             // iteration budget of this pass
+};
+
+// Per-system settings of one pass (set by the host before the pass): the
+// relative tolerance, the iteration budget, and whether the system runs at
+// all (an inactive system starts stopped and counts no iteration).
+struct PassParams {
+    double tolerance;
+    int max_iterations;
+    int active;
 };
 
 struct SolveParams {

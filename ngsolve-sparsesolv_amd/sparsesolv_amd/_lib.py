@@ -37,6 +37,7 @@ EXPORTED = (
     "ssa_ic_create", "ssa_ic_destroy", "ssa_ic_info_get", "ssa_ic_ordering",
     "ssa_ic_schedule", "ssa_ic_factor", "ssa_ic_apply", "ssa_spmv", "ssa_solve",
     "ssa_ic_profile", "ssa_batch_create", "ssa_batch_solve", "ssa_ic_systems",
+    "ssa_solve_multi", "ssa_ic_apply_multi", "ssa_spmv_multi", "ssa_ic_profile_multi",
 )
 
 PROFILE_SPMV = 0
@@ -127,6 +128,12 @@ def load() -> C.CDLL:
         L.ssa_batch_solve.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.POINTER(Config),
                                       C.POINTER(Result)]
         L.ssa_ic_systems.argtypes = [vp, C.POINTER(C.c_int)]
+        L.ssa_solve_multi.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.c_int, C.POINTER(Config),
+                                      C.POINTER(Result)]
+        L.ssa_ic_apply_multi.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+        L.ssa_spmv_multi.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+        L.ssa_ic_profile_multi.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double)]
         for name in EXPORTED:
             getattr(L, name).restype = C.c_char_p if name == "ssa_last_error" else (
                 None if name == "ssa_config_default" else C.c_int)
@@ -288,6 +295,46 @@ class IcHandle:
         check(self.lib.ssa_solve(self.h, method, C.c_void_p(b_ptr), C.c_void_p(x_ptr), 1,
                                  C.byref(cfg), C.byref(res), hp))
         return res
+
+    # ---- several right-hand sides (rows of a (nrhs, n) array) -------------
+    def _block(self, X):
+        X = np.ascontiguousarray(X, dtype=self.dtype)
+        if X.ndim != 2 or X.shape[1] != self.n or X.shape[0] < 1:
+            raise ValueError(f"(nrhs, {self.n}) array expected, got {X.shape}")
+        return X
+
+    def solve_multi(self, method: int, B, X0, cfg: Config):
+        """ssa_solve_multi: X (nrhs, n) and one Result per right-hand side."""
+        B = self._block(B)
+        X = np.array(self._block(np.zeros_like(B) if X0 is None else X0), dtype=self.dtype)
+        res = (Result * B.shape[0])()
+        check(self.lib.ssa_solve_multi(self.h, method, B.shape[0], B.ctypes.data, X.ctypes.data, 0,
+                                       C.byref(cfg), res))
+        return X, list(res)
+
+    def solve_multi_device(self, method: int, nrhs: int, b_ptr: int, x_ptr: int, cfg: Config):
+        res = (Result * nrhs)()
+        check(self.lib.ssa_solve_multi(self.h, method, nrhs, C.c_void_p(b_ptr), C.c_void_p(x_ptr), 1,
+                                       C.byref(cfg), res))
+        return list(res)
+
+    def apply_multi(self, X) -> np.ndarray:
+        X = self._block(X)
+        Y = np.empty_like(X)
+        check(self.lib.ssa_ic_apply_multi(self.h, X.shape[0], X.ctypes.data, Y.ctypes.data, 0))
+        return Y
+
+    def spmv_multi(self, X) -> np.ndarray:
+        X = self._block(X)
+        Y = np.empty_like(X)
+        check(self.lib.ssa_spmv_multi(self.h, X.shape[0], X.ctypes.data, Y.ctypes.data, 0))
+        return Y
+
+    def profile_multi(self, what: int, nrhs: int, reps: int = 20):
+        """profile() for a block of nrhs right-hand sides."""
+        us, by = C.c_double(), C.c_double()
+        check(self.lib.ssa_ic_profile_multi(self.h, what, nrhs, reps, C.byref(us), C.byref(by)))
+        return us.value, by.value
 
 
 class BatchHandle(IcHandle):

@@ -317,6 +317,17 @@ def eddy(m: int, iron_frac: float = 0.2, nu_iron: float = 1e-3, eps: float = 1e-
     return System(CSR.from_scipy(A), b, f"eddy{m}", {"m": m})
 
 
+def complex_gaussian_block(n: int, count: int, seed: int = 2) -> np.ndarray:
+    """`count` complex Gaussian vectors of length n, (count, n): vector k is
+    outputs 2nk .. 2n(k+1)-1 of the Gaussian stream of `seed`, paired as
+    (re, im); vector 0 is eddy()'s right-hand side for the same seed."""
+    out = np.empty((count, n), dtype=np.complex128)
+    for k in range(count):
+        z = gaussian(seed, 2 * n, offset=2 * n * k)
+        out[k] = z[0::2] + 1j * z[1::2]
+    return out
+
+
 def poisson(m: int, rhs_seed: int = 0) -> System:
     """7-point Laplacian with a uniform[-1,1] RHS."""
     A = laplace7(m)

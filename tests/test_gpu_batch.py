@@ -113,9 +113,18 @@ def test_batch_handle_rejects_single_system_calls():
         hb.solve(_lib.METHOD_ICCG, b, None, cfg)
     with pytest.raises(_lib.SsaError):
         hb.ordering()
-    with pytest.raises(_lib.SsaError):
-        _lib.BatchHandle([A for A, _ in systems],
-                         _lib.default_config(use_abmc=1, precision=_lib.PRECISION_MIXED))
+
+
+def test_batch_mixed_precision_matches_single_solves():
+    # per-system iterative refinement: each system restarts, stops improving
+    # and converges on its own, as in its own mixed-precision solve
+    systems = real_batch()[:4]
+    cfg = _lib.default_config(use_abmc=1, precision=_lib.PRECISION_MIXED, max_iterations=1500)
+    check_against_singles(systems, cfg, _lib.METHOD_ICCG)
+    cplx = [(P.eddy(m).A, P.eddy(m).b) for m in (5, 6)]
+    check_against_singles(cplx, _lib.default_config(use_abmc=1, precision=_lib.PRECISION_MIXED,
+                                                    abmc_num_colors=8, max_iterations=1500),
+                          _lib.METHOD_ICCOCR)
 
 
 def test_autotune_on_gpu():
