@@ -144,11 +144,6 @@ struct Ctrl {
 // its workgroups are [vec_g0[s], vec_g0[s+1]) and cover its rows exactly
 // as a one-system launch of that many workgroups would (same partials,
 // same bits); on the SpMV grid its workgroups are [spmv_g0[s], ...).
-//
-// Several right-hand sides of one system: right-hand side r owns the rows
-// [r*vstride, (r+1)*vstride) of every vector, state r, history r and the
-// partials [r*np, (r+1)*np) of a producer with np workgroups per
-// right-hand side.  The vector kernels run a (G, W) grid, blockIdx.y = r.
 struct Systems {
     const int* vec_sys;
     const int* vec_g0;
@@ -157,7 +152,6 @@ struct Systems {
     const int* row0;
     const int* nrows;
     int hist_stride;
-    index_t vstride;
 };
 
 // The rows a vector-grid workgroup owns: i = local*kBlock + lane, stepping
@@ -170,19 +164,15 @@ struct Span {
 
 __device__ inline Span vec_span(const Systems& y, index_t n)
 {
-    if (!y.vec_sys) {
-        const int r = int(blockIdx.y);
-        return Span{index_t(r) * y.vstride, n, index_t(blockIdx.x), index_t(gridDim.x), r,
-                    r * int(gridDim.x) + int(blockIdx.x)};
-    }
+    if (!y.vec_sys) return Span{0, n, index_t(blockIdx.x), index_t(gridDim.x), 0, int(blockIdx.x)};
     const int s = y.vec_sys[blockIdx.x];
     const int g0 = y.vec_g0[s];
     return Span{y.row0[s], y.nrows[s], index_t(blockIdx.x) - g0, index_t(y.vec_g0[s + 1] - g0), s,
                 int(blockIdx.x)};
 }
 
-// the control step as seen from a workgroup of state `sys` (batch system
-// or right-hand side) whose index among that state's workgroups is `local`
+// the control step as seen from a workgroup of state `sys` whose index
+// among that state's workgroups is `local`
 __device__ inline Ctrl ctrl_for(Ctrl c, const Systems& y, int sys, index_t local)
 {
     c.first = local == 0;

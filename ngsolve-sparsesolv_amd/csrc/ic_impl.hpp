@@ -25,10 +25,12 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
 #include "host_plan.hpp"
+#include "block_kernels.hpp"
 #include "kernels.hpp"
 #include "ic_handle.hpp"
 #include "sparsesolv_amd.h"
@@ -320,13 +322,12 @@ struct IcImpl final : IcBase {
     ssa::Systems systems() const
     {
         if (count == 1)
-            return ssa::Systems{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, hist_stride, n};
+            return ssa::Systems{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, hist_stride};
         return ssa::Systems{d_vec_sys.as<int>(), d_vec_g0.as<int>(), d_spmv_sys.as<int>(),
-                            d_spmv_g0.as<int>(), d_row0.as<int>(), d_nrows.as<int>(), hist_stride, 0};
+                            d_spmv_g0.as<int>(), d_row0.as<int>(), d_nrows.as<int>(), hist_stride};
     }
     // vector-grid workgroups per right-hand side, and the launch grid
     int vgrid() const { return count == 1 ? vec_grid(n) : vec_total; }
-    dim3 vgrid2() const { return dim3(unsigned(vgrid()), unsigned(W)); }
     // solver states: one per system of a batch, one per right-hand side
     int nstates() const { return count > 1 ? count : W; }
     int system_count() const override { return count; }
@@ -428,8 +429,9 @@ struct IcImpl final : IcBase {
 
     // a colour small enough to share a single-workgroup kernel with its
     // neighbours: no oversized block, at most two passes of the workgroup
-    static int tail_threads(int BR)
+    int tail_threads(int BR) const
     {
+        if (W > 1) return ssa::tail_threads<P, 1, 4>();
         return BR >= 16 ? ssa::tail_threads<P, 16>() : ssa::tail_threads<P, 1>();
     }
 
@@ -487,10 +489,7 @@ struct IcImpl final : IcBase {
         tp.count = int(p1 - p0);
         for (index_t q = p0; q <= p1; ++q) tp.slices[q - p0] = D.phase_slices[q];
         const ssa::SweepArrays A = sweep_arrays(D);
-        hipStream_t s = ctx->stream;
-        // one workgroup per right-hand side
-#define SSA_TAIL(BRV)                                                                           \
-    ssa::k_sweep_tail<P, F, BRV, FWD, 8><<<W, ssa::tail_threads<P, BRV>(), 0, s>>>(tp, A, in, out, n, st)
+#define SSA_TAIL(BRV) tail_br<FWD, BRV>(tp, A, in, out, st)
         switch (D.BR == 0 ? 1 : D.BR) {
         case 1: SSA_TAIL(1); break;
         case 2: SSA_TAIL(2); break;
@@ -503,6 +502,29 @@ struct IcImpl final : IcBase {
         HIP_CHECK(hipGetLastError());
     }
 
+    // a run of small colours in one workgroup, WV right-hand sides per lane
+    template <bool FWD, int BRV, int WV>
+    void tail_w(const ssa::TailPhases& tp, const ssa::SweepArrays& A, const P* in, P* out,
+                const SolverState* st)
+    {
+        constexpr int CH = WV == 1 ? 8 : sweep_chunk<P, WV>();
+        ssa::k_sweep_tail<P, F, BRV, FWD, CH, WV><<<1, ssa::tail_threads<P, BRV, WV>(), 0, ctx->stream>>>(
+            tp, A, in, out, st);
+    }
+
+    template <bool FWD, int BRV>
+    void tail_br(const ssa::TailPhases& tp, const ssa::SweepArrays& A, const P* in, P* out,
+                 const SolverState* st)
+    {
+        switch (W) {
+        case 1: tail_w<FWD, BRV, 1>(tp, A, in, out, st); break;
+        case 4: tail_w<FWD, BRV, 4>(tp, A, in, out, st); break;
+        case 8: tail_w<FWD, BRV, 8>(tp, A, in, out, st); break;
+        case 16: tail_w<FWD, BRV, 16>(tp, A, in, out, st); break;
+        default: throw SsaError(SSA_ERR_ARGUMENT, "unsupported block width");
+        }
+    }
+
     // one colour's regular slices (and the external part of its oversized
     // blocks) with WV right-hand sides per lane
     template <bool FWD, int BRV, int WV>
@@ -510,7 +532,7 @@ struct IcImpl final : IcBase {
                  const ssa::WalkExt& we, const SolverState* st)
     {
         ssa::k_sweep<P, F, BRV, FWD, sweep_chunk<P, WV>(), WV><<<g, ssa::kBlock, 0, ctx->stream>>>(
-            s0, s1, A, in, out, n, we, st);
+            s0, s1, A, in, out, we, st);
     }
 
     template <bool FWD, int BRV>
@@ -569,7 +591,7 @@ struct IcImpl final : IcBase {
     ssa::k_sweep_walk<P, F, FWD, KM, DEPTH><<<dim3(unsigned(b1 - b0), unsigned(W)), 64, lds, s>>>( \
         b0, D.w_step_off.as<index_t>(), D.w_row.as<index_t>(), D.w_pos.as<index_t>(),         \
         D.w_invd.as<F>(), D.w_v.as<F>(), D.w_ix.as<index_t>(), D.max_big_rows, out,           \
-        D.w_sa.as<P>(), D.w_sb.as<P>(), n, D.walk_slots, st,                                  \
+        D.w_sa.as<P>(), D.w_sb.as<P>(), W, D.walk_slots, st,                                  \
         D.batched ? D.walk_sys.as<int>() : nullptr)
             switch (D.walk_kmax) {
             case 2: SSA_WALK(2, 8); break;
@@ -584,7 +606,7 @@ struct IcImpl final : IcBase {
             ssa::k_sweep_big<P, F, FWD><<<dim3(unsigned(b1 - b0), unsigned(W)), 1024, use_lds ? lds : 0, s>>>(
                 b0, D.big_pos.as<index_t>(), D.big_lvl_off.as<index_t>(), D.big_lvl.as<index_t>(),
                 D.big_rows.as<index_t>(), D.big_rp.as<index_t>(), D.big_ci.as<index_t>(),
-                D.big_v.as<F>(), invd.as<F>(), in, out, use_lds, n, st);
+                D.big_v.as<F>(), invd.as<F>(), in, out, use_lds, W, st);
             HIP_CHECK(hipGetLastError());
         }
     }
@@ -620,13 +642,60 @@ struct IcImpl final : IcBase {
         launch_sweep<false>(bwd, tbuf, out, st);
     }
 
-    // dst[i] = src[map[i]] for i < len (map -1: zero), for the first m
-    // right-hand sides (vector r of src at r*sstride, of dst at r*dstride)
-    void launch_gather(const index_t* map, const T* src, index_t sstride, T* dst, index_t dstride,
-                       index_t len, int m = 1)
+    // ---- vector plumbing: one vector (W == 1, batches included) or a block
+    // of W right-hand sides in interleaved rows -------------------------------
+    template <typename Fn>
+    void with_width(Fn&& fn)
     {
-        ssa::k_gather<T><<<dim3(unsigned(vec_grid(len)), unsigned(m)), ssa::kBlock, 0, ctx->stream>>>(
-            len, map, src, sstride, dst, dstride);
+        switch (W) {
+        case 4: fn(std::integral_constant<int, 4>{}); break;
+        case 8: fn(std::integral_constant<int, 8>{}); break;
+        case 16: fn(std::integral_constant<int, 16>{}); break;
+        default: throw SsaError(SSA_ERR_ARGUMENT, "unsupported block width");
+        }
+    }
+    // element-wise kernels over count elements: up to 8192 workgroups
+    static int flat_grid(size_t count)
+    {
+        return int(std::max<size_t>(1, std::min<size_t>((count + ssa::kBlock - 1) / ssa::kBlock, 8192)));
+    }
+    size_t elements() const { return size_t(n) * size_t(W); }
+
+    // dst[i] = src[map[i]] for i < len (map -1: zero)
+    void launch_gather(const index_t* map, const T* src, T* dst, index_t len)
+    {
+        ssa::k_gather<T><<<vec_grid(len), ssa::kBlock, 0, ctx->stream>>>(len, map, src, 0, dst, 0);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // the caller's m vectors (tot values each, back to back) into device
+    // order; block padding columns become zero
+    void gather_in(const T* src, T* dst, int m)
+    {
+        if (W == 1) {
+            launch_gather(rev.as<index_t>(), src, dst, n);
+            return;
+        }
+        with_width([&](auto wc) {
+            constexpr int WV = decltype(wc)::value;
+            ssa::k_gather_blk<T, WV><<<flat_grid(elements()), ssa::kBlock, 0, ctx->stream>>>(
+                n, rev.as<index_t>(), src, size_t(tot), m, dst);
+        });
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // device order out to the caller's m vectors
+    void scatter_out(const T* src, T* dst, int m)
+    {
+        if (W == 1) {
+            launch_gather(perm.as<index_t>(), src, dst, tot);
+            return;
+        }
+        with_width([&](auto wc) {
+            constexpr int WV = decltype(wc)::value;
+            ssa::k_scatter_blk<T, WV><<<flat_grid(size_t(tot) * size_t(m)), ssa::kBlock, 0, ctx->stream>>>(
+                tot, perm.as<index_t>(), src, m, dst);
+        });
         HIP_CHECK(hipGetLastError());
     }
 
@@ -638,29 +707,66 @@ struct IcImpl final : IcBase {
     // the P copy a mixed-precision kernel also writes (none in FP64)
     P* low_copy() { return kMixed ? lo_a.as<P>() : nullptr; }
 
+    // dst = src converted (states that have stopped are left alone)
     template <typename S, typename D>
     void convert(const S* src, D* dst, const SolverState* st)
     {
-        ssa::k_copy<S, D><<<vgrid2(), ssa::kBlock, 0, ctx->stream>>>(n, src, dst, st, systems());
+        if (W == 1) {
+            ssa::k_copy<S, D><<<vgrid(), ssa::kBlock, 0, ctx->stream>>>(n, src, dst, st, systems());
+        } else {
+            with_width([&](auto wc) {
+                constexpr int WV = decltype(wc)::value;
+                ssa::k_copy_blk<S, D, WV><<<flat_grid(elements()), ssa::kBlock, 0, ctx->stream>>>(
+                    elements(), src, dst, st);
+            });
+        }
         HIP_CHECK(hipGetLastError());
     }
 
-    // zero the vectors of right-hand sides m..W-1 (block padding)
-    void zero_tail(DevBuf& b, int m)
+    // partials of conj?(a) . b per state (no fused control step)
+    template <typename TA, typename TB>
+    void launch_dot(const TA* a, const TB* b, int conj, T* part)
     {
-        if (m < W)
-            HIP_CHECK(hipMemsetAsync(b.as<T>() + size_t(m) * size_t(n), 0,
-                                     size_t(W - m) * size_t(n) * sizeof(T), ctx->stream));
+        if (W == 1) {
+            ssa::k_dot<T, TA, TB><<<vgrid(), ssa::kBlock, 0, ctx->stream>>>(n, a, b, conj, part, nullptr,
+                                                                          ssa::Ctrl{}, systems());
+        } else {
+            with_width([&](auto wc) {
+                constexpr int WV = decltype(wc)::value;
+                ssa::k_dot_blk<T, TA, TB, WV><<<vgrid(), ssa::kBlock, 0, ctx->stream>>>(n, a, b, conj, part);
+            });
+        }
+        HIP_CHECK(hipGetLastError());
     }
 
-    // device rows [first, first + count) of state q's vector
+    // partials of |a|^2 (and |b|^2) per state
+    void launch_norms(const T* a, const T* b, double* pa, double* pb)
+    {
+        if (W == 1) {
+            ssa::k_norms<T><<<vgrid(), ssa::kBlock, 0, ctx->stream>>>(n, a, b, pa, pb, systems());
+        } else {
+            with_width([&](auto wc) {
+                constexpr int WV = decltype(wc)::value;
+                ssa::k_norms_blk<T, WV><<<vgrid(), ssa::kBlock, 0, ctx->stream>>>(n, a, b, pa, pb);
+            });
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // device rows [first, first + count) of a batch system's vector
     std::pair<size_t, size_t> rows_of(int q) const
     {
         if (count > 1) return {size_t(h_row0[q]), size_t(h_nrows[q])};
-        return {size_t(q) * size_t(n), size_t(n)};
+        return {0, size_t(n)};
     }
+    // state q's part of src into dst (a batch system's rows, a block's column)
     void copy_rows(DevBuf& dst, const DevBuf& src, int q)
     {
+        if (W > 1) {
+            ssa::k_copy_column<T><<<vec_grid(n), ssa::kBlock, 0, ctx->stream>>>(n, W, q, src.as<T>(), dst.as<T>());
+            HIP_CHECK(hipGetLastError());
+            return;
+        }
         const auto rr = rows_of(q);
         HIP_CHECK(hipMemcpyAsync(dst.as<T>() + rr.first, src.as<T>() + rr.first, rr.second * sizeof(T),
                                  hipMemcpyDeviceToDevice, ctx->stream));
@@ -705,8 +811,7 @@ struct IcImpl final : IcBase {
             HIP_CHECK(hipMemcpyAsync(tmp.p, xin, vb, hipMemcpyHostToDevice, s));
             src = tmp.as<T>();
         }
-        launch_gather(rev.as<index_t>(), src, tot, r.as<T>(), n, n, m);
-        zero_tail(r, m);
+        gather_in(src, r.as<T>(), m);
         if (precond) {
             if (kMixed) convert(r.as<T>(), pre_in(false), nullptr);
             launch_precond(pre_in(false), pre_out(false), pre_tmp(), nullptr);
@@ -715,7 +820,7 @@ struct IcImpl final : IcBase {
             launch_spmv(r.as<T>(), z.as<T>(), 0, nullptr, nullptr);
         }
         T* dst = on_device ? yout : tmp2.as<T>();
-        launch_gather(perm.as<index_t>(), z.as<T>(), n, dst, tot, tot, m);
+        scatter_out(z.as<T>(), dst, m);
         if (!on_device) HIP_CHECK(hipMemcpyAsync(yout, tmp2.p, vb, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
@@ -723,17 +828,19 @@ struct IcImpl final : IcBase {
     // one Krylov iteration (kernels only; no host synchronisation)
     void enqueue_iteration(int method, const SolveParams& prm)
     {
+        if (W > 1) {
+            enqueue_block_iteration(method, prm);
+            return;
+        }
         hipStream_t s = ctx->stream;
         SolverState* st = state.as<SolverState>();
         const int G = vgrid();
-        const dim3 G2 = vgrid2();
         const ssa::Systems y = systems();
         const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
                          method == SSA_METHOD_NGS_ICCOCR;
         double* H = hist.as<double>();
         // control steps fused into the kernels that consume their result;
-        // `spmv`: the partials come from the SpMV grid; np counts the
-        // producer's workgroups per right-hand side
+        // `spmv`: the partials come from the SpMV grid
         const int np_spmv = std::max(1, spmv_grid());
         auto ctl = [&](const void* part, int np, bool spmv) {
             return ssa::Ctrl{st, H, part, np, spmv ? 1 : 0, prm, 0, 0};
@@ -746,25 +853,74 @@ struct IcImpl final : IcBase {
             // kernel (after the preconditioner, which cannot change x or r),
             // beta and the best-x copy in the p update
             launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
-            ssa::k_cg_update<T, P><<<G2, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv, true), low_copy(), y);
+            ssa::k_cg_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv, true), low_copy(), y);
             if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
-            else ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st, y);
-            ssa::k_dot<T, T, P><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G, false), y);
-            ssa::k_xpby<T, P><<<G2, ssa::kBlock, 0, s>>>(n, pre_out(c), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G, false), y);
+            else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st, y);
+            ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G, false), y);
+            ssa::k_xpby<T, P><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G, false), y);
         } else {
             // COCRSolver::do_iterate body (cocr_solver.hpp:96-162); z holds rt,
             // Ap holds q.  alpha in the update kernel, the check in the
             // t = A rt kernel, beta and the best-x copy in the p, q update
             if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
-            else ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st, y);
+            else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st, y);
             if (kMixed)   // (q, qt) with the narrow operand second
-                ssa::k_dot<T, T, P><<<G2, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
+                ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
             else
-                ssa::k_dot<T, P, T><<<G2, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
-            ssa::k_cocr_update<T, P><<<G2, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
+                ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
+            ssa::k_cocr_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
             launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G, false));
-            ssa::k_cocr_pq<T, P><<<G2, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
+            ssa::k_cocr_pq<T, P><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
         }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // The same iteration for a block of W right-hand sides: the same kernels'
+    // arithmetic per right-hand side, with each control step in its own
+    // W-workgroup kernel between producer and consumer (same state updates in
+    // the same order: alpha, check, beta).
+    void enqueue_block_iteration(int method, const SolveParams& prm)
+    {
+        hipStream_t s = ctx->stream;
+        SolverState* st = state.as<SolverState>();
+        const int G = vgrid();
+        const bool pre = method == SSA_METHOD_ICCG || method == SSA_METHOD_ICCOCR ||
+                         method == SSA_METHOD_NGS_ICCOCR;
+        double* H = hist.as<double>();
+        const int hs = hist_stride;
+        const int np_spmv = std::max(1, spmv_grid());
+        T* bx = prm.save_best ? best.as<T>() : nullptr;
+        const bool c = prm.cocr != 0;
+        const size_t ne = elements();
+        const int FG = flat_grid(ne);
+        constexpr int CT = ssa::kCtrlThreads;
+        with_width([&](auto wc) {
+            constexpr int WV = decltype(wc)::value;
+            if (!c) {
+                launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
+                ssa::k_ctrl_alpha_blk<T><<<WV, CT, 0, s>>>(part_a.as<T>(), np_spmv, st, prm, H, hs);
+                ssa::k_cg_update_blk<T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, low_copy());
+                ssa::k_ctrl_check_blk<<<WV, CT, 0, s>>>(part_b.as<double>(), G, st, prm, H, hs);
+                if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
+                else ssa::k_copy_blk<T, T, WV><<<FG, ssa::kBlock, 0, s>>>(ne, r.as<T>(), z.as<T>(), st);
+                ssa::k_dot_blk<T, T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>());
+                ssa::k_ctrl_beta_blk<T><<<WV, CT, 0, s>>>(part_c.as<T>(), G, st, prm);
+                ssa::k_xpby_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, pre_out(c), p.as<T>(), x.as<T>(), bx, st);
+            } else {
+                if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
+                else ssa::k_copy_blk<T, T, WV><<<FG, ssa::kBlock, 0, s>>>(ne, Ap.as<T>(), qt.as<T>(), st);
+                if (kMixed)   // (q, qt) with the narrow operand second
+                    ssa::k_dot_blk<T, T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>());
+                else
+                    ssa::k_dot_blk<T, P, T, WV><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>());
+                ssa::k_ctrl_alpha_blk<T><<<WV, CT, 0, s>>>(part_a.as<T>(), G, st, prm, H, hs);
+                ssa::k_cocr_update_blk<T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st);
+                ssa::k_ctrl_check_blk<<<WV, CT, 0, s>>>(part_b.as<double>(), G, st, prm, H, hs);
+                launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st);
+                ssa::k_ctrl_beta_blk<T><<<WV, CT, 0, s>>>(part_c.as<T>(), np_spmv, st, prm);
+                ssa::k_cocr_pq_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, low_copy());
+            }
+        });
         HIP_CHECK(hipGetLastError());
     }
 
@@ -773,8 +929,7 @@ struct IcImpl final : IcBase {
     std::vector<double> state_norms(const T* v)
     {
         hipStream_t s = ctx->stream;
-        ssa::k_norms<T><<<vgrid2(), ssa::kBlock, 0, s>>>(n, v, nullptr, part_d.as<double>(), nullptr, systems());
-        HIP_CHECK(hipGetLastError());
+        launch_norms(v, nullptr, part_d.as<double>(), nullptr);
         const int G = vgrid();
         std::vector<double> h(size_t(G) * size_t(W), 0.0);
         HIP_CHECK(hipMemcpyAsync(h.data(), part_d.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
@@ -793,7 +948,7 @@ struct IcImpl final : IcBase {
     // ||b - A x|| of every state in FP64 (uses Ap and r)
     std::vector<double> true_residuals()
     {
-        const index_t rows = n * index_t(W);
+        const index_t rows = index_t(elements());
         launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
         ssa::k_residual<T><<<vec_grid(rows), ssa::kBlock, 0, ctx->stream>>>(rows, bvec.as<T>(), Ap.as<T>(), r.as<T>());
         HIP_CHECK(hipGetLastError());
@@ -806,7 +961,6 @@ struct IcImpl final : IcBase {
     {
         hipStream_t s = ctx->stream;
         const int G = vgrid();
-        const dim3 G2 = vgrid2();
         const int S = nstates();
         const ssa::Systems y = systems();
         SolverState* st = state.as<SolverState>();
@@ -819,10 +973,10 @@ struct IcImpl final : IcBase {
 
         // prepare_iteration: r = b - A x, norms, normaliser (one control
         // workgroup per state)
-        const index_t rows = n * index_t(W);
+        const index_t rows = index_t(elements());
         launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
         ssa::k_residual<T><<<vec_grid(rows), ssa::kBlock, 0, s>>>(rows, bvec.as<T>(), Ap.as<T>(), r.as<T>());
-        ssa::k_norms<T><<<G2, ssa::kBlock, 0, s>>>(n, bvec.as<T>(), r.as<T>(), part_b.as<double>(), part_d.as<double>(), y);
+        launch_norms(bvec.as<T>(), r.as<T>(), part_b.as<double>(), part_d.as<double>());
         ssa::k_ctrl_init<<<S, ssa::kBlock, 0, s>>>(part_b.as<double>(), part_d.as<double>(), G, st, prm, H, y,
                                                    pass.as<ssa::PassParams>());
         HIP_CHECK(hipGetLastError());
@@ -833,9 +987,9 @@ struct IcImpl final : IcBase {
                 if (kMixed) convert(r.as<T>(), pre_in(c), nullptr);
                 launch_precond(pre_in(c), pre_out(c), pre_tmp(), nullptr);
             } else {
-                ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr, y);
+                convert(r.as<T>(), z.as<T>(), nullptr);
             }
-            ssa::k_dot<T, T, P><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), nullptr, ssa::Ctrl{}, y);
+            launch_dot(r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>());
             ssa::k_ctrl_store<T><<<S, ssa::kBlock, 0, s>>>(part_c.as<T>(), G, st, 0, y, 0);
             convert(pre_out(c), p.as<T>(), nullptr);
         } else {
@@ -846,9 +1000,9 @@ struct IcImpl final : IcBase {
                                pre_tmp(), nullptr);
                 if (kMixed) convert(pre_out(c), z.as<T>(), nullptr);
             } else {
-                ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), nullptr, y);
+                convert(r.as<T>(), z.as<T>(), nullptr);
             }
-            ssa::k_copy<T, T><<<G2, ssa::kBlock, 0, s>>>(n, z.as<T>(), p.as<T>(), nullptr, y);
+            convert(z.as<T>(), p.as<T>(), nullptr);
             launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_c.as<T>(), nullptr);
             ssa::k_ctrl_store<T><<<S, ssa::kBlock, 0, s>>>(part_c.as<T>(), std::max(1, spmv_grid()), st, 0, y, 1);
             if (kMixed) convert(Ap.as<T>(), pre_in(c), nullptr);
@@ -954,11 +1108,9 @@ struct IcImpl final : IcBase {
             bsrc = tmp.as<T>();
             xsrc = tmp2.as<T>();
         }
-        launch_gather(rev.as<index_t>(), bsrc, tot, bvec.as<T>(), n, n, m);
-        zero_tail(bvec, m);
+        gather_in(bsrc, bvec.as<T>(), m);
         if (prm.ngs) HIP_CHECK(hipMemsetAsync(x.p, 0, db, s));
-        else launch_gather(rev.as<index_t>(), xsrc, tot, x.as<T>(), n, n, m);
-        zero_tail(x, m);
+        else gather_in(xsrc, x.as<T>(), m);
         HIP_CHECK(hipMemsetAsync(best.p, 0, db, s));
 
         // per state: whether it is solved at all (block padding is not),
@@ -1058,7 +1210,7 @@ struct IcImpl final : IcBase {
             }
         }
         T* xdst = on_device ? xio : tmp2.as<T>();
-        launch_gather(perm.as<index_t>(), x.as<T>(), n, xdst, tot, tot, m);
+        scatter_out(x.as<T>(), xdst, m);
         if (!on_device) HIP_CHECK(hipMemcpyAsync(xio, tmp2.p, vb, hipMemcpyDeviceToHost, s));
         if (history && S == 1 && results[0].history_len > 0)
             HIP_CHECK(hipMemcpyAsync(history, H, sizeof(double) * results[0].history_len, hipMemcpyDeviceToHost, s));

@@ -22,13 +22,74 @@ constexpr index_t kPadColumn = -(1 << 30);   // padding slot of a sweep layout
 constexpr int kWavesPerBlock = kBlock / 64;
 
 // ---------------------------------------------------------------------------
+// Blocks of W right-hand sides: every vector holds n rows of W values,
+// element (i, r) at v[i*W + r], so a lane's W values are one contiguous,
+// 16-byte aligned run (W >= 4) moved as dwordx4 loads and stores.  W = 1 is
+// the one-vector layout.
+// ---------------------------------------------------------------------------
+template <typename P, int W>
+__device__ inline void load_row(const P* __restrict__ p, P (&d)[W])
+{
+    if constexpr (W == 1) {
+        d[0] = p[0];
+    } else {
+        static_assert((W * sizeof(P)) % 16 == 0, "a row is whole dwordx4 words");
+        constexpr int kWords = int(W * sizeof(P) / 16);
+        uint4 w[kWords];
+        const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+        for (int k = 0; k < kWords; ++k) w[k] = q[k];
+        __builtin_memcpy(d, w, sizeof(w));
+    }
+}
+
+template <typename P, int W>
+__device__ inline void store_row(P* p, const P (&d)[W])
+{
+    if constexpr (W == 1) {
+        p[0] = d[0];
+    } else {
+        constexpr int kWords = int(W * sizeof(P) / 16);
+        uint4 w[kWords];
+        __builtin_memcpy(w, d, sizeof(w));
+        uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+        for (int k = 0; k < kWords; ++k) q[k] = w[k];
+    }
+}
+
+// W values summed over the workgroup with block_sum's tree for each of them
+// (wave butterfly, then the waves in order); valid in threads 0..W-1, thread
+// r holding value r.  `scratch` holds kWavesPerBlock*W values.
+template <typename T, int W>
+__device__ inline T block_sum_multi(const T (&v)[W], T* scratch)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    T w[W];
+#pragma unroll
+    for (int r = 0; r < W; ++r) w[r] = wave_sum(v[r]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < W; ++r) scratch[wid * W + r] = w[r];
+    }
+    __syncthreads();
+    T s = zero<T>();
+    if (threadIdx.x < W) {
+        for (int q = 0; q < kWavesPerBlock; ++q) s = add(s, scratch[q * W + threadIdx.x]);
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
 // SpMV over SELL-64 (one wavefront per 64-row slice), optional fused dot
 // DOT: 0 none, 1 sum x_i y_i, 2 sum conj(x_i) y_i
 //
-// W right-hand sides (vector r at offset r*sy.vstride): each lane loads a
-// matrix entry once and applies it to all W vectors; the partials of
-// right-hand side r are [r*gridDim.x, (r+1)*gridDim.x).  Right-hand sides
-// that have stopped are still multiplied (their results are not read).
+// W right-hand sides (interleaved rows): each lane loads a matrix entry once
+// and applies it to all W vectors; the partials of right-hand side r are
+// [r*gridDim.x, (r+1)*gridDim.x).  Right-hand sides that have stopped are
+// still multiplied (their results are not read).  The fused convergence
+// check (`check`) is for one vector only.
 // ---------------------------------------------------------------------------
 template <typename T, int DOT, int W>
 __global__ __launch_bounds__(kBlock) void k_spmv(
@@ -38,11 +99,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const SolverState* __restrict__ st, Ctrl check, Systems sy)
 {
     const int sys = sy.spmv_sys ? sy.spmv_sys[blockIdx.x] : 0;
-    unsigned live = 0;   // bit r: right-hand side r (or the batch system) runs
+    int live = 0;   // some right-hand side (or the batch system) runs
 #pragma unroll
-    for (int r = 0; r < W; ++r)
-        if (!st || !st[sys + r].status) live |= 1u << r;
-    __shared__ T red[kWavesPerBlock];
+    for (int r = 0; r < W; ++r) live |= !st || !st[sys + r].status;
+    __shared__ T red[kWavesPerBlock * W];
     const int lane = threadIdx.x & 63;
     const index_t s = index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
     const index_t sc = s < nslices ? s : nslices;
@@ -53,7 +113,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         if (o0 < 0) y[0] = zero<T>();
         return;
     }
-    const size_t vs = size_t(sy.vstride);
     T d[W];
 #pragma unroll
     for (int r = 0; r < W; ++r) d[r] = zero<T>();
@@ -66,32 +125,35 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         for (index_t k = o0; k < o1; ++k) {
             const size_t at = size_t(k) * 64 + lane;
             const T v = val[at];
-            const index_t c = col[at];
+            T g[W];
+            load_row<T, W>(x + size_t(col[at]) * W, g);
 #pragma unroll
-            for (int r = 0; r < W; ++r) acc[r] = fma_(v, x[r * vs + c], acc[r]);
+            for (int r = 0; r < W; ++r) acc[r] = fma_(v, g[r], acc[r]);
         }
         const index_t row = s * 64 + lane;
         if (row < n) {
+            store_row<T, W>(y + size_t(row) * W, acc);
+            if (DOT) {
+                T xr[W];
+                load_row<T, W>(x + size_t(row) * W, xr);
 #pragma unroll
-            for (int r = 0; r < W; ++r) {
-                y[r * vs + row] = acc[r];
-                if (DOT) d[r] = mul(conj_if(x[r * vs + row], DOT == 2), acc[r]);
+                for (int r = 0; r < W; ++r) d[r] = mul(conj_if(xr[r], DOT == 2), acc[r]);
             }
         }
     }
     if (DOT) {
-#pragma unroll
-        for (int r = 0; r < W; ++r) {
-            T t = block_sum(d[r], red);
-            if (threadIdx.x == 0) partials[size_t(r) * gridDim.x + blockIdx.x] = t;
+        if constexpr (W == 1) {
+            T t = block_sum(d[0], red);
+            if (threadIdx.x == 0) partials[blockIdx.x] = t;
+        } else {
+            T t = block_sum_multi<T, W>(d, red);
+            if (threadIdx.x < W) partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
         }
     }
-    if (check.part) {
+    if (W == 1 && check.part) {
         __shared__ double redd[kWavesPerBlock];
         const index_t local = index_t(blockIdx.x) - (sy.spmv_g0 ? sy.spmv_g0[sys] : 0);
-#pragma unroll
-        for (int r = 0; r < W; ++r)
-            if (live & (1u << r)) ctrl_check_wg0(ctrl_for(check, sy, sys + r, local), redd);
+        ctrl_check_wg0(ctrl_for(check, sy, sys, local), redd);
     }
 }
 
@@ -110,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 // External part of the oversized blocks of this phase (walk slots t0..t1):
 //   FWD: a = in_i - sum_ext L_ik out_k      BWD: a = -sum_ext U_ik out_k
 // written to sa[t] (and in_i to sb[t] for BWD); the walk kernel finishes them.
-// Right-hand side r uses the slots [r*ws, (r+1)*ws).
+// Right-hand side r of a block uses the slots [r*ws, (r+1)*ws).
 struct WalkExt {
     size_t t0, t1;
     const index_t* row;
@@ -124,25 +186,27 @@ struct WalkExt {
 
 template <typename P, typename F, bool FWD, int W>
 __device__ inline void walk_external(const WalkExt& w, size_t t, const P* __restrict__ in,
-                                     const P* out, size_t vs)
+                                     const P* out)
 {
     if (t >= w.t1) return;
     const index_t row = w.row[t];
     if (row < 0) return;
     const F* ev = static_cast<const F*>(w.ext_v);
-    P a[W];
+    P xin[W], a[W];
+    load_row<P, W>(in + size_t(row) * W, xin);
 #pragma unroll
-    for (int r = 0; r < W; ++r) a[r] = FWD ? in[r * vs + row] : zero<P>();
+    for (int r = 0; r < W; ++r) a[r] = FWD ? xin[r] : zero<P>();
     for (index_t k = w.ext_rp[t]; k < w.ext_rp[t + 1]; ++k) {
         const P v = cvt<P>(ev[k]);
-        const index_t c = w.ext_ci[k];
+        P g[W];
+        load_row<P, W>(out + size_t(w.ext_ci[k]) * W, g);
 #pragma unroll
-        for (int r = 0; r < W; ++r) a[r] = fms_(v, out[r * vs + c], a[r]);
+        for (int r = 0; r < W; ++r) a[r] = fms_(v, g[r], a[r]);
     }
 #pragma unroll
     for (int r = 0; r < W; ++r) {
         static_cast<P*>(w.sa)[r * w.ws + t] = a[r];
-        if (!FWD) static_cast<P*>(w.sb)[r * w.ws + t] = in[r * vs + row];
+        if (!FWD) static_cast<P*>(w.sb)[r * w.ws + t] = xin[r];
     }
 }
 
@@ -170,11 +234,11 @@ struct SweepArrays {
 // One wavefront, one slice.  The slice's offsets, the lane's row and the
 // first batch of entries are requested before `stopped` is tested, so the
 // chain of dependent memory round trips is offsets -> entries -> gathers.
-// W right-hand sides (vector r at offset r*vs): each entry is loaded once
-// and applied to all of them; CH entries per batch.
+// W right-hand sides (interleaved rows): each entry is loaded once and
+// applied to all of them; CH entries per batch.
 template <typename P, typename F, int BR, bool FWD, int CH, int W>
 __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __restrict__ in,
-                                   P* out, size_t vs, int stopped)
+                                   P* out, int stopped)
 {
     const F* __restrict__ val = static_cast<const F*>(A.val);
     const index_t* __restrict__ col = A.col;
@@ -206,11 +270,14 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
     // right-hand sides it is loaded there (registers), with one up front
     constexpr bool kLateIn = !FWD && W > 1;
     P xin[W], acc[W];
+    if (kLateIn) {
 #pragma unroll
-    for (int r = 0; r < W; ++r) {
-        xin[r] = kLateIn ? zero<P>() : in[r * vs + rr];
-        acc[r] = FWD ? xin[r] : zero<P>();
+        for (int r = 0; r < W; ++r) xin[r] = zero<P>();
+    } else {
+        load_row<P, W>(in + size_t(rr) * W, xin);
     }
+#pragma unroll
+    for (int r = 0; r < W; ++r) acc[r] = FWD ? xin[r] : zero<P>();
     const P dinv = cvt<P>(static_cast<const F*>(A.invd)[rr]);
     F intv[BR > 1 ? BR - 1 : 1];
 #pragma unroll
@@ -223,8 +290,7 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
         for (int j = 0; j < CH; ++j) {
             if (k0 + j >= o1) c[j] = kPadColumn;
             const index_t cj = c[j] >= 0 ? c[j] : 0;
-#pragma unroll
-            for (int r = 0; r < W; ++r) g[j][r] = out[r * vs + cj];
+            load_row<P, W>(out + size_t(cj) * W, g[j]);
         }
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
@@ -248,10 +314,7 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
             v[j] = val[at];
         }
     }
-    if (kLateIn) {
-#pragma unroll
-        for (int r = 0; r < W; ++r) xin[r] = in[r * vs + rr];
-    }
+    if (kLateIn) load_row<P, W>(in + size_t(rr) * W, xin);
     const int base = lane - pos;
     P res[W];
 #pragma unroll
@@ -260,10 +323,8 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
     for (int j = 0; j < BR; ++j) {
         if (live && pos == j) {
 #pragma unroll
-            for (int r = 0; r < W; ++r) {
-                res[r] = FWD ? mul(acc[r], dinv) : fma_(acc[r], dinv, xin[r]);
-                out[r * vs + row] = res[r];
-            }
+            for (int r = 0; r < W; ++r) res[r] = FWD ? mul(acc[r], dinv) : fma_(acc[r], dinv, xin[r]);
+            store_row<P, W>(out + size_t(row) * W, res);
         }
         if (j < BR - 1) {
             // position j of this lane's block (clamped for idle lanes and
@@ -294,55 +355,47 @@ __device__ inline int all_stopped(const SolverState* __restrict__ st)
 template <typename P, typename F, int BR, bool FWD, int CH, int W>
 __global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, SweepArrays A,
                                                   const P* __restrict__ in, P* out,
-                                                  index_t vstride, WalkExt walk,
+                                                  WalkExt walk,
                                                   const SolverState* __restrict__ st)
 {
-    const size_t vs = size_t(vstride);
     const index_t nreg = (s1 - s0 + kWavesPerBlock - 1) / kWavesPerBlock;
     if (index_t(blockIdx.x) >= nreg) {
         // batches: computed for every system (the results of a stopped
         // system's sweep are never read)
         if (!A.slice_sys && all_stopped<W>(st)) return;
-        walk_external<P, F, FWD, W>(walk, walk.t0 + size_t(blockIdx.x - nreg) * kBlock + threadIdx.x, in, out, vs);
+        walk_external<P, F, FWD, W>(walk, walk.t0 + size_t(blockIdx.x - nreg) * kBlock + threadIdx.x, in, out);
         return;
     }
     const index_t s = s0 + index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
     if (s >= s1) return;                       // whole wavefront: no shuffle below
     const int stopped = all_stopped<W>(st + (A.slice_sys ? A.slice_sys[s] : 0));
-    sweep_slice<P, F, BR, FWD, CH, W>(s, A, in, out, vs, stopped);
+    sweep_slice<P, F, BR, FWD, CH, W>(s, A, in, out, stopped);
 }
 
 // Several consecutive small colours in one workgroup of NT threads: the
 // wavefronts share each colour's slices, a workgroup barrier separates the
 // colours (global writes of a workgroup are visible to it after the
 // barrier).  NT = 1024 for real values, 512 for complex values or 16-row
-// blocks (register budget).
-template <typename P, int BR> constexpr int tail_threads() { return (sizeof(P) > 8 || BR >= 16) ? 512 : 1024; }
+// blocks, 256 with several right-hand sides per lane (register budget).
+template <typename P, int BR, int W = 1>
+constexpr int tail_threads() { return W > 1 ? 256 : (sizeof(P) > 8 || BR >= 16) ? 512 : 1024; }
 constexpr int kTailMaxPhases = 16;
 struct TailPhases {
     int count;
     index_t slices[kTailMaxPhases + 1];
 };
 
-// Several right-hand sides: one workgroup each (blockIdx.x = r, vector r
-// at offset r*vs); the workgroups run side by side, so the chain of
-// colours costs what it costs for one right-hand side.
-template <typename P, typename F, int BR, bool FWD, int CH>
-__global__ __launch_bounds__((tail_threads<P, BR>())) void k_sweep_tail(TailPhases T, SweepArrays A,
-                                                                  const P* __restrict__ in, P* out,
-                                                                  index_t vstride,
-                                                                  const SolverState* __restrict__ st)
+template <typename P, typename F, int BR, bool FWD, int CH, int W>
+__global__ __launch_bounds__((tail_threads<P, BR, W>())) void k_sweep_tail(TailPhases T, SweepArrays A,
+                                                                     const P* __restrict__ in, P* out,
+                                                                     const SolverState* __restrict__ st)
 {
-    const int r = int(blockIdx.x);
-    in += size_t(r) * size_t(vstride);
-    out += size_t(r) * size_t(vstride);
-    st += r;
-    if (!A.slice_sys && st->status) return;
+    if (!A.slice_sys && all_stopped<W>(st)) return;
     const index_t w = index_t(threadIdx.x >> 6);
-    constexpr index_t kWaves = tail_threads<P, BR>() / 64;
+    constexpr index_t kWaves = tail_threads<P, BR, W>() / 64;
     for (int ph = 0; ph < T.count; ++ph) {
         for (index_t s = T.slices[ph] + w; s < T.slices[ph + 1]; s += kWaves)
-            sweep_slice<P, F, BR, FWD, CH, 1>(s, A, in, out, 0, A.slice_sys ? st[A.slice_sys[s]].status : 0);
+            sweep_slice<P, F, BR, FWD, CH, W>(s, A, in, out, A.slice_sys ? st[A.slice_sys[s]].status : 0);
         __syncthreads();
     }
 }
@@ -359,11 +412,11 @@ __global__ __launch_bounds__(1024) void k_sweep_big(
     const index_t* __restrict__ big_lvl, const index_t* __restrict__ big_rows,
     const index_t* __restrict__ big_rp, const index_t* __restrict__ big_ci,
     const F* __restrict__ big_v, const F* __restrict__ invd, const P* __restrict__ in,
-    P* out, int use_lds, index_t vstride, const SolverState* __restrict__ st)
+    P* out, int use_lds, index_t es, const SolverState* __restrict__ st)
 {
-    // right-hand side blockIdx.y
-    in += size_t(blockIdx.y) * size_t(vstride);
-    out += size_t(blockIdx.y) * size_t(vstride);
+    // right-hand side r = blockIdx.y of a block: row i at i*es + r
+    in += blockIdx.y;
+    out += blockIdx.y;
     if (st && st[blockIdx.y].status) return;
     extern __shared__ unsigned char lds_raw[];
     P* lds = reinterpret_cast<P*>(lds_raw);
@@ -375,18 +428,18 @@ __global__ __launch_bounds__(1024) void k_sweep_big(
         for (index_t q = lv[l] + index_t(threadIdx.x); q < lv[l + 1]; q += index_t(blockDim.x)) {
             const index_t gp = p0 + q;
             const index_t row = big_rows[gp];
-            P acc = FWD ? in[row] : zero<P>();
+            P acc = FWD ? in[size_t(row) * es] : zero<P>();
             for (index_t k = big_rp[gp]; k < big_rp[gp + 1]; ++k) {
                 const index_t c = big_ci[k];
                 P xv;
-                if (c >= 0) xv = out[c];
+                if (c >= 0) xv = out[size_t(c) * es];
                 else if (use_lds) xv = lds[-1 - c];
-                else xv = out[big_rows[p0 + (-1 - c)]];
+                else xv = out[size_t(big_rows[p0 + (-1 - c)]) * es];
                 acc = fms_(cvt<P>(big_v[k]), xv, acc);
             }
             const P dinv = cvt<P>(invd[row]);
-            const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, in[row]);
-            out[row] = res;
+            const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, in[size_t(row) * es]);
+            out[size_t(row) * es] = res;
             if (use_lds) lds[q] = res;
         }
         __syncthreads();
@@ -400,9 +453,9 @@ __global__ __launch_bounds__(1024) void k_sweep_big(
 // read the values of earlier steps from LDS (indexed by level position).
 // The operands of step s+D are loaded while step s computes (D register
 // sets, rotated by the unrolled j loop), so the chain of steps pays LDS
-// latency, not memory latency.  Several right-hand sides: one wavefront
-// per (block, right-hand side), blockIdx.y = r; walk slots of r start at
-// r*ws, its vector at r*vs.
+// latency, not memory latency.  A block of right-hand sides: one wavefront
+// per (block, right-hand side r = blockIdx.y); r's walk slots start at r*ws
+// and its rows are out[i*es + r].
 // ---------------------------------------------------------------------------
 template <typename P, typename F, int KMAX>
 struct WalkBuf {
@@ -418,12 +471,12 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
     index_t b0, const index_t* __restrict__ step_off, const index_t* __restrict__ wrow,
     const index_t* __restrict__ wpos, const F* __restrict__ winvd, const F* __restrict__ wv,
     const index_t* __restrict__ wix, index_t zero_pos, P* out, const P* __restrict__ sa,
-    const P* __restrict__ sb, index_t vstride, size_t ws, const SolverState* __restrict__ st,
+    const P* __restrict__ sb, index_t es, size_t ws, const SolverState* __restrict__ st,
     const int* __restrict__ walk_sys)
 {
     const index_t bb = b0 + index_t(blockIdx.x);
     const int r = int(blockIdx.y);
-    out += size_t(r) * size_t(vstride);
+    out += r;
     sa += size_t(r) * ws;
     sb += size_t(r) * ws;
     if (st && st[(walk_sys ? walk_sys[bb] : 0) + r].status) return;
@@ -466,7 +519,7 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
             // wavefront's LDS operations execute in order, and the compiler
             // keeps a possibly aliasing LDS read after this write.
             lds[B.pos >= 0 ? B.pos : trash] = res;
-            if (B.pos >= 0) out[B.row] = res;
+            if (B.pos >= 0) out[size_t(B.row) * es] = res;
             __builtin_amdgcn_wave_barrier();
             load(s + j + D, B);
         }
