@@ -37,7 +37,12 @@ Extra workloads (`extra`, skipped with --quick):
              and the colour sweeps load each matrix and factor entry once for
              all 16), with each right-hand side's error against its FP64
              solution, beside the FP64 block solve and the one-vector FP64
-             and mixed solves.
+             and mixed solves;
+  eddy78-lossy-multi16  the same on eddy78's grid with the conductor
+             filling the box (A = K + j*0.1*M_e on every edge): COCR reaches
+             1e-10 there, so the block solve is a time to tolerance and the
+             errors compare converged solutions (and the reference's, for the
+             first right-hand side).
 
 The JSON line also carries
   roofline      the dominant kernel by time, the ABMC colour kernel k_sweep:
@@ -90,6 +95,14 @@ WORKLOADS = {
         method="cocr",
         cfg=dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=8,
                  tolerance=1e-10, max_iterations=1000)),
+    "eddy78-lossy": dict(
+        desc=("complex-symmetric eddy current in a lossy medium, 78^3 hex grid (n=1,387,386, "
+              "nnz=17,821,650): eddy78's operator with the conductor filling the box, "
+              "A = K + j*0.1*M_e on every edge; ABMC-IC(0)-COCR complex<double>, shift 1.05, "
+              "block 4, 8 colours, tol 1e-10, max 1000 iterations"),
+        method="cocr",
+        cfg=dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=8,
+                 tolerance=1e-10, max_iterations=1000)),
 }
 
 
@@ -125,6 +138,8 @@ def make_system(name):
             _SYSTEMS[base] = P.poisson(48)
         elif base == "eddy78":
             _SYSTEMS[base] = P.eddy(78)
+        elif base == "eddy78-lossy":
+            _SYSTEMS[base] = P.eddy(78, cond_frac=1.0, sigma=0.1)
         else:
             raise SystemExit(f"unknown workload {name}")
     return _SYSTEMS[base]
@@ -332,23 +347,23 @@ def run_batch(steps, warmup, cpu=True, budget_s=25.0, profile_reps=10, tune=True
     return out
 
 
-MULTI_DESC = ("complex-symmetric eddy current, 78^3 hex grid (n=1,387,386, nnz=17,821,650), "
-              "16 seeded complex-Gaussian right-hand sides (seed 2; the first is eddy78's); "
-              "ABMC-IC(0)-COCR, shift 1.05, block 4, 8 colours, tol 1e-10, max 1000 "
-              "iterations; IC factor and sweeps in complex<float> inside the complex<double> "
-              "COCR with iterative refinement, all 16 in one block solve")
+MULTI_DESC = ("{sys}; 16 seeded complex-Gaussian right-hand sides (seed 2; the first is "
+              "the one-vector workload's); IC factor and sweeps in complex<float> inside the "
+              "complex<double> COCR with iterative refinement, all 16 in one block solve")
 
 
-def run_multi(steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_s=40.0):
-    """eddy78 with a block of nrhs right-hand sides: the mixed-precision block
-    solve per step (device-resident vectors), the FP64 block solve (each
-    right-hand side's FP64 solution, for its error), and the one-vector FP64
-    and mixed solves of the first right-hand side for the time per vector."""
+def run_multi(system, steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_s=40.0):
+    """`system` with a block of nrhs right-hand sides: the mixed-precision
+    block solve per step (device-resident vectors), the FP64 block solve
+    (each right-hand side's FP64 solution, for its error), and the one-vector
+    FP64 and mixed solves of the first right-hand side for the time per
+    vector."""
     import torch
     from sparsesolv_amd import _lib
     from sparsesolv_amd import problems as P
-    w = WORKLOADS["eddy78"]
-    sysm = make_system("eddy78")
+    w = WORKLOADS[system]
+    tag = f"[{system}-multi{nrhs}]"
+    sysm = make_system(system)
     A = sysm.A
     B = P.complex_gaussian_block(A.n, nrhs, 2)
     kw = dict(w["cfg"])
@@ -389,16 +404,16 @@ def run_multi(steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_s=40.0):
     for k in range(steps):
         t, res = block(hm, cfgm)
         times.append(t)
-        log(f"[multi16] step {k}: {t:.4f}s iterations max={max(r.iterations for r in res)} "
+        log(f"{tag} step {k}: {t:.4f}s iterations max={max(r.iterations for r in res)} "
             f"all converged={all(r.converged for r in res)}")
     Xm = X_d.cpu().numpy()
     block(h64, cfg64)
     t64, res64 = block(h64, cfg64)
     X64 = X_d.cpu().numpy()
-    log(f"[multi16] fp64 block: {t64:.4f}s iterations max={max(r.iterations for r in res64)}")
+    log(f"{tag} fp64 block: {t64:.4f}s iterations max={max(r.iterations for r in res64)}")
     t1_64, r1_64 = single(h64, cfg64)
     t1_m, r1_m = single(hm, cfgm)
-    log(f"[multi16] one vector: fp64 {t1_64:.4f}s ({r1_64.iterations} it), "
+    log(f"{tag} one vector: fp64 {t1_64:.4f}s ({r1_64.iterations} it), "
         f"mixed {t1_m:.4f}s ({r1_m.iterations} it)")
     As = A.to_scipy()
     err, trel = [], []
@@ -419,11 +434,12 @@ def run_multi(steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_s=40.0):
             continue
         kern[label] = {"us": round(us, 2), "bytes": int(by), "GB_s": round(by / us / 1e3, 1),
                        "frac": round(by / us / 1e3 / PEAK_GBS, 4)}
-        log(f"[multi16] {label} (x{nrhs}): {us:.2f} us  {by / 1e6:.2f} MB  {by / us / 1e3:.0f} GB/s")
+        log(f"{tag} {label} (x{nrhs}): {us:.2f} us  {by / 1e6:.2f} MB  {by / us / 1e3:.0f} GB/s")
     model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], 16, True, 8, nrhs)
     it_mean = float(np.mean(its))
     out = {
-        "workload": "eddy78-multi16", "desc": MULTI_DESC, "nrhs": nrhs, "n": A.n,
+        "workload": f"{system}-multi{nrhs}", "desc": MULTI_DESC.format(sys=w["desc"]),
+        "nrhs": nrhs, "n": A.n,
         "seconds": sec, "times": [round(t, 5) for t in times],
         "seconds_per_rhs": round(sec / nrhs, 5),
         "iterations": its, "all_converged": bool(all(r.converged for r in res)),
@@ -443,14 +459,16 @@ def run_multi(steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_s=40.0):
         "kernels": kern,
     }
     if cpu:
-        out["cpu_reference_rhs0"] = multi_cpu_check(A, B[0], kw, Xm[0], X64[0], budget_s)
+        out["cpu_reference_rhs0"] = multi_cpu_check(A, B[0], kw, Xm[0], X64[0], int(r1_64.iterations),
+                                                    budget_s)
     return out
 
 
-def multi_cpu_check(A, b, kw, xm, x64, budget_s):
+def multi_cpu_check(A, b, kw, xm, x64, gpu_iters, budget_s):
     """The reference's FP64 COCR on the first right-hand side, when a full
-    solve fits the budget (estimated from 5 timed iterations): the errors of
-    the GPU's mixed and FP64 solutions against it."""
+    solve fits the budget (estimated from 5 timed iterations and the GPU's
+    FP64 iteration count): the errors of the GPU's mixed and FP64 solutions
+    against it."""
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -460,7 +478,7 @@ def multi_cpu_check(A, b, kw, xm, x64, budget_s):
     fields = dict(orc.config()._fields_)
     cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
     t_probe, setup = orc.time_iterations(A, b, cfg, 5, "COCR")
-    est = setup + t_probe / 5 * kw.get("max_iterations", 1000)
+    est = setup + t_probe / 5 * 1.2 * min(gpu_iters, kw.get("max_iterations", 1000))
     if est > budget_s:
         return {"skipped": f"estimated {est:.0f}s for a full reference solve > {budget_s:.0f}s"}
     t = time.perf_counter()
@@ -606,7 +624,8 @@ def main():
                                                  args.budget, args.profile_reps)
     extra = {"primary": main_out}
     names = ("curlcurl64-mixed", "curlcurl64-level", "laplace48", "laplace48-mixed",
-             "laplace48-level", "eddy78", "eddy78-mixed", "batch64", "eddy78-multi16")
+             "laplace48-level", "eddy78", "eddy78-mixed", "batch64", "eddy78-multi16",
+             "eddy78-lossy-multi16")
     wanted = set(names if args.extras is None else args.extras.split(","))
     if not args.quick:
         for other in names[:7]:
@@ -623,10 +642,12 @@ def main():
         extra["batch64"] = run_batch(max(1, args.steps), 1, cpu=not args.no_cpu,
                                      profile_reps=args.profile_reps)
         log(f"[batch64] {json.dumps({k: v for k, v in extra['batch64'].items() if k != 'desc'})}")
-    if not args.quick and "eddy78-multi16" in wanted:
-        extra["eddy78-multi16"] = run_multi(max(1, args.steps), 1, cpu=not args.no_cpu,
-                                            profile_reps=args.profile_reps)
-        log(f"[multi16] {json.dumps({k: v for k, v in extra['eddy78-multi16'].items() if k != 'desc'})}")
+    for system in ("eddy78", "eddy78-lossy"):
+        key = f"{system}-multi16"
+        if not args.quick and key in wanted:
+            extra[key] = run_multi(system, max(1, args.steps), 1, cpu=not args.no_cpu,
+                                   profile_reps=args.profile_reps)
+            log(f"[{key}] {json.dumps({k: v for k, v in extra[key].items() if k != 'desc'})}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -4,7 +4,8 @@
 // Everything lives in ABMC order: the Krylov vectors, the operator A and
 // both triangular factors.  Two layouts are built here.
 //
-// 1. SpmvLayout: SELL-64.  Rows are taken 64 at a time (one wavefront);
+// 1. SpmvLayout: SELL-64.  Rows are taken 64 at a time (one wavefront), in
+//    the matrix's original order (a map gives each lane its row);
 //    slice s stores K_s = max row length entry slots, slot-major, so lane l
 //    of slot k reads val[(off_s + k)*64 + l] -- every load of the wavefront
 //    is one contiguous 512 B (f64) / 1 KiB (c128) segment.  Short rows are
@@ -57,17 +58,29 @@ struct SpmvLayout {
     std::vector<index_t> slice_off;   // nslices+1, in entry slots
     std::vector<F> val;               // slots*64
     std::vector<index_t> col;
+    std::vector<index_t> row;         // nslices*64: the lane's row (-1: none)
 
+    // Slices take the rows in the order `order` (order[k] = the row at
+    // position k; empty: ascending).  With order = the matrix's original
+    // numbering, a wavefront's rows, and the workgroups running at one time,
+    // cover a compact part of the domain, so their gathers of x come from a
+    // compact part of it too; in the colour order the rows in flight span
+    // whole colours and reference nearly all of x.
     template <typename T>
-    void build(const CsrHost<T>& A)
+    void build(const CsrHost<T>& A, const std::vector<index_t>& order = {})
     {
         n = A.n;
         nslices = (n + kWave - 1) / kWave;
+        auto row_at = [&](index_t k) { return order.empty() ? k : order[size_t(k)]; };
         slice_off.assign(size_t(nslices) + 1, 0);
+        row.assign(size_t(nslices) * kWave, -1);
         for (index_t s = 0; s < nslices; ++s) {
             index_t K = 0;
-            for (index_t r = s * kWave; r < std::min<index_t>(n, (s + 1) * kWave); ++r)
+            for (index_t k = s * kWave; k < std::min<index_t>(n, (s + 1) * kWave); ++k) {
+                const index_t r = row_at(k);
+                row[size_t(k)] = r;
                 K = std::max(K, A.rp[r + 1] - A.rp[r]);
+            }
             slice_off[s + 1] = slice_off[s] + K;
         }
         const size_t slots = size_t(slice_off[nslices]) * kWave;
@@ -76,14 +89,14 @@ struct SpmvLayout {
         for (index_t s = 0; s < nslices; ++s) {
             const index_t K = slice_off[s + 1] - slice_off[s];
             for (int l = 0; l < kWave; ++l) {
-                const index_t r = s * kWave + l;
+                const index_t r = row[size_t(s) * kWave + l];
                 for (index_t k = 0; k < K; ++k) {
                     const size_t at = (size_t(slice_off[s]) + k) * kWave + l;
-                    if (r < n && k < A.rp[r + 1] - A.rp[r]) {
+                    if (r >= 0 && k < A.rp[r + 1] - A.rp[r]) {
                         val[at] = static_cast<F>(A.v[A.rp[r] + k]);
                         col[at] = A.ci[A.rp[r] + k];
                     } else {
-                        col[at] = r < n ? r : 0;
+                        col[at] = r >= 0 ? r : 0;   // padding: value 0
                     }
                 }
             }
@@ -112,10 +125,15 @@ SpmvLayout<F> combine_spmv(const std::vector<const SpmvLayout<F>*>& parts, index
             out.col.resize(dst + size_t(K) * kWave);
             for (size_t q = 0; q < size_t(K) * kWave; ++q) out.col[dst + q] = P->col[src + q] + r0;
             out.slice_off.push_back(out.slice_off.back() + K);
+            for (int l = 0; l < kWave; ++l) {
+                const index_t r = P->row[size_t(s) * kWave + l];
+                out.row.push_back(r >= 0 ? r + r0 : -1);
+            }
         }
         index_t ns = P->nslices;
         while (ns % align) {
             out.slice_off.push_back(out.slice_off.back());
+            out.row.insert(out.row.end(), kWave, -1);
             ++ns;
         }
         row0.push_back(r0 + ns * kWave);
@@ -123,6 +141,7 @@ SpmvLayout<F> combine_spmv(const std::vector<const SpmvLayout<F>*>& parts, index
     out.nslices = index_t(out.slice_off.size()) - 1;
     out.n = row0.back();
     if (out.val.empty()) { out.val.assign(kWave, F(0)); out.col.assign(kWave, 0); }
+    if (out.row.empty()) out.row.assign(kWave, -1);
     return out;
 }
 

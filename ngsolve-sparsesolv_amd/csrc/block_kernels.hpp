@@ -2,13 +2,13 @@
 // right-hand sides of one system (interleaved rows: element (i, r) at
 // v[i*W + r]; the SpMV and the sweeps take W as a template in kernels.hpp).
 //
-// The kernels that reduce (dots, norms, the residual updates) give each
-// thread whole rows: the partial of right-hand side r in workgroup g sums
-// rows g*kBlock + t, stepping by G*kBlock, with block_sum's tree -- the same
-// rows in the same order as workgroup g of the one-vector kernel, so a block
-// solve reproduces the one-vector solves bit for bit.  Element-wise kernels
-// take the flat, coalesced mapping (thread e, element e; G*kBlock is a
-// multiple of W, so a thread keeps one right-hand side).  The control steps
+// The kernels that reduce (dots, norms, the residual updates) sum, for
+// right-hand side r in workgroup g, the contributions of rows g*kBlock + t
+// (stepping by G*kBlock) in thread t, in the same order, with block_sum's
+// tree -- as workgroup g of the one-vector kernel does -- so a block solve
+// reproduces the one-vector solves bit for bit (tile_reduce).  Element-wise
+// kernels take the flat, coalesced mapping (thread e, element e; G*kBlock is
+// a multiple of W, so a thread keeps one right-hand side).  The control steps
 // run as W one-workgroup kernels between producer and consumer: a fused
 // step would have every consuming workgroup re-sum W sets of partials.
 // Partials of right-hand side r occupy [r*np, (r+1)*np) for a producer of
@@ -19,15 +19,55 @@
 
 namespace ssa {
 
-// values of a row moved at once: 64 bytes (four dwordx4) or the whole row
-template <typename T, int W>
-constexpr int row_chunk() { return int(64 / sizeof(T)) < W ? int(64 / sizeof(T)) : W; }
-
-template <int W>
-__device__ inline void running_mask(const SolverState* __restrict__ st, bool (&run)[W])
+// Reductions over a block keep the one-vector kernel's tree: in workgroup g,
+// thread t owns rows g*kBlock + t (stepping by G*kBlock) and adds their
+// contributions in that order, then block_sum combines the threads.  The
+// memory side stays coalesced: a tile of kBlock rows is processed element by
+// element (consecutive threads, consecutive elements), each contribution is
+// written to an LDS tile, and after a barrier thread t adds the
+// contributions of its row.  Columns go CP at a time (the tile is
+// kBlock x (CP+1) values, one pad column against bank conflicts).
+template <typename V, int W>
+constexpr int pass_cols()
 {
+    constexpr int lim = int(160 / sizeof(V)) - 1;
+    return W <= lim ? W : lim >= 8 ? 8 : lim >= 4 ? 4 : 1;
+}
+
+// One tile: rows [base, base + rows) (rows <= kBlock).  fn(e, h) does the
+// element-wise work of element e (index into the interleaved vectors; its
+// column is h*CP + (threadIdx.x & (CP-1))) and returns the contribution.
+template <typename V, int W, typename Fn>
+__device__ inline void tile_reduce(index_t base, index_t rows, V* tile, V (&s)[W], Fn&& fn)
+{
+    constexpr int CP = pass_cols<V, W>();
+    const int t = threadIdx.x;
 #pragma unroll
-    for (int r = 0; r < W; ++r) run[r] = st[r].status == 0;
+    for (int h = 0; h < W / CP; ++h) {
+        const int c = h * CP + (t & (CP - 1));
+#pragma unroll
+        for (int j = 0; j < CP; ++j) {
+            const int el = j * kBlock + t;
+            const int lr = el / CP;
+            V v = zero<V>();
+            if (lr < rows) v = fn(size_t(base + lr) * W + c, h);
+            tile[lr * (CP + 1) + (el & (CP - 1))] = v;
+        }
+        __syncthreads();
+        if (t < rows) {
+#pragma unroll
+            for (int k = 0; k < CP; ++k) s[h * CP + k] = add(s[h * CP + k], tile[t * (CP + 1) + k]);
+        }
+        __syncthreads();
+    }
+}
+
+// the column of thread t in pass h of a tile with CP columns per pass
+template <typename V, int W>
+__device__ inline int pass_column(int h)
+{
+    constexpr int CP = pass_cols<V, W>();
+    return h * CP + int(threadIdx.x & (CP - 1));
 }
 
 // partial sums of conj?(a_i) b_i, accumulated in T (operands converted)
@@ -37,22 +77,15 @@ __global__ __launch_bounds__(kBlock) void k_dot_blk(index_t n, const TA* __restr
                                                    T* __restrict__ partials)
 {
     __shared__ T red[kWavesPerBlock * W];
+    __shared__ T tile[kBlock * (pass_cols<T, W>() + 1)];
     T d[W];
 #pragma unroll
     for (int r = 0; r < W; ++r) d[r] = zero<T>();
-    constexpr int C = row_chunk<TA, W>() < row_chunk<TB, W>() ? row_chunk<TA, W>() : row_chunk<TB, W>();
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock) {
-#pragma unroll
-        for (int c0 = 0; c0 < W; c0 += C) {
-            TA av[C];
-            TB bv[C];
-            load_row<TA, C>(a + size_t(i) * W + c0, av);
-            load_row<TB, C>(b + size_t(i) * W + c0, bv);
-#pragma unroll
-            for (int r = 0; r < C; ++r)
-                d[c0 + r] = fma_(conj_if(cvt<T>(av[r]), conj != 0), cvt<T>(bv[r]), d[c0 + r]);
-        }
+    for (index_t base = index_t(blockIdx.x) * kBlock; base < n; base += index_t(gridDim.x) * kBlock) {
+        const index_t rows = n - base < kBlock ? n - base : kBlock;
+        tile_reduce<T, W>(base, rows, tile, d, [&](size_t e, int) {
+            return mul(conj_if(cvt<T>(a[e]), conj != 0), cvt<T>(b[e]));
+        });
     }
     const T t = block_sum_multi<T, W>(d, red);
     if (threadIdx.x < W) partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
@@ -66,25 +99,14 @@ __global__ __launch_bounds__(kBlock) void k_norms_blk(index_t n, const T* __rest
                                                      double* __restrict__ pb)
 {
     __shared__ double red[kWavesPerBlock * W];
+    __shared__ double tile[kBlock * (pass_cols<double, W>() + 1)];
     double sa[W], sb[W];
 #pragma unroll
     for (int r = 0; r < W; ++r) sa[r] = sb[r] = 0.0;
-    constexpr int C = row_chunk<T, W>();
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock) {
-#pragma unroll
-        for (int c0 = 0; c0 < W; c0 += C) {
-            T av[C];
-            load_row<T, C>(a + size_t(i) * W + c0, av);
-#pragma unroll
-            for (int r = 0; r < C; ++r) sa[c0 + r] += norm2(av[r]);
-            if (b) {
-                T bv[C];
-                load_row<T, C>(b + size_t(i) * W + c0, bv);
-#pragma unroll
-                for (int r = 0; r < C; ++r) sb[c0 + r] += norm2(bv[r]);
-            }
-        }
+    for (index_t base = index_t(blockIdx.x) * kBlock; base < n; base += index_t(gridDim.x) * kBlock) {
+        const index_t rows = n - base < kBlock ? n - base : kBlock;
+        tile_reduce<double, W>(base, rows, tile, sa, [&](size_t e, int) { return norm2(a[e]); });
+        if (b) tile_reduce<double, W>(base, rows, tile, sb, [&](size_t e, int) { return norm2(b[e]); });
     }
     double t = block_sum_multi<double, W>(sa, red);
     if (threadIdx.x < W) pa[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
@@ -106,41 +128,30 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_blk(index_t n, T* __restri
 {
     if (all_stopped<W>(st)) return;
     __shared__ double red[kWavesPerBlock * W];
-    bool run[W];
-    running_mask<W>(st, run);
-    T alpha[W];
+    constexpr int CP = pass_cols<double, W>();
+    __shared__ double tile[kBlock * (CP + 1)];
+    // the column a thread updates in each pass, its alpha and whether it runs
+    T alpha[W / CP];
+    bool run[W / CP];
+#pragma unroll
+    for (int h = 0; h < W / CP; ++h) {
+        const int c = pass_column<double, W>(h);
+        run[h] = st[c].status == 0;
+        alpha[h] = from_pair<T>(st[c].alpha);
+    }
     double s[W];
 #pragma unroll
-    for (int q = 0; q < W; ++q) {
-        alpha[q] = from_pair<T>(st[q].alpha);
-        s[q] = 0.0;
-    }
-    constexpr int C = row_chunk<T, W>();
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock) {
-#pragma unroll
-        for (int c0 = 0; c0 < W; c0 += C) {
-            const size_t at = size_t(i) * W + c0;
-            T xv[C], rv[C], pv[C], av[C];
-            P lv[C];
-            load_row<T, C>(x + at, xv);
-            load_row<T, C>(r + at, rv);
-            load_row<T, C>(p + at, pv);
-            load_row<T, C>(Ap + at, av);
-#pragma unroll
-            for (int k = 0; k < C; ++k) {
-                const int q = c0 + k;
-                if (run[q]) {
-                    xv[k] = fma_(alpha[q], pv[k], xv[k]);
-                    rv[k] = fms_(alpha[q], av[k], rv[k]);
-                    s[q] += norm2(rv[k]);
-                }
-                lv[k] = cvt<P>(rv[k]);
-            }
-            store_row<T, C>(x + at, xv);
-            store_row<T, C>(r + at, rv);
-            if (r_low) store_row<P, C>(r_low + at, lv);
-        }
+    for (int q = 0; q < W; ++q) s[q] = 0.0;
+    for (index_t base = index_t(blockIdx.x) * kBlock; base < n; base += index_t(gridDim.x) * kBlock) {
+        const index_t rows = n - base < kBlock ? n - base : kBlock;
+        tile_reduce<double, W>(base, rows, tile, s, [&](size_t e, int h) {
+            if (!run[h]) return 0.0;
+            x[e] = fma_(alpha[h], p[e], x[e]);
+            const T ri = fms_(alpha[h], Ap[e], r[e]);
+            r[e] = ri;
+            if (r_low) r_low[e] = cvt<P>(ri);
+            return norm2(ri);
+        });
     }
     const double t = block_sum_multi<double, W>(s, red);
     if (threadIdx.x < W) partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
@@ -159,43 +170,29 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
 {
     if (all_stopped<W>(st)) return;
     __shared__ double red[kWavesPerBlock * W];
-    bool run[W];
-    running_mask<W>(st, run);
-    T alpha[W];
+    constexpr int CP = pass_cols<double, W>();
+    __shared__ double tile[kBlock * (CP + 1)];
+    T alpha[W / CP];
+    bool run[W / CP];
+#pragma unroll
+    for (int h = 0; h < W / CP; ++h) {
+        const int c = pass_column<double, W>(h);
+        run[h] = st[c].status == 0;
+        alpha[h] = from_pair<T>(st[c].alpha);
+    }
     double s[W];
 #pragma unroll
-    for (int k = 0; k < W; ++k) {
-        alpha[k] = from_pair<T>(st[k].alpha);
-        s[k] = 0.0;
-    }
-    constexpr int C = row_chunk<T, W>();
-    for (index_t i = index_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += index_t(gridDim.x) * kBlock) {
-#pragma unroll
-        for (int c0 = 0; c0 < W; c0 += C) {
-            const size_t at = size_t(i) * W + c0;
-            T xv[C], rv[C], tv[C], pv[C], qv[C];
-            TQ qtv[C];
-            load_row<T, C>(x + at, xv);
-            load_row<T, C>(r + at, rv);
-            load_row<T, C>(rt + at, tv);
-            load_row<T, C>(p + at, pv);
-            load_row<T, C>(q + at, qv);
-            load_row<TQ, C>(qt + at, qtv);
-#pragma unroll
-            for (int k = 0; k < C; ++k) {
-                const int j = c0 + k;
-                if (run[j]) {
-                    xv[k] = fma_(alpha[j], pv[k], xv[k]);
-                    rv[k] = fms_(alpha[j], qv[k], rv[k]);
-                    tv[k] = fms_(alpha[j], cvt<T>(qtv[k]), tv[k]);
-                    s[j] += norm2(rv[k]);
-                }
-            }
-            store_row<T, C>(x + at, xv);
-            store_row<T, C>(r + at, rv);
-            store_row<T, C>(rt + at, tv);
-        }
+    for (int k = 0; k < W; ++k) s[k] = 0.0;
+    for (index_t base = index_t(blockIdx.x) * kBlock; base < n; base += index_t(gridDim.x) * kBlock) {
+        const index_t rows = n - base < kBlock ? n - base : kBlock;
+        tile_reduce<double, W>(base, rows, tile, s, [&](size_t e, int h) {
+            if (!run[h]) return 0.0;
+            x[e] = fma_(alpha[h], p[e], x[e]);
+            const T ri = fms_(alpha[h], q[e], r[e]);
+            r[e] = ri;
+            rt[e] = fms_(alpha[h], cvt<T>(qt[e]), rt[e]);
+            return norm2(ri);
+        });
     }
     const double t = block_sum_multi<double, W>(s, red);
     if (threadIdx.x < W) partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;

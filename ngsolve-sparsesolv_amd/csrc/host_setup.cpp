@@ -113,7 +113,9 @@ void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_
         Udev = &Up;
         block_size = 1;
     }
-    plan.spmv.build(plan.fac.A_perm);
+    // SpMV slices in the original row order (ordering[i]: device row of
+    // original row i)
+    plan.spmv.build(plan.fac.A_perm, plan.fac.sched.ordering);
     // the sweep layouts are built from the factor converted to F
     sparsesolv_amd::CsrHost<F> L, U;
     L.n = U.n = n;

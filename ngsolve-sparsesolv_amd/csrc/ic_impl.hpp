@@ -99,10 +99,10 @@ inline int rhs_width(int nrhs) { return nrhs <= 1 ? 1 : nrhs <= 4 ? 4 : nrhs <= 
 static_assert(sparsesolv_amd::kSweepChunk >= 16, "layout padding shorter than a batch");
 // Krylov iterations per captured graph
 constexpr int kGraphIterations = 8;
-// mixed precision: refinement passes per solve, and the residual reduction
-// a COCR pass aims for before the next restart
+// mixed precision: refinement passes per solve, and the iterations between
+// two residual replacements of mixed-precision COCR (one per captured group)
 constexpr int kRefinePasses = 8;
-constexpr double kInnerReduction = 1e-5;
+constexpr int kReplaceEvery = kGraphIterations;
 
 inline int vec_grid(index_t n)
 {
@@ -218,7 +218,7 @@ struct IcImpl final : IcBase {
     int hist_stride = 0;
 
     // device: operator, permutations, factor
-    DevBuf A_off, A_val, A_col, perm, rev, invd;
+    DevBuf A_off, A_val, A_col, A_row, perm, rev, invd;
     index_t A_nslices = 0;
     size_t spmv_bytes = 0;
     DevSweep fwd, bwd;
@@ -275,6 +275,7 @@ struct IcImpl final : IcBase {
         A_off.upload(plan.spmv.slice_off, s);
         A_val.upload(plan.spmv.val, s);
         A_col.upload(plan.spmv.col, s);
+        A_row.upload(plan.spmv.row, s);
         A_nslices = plan.spmv.nslices;
         spmv_bytes = plan.spmv.val.size() * sizeof(HT) + plan.spmv.col.size() * 4;
         perm.upload(plan.gather_out, s);
@@ -411,13 +412,14 @@ struct IcImpl final : IcBase {
         const ssa::Systems sy = systems();
         const index_t* off = A_off.as<index_t>();
         const index_t* col = A_col.as<index_t>();
+        const index_t* rows = A_row.as<index_t>();
         const T* val = A_val.as<T>();
         if (dot == 0)
-            ssa::k_spmv<T, 0, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, n, A_nslices, xin, y, part, st, ctrl, sy);
+            ssa::k_spmv<T, 0, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
         else if (dot == 1)
-            ssa::k_spmv<T, 1, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, n, A_nslices, xin, y, part, st, ctrl, sy);
+            ssa::k_spmv<T, 1, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
         else
-            ssa::k_spmv<T, 2, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, n, A_nslices, xin, y, part, st, ctrl, sy);
+            ssa::k_spmv<T, 2, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
     }
 
     ssa::SweepArrays sweep_arrays(DevSweep& D)
@@ -825,11 +827,27 @@ struct IcImpl final : IcBase {
         HIP_CHECK(hipStreamSynchronize(s));
     }
 
+    // Mixed-precision COCR carries rt = M^{-1} r by the recurrence
+    // rt -= alpha qt, and qt comes from the float preconditioner: the error
+    // of the recurrence grows until the iteration stalls (near 1e-7 of the
+    // starting residual).  Every kReplaceEvery iterations rt is therefore
+    // recomputed from the FP64 residual, rt = M^{-1} r (residual
+    // replacement); t = A rt follows, so q = A p stays exact.  qt (lo_b) is
+    // consumed by then, lo_t is scratch.
+    void replace_rt(const SolverState* st)
+    {
+        convert(r.as<T>(), lo_b.as<P>(), st);
+        launch_precond(lo_b.as<P>(), lo_b.as<P>(), lo_t.as<P>(), st);
+        convert(lo_b.as<P>(), z.as<T>(), st);
+    }
+    // whether iteration k of a pass (counted from 0) replaces rt
+    static bool replace_at(int k) { return k % kReplaceEvery == kReplaceEvery - 1; }
+
     // one Krylov iteration (kernels only; no host synchronisation)
-    void enqueue_iteration(int method, const SolveParams& prm)
+    void enqueue_iteration(int method, const SolveParams& prm, bool replace)
     {
         if (W > 1) {
-            enqueue_block_iteration(method, prm);
+            enqueue_block_iteration(method, prm, replace);
             return;
         }
         hipStream_t s = ctx->stream;
@@ -869,6 +887,7 @@ struct IcImpl final : IcBase {
             else
                 ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
             ssa::k_cocr_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
+            if (kMixed && pre && replace) replace_rt(st);
             launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G, false));
             ssa::k_cocr_pq<T, P><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
         }
@@ -879,7 +898,7 @@ struct IcImpl final : IcBase {
     // arithmetic per right-hand side, with each control step in its own
     // W-workgroup kernel between producer and consumer (same state updates in
     // the same order: alpha, check, beta).
-    void enqueue_block_iteration(int method, const SolveParams& prm)
+    void enqueue_block_iteration(int method, const SolveParams& prm, bool replace)
     {
         hipStream_t s = ctx->stream;
         SolverState* st = state.as<SolverState>();
@@ -916,6 +935,7 @@ struct IcImpl final : IcBase {
                 ssa::k_ctrl_alpha_blk<T><<<WV, CT, 0, s>>>(part_a.as<T>(), G, st, prm, H, hs);
                 ssa::k_cocr_update_blk<T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st);
                 ssa::k_ctrl_check_blk<<<WV, CT, 0, s>>>(part_b.as<double>(), G, st, prm, H, hs);
+                if (kMixed && pre && replace) replace_rt(st);
                 launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st);
                 ssa::k_ctrl_beta_blk<T><<<WV, CT, 0, s>>>(part_c.as<T>(), np_spmv, st, prm);
                 ssa::k_cocr_pq_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, low_copy());
@@ -1019,11 +1039,11 @@ struct IcImpl final : IcBase {
         hipGraphExec_t exec = nullptr;
         if (cfg.use_graph) {
             HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int u = 0; u < unroll; ++u) enqueue_iteration(method, prm);
+            for (int u = 0; u < unroll; ++u) enqueue_iteration(method, prm, replace_at(u));
             HIP_CHECK(hipStreamEndCapture(s, &graph));
             HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
-        int done = 0;
+        int done = 0, enqueued = 0;
         // runs while any state runs (a stopped state's kernels return at once)
         auto running = [&] {
             HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState) * S, hipMemcpyDeviceToHost, s));
@@ -1038,7 +1058,7 @@ struct IcImpl final : IcBase {
             const int groups = (k + unroll - 1) / unroll;
             for (int i = 0; i < groups; ++i) {
                 if (exec) HIP_CHECK(hipGraphLaunch(exec, s));
-                else enqueue_iteration(method, prm);
+                else enqueue_iteration(method, prm, replace_at(enqueued++));
             }
             done += groups * unroll;
             more = running();
@@ -1132,9 +1152,7 @@ struct IcImpl final : IcBase {
             // Mixed precision, iterative refinement, per state: the residual
             // is computed in FP64 from x, each pass runs the preconditioned
             // Krylov method from x until its residual falls below the
-            // tolerance or (COCR) below kInnerReduction times the residual the
-            // pass started from -- COCR carries M^{-1} r by a recurrence that,
-            // with a float preconditioner, stalls near float precision -- and
+            // tolerance (COCR with residual replacement, replace_rt), and
             // the next pass restarts from the new x.  A pass that ends
             // unconverged leaves its best iterate (when save_best_result); a
             // pass that does not lower the true residual is undone, so x and
@@ -1155,9 +1173,7 @@ struct IcImpl final : IcBase {
                     run[u] = active[u] && !done[u] && !(rel[u] < cfg.tolerance) &&
                              iters[u] < cfg.max_iterations;
                     any = any || run[u];
-                    const double tol = prm.cocr ? std::max(cfg.tolerance, kInnerReduction * rel[u])
-                                                : cfg.tolerance;
-                    h_pass[q] = ssa::PassParams{tol, cfg.max_iterations - iters[u], run[u] ? 1 : 0};
+                    h_pass[q] = ssa::PassParams{cfg.tolerance, cfg.max_iterations - iters[u], run[u] ? 1 : 0};
                 }
                 if (!any) break;
                 HIP_CHECK(hipMemcpyAsync(tmp.p, x.p, db, hipMemcpyDeviceToDevice, s));
@@ -1334,7 +1350,7 @@ struct IcImpl final : IcBase {
         o.shift_trials = shift_trials;
         o.actual_shift = actual_shift;
         o.setup_seconds = setup_seconds;
-        o.device_bytes = A_off.bytes + A_val.bytes + A_col.bytes + perm.bytes + rev.bytes +
+        o.device_bytes = A_off.bytes + A_val.bytes + A_col.bytes + A_row.bytes + perm.bytes + rev.bytes +
                          invd.bytes + fwd.bytes() + bwd.bytes();
         o.factor_bytes = fwd.stream_bytes;
         o.spmv_bytes = spmv_bytes;

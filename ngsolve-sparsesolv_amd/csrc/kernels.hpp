@@ -82,7 +82,8 @@ __device__ inline T block_sum_multi(const T (&v)[W], T* scratch)
 }
 
 // ---------------------------------------------------------------------------
-// SpMV over SELL-64 (one wavefront per 64-row slice), optional fused dot
+// SpMV over SELL-64 (one wavefront per 64-row slice; lane l of slice s
+// computes row rows[s*64 + l], -1: none), optional fused dot
 // DOT: 0 none, 1 sum x_i y_i, 2 sum conj(x_i) y_i
 //
 // W right-hand sides (interleaved rows): each lane loads a matrix entry once
@@ -94,7 +95,7 @@ __device__ inline T block_sum_multi(const T (&v)[W], T* scratch)
 template <typename T, int DOT, int W>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const index_t* __restrict__ slice_off, const T* __restrict__ val,
-    const index_t* __restrict__ col, index_t n, index_t nslices,
+    const index_t* __restrict__ col, const index_t* __restrict__ rows, index_t nslices,
     const T* __restrict__ x, T* __restrict__ y, T* __restrict__ partials,
     const SolverState* __restrict__ st, Ctrl check, Systems sy)
 {
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const index_t s = index_t(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
     const index_t sc = s < nslices ? s : nslices;
     const index_t o0 = slice_off[sc], o1 = s < nslices ? slice_off[s + 1] : o0;
+    const index_t row = s < nslices ? rows[size_t(s) * 64 + lane] : -1;
     if (!live) {
         // consuming the offset keeps its load ahead of the status test
         // (offsets are >= 0)
@@ -130,8 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 #pragma unroll
             for (int r = 0; r < W; ++r) acc[r] = fma_(v, g[r], acc[r]);
         }
-        const index_t row = s * 64 + lane;
-        if (row < n) {
+        if (row >= 0) {
             store_row<T, W>(y + size_t(row) * W, acc);
             if (DOT) {
                 T xr[W];
@@ -530,7 +531,8 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
 // Vector kernels (grid-stride; one partial per workgroup)
 // ---------------------------------------------------------------------------
 
-// partial sums of conj?(a_i) b_i, accumulated in T (operands converted)
+// partial sums of conj?(a_i) b_i, accumulated in T (operands converted;
+// each product rounded, then added, as the block kernel adds them)
 template <typename T, typename TA, typename TB>
 __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const TA* __restrict__ a,
                                                const TB* __restrict__ b, int conj,
@@ -543,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const TA* __restrict_
     __shared__ T red[kWavesPerBlock];
     T d = zero<T>();
     for (index_t i = sp.local * kBlock + threadIdx.x; i < sp.n; i += sp.G * kBlock)
-        d = fma_(conj_if(cvt<T>(a[sp.row0 + i]), conj != 0), cvt<T>(b[sp.row0 + i]), d);
+        d = add(d, mul(conj_if(cvt<T>(a[sp.row0 + i]), conj != 0), cvt<T>(b[sp.row0 + i])));
     T t = block_sum(d, red);
     if (threadIdx.x == 0) partials[sp.slot] = t;
     if (check.part) {
