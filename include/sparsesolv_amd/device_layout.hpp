@@ -34,6 +34,7 @@
 #include <cstdint>
 #include <numeric>
 #include <stdexcept>
+#include <cstring>
 #include <vector>
 
 #include "abmc_ordering.hpp"
@@ -45,9 +46,17 @@ constexpr int kWave = 64;
 // column of a padding slot in the sweep layouts: neither a row (>= 0) nor an
 // in-block position (-1 - position, position < 64)
 constexpr index_t kPadCol = -(1 << 30);
-// steps of the straggler walk prefetched ahead (the step lists are padded to
-// a multiple of it)
-constexpr int kWalkDepth = 8;
+// steps of the straggler walk prefetched ahead, at most (the step lists are
+// padded to a multiple of it, plus as many idle steps at the end; a kernel
+// prefetching 8 or 16 steps fits both)
+constexpr int kWalkDepth = 16;
+// one 16-byte word of a packed walk record
+struct WalkWord {
+    unsigned char b[16];
+};
+// 16-byte words of a walk record for kmax internal entries of fs-byte
+// values: kmax values, D^-1, then int32 row, position and kmax positions
+constexpr int walk_record_words(int kmax, int fs) { return ((kmax + 1) * fs + (2 + kmax) * 4 + 15) / 16; }
 // entries a sweep kernel loads per batch; the entry arrays end with this
 // many padding slots so a batch never needs a bounds guard
 constexpr int kSweepChunk = 16;
@@ -190,6 +199,14 @@ struct SweepLayout {
     std::vector<F> walk_invd;            // slots
     std::vector<F> walk_v;               // (step*kmax + k)*64 + lane
     std::vector<index_t> walk_ix;
+    // The same per-slot operands packed for the device (pack_walk): one
+    // record of walk_rec_words 16-byte words per slot -- the kmax values,
+    // D^-1, then int32 row, position and the kmax LDS positions -- stored
+    // word-major per step (word w of lane l of step s at (s*words + w)*64 +
+    // l), so each word of a step is one coalesced load and a step costs
+    // walk_rec_words loads instead of 2*kmax + 3.
+    std::vector<WalkWord> walk_rec;
+    int walk_rec_words = 0;
 
     index_t nslices() const { return phase_slices.empty() ? 0 : phase_slices.back(); }
 
@@ -459,7 +476,37 @@ private:
         if (walk_row.empty()) { walk_row.assign(1, -1); walk_pos.assign(1, -1); walk_invd.assign(1, F(0)); walk_ext_rp.push_back(0); }
         if (walk_ext_ci.empty()) { walk_ext_ci.push_back(0); walk_ext_v.push_back(F(0)); }
         if (walk_v.empty()) { walk_v.push_back(F(0)); walk_ix.push_back(0); }
+        pack_walk();
         (void)M; (void)forward;
+    }
+
+    void pack_walk()
+    {
+        const int K = walk_kmax;
+        walk_rec_words = K > 0 ? walk_record_words(K, int(sizeof(F))) : 0;
+        if (K == 0) {
+            walk_rec.assign(1, WalkWord{});
+            return;
+        }
+        const size_t W = size_t(walk_rec_words), nsteps = walk_row.size() / kWave;
+        const size_t FS = sizeof(F);
+        walk_rec.assign(nsteps * W * kWave, WalkWord{});
+        std::vector<unsigned char> rec(W * 16);
+        for (size_t st = 0; st < nsteps; ++st)
+            for (int l = 0; l < kWave; ++l) {
+                std::fill(rec.begin(), rec.end(), 0);
+                const size_t t = st * kWave + size_t(l);
+                for (int k = 0; k < K; ++k)
+                    std::memcpy(&rec[size_t(k) * FS], &walk_v[(st * size_t(K) + size_t(k)) * kWave + size_t(l)], FS);
+                std::memcpy(&rec[size_t(K) * FS], &walk_invd[t], FS);
+                const size_t ib = size_t(K + 1) * FS;
+                std::memcpy(&rec[ib], &walk_row[t], 4);
+                std::memcpy(&rec[ib + 4], &walk_pos[t], 4);
+                for (int k = 0; k < K; ++k)
+                    std::memcpy(&rec[ib + 8 + size_t(k) * 4], &walk_ix[(st * size_t(K) + size_t(k)) * kWave + size_t(l)], 4);
+                for (size_t w = 0; w < W; ++w)
+                    std::memcpy(walk_rec[(st * W + w) * kWave + size_t(l)].b, &rec[w * 16], 16);
+            }
     }
 
 public:
@@ -578,6 +625,7 @@ public:
         if (out.walk_row.empty()) { out.walk_row.assign(1, -1); out.walk_pos.assign(1, -1); out.walk_invd.assign(1, F(0)); out.walk_ext_rp.push_back(0); }
         if (out.walk_ext_ci.empty()) { out.walk_ext_ci.push_back(0); out.walk_ext_v.push_back(F(0)); }
         if (out.walk_v.empty()) { out.walk_v.push_back(F(0)); out.walk_ix.push_back(0); }
+        out.pack_walk();
         return out;
     }
 };

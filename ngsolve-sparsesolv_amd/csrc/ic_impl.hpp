@@ -126,7 +126,7 @@ struct DevSweep {
     DevBuf slice_sys, walk_sys;   // batches: system of every slice / oversized block
     bool batched = false;
     DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
-    DevBuf w_step_off, w_row, w_pos, w_ext_rp, w_ext_ci, w_ext_v, w_invd, w_v, w_ix, w_sa, w_sb;
+    DevBuf w_step_off, w_row, w_ext_rp, w_ext_ci, w_ext_v, w_rec, w_sa, w_sb;
     std::vector<index_t> phase_slices, phase_big, phase_rows;
     std::vector<int64_t> phase_entries;
     int BR = 1;
@@ -154,13 +154,10 @@ struct DevSweep {
         big_v.upload(L.big_v, s);
         w_step_off.upload(L.walk_step_off, s);
         w_row.upload(L.walk_row, s);
-        w_pos.upload(L.walk_pos, s);
         w_ext_rp.upload(L.walk_ext_rp, s);
         w_ext_ci.upload(L.walk_ext_ci, s);
         w_ext_v.upload(L.walk_ext_v, s);
-        w_invd.upload(L.walk_invd, s);
-        w_v.upload(L.walk_v, s);
-        w_ix.upload(L.walk_ix, s);
+        w_rec.upload(L.walk_rec, s);
         walk_kmax = L.walk_kmax;
         h_step_off = L.walk_step_off;
         walk_slots = L.walk_row.size();
@@ -189,8 +186,7 @@ struct DevSweep {
         return lrow.bytes + lpos.bytes + slot.bytes + val.bytes + col.bytes +
                big_pos.bytes + big_lvl_off.bytes + big_lvl.bytes + big_rows.bytes +
                big_rp.bytes + big_ci.bytes + big_v.bytes + w_step_off.bytes + w_row.bytes +
-               w_pos.bytes + w_ext_rp.bytes + w_ext_ci.bytes + w_ext_v.bytes + w_invd.bytes +
-               w_v.bytes + w_ix.bytes;
+               w_ext_rp.bytes + w_ext_ci.bytes + w_ext_v.bytes + w_rec.bytes;
     }
 };
 
@@ -236,6 +232,12 @@ struct IcImpl final : IcBase {
     // Mixed precision (P narrower than T): the preconditioner's input, output
     // and intermediate vectors in P.  In FP64 they alias the Krylov vectors.
     static constexpr bool kMixed = !std::is_same<T, P>::value;
+    // steps the straggler walk keeps in flight: the compiler waits for all
+    // of them at the start of each group, so deeper groups pay that round
+    // trip less often; complex<double> operands, and rows of 8 internal
+    // entries, take too many registers for 16 (those walks prefetch 8)
+    static constexpr int kWalkPrefetch = sizeof(P) > 8 ? 8 : 16;
+    static_assert(sparsesolv_amd::kWalkDepth % kWalkPrefetch == 0, "walk padding");
     DevBuf lo_a, lo_b, lo_t;
 
     SolverState* h_state = nullptr;  // pinned, one per state
@@ -283,6 +285,16 @@ struct IcImpl final : IcBase {
         invd.upload(plan.invd, s);
         fwd.upload(plan.fwd, s);
         bwd.upload(plan.bwd, s);
+        // the host packs walk records for values of sizeof(HF) bytes, the
+        // kernel reads them as F: same size, same layout
+        static_assert(sizeof(HF) == sizeof(F), "walk record value size");
+        for (const auto* L : {&plan.fwd, &plan.bwd}) {
+            const int K = L->walk_kmax;
+            const int expect = K == 2 ? ssa::walk_words<F, 2>() : K == 4 ? ssa::walk_words<F, 4>()
+                             : K == 8 ? ssa::walk_words<F, 8>() : 0;
+            if (L->walk_rec_words != expect)
+                throw SsaError(SSA_ERR_RUNTIME, "walk record layout mismatch");
+        }
         if (count > 1) {
             // each system keeps the grid a one-system solve would use, so its
             // reductions give the same bits
@@ -591,13 +603,12 @@ struct IcImpl final : IcBase {
             const size_t lds = (size_t(D.max_big_rows) + 2) * sizeof(P);
 #define SSA_WALK(KM, DEPTH)                                                                    \
     ssa::k_sweep_walk<P, F, FWD, KM, DEPTH><<<dim3(unsigned(b1 - b0), unsigned(W)), 64, lds, s>>>( \
-        b0, D.w_step_off.as<index_t>(), D.w_row.as<index_t>(), D.w_pos.as<index_t>(),         \
-        D.w_invd.as<F>(), D.w_v.as<F>(), D.w_ix.as<index_t>(), D.max_big_rows, out,           \
+        b0, D.w_step_off.as<index_t>(), D.w_rec.as<uint4>(), D.max_big_rows, out,              \
         D.w_sa.as<P>(), D.w_sb.as<P>(), W, D.walk_slots, st,                                  \
         D.batched ? D.walk_sys.as<int>() : nullptr)
             switch (D.walk_kmax) {
-            case 2: SSA_WALK(2, 8); break;
-            case 4: SSA_WALK(4, 8); break;
+            case 2: SSA_WALK(2, kWalkPrefetch); break;
+            case 4: SSA_WALK(4, kWalkPrefetch); break;
             default: SSA_WALK(8, 8); break;
             }
 #undef SSA_WALK
@@ -627,11 +638,11 @@ struct IcImpl final : IcBase {
         D.walk_fits = need <= size_t(optin);
         if (D.walk_fits && need > 64 * 1024) {
             const int bytes = int(need);
-            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 2, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 4, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 2, kWalkPrefetch>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 4, kWalkPrefetch>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
             HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, true, 8, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 2, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 4, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 2, kWalkPrefetch>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 4, kWalkPrefetch>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
             HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ssa::k_sweep_walk<P, F, false, 8, 8>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         }
         return D.walk_fits;

@@ -458,23 +458,26 @@ __global__ __launch_bounds__(1024) void k_sweep_big(
 // per (block, right-hand side r = blockIdx.y); r's walk slots start at r*ws
 // and its rows are out[i*es + r].
 // ---------------------------------------------------------------------------
+// a step's operands: the external part (and, backward, the input) from the
+// phase kernel, and the slot's packed record of R 16-byte words (K values,
+// D^-1, int32 row, position and K LDS positions; word w of lane l of step s
+// at rec[(s*R + w)*64 + l])
+template <typename F, int KMAX>
+constexpr int walk_words() { return int(((KMAX + 1) * sizeof(F) + (2 + KMAX) * 4 + 15) / 16); }
+
 template <typename P, typename F, int KMAX>
 struct WalkBuf {
     P a, b;
-    F d;
-    index_t row, pos;
-    F v[KMAX];
-    index_t ix[KMAX];
+    uint4 w[walk_words<F, KMAX>()];
 };
 
 template <typename P, typename F, bool FWD, int KMAX, int D>
 __global__ __launch_bounds__(64) void k_sweep_walk(
-    index_t b0, const index_t* __restrict__ step_off, const index_t* __restrict__ wrow,
-    const index_t* __restrict__ wpos, const F* __restrict__ winvd, const F* __restrict__ wv,
-    const index_t* __restrict__ wix, index_t zero_pos, P* out, const P* __restrict__ sa,
-    const P* __restrict__ sb, index_t es, size_t ws, const SolverState* __restrict__ st,
-    const int* __restrict__ walk_sys)
+    index_t b0, const index_t* __restrict__ step_off, const uint4* __restrict__ rec,
+    index_t zero_pos, P* out, const P* __restrict__ sa, const P* __restrict__ sb, index_t es,
+    size_t ws, const SolverState* __restrict__ st, const int* __restrict__ walk_sys)
 {
+    constexpr int R = walk_words<F, KMAX>();
     const index_t bb = b0 + index_t(blockIdx.x);
     const int r = int(blockIdx.y);
     out += r;
@@ -484,26 +487,20 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
     extern __shared__ unsigned char lds_raw[];
     P* lds = reinterpret_cast<P*>(lds_raw);   // [0, zero_pos): values; zero_pos: 0; +1: idle lanes
     const index_t s0 = step_off[bb], s1 = step_off[bb + 1];   // s1 - s0 is a multiple of D
-    const int lane = threadIdx.x;
+    const unsigned ul = threadIdx.x;
     const index_t trash = zero_pos + 1;
-    if (lane == 0) lds[zero_pos] = zero<P>();
+    if (ul == 0) lds[zero_pos] = zero<P>();
     __syncthreads();
     WalkBuf<P, F, KMAX> buf[D];
     // the step arrays extend D idle steps past the last block, so every
     // prefetch below is in bounds and none needs a guard
     auto load = [&](index_t s, WalkBuf<P, F, KMAX>& B) {
-        const size_t t = size_t(s) * 64 + lane;
-        B.a = sa[t];
-        B.b = FWD ? zero<P>() : sb[t];
-        B.d = winvd[t];
-        B.row = wrow[t];
-        B.pos = wpos[t];
+        const size_t t0 = size_t(s) * 64;
+        B.a = (sa + t0)[ul];
+        B.b = FWD ? zero<P>() : (sb + t0)[ul];
+        const uint4* q = rec + size_t(s) * R * 64;
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            const size_t at = (size_t(s) * KMAX + k) * 64 + lane;
-            B.v[k] = wv[at];
-            B.ix[k] = wix[at];
-        }
+        for (unsigned w = 0; w < unsigned(R); ++w) B.w[w] = q[w * 64u + ul];
     };
 #pragma unroll
     for (int j = 0; j < D; ++j) load(s0 + j, buf[j]);
@@ -511,18 +508,35 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             WalkBuf<P, F, KMAX>& B = buf[j];
-            P acc = B.a;
+            // the record's fields (register moves)
+            F v[KMAX], d;
+            index_t ints[2 + KMAX];
+            __builtin_memcpy(v, B.w, sizeof(v));
+            __builtin_memcpy(&d, reinterpret_cast<const unsigned char*>(B.w) + KMAX * sizeof(F), sizeof(F));
+            __builtin_memcpy(ints, reinterpret_cast<const unsigned char*>(B.w) + (KMAX + 1) * sizeof(F), sizeof(ints));
+            const index_t row = ints[0], pos = ints[1];
+            // the chain of the walk is LDS write -> next step's LDS reads:
+            // issue this step's reads first, then refill the slot the
+            // previous step consumed while they are in flight (step
+            // s + j - 1 + D; in the first group that slot still holds its
+            // own step and is reloaded unchanged; the last refills reach
+            // D - 2 steps past the block, inside the padded step arrays)
+            P g[KMAX];
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) acc = fms_(cvt<P>(B.v[k]), lds[B.ix[k]], acc);
-            const P dinv = cvt<P>(B.d);
-            const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, B.b);
+            for (int k = 0; k < KMAX; ++k) g[k] = lds[ints[2 + k]];
+            const P a = B.a, xb = B.b;
+            load(s + j - 1 + D, buf[(j + D - 1) % D]);
+            P acc = a;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) acc = fms_(cvt<P>(v[k]), g[k], acc);
+            const P dinv = cvt<P>(d);
+            const P res = FWD ? mul(acc, dinv) : fma_(acc, dinv, xb);
             // Later steps read these LDS values from other lanes.  One
             // wavefront's LDS operations execute in order, and the compiler
             // keeps a possibly aliasing LDS read after this write.
-            lds[B.pos >= 0 ? B.pos : trash] = res;
-            if (B.pos >= 0) out[size_t(B.row) * es] = res;
+            lds[pos >= 0 ? pos : trash] = res;
+            if (pos >= 0) out[size_t(row) * es] = res;
             __builtin_amdgcn_wave_barrier();
-            load(s + j + D, B);
         }
     }
 }
