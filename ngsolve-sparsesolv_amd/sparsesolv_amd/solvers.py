@@ -126,11 +126,16 @@ class _ICPreconditioner:
             self.Update()
         return self._handle
 
+    def _apply(self, x):
+        x = np.asarray(x, dtype=self._dtype)
+        # a (k, n) array: k vectors in one block application
+        return self.handle.apply_multi(x) if x.ndim == 2 else self.handle.apply(x)
+
     def Mult(self, x, y=None):
         """y = M^{-1} x, constrained entries zeroed (Mult, :123-136)."""
-        out = self.handle.apply(np.asarray(x, dtype=self._dtype))
+        out = self._apply(x)
         if self.freedofs is not None:
-            out[~self.freedofs] = 0
+            out[..., ~self.freedofs] = 0
         if y is not None:
             y[...] = out
             return y
@@ -138,9 +143,9 @@ class _ICPreconditioner:
 
     def MultAdd(self, s, x, y):
         """y += s * M^{-1} x on free entries (MultAdd, :138-160)."""
-        out = self.handle.apply(np.asarray(x, dtype=self._dtype))
+        out = self._apply(x)
         if self.freedofs is not None:
-            out[~self.freedofs] = 0
+            out[..., ~self.freedofs] = 0
         y += s * out
         return y
 
@@ -294,18 +299,52 @@ class _SparseSolvSolver:
         self._last_result = result
         return result
 
+    def SolveBlock(self, rhs, sol) -> list:
+        """Several right-hand sides at once: rows of rhs and sol (shape
+        (k, n)), sol holding the initial guesses; row i of the result is what
+        Solve(rhs[i], sol[i]) gives, bit for bit.  GPU extension (the
+        reference solves one vector per call): the block's SpMV and sweeps
+        load each matrix and factor entry once for up to 16 rows."""
+        code = self._method_code()
+        if code not in (_lib.METHOD_ICCG, _lib.METHOD_ICCOCR, _lib.METHOD_CG, _lib.METHOD_COCR):
+            raise RuntimeError(f"SparseSolvSolver.SolveBlock: method {self.method} runs one vector")
+        cfg = self._config()
+        B = np.array(rhs, dtype=self._dtype, ndmin=2)
+        X0 = np.array(sol, dtype=self._dtype, ndmin=2)
+        if B.shape != X0.shape or B.shape[1] != self.mat.n:
+            raise ValueError(f"SolveBlock: rhs and sol must both be (k, {self.mat.n})")
+        if self.freedofs is not None:
+            B = np.where(self.freedofs, B, 0)
+            X0 = np.where(self.freedofs, X0, 0)
+        if code in (_lib.METHOD_CG, _lib.METHOD_COCR) and not cfg.use_abmc:
+            cfg = _lib.default_config(**{k: getattr(cfg, k) for k, _ in _lib.Config._fields_})
+            cfg.use_abmc = 1
+        h = self._get_handle(cfg)
+        X, res = h.solve_multi(code, B, X0, cfg)
+        if self.freedofs is not None:
+            sol[:, self.freedofs] = X[:, self.freedofs]
+        else:
+            sol[...] = X
+        results = [SparseSolvResult(r.converged, r.iterations, r.final_residual, []) for r in res]
+        self._last_result = results[-1]
+        return results
+
     def Mult(self, x, y=None):
-        """y = A^{-1} x with a zero initial guess (Mult, :60-63)."""
-        out = np.zeros(self.mat.n, dtype=self._dtype)
-        self.Solve(x, out)
+        """y = A^{-1} x with a zero initial guess (Mult, :60-63); a (k, n)
+        array is solved as a block of k right-hand sides."""
+        x = np.asarray(x)
+        out = np.zeros(x.shape if x.ndim == 2 else self.mat.n, dtype=self._dtype)
+        if x.ndim == 2:
+            self.SolveBlock(x, out)
+        else:
+            self.Solve(x, out)
         if y is not None:
             y[...] = out
             return y
         return out
 
     def MultAdd(self, s, x, y):
-        tmp = np.zeros(self.mat.n, dtype=self._dtype)
-        self.Solve(x, tmp)
+        tmp = self.Mult(x)
         y += s * tmp
         return y
 

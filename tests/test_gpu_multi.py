@@ -173,3 +173,28 @@ def test_multi_rejections():
     hb = _lib.BatchHandle([A, varcoef(5)], cfg)
     with pytest.raises(_lib.SsaError):
         hb.solve_multi(_lib.METHOD_ICCG, np.zeros((2, hb.n)), None, cfg)
+
+
+def test_python_front_end_block_solve():
+    # SparseSolvSolver.SolveBlock / Mult on a (k, n) array and the
+    # preconditioner's Mult on a block: each row as the one-vector call gives it
+    import sparsesolv_amd as ss
+    s = P.eddy(5)
+    B = rhs_block(s.A.n, 3, 160, complex_=True)
+    solver = ss.SparseSolvSolver(s.A, method="COCR", tol=1e-10, maxiter=400)
+    solver.use_abmc = True
+    solver.abmc_num_colors = 8
+    X = np.zeros_like(B)
+    results = solver.SolveBlock(B, X)
+    assert len(results) == 3
+    for k in range(3):
+        xk = np.zeros(s.A.n, dtype=complex)
+        rk = solver.Solve(B[k], xk)
+        assert np.array_equal(X[k], xk)
+        assert results[k].iterations == rk.iterations
+    assert np.array_equal(solver * B, X)
+    pre = ss.ICPreconditioner(s.A)
+    pre.use_abmc = True
+    Y = pre * B
+    for k in range(3):
+        assert np.array_equal(Y[k], pre * B[k])
