@@ -142,15 +142,40 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         T acc[W];
 #pragma unroll
         for (int r = 0; r < W; ++r) acc[r] = zero<T>();
-        constexpr int kUnroll = W >= 8 ? 1 : W >= 4 ? 2 : 4;
-#pragma unroll kUnroll
-        for (index_t k = o0; k < o1; ++k) {
-            const size_t at = size_t(k) * 64 + lane;
-            const T v = val[at];
-            T g[W];
-            load_row<T, W>(x + size_t(col[at]) * W, g);
+        if constexpr (W == 1) {
+#pragma unroll 4
+            for (index_t k = o0; k < o1; ++k) {
+                const size_t at = size_t(k) * 64 + lane;
+                const T v = val[at];
+                T g[W];
+                load_row<T, W>(x + size_t(col[at]) * W, g);
 #pragma unroll
-            for (int r = 0; r < W; ++r) acc[r] = fma_(v, g[r], acc[r]);
+                for (int r = 0; r < W; ++r) acc[r] = fma_(v, g[r], acc[r]);
+            }
+        } else {
+            // one entry's W gathered values fill the registers: the next
+            // entry's column and value load while this entry's gathers are
+            // in flight, so an entry costs one memory round trip, not two
+            // (K is uniform over the slice, so the guard is a scalar branch)
+            index_t cn = 0;
+            T vn = zero<T>();
+            if (o0 < o1) {
+                cn = col[size_t(o0) * 64 + lane];
+                vn = val[size_t(o0) * 64 + lane];
+            }
+            for (index_t k = o0; k < o1; ++k) {
+                const index_t c = cn;
+                const T v = vn;
+                if (k + 1 < o1) {
+                    const size_t at = size_t(k + 1) * 64 + lane;
+                    cn = col[at];
+                    vn = val[at];
+                }
+                T g[W];
+                load_row<T, W>(x + size_t(c) * W, g);
+#pragma unroll
+                for (int r = 0; r < W; ++r) acc[r] = fma_(v, g[r], acc[r]);
+            }
         }
         if (row >= 0) {
             store_row<T, W>(y + size_t(row) * W, acc);
@@ -313,6 +338,27 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
             const index_t cj = c[j] >= 0 ? c[j] : 0;
             load_row<P, W>(out + size_t(cj) * W, g[j]);
         }
+        // With several right-hand sides the gathered values fill the
+        // registers and a slice takes several chunks: the next chunk's
+        // entries load while this chunk's gathers are in flight (one memory
+        // round trip per chunk instead of two).  The guard is uniform over
+        // the slice.
+        const index_t k1 = k0 + CH;
+        index_t cn[CH];
+        F vn[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            cn[j] = kPadColumn;
+            vn[j] = zero<F>();
+        }
+        if (W > 1 && k1 < o1) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const size_t at = (size_t(k1) + j) * 64 + lane;
+                cn[j] = col[at];
+                vn[j] = val[at];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
             const P vj = cvt<P>(v[j]);
@@ -326,13 +372,18 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
 #pragma unroll
             for (int q = 0; q < BR - 1; ++q) intv[q] = select_(c[j] == -1 - q, v[j], intv[q]);
         }
-        k0 += CH;
+        k0 = k1;
         if (k0 >= o1) break;
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
-            const size_t at = (size_t(k0) + j) * 64 + lane;
-            c[j] = col[at];
-            v[j] = val[at];
+            if (W > 1) {
+                c[j] = cn[j];
+                v[j] = vn[j];
+            } else {
+                const size_t at = (size_t(k0) + j) * 64 + lane;
+                c[j] = col[at];
+                v[j] = val[at];
+            }
         }
     }
     if (kLateIn) load_row<P, W>(in + size_t(rr) * W, xin);
