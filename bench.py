@@ -563,16 +563,39 @@ def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0):
         if x_gpu is not None:
             out["gpu_vs_cpu_rel_diff"] = float(np.linalg.norm(x_gpu - x) /
                                                max(np.linalg.norm(x), 1e-300))
-        return out
-    iters = int(max(probe, min(gpu_iters, (budget_s - setup) / per_it)))
-    t_it, setup = orc.time_iterations(A, b, cfg, iters, mname)
-    per_it = t_it / iters
-    out.update(value=round(per_it * gpu_iters, 4), setup_s=round(setup, 3),
-               sample=(f"setup + {iters} {mname} iterations of the same system timed "
-                       f"({setup:.2f}s + {t_it:.2f}s), extrapolated to the GPU's {gpu_iters} "
-                       f"iterations; value excludes setup"),
-               ms_per_iteration=round(per_it * 1e3, 3))
+    else:
+        iters = int(max(probe, min(gpu_iters, (budget_s - setup) / per_it)))
+        t_it, setup = orc.time_iterations(A, b, cfg, iters, mname)
+        per_it = t_it / iters
+        out.update(value=round(per_it * gpu_iters, 4), setup_s=round(setup, 3),
+                   sample=(f"setup + {iters} {mname} iterations of the same system timed "
+                           f"({setup:.2f}s + {t_it:.2f}s), extrapolated to the GPU's {gpu_iters} "
+                           f"iterations; value excludes setup"),
+                   ms_per_iteration=round(per_it * 1e3, 3))
+    if kind == "reference" and out["cores"] > 1:
+        out["single_thread"] = cpu_single_thread(orc, A, b, cfg, mname,
+                                                 out.get("iterations", gpu_iters), out["cores"],
+                                                 budget_s=0.4 * budget_s)
     return out
+
+
+def cpu_single_thread(orc, A, b, cfg, mname, gpu_iters, threads, budget_s):
+    """The same reference build on one thread: setup + a bounded number of
+    iterations, extrapolated to the GPU's iteration count."""
+    orc.set_threads(1)
+    try:
+        t_probe, setup = orc.time_iterations(A, b, cfg, 2, mname)
+        per_it = max(t_probe / 2, 1e-6)
+        iters = int(max(2, min(gpu_iters, (budget_s - setup) / per_it)))
+        t_it, setup = orc.time_iterations(A, b, cfg, iters, mname)
+    finally:
+        orc.set_threads(threads)
+    per_it = t_it / iters
+    return {"cores": 1, "value": round(per_it * gpu_iters, 4), "setup_s": round(setup, 3),
+            "ms_per_iteration": round(per_it * 1e3, 3),
+            "sample": (f"setup + {iters} {mname} iterations on one thread "
+                       f"({setup:.2f}s + {t_it:.2f}s), extrapolated to {gpu_iters} iterations; "
+                       f"value excludes setup")}
 
 
 def recorded_traffic(workload):

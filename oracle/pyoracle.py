@@ -169,6 +169,11 @@ class Oracle:
     def num_threads(self) -> int:
         return self.lib.orc_num_threads() if self.has_threads else 1
 
+    def set_threads(self, n: int) -> None:
+        """Threads of the library's later OpenMP regions (OpenMP builds only)."""
+        if hasattr(self.lib, "orc_set_threads"):
+            self.lib.orc_set_threads(int(n))
+
 
 class IC:
     """An IC(0) preconditioner built by the checker library."""

@@ -409,4 +409,14 @@ int orc_num_threads(void)
 #endif
 }
 
+// threads of the following parallel regions (OpenMP build; no effect otherwise)
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 }  // extern "C"

@@ -29,3 +29,18 @@ def test_bytes_per_iteration_model():
     # the same system with a complex<float> factor and sweeps
     assert bench.bytes_per_iteration(n, nnz, (nnz + n) // 2, 16, True, 8) == \
         1_267_820_412 - 2 * ((nnz + n) // 2) * 8 - n * (2 * 8 + 8)
+
+
+def test_cpu_baseline_reports_one_thread_beside_all_threads():
+    import pytest
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if not pyoracle.available("ref_omp"):
+        pytest.skip("the reference's OpenMP build (oracle/_ref/libssref_omp.so) is not present")
+    from sparsesolv_amd import problems as P
+    s = P.poisson(12)
+    out = bench.cpu_baseline(s.A, s.b, dict(use_abmc=1), "iccg", 30, budget_s=5.0)
+    assert out["kind"] == "reference" and out["value"] > 0
+    if out["cores"] > 1:
+        one = out["single_thread"]
+        assert one["cores"] == 1 and one["value"] > 0 and one["ms_per_iteration"] > 0
