@@ -576,6 +576,12 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
     };
 #pragma unroll
     for (int j = 0; j < D; ++j) load(s0 + j, buf[j]);
+    // One full wait here, once: the compiler orders the loads above as it
+    // likes, and the wait counts at the loop head merge this entry with the
+    // back edge -- without it, every group of D steps would wait for nearly
+    // all of its prefetches (the slot the first step reads may be the last
+    // one loaded here).
+    __builtin_amdgcn_s_waitcnt(0);
     for (index_t s = s0; s < s1; s += D) {
 #pragma unroll
         for (int j = 0; j < D; ++j) {
