@@ -84,3 +84,20 @@ def test_layouts_level_path(tmp_path, name, builder):
     # device rows in level order and the factor in the original order
     info = run_check(tmp_path, builder(), abmc=False)
     assert info["BR"] == 1 and info["phases"] > 4
+
+
+@pytest.mark.parametrize("abmc", [True, False], ids=["abmc", "level"])
+@pytest.mark.parametrize("name", ["n1", "n1c", "diag100", "tri300", "n5"])
+def test_layouts_degenerate_shapes(tmp_path, name, abmc):
+    # one row; no off-diagonal entries (on the ABMC path the stragglers form
+    # one oversized block of a single level); fewer rows than a wavefront
+    import scipy.sparse as sp
+    M = {"n1": lambda: sp.csr_matrix(np.array([[4.0]])),
+         "n1c": lambda: sp.csr_matrix(np.array([[4.0 + 1.0j]])),
+         "diag100": lambda: sp.diags(np.linspace(1.0, 5.0, 100)).tocsr(),
+         "tri300": lambda: sp.diags([-np.ones(299), 2.0 * np.ones(300), -np.ones(299)],
+                                    [-1, 0, 1]).tocsr(),
+         "n5": lambda: sp.csr_matrix(np.eye(5) * 4.0 - np.eye(5, k=1) - np.eye(5, k=-1))}[name]()
+    info = run_check(tmp_path, P.CSR.from_scipy(M), abmc=abmc)
+    if name == "diag100" and abmc:
+        assert info["big"] == 1 and info["big_rows"] == 76
