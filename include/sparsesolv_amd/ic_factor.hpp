@@ -16,15 +16,25 @@
 // The arithmetic is the reference's, term for term and in the same order,
 // so a real factor is bit-identical when both are compiled without fused
 // multiply-add (this header is compiled with -ffp-contract=off).  The search
-// for L(j,k) walks row j against a scatter of row i instead of the
-// reference's linear search; it visits the same k in the same order.
+// for L(j,k) walks row j against row i (a merge of the two sorted column
+// lists on the ABMC branch, a scatter of row i on the level branch) instead
+// of the reference's linear search; it visits the same k in the same order.
+//
+// Host threads: the rows of the permuted matrix, and on the ABMC branch the
+// blocks of one colour (independent, as in the reference's colour-parallel
+// loop), are split over up to 16 threads.  Every row computes the same
+// values in the same order whichever thread runs it.
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <exception>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "abmc_ordering.hpp"
@@ -57,6 +67,49 @@ struct CsrHost {
     std::vector<T> v;
 };
 
+// Threads of the host setup's parallel loops: up to 16, fewer when the
+// calling thread lowers its cap (a pool that already plans several systems
+// side by side sets 1 in its workers).
+inline int& setup_thread_cap()
+{
+    static thread_local int cap = 16;
+    return cap;
+}
+
+inline int setup_threads()
+{
+    const unsigned h = std::thread::hardware_concurrency();
+    return std::max(1, std::min(setup_thread_cap(), int(std::min(16u, std::max(1u, h)))));
+}
+
+// fn(lo, hi) over [0, count) in contiguous chunks of at least `grain`, one
+// chunk per thread.  An exception of the lowest chunk that throws is
+// rethrown (the serial loop's first failure).
+template <typename Fn>
+inline void parallel_chunks(index_t count, index_t grain, Fn fn)
+{
+    const index_t nt = std::max<index_t>(
+        1, std::min<index_t>(setup_threads(), count / std::max<index_t>(grain, 1)));
+    if (nt <= 1) {
+        fn(index_t(0), count);
+        return;
+    }
+    std::vector<std::exception_ptr> err(static_cast<size_t>(nt));
+    std::vector<std::thread> pool;
+    pool.reserve(size_t(nt));
+    for (index_t t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            try {
+                fn(index_t(int64_t(count) * t / nt), index_t(int64_t(count) * (t + 1) / nt));
+            } catch (...) {
+                err[size_t(t)] = std::current_exception();
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
 // B[perm i][perm j] = A[i][j], each row sorted by column (unique columns).
 template <typename T>
 CsrHost<T> permute_csr(index_t n, const index_t* rp, const index_t* ci, const T* v,
@@ -69,16 +122,18 @@ CsrHost<T> permute_csr(index_t n, const index_t* rp, const index_t* ci, const T*
     for (index_t i = 0; i < n; ++i) out.rp[i + 1] += out.rp[i];
     out.ci.resize(out.rp[n]);
     out.v.resize(out.rp[n]);
-    std::vector<std::pair<index_t, index_t>> tmp;
-    for (index_t i = 0; i < n; ++i) {
-        const index_t ni = perm[i];
-        tmp.clear();
-        for (index_t k = rp[i]; k < rp[i + 1]; ++k) tmp.emplace_back(perm[ci[k]], k);
-        std::sort(tmp.begin(), tmp.end(),
-                  [](const auto& a, const auto& b) { return a.first < b.first; });
-        index_t pos = out.rp[ni];
-        for (const auto& e : tmp) { out.ci[pos] = e.first; out.v[pos] = v[e.second]; ++pos; }
-    }
+    parallel_chunks(n, 4096, [&](index_t lo, index_t hi) {
+        std::vector<std::pair<index_t, index_t>> tmp;
+        for (index_t i = lo; i < hi; ++i) {
+            const index_t ni = perm[i];
+            tmp.clear();
+            for (index_t k = rp[i]; k < rp[i + 1]; ++k) tmp.emplace_back(perm[ci[k]], k);
+            std::sort(tmp.begin(), tmp.end(),
+                      [](const auto& a, const auto& b) { return a.first < b.first; });
+            index_t pos = out.rp[ni];
+            for (const auto& e : tmp) { out.ci[pos] = e.first; out.v[pos] = v[e.second]; ++pos; }
+        }
+    });
     return out;
 }
 
@@ -157,6 +212,60 @@ private:
                          static_cast<T>(scaling[L.ci[k]]);
     }
 
+    // Off-diagonal entries of row i (ABMC branch): L(j,k) is looked up by
+    // merging row j with row i, both sorted by column (the permuted matrix
+    // sorts every row), which visits the same k in the same order as the
+    // scatter of row i in the level branch below.
+    void update_row_merge(index_t i)
+    {
+        const index_t rs = L.rp[i], re = L.rp[i + 1];
+        for (index_t kk = rs; kk < re - 1; ++kk) {
+            const index_t j = L.ci[kk];
+            if (j >= i) break;
+            T s = L.v[kk];
+            index_t p = rs;
+            // terms k < j common to rows i and j, ascending k
+            for (index_t jj = L.rp[j]; jj < L.rp[j + 1]; ++jj) {
+                const index_t k = L.ci[jj];
+                if (k >= j) break;
+                while (p < kk && L.ci[p] < k) ++p;
+                if (p < kk && L.ci[p] == k) s -= L.v[p] * L.v[jj] * inv_diag[k];
+            }
+            L.v[kk] = s;
+        }
+    }
+
+    // Pivot of row i: d_i = shift * a_ii - sum L_ik^2 / d_k, stored with
+    // its inverse (clamped).  Returns whether it falls under the auto-shift
+    // threshold (the caller decides about the restart).
+    bool pivot_row(index_t i, double shift, const IcSettings& cfg, bool clamp_on_breakdown)
+    {
+        const index_t rs = L.rp[i], re = L.rp[i + 1];
+        const index_t dp = re - 1;
+        if (dp < rs || L.ci[dp] != i)
+            throw std::runtime_error("IC decomposition failed: missing diagonal at row " +
+                                     std::to_string(i));
+        const T orig = L.v[dp];
+        T s = orig * static_cast<T>(shift);
+        for (index_t kk = rs; kk < dp; ++kk) {
+            const index_t k = L.ci[kk];
+            if (k >= i) break;
+            s -= L.v[kk] * L.v[kk] * inv_diag[k];
+        }
+        L.v[dp] = s;
+        const double abs_s = abs_of(s);
+        const bool low = cfg.auto_shift && abs_s < cfg.min_diagonal_threshold && abs_of(orig) > 0.0;
+        if (low && !clamp_on_breakdown) return true;
+        if (abs_s > cfg.zero_diagonal_replacement) {
+            inv_diag[i] = T(1) / s;
+        } else {
+            const T safe = static_cast<T>(cfg.zero_diagonal_replacement);
+            L.v[dp] = safe;
+            inv_diag[i] = T(1) / safe;
+        }
+        return low;
+    }
+
     void factor(const IcSettings& cfg, const std::vector<T>& original)
     {
         inv_diag.assign(n, T(0));
@@ -165,18 +274,38 @@ private:
         int trials = 0;
         const bool abmc = sched.built();
         const index_t nc = abmc ? sched.num_colors() : 1;
-        // where[c] = position of column c in the current row i, or -1
-        std::vector<index_t> where(n, -1);
         bool restart = true;
         while (restart) {
             restart = false;
             if (trials > 0) L.v = original;
             if (cfg.diagonal_scaling) scale_into(original);
-            for (index_t c = 0; c < nc && !restart; ++c) {
-                const index_t r0 = abmc ? sched.block_offsets[sched.color_offsets[c]] : 0;
-                const index_t r1 = abmc ? sched.block_offsets[sched.color_offsets[c + 1]] : n;
-                bool breakdown = false;
-                for (index_t i = r0; i < r1; ++i) {
+            if (abmc) {
+                // the blocks of a colour are independent: split over threads;
+                // a pivot under the threshold restarts after the colour
+                for (index_t c = 0; c < nc && !restart; ++c) {
+                    const index_t b0 = sched.color_offsets[c], b1 = sched.color_offsets[c + 1];
+                    std::atomic<bool> low{false};
+                    parallel_chunks(b1 - b0, 64, [&](index_t lo, index_t hi) {
+                        bool any = false;
+                        for (index_t b = b0 + lo; b < b0 + hi; ++b)
+                            for (index_t i = sched.block_offsets[b]; i < sched.block_offsets[b + 1]; ++i) {
+                                update_row_merge(i);
+                                any |= pivot_row(i, shift, cfg, true);
+                            }
+                        if (any) low.store(true);
+                    });
+                    const bool breakdown = low.load();
+                    if (breakdown && shift < cfg.max_shift_value && trials < cfg.max_shift_trials) {
+                        shift += inc; inc *= 2; ++trials;
+                        restart = true;
+                    }
+                }
+            } else {
+                // natural order, one row after the other; the restart comes at
+                // the failing row.  where[c] = position of column c in row i,
+                // or -1
+                std::vector<index_t> where(n, -1);
+                for (index_t i = 0; i < n; ++i) {
                     const index_t rs = L.rp[i], re = L.rp[i + 1];
                     for (index_t q = rs; q < re; ++q) where[L.ci[q]] = q;
                     for (index_t kk = rs; kk < re - 1; ++kk) {
@@ -194,40 +323,23 @@ private:
                         L.v[kk] = s;
                     }
                     for (index_t q = rs; q < re; ++q) where[L.ci[q]] = -1;
-                    const index_t dp = re - 1;
-                    if (dp < rs || L.ci[dp] != i)
-                        throw std::runtime_error(
-                            "IC decomposition failed: missing diagonal at row " + std::to_string(i));
-                    const T orig = L.v[dp];
-                    T s = orig * static_cast<T>(shift);
-                    for (index_t kk = rs; kk < dp; ++kk) {
-                        const index_t k = L.ci[kk];
-                        if (k >= i) break;
-                        s -= L.v[kk] * L.v[kk] * inv_diag[k];
-                    }
-                    L.v[dp] = s;
-                    const double abs_s = abs_of(s);
-                    if (cfg.auto_shift && abs_s < cfg.min_diagonal_threshold && abs_of(orig) > 0.0) {
-                        if (abmc) {
-                            breakdown = true;
-                        } else if (shift < cfg.max_shift_value && trials < cfg.max_shift_trials) {
+                    if (pivot_row(i, shift, cfg, false)) {
+                        if (shift < cfg.max_shift_value && trials < cfg.max_shift_trials) {
                             shift += inc; inc *= 2; ++trials;
                             restart = true;
                             break;
                         }
+                        // no restart left: clamp as usual
+                        const index_t dp = L.rp[i + 1] - 1;
+                        const double abs_s = abs_of(L.v[dp]);
+                        if (abs_s > cfg.zero_diagonal_replacement) {
+                            inv_diag[i] = T(1) / L.v[dp];
+                        } else {
+                            const T safe = static_cast<T>(cfg.zero_diagonal_replacement);
+                            L.v[dp] = safe;
+                            inv_diag[i] = T(1) / safe;
+                        }
                     }
-                    if (abs_s > cfg.zero_diagonal_replacement) {
-                        inv_diag[i] = T(1) / s;
-                    } else {
-                        const T safe = static_cast<T>(cfg.zero_diagonal_replacement);
-                        L.v[dp] = safe;
-                        inv_diag[i] = T(1) / safe;
-                    }
-                }
-                if (abmc && breakdown && shift < cfg.max_shift_value &&
-                    trials < cfg.max_shift_trials) {
-                    shift += inc; inc *= 2; ++trials;
-                    restart = true;
                 }
             }
         }

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <exception>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -115,7 +116,6 @@ void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_
     }
     // SpMV slices in the original row order (ordering[i]: device row of
     // original row i)
-    plan.spmv.build(plan.fac.A_perm, plan.fac.sched.ordering);
     // the sweep layouts are built from the factor converted to F
     sparsesolv_amd::CsrHost<F> L, U;
     L.n = U.n = n;
@@ -127,8 +127,40 @@ void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_
     for (size_t k = 0; k < U.v.size(); ++k) U.v[k] = narrow<F>(Udev->v[k]);
     plan.invd.resize(n);
     for (int i = 0; i < n; ++i) plan.invd[i] = narrow<F>(invd_dev[i]);
-    plan.fwd.build(plan.fac.sched, L, /*forward=*/true, block_size, plan.invd);
-    plan.bwd.build(plan.fac.sched, U, /*forward=*/false, block_size, plan.invd);
+    // the three layouts are independent: built on three threads (one when
+    // the caller's thread cap says so)
+    if (sparsesolv_amd::setup_threads() < 3) {
+        plan.spmv.build(plan.fac.A_perm, plan.fac.sched.ordering);
+        plan.fwd.build(plan.fac.sched, L, /*forward=*/true, block_size, plan.invd);
+        plan.bwd.build(plan.fac.sched, U, /*forward=*/false, block_size, plan.invd);
+        plan.seconds = std::chrono::duration<double>(clk::now() - t0).count();
+        return;
+    }
+    std::exception_ptr err[2];
+    std::thread tf([&] {
+        try {
+            plan.fwd.build(plan.fac.sched, L, /*forward=*/true, block_size, plan.invd);
+        } catch (...) {
+            err[0] = std::current_exception();
+        }
+    });
+    std::thread tb([&] {
+        try {
+            plan.bwd.build(plan.fac.sched, U, /*forward=*/false, block_size, plan.invd);
+        } catch (...) {
+            err[1] = std::current_exception();
+        }
+    });
+    std::exception_ptr err_spmv;
+    try {
+        plan.spmv.build(plan.fac.A_perm, plan.fac.sched.ordering);
+    } catch (...) {
+        err_spmv = std::current_exception();
+    }
+    tf.join();
+    tb.join();
+    for (const auto& e : {err_spmv, err[0], err[1]})
+        if (e) std::rethrow_exception(e);
     plan.seconds = std::chrono::duration<double>(clk::now() - t0).count();
 }
 
@@ -147,6 +179,11 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
     std::vector<std::string> errors(static_cast<size_t>(count));
     std::atomic<int> next{0};
     auto work = [&] {
+        // the systems run side by side: each plans on one thread (the cap
+        // is restored, the calling thread runs this too)
+        int& cap = sparsesolv_amd::setup_thread_cap();
+        const int saved = cap;
+        if (count > 1) cap = 1;
         for (int s; (s = next.fetch_add(1)) < count;) {
             try {
                 build_plan<T, F>(plans[s], n[s], row_ptr[s], col_idx[s], values[s], settings);
@@ -154,8 +191,9 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
                 errors[s] = e.what();
             }
         }
+        cap = saved;
     };
-    const int nthreads = std::max(1, std::min<int>(count, int(std::thread::hardware_concurrency())));
+    const int nthreads = std::max(1, std::min<int>(count, sparsesolv_amd::setup_threads()));
     std::vector<std::thread> pool;
     for (int k = 1; k < nthreads; ++k) pool.emplace_back(work);
     work();
@@ -182,10 +220,11 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
         bp.row0 = {0, n[0]};
         bp.gather_in = P.fac.sched.reverse_ordering;
         bp.gather_out = P.fac.sched.ordering;
-        bp.spmv = P.spmv;
-        bp.fwd = P.fwd;
-        bp.bwd = P.bwd;
-        bp.invd = P.invd;
+        // the layouts move; the plan keeps its factor for the queries
+        bp.spmv = std::move(P.spmv);
+        bp.fwd = std::move(P.fwd);
+        bp.bwd = std::move(P.bwd);
+        bp.invd = std::move(P.invd);
         bp.single = std::move(P);
     } else {
         std::vector<const sparsesolv_amd::SpmvLayout<T>*> sp;

@@ -262,14 +262,15 @@ struct IcImpl final : IcBase {
         h_row0 = plan.row0;
         h_nrows = plan.nrows;
         if (count == 1) {
-            const auto& f = plan.single.fac;
-            h_ordering = f.sched.ordering;
-            h_color_offsets = f.sched.color_offsets;
-            h_block_offsets = f.sched.block_offsets;
-            h_Lrp = f.L.rp;
-            h_Lci = f.L.ci;
-            h_Lv = f.L.v;
-            h_invd = f.inv_diag;
+            // the plan is local: its schedule and factor move to the handle
+            auto& f = plan.single.fac;
+            h_ordering = std::move(f.sched.ordering);
+            h_color_offsets = std::move(f.sched.color_offsets);
+            h_block_offsets = std::move(f.sched.block_offsets);
+            h_Lrp = std::move(f.L.rp);
+            h_Lci = std::move(f.L.ci);
+            h_Lv = std::move(f.L.v);
+            h_invd = std::move(f.inv_diag);
         }
 
         if (!ctx) return;   // host-only handle: schedule and factor queries only

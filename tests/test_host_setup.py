@@ -2,6 +2,7 @@
 against the oracle, through the C ABI with a host-only handle (no GPU)."""
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from sparsesolv_amd import _lib
 from sparsesolv_amd import problems as P
@@ -16,6 +17,17 @@ CASES = [
     ("lap8_auto03", lambda: P.laplace7(8), dict(shift_parameter=0.3, auto_shift=1)),
     ("eddy6", lambda: P.eddy(6).A, dict(abmc_num_colors=8)),
     ("eddy5_b2", lambda: P.eddy(5).A, dict(abmc_block_size=2)),
+    # colours of hundreds of blocks: the factor runs on host threads
+    ("lap24", lambda: P.laplace7(24), dict()),
+    ("lap24_auto03", lambda: P.laplace7(24), dict(shift_parameter=0.3, auto_shift=1)),
+    ("cc16_auto", lambda: P.curlcurl(16).A, dict(auto_shift=1)),
+    ("eddy16", lambda: P.eddy(16).A, dict(abmc_num_colors=8)),
+    # 10,000 uncoupled 2x2 blocks [[1, 1], [1, 1]] at shift 1.0: every
+    # second pivot is 1 - 1/1 = 0, so the auto-shift restarts (at 1.05)
+    # after a colour of 5,000 blocks factored on host threads
+    ("pairs_restart", lambda: P.CSR.from_scipy(
+        sp.kron(sp.identity(10000), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr()),
+     dict(shift_parameter=1.0, auto_shift=1)),
 ]
 
 
