@@ -178,18 +178,23 @@ private:
     {
         L.n = nn;
         L.rp.assign(size_t(nn) + 1, 0);
-        for (index_t i = 0; i < nn; ++i) {
-            index_t c = 0;
-            for (index_t k = rp[i]; k < rp[i + 1]; ++k) c += ci[k] <= i;
-            L.rp[i + 1] = L.rp[i] + c;
-        }
+        parallel_chunks(nn, 4096, [&](index_t lo, index_t hi) {
+            for (index_t i = lo; i < hi; ++i) {
+                index_t c = 0;
+                for (index_t k = rp[i]; k < rp[i + 1]; ++k) c += ci[k] <= i;
+                L.rp[i + 1] = c;
+            }
+        });
+        for (index_t i = 0; i < nn; ++i) L.rp[i + 1] += L.rp[i];
         L.ci.resize(L.rp[nn]);
         L.v.resize(L.rp[nn]);
-        for (index_t i = 0; i < nn; ++i) {
-            index_t pos = L.rp[i];
-            for (index_t k = rp[i]; k < rp[i + 1]; ++k)
-                if (ci[k] <= i) { L.ci[pos] = ci[k]; L.v[pos] = v[k]; ++pos; }
-        }
+        parallel_chunks(nn, 4096, [&](index_t lo, index_t hi) {
+            for (index_t i = lo; i < hi; ++i) {
+                index_t pos = L.rp[i];
+                for (index_t k = rp[i]; k < rp[i + 1]; ++k)
+                    if (ci[k] <= i) { L.ci[pos] = ci[k]; L.v[pos] = v[k]; ++pos; }
+            }
+        });
     }
 
     void scaling_factors()
@@ -347,6 +352,9 @@ private:
         shift_trials = trials;
     }
 
+    // U = L^T, each row of U in ascending column order.  A thread owns a
+    // range [c0, c1) of U's rows (L's columns) and scans the rows i >= c0 of
+    // L (L is lower triangular, rows sorted by column) in ascending order.
     void transpose()
     {
         U.n = n;
@@ -354,15 +362,24 @@ private:
         U.rp.assign(size_t(n) + 1, 0);
         U.ci.resize(nnz);
         U.v.resize(nnz);
-        for (index_t k = 0; k < nnz; ++k) U.rp[L.ci[k] + 1]++;
+        auto in_range = [&](index_t i, index_t c0) {
+            return index_t(std::lower_bound(L.ci.begin() + L.rp[i], L.ci.begin() + L.rp[i + 1], c0) -
+                           L.ci.begin());
+        };
+        parallel_chunks(n, 4096, [&](index_t c0, index_t c1) {
+            for (index_t i = c0; i < n; ++i)
+                for (index_t k = in_range(i, c0); k < L.rp[i + 1] && L.ci[k] < c1; ++k) U.rp[L.ci[k] + 1]++;
+        });
         for (index_t i = 0; i < n; ++i) U.rp[i + 1] += U.rp[i];
         std::vector<index_t> fill(U.rp.begin(), U.rp.end() - 1);
-        for (index_t i = 0; i < n; ++i)
-            for (index_t k = L.rp[i]; k < L.rp[i + 1]; ++k) {
-                const index_t p = fill[L.ci[k]]++;
-                U.ci[p] = i;
-                U.v[p] = L.v[k];
-            }
+        parallel_chunks(n, 4096, [&](index_t c0, index_t c1) {
+            for (index_t i = c0; i < n; ++i)
+                for (index_t k = in_range(i, c0); k < L.rp[i + 1] && L.ci[k] < c1; ++k) {
+                    const index_t p = fill[L.ci[k]]++;
+                    U.ci[p] = i;
+                    U.v[p] = L.v[k];
+                }
+        });
     }
 };
 

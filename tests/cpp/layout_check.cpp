@@ -291,6 +291,21 @@ int run(const char* path, int block, int colours, bool abmc)
     // order (ABMC order; the original order on the level path)
     const auto& L = plan.fac.L;
     const auto& U = plan.fac.U;
+    {
+        // U is L^T, each row in ascending column order (a serial transpose)
+        std::vector<index_t> rp(static_cast<size_t>(n) + 1, 0);
+        for (index_t c : L.ci) rp[size_t(c) + 1]++;
+        for (int i = 0; i < n; ++i) rp[size_t(i) + 1] += rp[size_t(i)];
+        std::vector<index_t> ci(L.ci.size()), fill(rp.begin(), rp.end() - 1);
+        std::vector<T> uv(L.v.size());
+        for (int i = 0; i < n; ++i)
+            for (int k = L.rp[size_t(i)]; k < L.rp[size_t(i) + 1]; ++k) {
+                const index_t q = fill[size_t(L.ci[size_t(k)])]++;
+                ci[size_t(q)] = i;
+                uv[size_t(q)] = L.v[size_t(k)];
+            }
+        CHECK(U.rp == rp && U.ci == ci && U.v == uv);
+    }
     const auto& d = plan.fac.inv_diag;
     const size_t N = static_cast<size_t>(n);
     std::vector<T> xf(N), tf(N), zf(N);

@@ -49,6 +49,9 @@ CASES = [
     ("cc6_b16", lambda: P.curlcurl(6).A, 16, 4),
     ("eddy6", lambda: P.eddy(6).A, 4, 8),
     ("eddy5_b2", lambda: P.eddy(5).A, 2, 4),
+    # tens of thousands of rows: the host setup (permutation, factor,
+    # transpose) runs on threads
+    ("cc24", lambda: P.curlcurl(24).A, 4, 4),
 ]
 
 
@@ -78,7 +81,8 @@ def test_layouts_wide_blocks_level_fallback(tmp_path):
     ("varcoef9", lambda: varcoef(9, seed=7)),
     ("cc5", lambda: P.curlcurl(5).A),
     ("eddy5", lambda: P.eddy(5).A),
-], ids=["varcoef9", "cc5", "eddy5"])
+    ("cc24", lambda: P.curlcurl(24).A),
+], ids=["varcoef9", "cc5", "eddy5", "cc24"])
 def test_layouts_level_path(tmp_path, name, builder):
     # use_abmc = 0: one-row blocks, one colour per dependency level, the
     # device rows in level order and the factor in the original order
