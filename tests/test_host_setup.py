@@ -58,6 +58,24 @@ def test_host_setup_missing_diagonal():
     assert e.value.code == _lib.SSA_ERR_RUNTIME
 
 
+def test_host_setup_missing_diagonal_on_threads(oracle):
+    # three missing diagonals in a 24^3 Laplacian: the colours run on host
+    # threads, and the error names the first failing row of the serial
+    # order, as the reference's message does
+    A = P.laplace7(24).to_scipy().tolil()
+    for r in (5000, 9000, 12000):
+        A[r, r] = 0.0
+    A = A.tocsr()
+    A.eliminate_zeros()
+    C = P.CSR.from_scipy(A)
+    with pytest.raises(Exception) as eo:
+        oracle.ic(C, oracle.config(use_abmc=1)).factor()
+    with pytest.raises(_lib.SsaError) as e:
+        _lib.IcHandle(C, _lib.default_config(use_abmc=1), host_only=True)
+    assert "missing diagonal at row" in str(eo.value)
+    assert str(e.value).endswith(str(eo.value).split("missing diagonal")[1])
+
+
 def test_host_setup_rejects_bad_input():
     from sparsesolv_amd.problems import CSR
     A = P.laplace7(3)
