@@ -302,15 +302,18 @@ class _SparseSolvSolver:
     def SolveBlock(self, rhs, sol) -> list:
         """Several right-hand sides at once: rows of rhs and sol (shape
         (k, n)), sol holding the initial guesses; row i of the result is what
-        Solve(rhs[i], sol[i]) gives, bit for bit.  GPU extension (the
-        reference solves one vector per call): the block's SpMV and sweeps
-        load each matrix and factor entry once for up to 16 rows."""
+        Solve(rhs[i], sol[i]) gives, bit for bit (residual histories are not
+        kept for blocks).  GPU extension (the reference solves one vector per
+        call): the block's SpMV and sweeps load each matrix and factor entry
+        once for up to 16 rows."""
         code = self._method_code()
         if code not in (_lib.METHOD_ICCG, _lib.METHOD_ICCOCR, _lib.METHOD_CG, _lib.METHOD_COCR):
             raise RuntimeError(f"SparseSolvSolver.SolveBlock: method {self.method} runs one vector")
         cfg = self._config()
         B = np.array(rhs, dtype=self._dtype, ndmin=2)
-        X0 = np.array(sol, dtype=self._dtype, ndmin=2)
+        if not isinstance(sol, np.ndarray) or sol.ndim != 2:
+            raise ValueError(f"SolveBlock: sol must be a (k, {self.mat.n}) array (it receives the solutions)")
+        X0 = np.array(sol, dtype=self._dtype)
         if B.shape != X0.shape or B.shape[1] != self.mat.n:
             raise ValueError(f"SolveBlock: rhs and sol must both be (k, {self.mat.n})")
         if self.freedofs is not None:

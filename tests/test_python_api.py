@@ -105,3 +105,16 @@ def test_mixed_precision_switch_and_host_setup():
         f.append(h.factor())
     for k in f[0]:
         assert np.array_equal(f[0][k], f[1][k]), k
+
+
+def test_solve_block_checks_its_arrays():
+    # no GPU needed: the shapes are checked before any handle is made
+    A = P.laplace7(4)
+    s = ss.SparseSolvSolver(A, method="ICCG")
+    B = np.ones((3, A.n))
+    with pytest.raises(ValueError):
+        s.SolveBlock(B, np.zeros(A.n))              # sol must be (k, n)
+    with pytest.raises(ValueError):
+        s.SolveBlock(B, np.zeros((2, A.n)))         # one row per right-hand side
+    with pytest.raises(ValueError):
+        s.SolveBlock(np.ones((3, A.n + 1)), np.zeros((3, A.n + 1)))
