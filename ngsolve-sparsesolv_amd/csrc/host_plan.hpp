@@ -20,6 +20,7 @@ struct HostPlan {
     sparsesolv_amd::SpmvLayout<T> spmv;
     sparsesolv_amd::SweepLayout<F> fwd, bwd;
     std::vector<F> invd;                   // inv_diag in the factor precision
+    std::vector<double> scale;             // diagonal_scaling: 1/sqrt|a_ii| per device row
     double seconds = 0.0;                  // host setup wall time
 };
 
@@ -48,6 +49,7 @@ struct BatchPlan {
     sparsesolv_amd::SpmvLayout<T> spmv;
     sparsesolv_amd::SweepLayout<F> fwd, bwd;
     std::vector<F> invd;
+    std::vector<double> scale;      // diagonal_scaling (one system only)
     // per system: setup facts kept after the layouts are combined
     std::vector<double> actual_shift, seconds;
     std::vector<int> shift_trials, num_colors, num_blocks, nnz, nnz_l;

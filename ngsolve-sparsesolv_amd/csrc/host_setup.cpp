@@ -127,6 +127,15 @@ void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_
     for (size_t k = 0; k < U.v.size(); ++k) U.v[k] = narrow<F>(Udev->v[k]);
     plan.invd.resize(n);
     for (int i = 0; i < n; ++i) plan.invd[i] = narrow<F>(invd_dev[i]);
+    // diagonal_scaling: the factor is of S A S, S = diag(1/sqrt|a_ii|) in
+    // the factor's order (ic_preconditioner.hpp:1120-1149); device order as
+    // D^-1 above
+    if (settings.diagonal_scaling) {
+        plan.scale.assign(size_t(n), 1.0);
+        const std::vector<int>& ord = plan.fac.sched.ordering;
+        for (int i = 0; i < n; ++i)
+            plan.scale[settings.use_abmc ? size_t(i) : size_t(ord[i])] = plan.fac.scaling[size_t(i)];
+    }
     // the three layouts are independent: built on three threads (one when
     // the caller's thread cap says so)
     if (sparsesolv_amd::setup_threads() < 3) {
@@ -172,6 +181,8 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     if (count < 1) throw std::invalid_argument("batch: count must be >= 1");
+    if (count > 1 && settings.diagonal_scaling)
+        throw std::domain_error("batch: diagonal_scaling is not supported for several systems");
     bp = BatchPlan<T, F>{};
     bp.count = count;
     std::vector<HostPlan<T, F>> plans(static_cast<size_t>(count));
@@ -225,6 +236,7 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
         bp.fwd = std::move(P.fwd);
         bp.bwd = std::move(P.bwd);
         bp.invd = std::move(P.invd);
+        bp.scale = std::move(P.scale);
         bp.single = std::move(P);
     } else {
         std::vector<const sparsesolv_amd::SpmvLayout<T>*> sp;

@@ -239,12 +239,21 @@ struct IcImpl final : IcBase {
     static constexpr int kWalkPrefetch = sizeof(P) > 8 ? 8 : 16;
     static_assert(sparsesolv_amd::kWalkDepth % kWalkPrefetch == 0, "walk padding");
     DevBuf lo_a, lo_b, lo_t;
+    // diagonal_scaling: S = diag(1/sqrt|a_ii|) per device row and the
+    // preconditioner's scaled input.  The reference applies S M^-1 S on the
+    // level path and in ICCG's reordered-space apply (Path 1,
+    // abmc_reorder_spmv), and M^-1 of the scaled factor unscaled in the
+    // ABMC apply (the composite scaling vector it uses is built before the
+    // factors, ic_preconditioner.hpp:125 vs :162); scale_apply holds the
+    // current operation's choice.
+    DevBuf scale_d, sbuf;
+    bool has_scale = false, abmc_order = true, scale_apply = false;
 
     SolverState* h_state = nullptr;  // pinned, one per state
 
     IcImpl(ssa_context* c, int count_, const int* ns, const int32_t* const* rp,
            const int32_t* const* ci, const HT* const* v, const sparsesolv_amd::IcSettings& st)
-        : ctx(c), count(count_)
+        : ctx(c), count(count_), abmc_order(st.use_abmc)
     {
         ssa_host::BatchPlan<HT, HF> plan;
         ssa_host::build_batch<HT, HF>(plan, count, ns, rp, ci, v, st);
@@ -284,6 +293,10 @@ struct IcImpl final : IcBase {
         perm.upload(plan.gather_out, s);
         rev.upload(plan.gather_in, s);
         invd.upload(plan.invd, s);
+        if (!plan.scale.empty()) {
+            scale_d.upload(plan.scale, s);
+            has_scale = true;
+        }
         fwd.upload(plan.fwd, s);
         bwd.upload(plan.bwd, s);
         // the host packs walk records for values of sizeof(HF) bytes, the
@@ -394,6 +407,7 @@ struct IcImpl final : IcBase {
             for (DevBuf* b : {&lo_a, &lo_b, &lo_t})
                 if (b->reserve(size_t(n) * size_t(W) * sizeof(P)))
                     HIP_CHECK(hipMemsetAsync(b->p, 0, b->bytes, s));
+        if (has_scale) sbuf.reserve(size_t(n) * size_t(W) * sizeof(P));
         if (idle.reserve(sizeof(SolverState) * nst)) HIP_CHECK(hipMemsetAsync(idle.p, 0, idle.bytes, s));
         fwd.walk_scratch(W, s);
         bwd.walk_scratch(W, s);
@@ -652,8 +666,23 @@ struct IcImpl final : IcBase {
     // out = M^{-1} in (ABMC order); t is the forward result
     void launch_precond(const P* in, P* out, P* tbuf, const SolverState* st)
     {
+        if (scale_apply) {
+            // S M^-1 S: the sweeps between two scalings
+            launch_scale(in, sbuf.as<P>());
+            launch_sweep<true>(fwd, sbuf.as<P>(), tbuf, st);
+            launch_sweep<false>(bwd, tbuf, out, st);
+            launch_scale(out, out);
+            return;
+        }
         launch_sweep<true>(fwd, in, tbuf, st);
         launch_sweep<false>(bwd, tbuf, out, st);
+    }
+
+    void launch_scale(const P* in, P* out)
+    {
+        ssa::k_scale_rows<P><<<flat_grid(elements()), ssa::kBlock, 0, ctx->stream>>>(
+            elements(), W, scale_d.as<double>(), in, out);
+        HIP_CHECK(hipGetLastError());
     }
 
     // ---- vector plumbing: one vector (W == 1, batches included) or a block
@@ -818,6 +847,8 @@ struct IcImpl final : IcBase {
     void operate(const T* xin, T* yout, bool on_device, int m, bool precond)
     {
         ensure_workspace(rhs_width(m));
+        // the preconditioner's own apply: scaled on the level path only
+        scale_apply = has_scale && !abmc_order;
         hipStream_t s = ctx->stream;
         const size_t vb = size_t(tot) * size_t(m) * sizeof(T);   // the caller's vectors
         const T* src = xin;
@@ -1095,6 +1126,10 @@ struct IcImpl final : IcBase {
                      ssa_result* results, double* history, int m)
     {
         ensure_workspace(rhs_width(m));
+        // scaled apply: the level path, and ICCG in the reordered space
+        // (Path 1, sparsesolv.hpp:58-70); the others use the ABMC apply
+        scale_apply = has_scale &&
+                      (!abmc_order || (method == SSA_METHOD_ICCG && cfg.abmc_reorder_spmv));
         hipStream_t s = ctx->stream;
         const int S = nstates();
         const int nres = count > 1 ? count : m;      // results the caller receives
@@ -1296,6 +1331,7 @@ struct IcImpl final : IcBase {
         if (nrhs < 1 || nrhs > kMaxRhs)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_profile: nrhs must be 1.." + std::to_string(kMaxRhs));
         ensure_workspace(rhs_width(nrhs));
+        scale_apply = false;   // the sweeps themselves are timed
         hipStream_t s = ctx->stream;
         // operands of all W right-hand sides (a block pads nrhs up to W)
         const size_t rows = size_t(n) * size_t(W);

@@ -679,6 +679,22 @@ __global__ __launch_bounds__(kBlock) void k_residual(index_t n, const T* __restr
 }
 
 // dst[i] = src[map[i]]; vector blockIdx.y of several at the given strides
+// out = S in, S = diag(s) per row (diagonal scaling; W values per row,
+// interleaved).  Each component is multiplied by s in the vector's precision
+// (in and out may be the same vector).
+__device__ inline double scale_by(double v, double s) { return v * s; }
+__device__ inline float scale_by(float v, double s) { return v * float(s); }
+__device__ inline c128 scale_by(c128 v, double s) { return c128{v.x * s, v.y * s}; }
+__device__ inline c64 scale_by(c64 v, double s) { return c64{v.x * float(s), v.y * float(s)}; }
+
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_scale_rows(size_t count, int W, const double* __restrict__ s,
+                                                      const P* in, P* out)
+{
+    for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < count; e += size_t(gridDim.x) * kBlock)
+        out[e] = scale_by(in[e], s[e / size_t(W)]);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gather(index_t n, const index_t* __restrict__ map,
                                                   const T* __restrict__ src, index_t src_stride,

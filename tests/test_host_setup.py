@@ -153,3 +153,23 @@ def test_batch_host_handle():
         h.factor()
     with pytest.raises(TypeError):
         _lib.BatchHandle([mats[0], P.eddy(4).A], _lib.default_config(use_abmc=1), host_only=True)
+
+
+@pytest.mark.parametrize("use_abmc", [1, 0])
+def test_host_setup_diagonal_scaling(oracle, use_abmc):
+    # the factor of S A S, S = diag(1/sqrt|a_ii|), bit for bit
+    m = 10
+    A = P.laplace7_varcoef(m, 10.0 ** P.uniform(21, m ** 3, -2.0, 2.0))
+    kw = dict(use_abmc=use_abmc, diagonal_scaling=1)
+    h = _lib.IcHandle(A, _lib.default_config(**kw), host_only=True)
+    f = oracle.ic(A, oracle.config(**kw)).factor()
+    g = h.factor()
+    for k in ("L_row_ptr", "L_col_idx", "L_values", "inv_diag"):
+        np.testing.assert_array_equal(g[k], f[k], err_msg=k)
+
+
+def test_batch_rejects_diagonal_scaling():
+    cfg = _lib.default_config(use_abmc=1, diagonal_scaling=1)
+    with pytest.raises(_lib.SsaError) as e:
+        _lib.BatchHandle([P.laplace7(4), P.laplace7(5)], cfg, host_only=True)
+    assert e.value.code == _lib.SSA_ERR_UNSUPPORTED
