@@ -77,7 +77,7 @@ typedef struct ssa_config {
     int abmc_block_size;              /* 4 */
     int abmc_num_colors;              /* 4 */
     int abmc_reorder_spmv;            /* 0 (GPU Krylov always runs in ABMC order) */
-    int abmc_use_rcm;                 /* 0 (unsupported: error) */
+    int abmc_use_rcm;                 /* 0; 1: ABMC of the RCM-reordered matrix */
     /* GPU-only fields */
     int precision;                    /* SSA_PRECISION_FP64 */
     int use_graph;                    /* 1: replay a captured iteration */

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "abmc_ordering.hpp"
+#include "rcm_ordering.hpp"
 
 namespace sparsesolv_amd {
 
@@ -55,6 +56,7 @@ struct IcSettings {
     bool use_abmc = true;
     int abmc_block_size = 4;
     int abmc_num_colors = 4;
+    bool abmc_use_rcm = false;
 };
 
 template <typename T>
@@ -154,7 +156,21 @@ struct IcFactorHost {
     {
         n = n_;
         if (cfg.use_abmc) {
-            sched.build(rp, ci, n, cfg.abmc_block_size, cfg.abmc_num_colors);
+            if (cfg.abmc_use_rcm) {
+                // ABMC of the RCM-reordered matrix (ic_preconditioner.hpp:97-113):
+                // the schedule's ordering becomes the composite original ->
+                // RCM -> ABMC one (:588-595), the matrix is permuted by it
+                std::vector<index_t> rcm, rcm_rev;
+                rcm_ordering(rp, ci, n, rcm, rcm_rev);
+                const CsrHost<T> R = permute_csr(n, rp, ci, v, rcm);
+                sched.build(R.rp.data(), R.ci.data(), n, cfg.abmc_block_size, cfg.abmc_num_colors);
+                std::vector<index_t> comp(static_cast<size_t>(n));
+                for (index_t i = 0; i < n; ++i) comp[size_t(i)] = sched.ordering[size_t(rcm[size_t(i)])];
+                for (index_t i = 0; i < n; ++i) sched.reverse_ordering[size_t(comp[size_t(i)])] = i;
+                sched.ordering = std::move(comp);
+            } else {
+                sched.build(rp, ci, n, cfg.abmc_block_size, cfg.abmc_num_colors);
+            }
             CsrHost<T> P = permute_csr(n, rp, ci, v, sched.ordering);
             extract_lower(P.n, P.rp.data(), P.ci.data(), P.v.data());
             if (keep_permuted_matrix) A_perm = std::move(P);

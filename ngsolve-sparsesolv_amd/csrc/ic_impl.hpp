@@ -247,13 +247,13 @@ struct IcImpl final : IcBase {
     // factors, ic_preconditioner.hpp:125 vs :162); scale_apply holds the
     // current operation's choice.
     DevBuf scale_d, sbuf;
-    bool has_scale = false, abmc_order = true, scale_apply = false;
+    bool has_scale = false, abmc_order = true, abmc_rcm = false, scale_apply = false;
 
     SolverState* h_state = nullptr;  // pinned, one per state
 
     IcImpl(ssa_context* c, int count_, const int* ns, const int32_t* const* rp,
            const int32_t* const* ci, const HT* const* v, const sparsesolv_amd::IcSettings& st)
-        : ctx(c), count(count_), abmc_order(st.use_abmc)
+        : ctx(c), count(count_), abmc_order(st.use_abmc), abmc_rcm(st.use_abmc && st.abmc_use_rcm)
     {
         ssa_host::BatchPlan<HT, HF> plan;
         ssa_host::build_batch<HT, HF>(plan, count, ns, rp, ci, v, st);
@@ -1127,9 +1127,12 @@ struct IcImpl final : IcBase {
     {
         ensure_workspace(rhs_width(m));
         // scaled apply: the level path, and ICCG in the reordered space
-        // (Path 1, sparsesolv.hpp:58-70); the others use the ABMC apply
+        // (Path 1, sparsesolv.hpp:58-76) or in the RCM space (Path 2,
+        // :78-96, whose adapter scales, ic_preconditioner.hpp:327-340); the
+        // others use the ABMC apply
         scale_apply = has_scale &&
-                      (!abmc_order || (method == SSA_METHOD_ICCG && cfg.abmc_reorder_spmv));
+                      (!abmc_order || (method == SSA_METHOD_ICCG &&
+                                       (cfg.abmc_reorder_spmv || abmc_rcm)));
         hipStream_t s = ctx->stream;
         const int S = nstates();
         const int nres = count > 1 ? count : m;      // results the caller receives

@@ -136,8 +136,6 @@ static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int c
     if (!cfg) throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": null argument");
     if (dtype != SSA_F64 && dtype != SSA_C128)
         throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": dtype must be SSA_F64 or SSA_C128");
-    if (cfg->use_abmc && cfg->abmc_use_rcm)
-        throw SsaError(SSA_ERR_UNSUPPORTED, std::string(who) + ": abmc_use_rcm is not supported");
     if (cfg->diagonal_scaling && count > 1)
         throw SsaError(SSA_ERR_UNSUPPORTED,
                        std::string(who) + ": diagonal_scaling is not supported for several systems");
@@ -157,6 +155,7 @@ static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int c
     st.use_abmc = cfg->use_abmc != 0;
     st.abmc_block_size = cfg->abmc_block_size;
     st.abmc_num_colors = cfg->abmc_num_colors;
+    st.abmc_use_rcm = cfg->abmc_use_rcm != 0;
     auto h = std::make_unique<ssa_ic>();
     h->dtype = dtype;
     // mixed: the factor (computed in FP64) is stored and applied in

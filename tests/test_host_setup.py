@@ -84,9 +84,6 @@ def test_host_setup_rejects_bad_input():
     with pytest.raises(_lib.SsaError) as e:
         _lib.IcHandle(bad, _lib.default_config(use_abmc=1), host_only=True)
     assert e.value.code == _lib.SSA_ERR_ARGUMENT
-    with pytest.raises(_lib.SsaError) as e:
-        _lib.IcHandle(A, _lib.default_config(use_abmc=1, abmc_use_rcm=1), host_only=True)
-    assert e.value.code == _lib.SSA_ERR_UNSUPPORTED
 
 
 def test_host_only_handle_refuses_device_work():
@@ -173,3 +170,36 @@ def test_batch_rejects_diagonal_scaling():
     with pytest.raises(_lib.SsaError) as e:
         _lib.BatchHandle([P.laplace7(4), P.laplace7(5)], cfg, host_only=True)
     assert e.value.code == _lib.SSA_ERR_UNSUPPORTED
+
+
+RCM_CASES = [
+    ("lap9", lambda: P.laplace7(9), dict()),
+    ("lap24", lambda: P.laplace7(24), dict()),
+    ("cc8_auto", lambda: P.curlcurl(8).A, dict(auto_shift=1)),
+    ("eddy6", lambda: P.eddy(6).A, dict(abmc_num_colors=8)),
+    ("varcoef10_b8", lambda: P.laplace7_varcoef(10, 10.0 ** P.uniform(21, 1000, -2.0, 2.0)),
+     dict(abmc_block_size=8, abmc_num_colors=6)),
+]
+
+
+@pytest.mark.parametrize("name,builder,kw", RCM_CASES, ids=[c[0] for c in RCM_CASES])
+def test_host_setup_rcm_matches_reference(ref, name, builder, kw):
+    # abmc_use_rcm: ABMC of the RCM-reordered matrix.  The reference build
+    # pins it (the C restatement has no RCM): its schedule and factor, bit
+    # for bit; our ordering is the composite original -> RCM -> ABMC one
+    A = builder()
+    cfgk = dict(use_abmc=1, abmc_use_rcm=1, **kw)
+    h = _lib.IcHandle(A, _lib.default_config(**cfgk), host_only=True)
+    f = ref.ic(A, ref.config(**cfgk)).factor()
+    co, bo = h.schedule()
+    np.testing.assert_array_equal(co, f["color_offsets"])
+    np.testing.assert_array_equal(bo, f["block_offsets"])
+    g = h.factor()
+    for k in ("L_row_ptr", "L_col_idx", "L_values", "inv_diag"):
+        np.testing.assert_array_equal(g[k], f[k], err_msg=k)
+    assert g["actual_shift"] == f["actual_shift"]
+    # the composite ordering is a permutation and differs from plain ABMC's
+    o = h.ordering()
+    assert np.array_equal(np.sort(o), np.arange(A.n))
+    plain = _lib.IcHandle(A, _lib.default_config(use_abmc=1, **kw), host_only=True).ordering()
+    assert not np.array_equal(o, plain)
