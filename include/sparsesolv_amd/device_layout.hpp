@@ -18,11 +18,15 @@
 //    gets a lane; the rows of a block sit in consecutive lanes of one slice,
 //    in processing order (ascending rows for the forward sweep over L,
 //    descending for the backward sweep over L^T).  Blocks are sorted by
-//    their longest row and packed into 64-lane slices; a slice stores
-//    K = its longest row entry slots, laid out like SpmvLayout.  A column
-//    c >= 0 is a finished row of an earlier phase (gathered from HBM); c < 0
-//    refers to position -1-c of the same block, resolved by wavefront
-//    shuffles once that row is final.
+//    their longest row and packed into 64-lane slices; a slice holds
+//    K = its longest row entry slots, laid out like SpmvLayout, and every
+//    slice of a colour starts stride = the colour's largest K slots after
+//    the previous one, so a kernel finds a slice's first entries from the
+//    colour's first slot and stride alone (no offsets load before them);
+//    each lane's info word carries its position in the block and its
+//    slice's K.  A column c >= 0 is a finished row of an earlier phase
+//    (gathered from HBM); c < 0 refers to position -1-c of the same block,
+//    resolved by wavefront shuffles once that row is final.
 //    The ABMC aggregation can leave one oversized last block (the rows left
 //    unassigned once ceil(n/B) blocks exist).  Such a block ("big block")
 //    is scheduled by its own dependency levels: rows grouped level by level,
@@ -161,8 +165,10 @@ struct SweepLayout {
     index_t nphases = 0;                // colours, in processing order
     std::vector<index_t> phase_slices;  // nphases+1: slice ranges
     std::vector<index_t> lane_row;      // nslices*64: row of the lane (-1: idle)
-    std::vector<index_t> lane_pos;      // nslices*64: position of the row in its block
-    std::vector<index_t> slice_slot;    // nslices+1: entry-slot offsets
+    std::vector<index_t> lane_info;     // nslices*64: position in the block | slice's K << 8
+    std::vector<index_t> slice_slot;    // nslices+1: first entry slot of each slice, end
+    std::vector<index_t> phase_slot;    // nphases: first entry slot of the colour's slices
+    std::vector<index_t> phase_stride;  // nphases: entry slots from one slice to the next
     std::vector<F> val;                 // slots*64
     std::vector<index_t> col;
     // per colour, rows of the regular slices and their factor entries
@@ -209,6 +215,12 @@ struct SweepLayout {
     int walk_rec_words = 0;
 
     index_t nslices() const { return phase_slices.empty() ? 0 : phase_slices.back(); }
+    // lane info word: the row's position in its block (< 64) and the
+    // slice's entry slots K
+    static constexpr int kInfoShift = 8;
+    static index_t info_word(index_t pos, index_t K) { return pos | (K << kInfoShift); }
+    static index_t info_pos(index_t w) { return w & ((1 << kInfoShift) - 1); }
+    static index_t info_len(index_t w) { return w >> kInfoShift; }
 
     // forward: rows of L below the diagonal; backward: rows of U = L^T above.
     template <typename T>
@@ -232,7 +244,9 @@ struct SweepLayout {
         big_pos.assign(1, 0);
         big_lvl_off.assign(1, 0);
         big_lvl.clear(); big_rows.clear(); big_rp.assign(1, 0); big_ci.clear(); big_v.clear();
-        lane_row.clear(); lane_pos.clear(); slice_slot.assign(1, 0);
+        lane_row.clear(); lane_info.clear(); slice_slot.assign(1, 0);
+        phase_slot.assign(size_t(nc), 0);
+        phase_stride.assign(size_t(nc), 0);
         val.clear(); col.clear();
         max_big_rows = 0; max_levels = 0;
 
@@ -266,12 +280,20 @@ struct SweepLayout {
             for (index_t b : regular) keyed.emplace_back(longest(b), b);
             std::stable_sort(keyed.begin(), keyed.end(),
                              [](const auto& x, const auto& y) { return x.first > y.first; });
-            index_t nsl = 0;
+            // fill slices with whole blocks, then lay them out stride slots
+            // apart
+            struct Slice {
+                index_t row[kWave], pos[kWave], b0[kWave], b1[kWave];
+                index_t K = 0;
+            };
+            std::vector<Slice> slices;
             size_t q = 0;
             while (q < keyed.size()) {
-                // fill one slice with whole blocks
-                std::vector<index_t> lanes_row(kWave, -1), lanes_pos(kWave, 0);
-                std::vector<index_t> lanes_b0(kWave, 0), lanes_b1(kWave, 0);
+                Slice sl;
+                std::fill(sl.row, sl.row + kWave, -1);
+                std::fill(sl.pos, sl.pos + kWave, 0);
+                std::fill(sl.b0, sl.b0 + kWave, 0);
+                std::fill(sl.b1, sl.b1 + kWave, 0);
                 int used = 0;
                 while (q < keyed.size()) {
                     const index_t b = keyed[q].second;
@@ -279,42 +301,51 @@ struct SweepLayout {
                     const int sz = int(b1 - b0);
                     if (used + sz > kWave) break;
                     for (int r = 0; r < sz; ++r) {
-                        lanes_row[used + r] = forward ? b0 + r : b1 - 1 - r;
-                        lanes_pos[used + r] = r;
-                        lanes_b0[used + r] = b0;
-                        lanes_b1[used + r] = b1;
+                        sl.row[used + r] = forward ? b0 + r : b1 - 1 - r;
+                        sl.pos[used + r] = r;
+                        sl.b0[used + r] = b0;
+                        sl.b1[used + r] = b1;
                     }
                     used += sz;
                     ++q;
                 }
-                index_t K = 0;
                 for (int l = 0; l < kWave; ++l) {
-                    if (lanes_row[l] < 0) continue;
+                    if (sl.row[l] < 0) continue;
                     index_t a, e;
-                    row_span(lanes_row[l], a, e);
-                    K = std::max(K, e - a);
+                    row_span(sl.row[l], a, e);
+                    sl.K = std::max(sl.K, e - a);
                 }
-                const size_t base = size_t(slice_slot.back()) * kWave;
-                slice_slot.push_back(slice_slot.back() + K);
-                val.resize(base + size_t(K) * kWave, F(0));
-                col.resize(base + size_t(K) * kWave, kPadCol);
+                slices.push_back(sl);
+            }
+            const index_t nsl = index_t(slices.size());
+            index_t stride = 0;
+            for (const Slice& sl : slices) stride = std::max(stride, sl.K);
+            const index_t first = slice_slot.back();
+            phase_slot[p] = first;
+            phase_stride[p] = stride;
+            slice_slot.pop_back();
+            for (index_t k = 0; k <= nsl; ++k) slice_slot.push_back(first + k * stride);
+            val.resize(size_t(first + nsl * stride) * kWave, F(0));
+            col.resize(size_t(first + nsl * stride) * kWave, kPadCol);
+            for (index_t k = 0; k < nsl; ++k) {
+                const Slice& sl = slices[size_t(k)];
+                const size_t base = size_t(first + k * stride) * kWave;
                 for (int l = 0; l < kWave; ++l) {
-                    lane_row.push_back(lanes_row[l]);
-                    lane_pos.push_back(lanes_pos[l]);
-                    if (lanes_row[l] < 0) continue;
+                    lane_row.push_back(sl.row[l]);
+                    lane_info.push_back(info_word(sl.pos[l], sl.K));
+                    if (sl.row[l] < 0) continue;
                     index_t a, e;
-                    row_span(lanes_row[l], a, e);
-                    const index_t b0 = lanes_b0[l], b1 = lanes_b1[l];
-                    for (index_t k = a; k < e; ++k) {
-                        const index_t cc = M.ci[k];
+                    row_span(sl.row[l], a, e);
+                    const index_t b0 = sl.b0[l], b1 = sl.b1[l];
+                    for (index_t k2 = a; k2 < e; ++k2) {
+                        const index_t cc = M.ci[k2];
                         const bool internal = cc >= b0 && cc < b1;
                         const index_t pos = forward ? cc - b0 : b1 - 1 - cc;
-                        const size_t at = base + size_t(k - a) * kWave + l;
-                        val[at] = static_cast<F>(M.v[k]);
+                        const size_t at = base + size_t(k2 - a) * kWave + l;
+                        val[at] = static_cast<F>(M.v[k2]);
                         col[at] = internal ? -1 - pos : cc;
                     }
                 }
-                ++nsl;
             }
             phase_slices[p + 1] = phase_slices[p] + nsl;
 
@@ -385,7 +416,7 @@ struct SweepLayout {
         if (big_ci.empty()) { big_ci.push_back(0); big_v.push_back(F(0)); }
         if (big_rows.empty()) big_rows.push_back(0);
         if (big_lvl.empty()) big_lvl.push_back(0);
-        if (lane_row.empty()) { lane_row.assign(kWave, -1); lane_pos.assign(kWave, 0); }
+        if (lane_row.empty()) { lane_row.assign(kWave, -1); lane_info.assign(kWave, 0); }
     }
 
 private:
@@ -541,32 +572,47 @@ public:
         out.phase_slices.assign(1, 0);
         out.phase_big.assign(1, 0);
         out.slice_slot.assign(1, 0);
+        out.phase_slot.assign(size_t(out.nphases), 0);
+        out.phase_stride.assign(size_t(out.nphases), 0);
         out.walk_step_off.assign(1, 0);
         out.walk_ext_rp.assign(1, 0);
         out.phase_rows.assign(size_t(out.nphases), 0);
         out.phase_entries.assign(size_t(out.nphases), 0);
         for (index_t p = 0; p < out.nphases; ++p) {
+            // the phase's stride: the longest slice of any system
+            index_t stride = 0, nsl = 0;
+            for (const SweepLayout* P : parts) {
+                if (p >= P->nphases) continue;
+                for (index_t sl = P->phase_slices[p]; sl < P->phase_slices[p + 1]; ++sl, ++nsl)
+                    stride = std::max(stride, info_len(P->lane_info[size_t(sl) * kWave]));
+            }
+            const index_t first = out.slice_slot.back();
+            out.phase_slot[p] = first;
+            out.phase_stride[p] = stride;
+            out.slice_slot.pop_back();
+            for (index_t k = 0; k <= nsl; ++k) out.slice_slot.push_back(first + k * stride);
+            out.val.resize(size_t(first + nsl * stride) * kWave, F(0));
+            out.col.resize(size_t(first + nsl * stride) * kWave, kPadCol);
+            index_t k = 0;
             for (size_t si = 0; si < parts.size(); ++si) {
                 const SweepLayout& P = *parts[si];
                 if (p >= P.nphases) continue;
                 const index_t r0 = row0[si];
                 out.phase_rows[p] += P.phase_rows[p];
                 out.phase_entries[p] += P.phase_entries[p];
-                for (index_t sl = P.phase_slices[p]; sl < P.phase_slices[p + 1]; ++sl) {
-                    const index_t Ks = P.slice_slot[sl + 1] - P.slice_slot[sl];
+                for (index_t sl = P.phase_slices[p]; sl < P.phase_slices[p + 1]; ++sl, ++k) {
+                    const index_t Ks = info_len(P.lane_info[size_t(sl) * kWave]);
                     const size_t src = size_t(P.slice_slot[sl]) * kWave;
-                    const size_t dst = size_t(out.slice_slot.back()) * kWave;
-                    out.slice_slot.push_back(out.slice_slot.back() + Ks);
-                    out.val.insert(out.val.end(), P.val.begin() + src, P.val.begin() + src + size_t(Ks) * kWave);
-                    out.col.resize(dst + size_t(Ks) * kWave);
+                    const size_t dst = size_t(first + k * stride) * kWave;
                     for (size_t q = 0; q < size_t(Ks) * kWave; ++q) {
                         const index_t c = P.col[src + q];
                         out.col[dst + q] = c >= 0 ? c + r0 : c;
+                        out.val[dst + q] = P.val[src + q];
                     }
                     for (int l = 0; l < kWave; ++l) {
                         const index_t r = P.lane_row[size_t(sl) * kWave + l];
                         out.lane_row.push_back(r >= 0 ? r + r0 : r);
-                        out.lane_pos.push_back(P.lane_pos[size_t(sl) * kWave + l]);
+                        out.lane_info.push_back(P.lane_info[size_t(sl) * kWave + l]);
                     }
                     slice_sys.push_back(int(si));
                 }
@@ -621,7 +667,7 @@ public:
         out.big_rp.assign(1, 0);
         out.big_ci.assign(1, 0);
         out.big_v.assign(1, F(0));
-        if (out.lane_row.empty()) { out.lane_row.assign(kWave, -1); out.lane_pos.assign(kWave, 0); }
+        if (out.lane_row.empty()) { out.lane_row.assign(kWave, -1); out.lane_info.assign(kWave, 0); }
         if (out.walk_row.empty()) { out.walk_row.assign(1, -1); out.walk_pos.assign(1, -1); out.walk_invd.assign(1, F(0)); out.walk_ext_rp.push_back(0); }
         if (out.walk_ext_ci.empty()) { out.walk_ext_ci.push_back(0); out.walk_ext_v.push_back(F(0)); }
         if (out.walk_v.empty()) { out.walk_v.push_back(F(0)); out.walk_ix.push_back(0); }

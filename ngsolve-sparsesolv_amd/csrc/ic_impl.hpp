@@ -122,12 +122,12 @@ template <> struct HostOf<ssa::c128> { using type = std::complex<double>; };
 template <> struct HostOf<ssa::c64> { using type = std::complex<float>; };
 
 struct DevSweep {
-    DevBuf lrow, lpos, slot, val, col;
+    DevBuf lrow, linfo, val, col;
     DevBuf slice_sys, walk_sys;   // batches: system of every slice / oversized block
     bool batched = false;
     DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
     DevBuf w_step_off, w_row, w_ext_rp, w_ext_ci, w_ext_v, w_rec, w_sa, w_sb;
-    std::vector<index_t> phase_slices, phase_big, phase_rows;
+    std::vector<index_t> phase_slices, phase_big, phase_rows, phase_slot, phase_stride;
     std::vector<int64_t> phase_entries;
     int BR = 1;
     int walk_kmax = 0;
@@ -141,8 +141,7 @@ struct DevSweep {
     void upload(const sparsesolv_amd::SweepLayout<F>& L, hipStream_t s)
     {
         lrow.upload(L.lane_row, s);
-        lpos.upload(L.lane_pos, s);
-        slot.upload(L.slice_slot, s);
+        linfo.upload(L.lane_info, s);
         val.upload(L.val, s);
         col.upload(L.col, s);
         big_pos.upload(L.big_pos, s);
@@ -163,6 +162,8 @@ struct DevSweep {
         walk_slots = L.walk_row.size();
         walk_scratch(1, s);
         phase_slices = L.phase_slices;
+        phase_slot = L.phase_slot;
+        phase_stride = L.phase_stride;
         phase_big = L.phase_big;
         phase_rows = L.phase_rows;
         phase_entries = L.phase_entries;
@@ -183,7 +184,7 @@ struct DevSweep {
     }
     size_t bytes() const
     {
-        return lrow.bytes + lpos.bytes + slot.bytes + val.bytes + col.bytes +
+        return lrow.bytes + linfo.bytes + val.bytes + col.bytes +
                big_pos.bytes + big_lvl_off.bytes + big_lvl.bytes + big_rows.bytes +
                big_rp.bytes + big_ci.bytes + big_v.bytes + w_step_off.bytes + w_row.bytes +
                w_ext_rp.bytes + w_ext_ci.bytes + w_ext_v.bytes + w_rec.bytes;
@@ -451,7 +452,7 @@ struct IcImpl final : IcBase {
 
     ssa::SweepArrays sweep_arrays(DevSweep& D)
     {
-        return ssa::SweepArrays{D.lrow.as<index_t>(), D.lpos.as<index_t>(), D.slot.as<index_t>(),
+        return ssa::SweepArrays{D.lrow.as<index_t>(), D.linfo.as<index_t>(),
                                 D.val.p, D.col.as<index_t>(), invd.p,
                                 D.batched ? D.slice_sys.as<int>() : nullptr};
     }
@@ -517,6 +518,10 @@ struct IcImpl final : IcBase {
         ssa::TailPhases tp{};
         tp.count = int(p1 - p0);
         for (index_t q = p0; q <= p1; ++q) tp.slices[q - p0] = D.phase_slices[q];
+        for (index_t q = p0; q < p1; ++q) {
+            tp.slot[q - p0] = D.phase_slot[q];
+            tp.stride[q - p0] = D.phase_stride[q];
+        }
         const ssa::SweepArrays A = sweep_arrays(D);
 #define SSA_TAIL(BRV) tail_br<FWD, BRV>(tp, A, in, out, st)
         switch (D.BR == 0 ? 1 : D.BR) {
@@ -557,22 +562,24 @@ struct IcImpl final : IcBase {
     // one colour's regular slices (and the external part of its oversized
     // blocks) with WV right-hand sides per lane
     template <bool FWD, int BRV, int WV>
-    void sweep_w(int g, index_t s0, index_t s1, const ssa::SweepArrays& A, const P* in, P* out,
-                 const ssa::WalkExt& we, const SolverState* st)
+    void sweep_w(int g, index_t s0, index_t s1, index_t slot0, index_t stride,
+                 const ssa::SweepArrays& A, const P* in, P* out, const ssa::WalkExt& we,
+                 const SolverState* st)
     {
         ssa::k_sweep<P, F, BRV, FWD, sweep_chunk<P, WV>(), WV><<<g, ssa::kBlock, 0, ctx->stream>>>(
-            s0, s1, A, in, out, we, st);
+            s0, s1, slot0, stride, A, in, out, we, st);
     }
 
     template <bool FWD, int BRV>
-    void sweep_br(int g, index_t s0, index_t s1, const ssa::SweepArrays& A, const P* in, P* out,
-                  const ssa::WalkExt& we, const SolverState* st)
+    void sweep_br(int g, index_t s0, index_t s1, index_t slot0, index_t stride,
+                  const ssa::SweepArrays& A, const P* in, P* out, const ssa::WalkExt& we,
+                  const SolverState* st)
     {
         switch (W) {
-        case 1: sweep_w<FWD, BRV, 1>(g, s0, s1, A, in, out, we, st); break;
-        case 4: sweep_w<FWD, BRV, 4>(g, s0, s1, A, in, out, we, st); break;
-        case 8: sweep_w<FWD, BRV, 8>(g, s0, s1, A, in, out, we, st); break;
-        case 16: sweep_w<FWD, BRV, 16>(g, s0, s1, A, in, out, we, st); break;
+        case 1: sweep_w<FWD, BRV, 1>(g, s0, s1, slot0, stride, A, in, out, we, st); break;
+        case 4: sweep_w<FWD, BRV, 4>(g, s0, s1, slot0, stride, A, in, out, we, st); break;
+        case 8: sweep_w<FWD, BRV, 8>(g, s0, s1, slot0, stride, A, in, out, we, st); break;
+        case 16: sweep_w<FWD, BRV, 16>(g, s0, s1, slot0, stride, A, in, out, we, st); break;
         default: throw SsaError(SSA_ERR_ARGUMENT, "unsupported block width");
         }
     }
@@ -602,7 +609,8 @@ struct IcImpl final : IcBase {
         if (s1 > s0 || gw > 0) {
             const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
             const ssa::SweepArrays A = sweep_arrays(D);
-#define SSA_SWEEP(BRV) sweep_br<FWD, BRV>(g, s0, s1, A, in, out, we, st)
+#define SSA_SWEEP(BRV) \
+    sweep_br<FWD, BRV>(g, s0, s1, D.phase_slot[ph], D.phase_stride[ph], A, in, out, we, st)
             switch (D.BR == 0 ? 1 : D.BR) {
             case 1: SSA_SWEEP(1); break;
             case 2: SSA_SWEEP(2); break;

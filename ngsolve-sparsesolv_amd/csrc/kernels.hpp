@@ -269,47 +269,55 @@ __device__ inline c64 shfl<c64>(c64 v, int src) { return c64{__shfl(v.x, src, 64
 
 struct SweepArrays {
     const index_t* lane_row;
-    const index_t* lane_pos;
-    const index_t* slice_slot;
+    const index_t* lane_info;   // position in the block | slice's entry slots << 8
     const void* val;
     const index_t* col;
     const void* invd;
     const int* slice_sys;   // batches: system of every slice (null: one system)
 };
 
-// One wavefront, one slice.  The slice's offsets, the lane's row and the
-// first batch of entries are requested before `stopped` is tested, so the
-// chain of dependent memory round trips is offsets -> entries -> gathers.
-// W right-hand sides (interleaved rows): each entry is loaded once and
-// applied to all of them; CH entries per batch.
+// One wavefront, one slice, whose entries start at slot o0 (wavefront-
+// uniform, from the colour's first slot and stride: no load).  The lane's
+// row and info word and the first batch of entries -- as many as the
+// colour's stride, the longest slice -- are requested before `stopped` is
+// tested, so the chain of dependent memory round trips is entries ->
+// gathers.  The info word gives the slice's own entry count K: only the
+// first K entries are gathered.  W right-hand sides (interleaved rows):
+// each entry is loaded once and applied to all of them; CH entries per
+// batch.
 template <typename P, typename F, int BR, bool FWD, int CH, int W>
-__device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __restrict__ in,
-                                   P* out, int stopped)
+__device__ inline void sweep_slice(index_t s, index_t o0, index_t stride, const SweepArrays& A,
+                                   const P* __restrict__ in, P* out, int stopped)
 {
     const F* __restrict__ val = static_cast<const F*>(A.val);
     const index_t* __restrict__ col = A.col;
     const int lane = threadIdx.x & 63;
     const size_t li = size_t(s) * 64 + lane;
-    const index_t o0 = A.slice_slot[s], o1 = A.slice_slot[s + 1];
     const index_t row = A.lane_row[li];
-    const int pos = A.lane_pos[li];
-    // entries at or past o1 belong to the next slice or to the padding at
-    // the end of the arrays: loaded, then masked
+    const index_t info = A.lane_info[li];
+    // slots in [o0, o0 + stride) are this slice's, padding past its K
     index_t c[CH];
     F v[CH];
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
-        const size_t at = (size_t(o0) + j) * 64 + lane;
-        c[j] = col[at];
-        v[j] = val[at];
+        c[j] = kPadColumn;
+        v[j] = zero<F>();
+        if (j < stride) {
+            const size_t at = (size_t(o0) + j) * 64 + lane;
+            c[j] = col[at];
+            v[j] = val[at];
+        }
     }
     if (stopped) {
-        // consuming the offset and the first entry on this path keeps their
-        // loads ahead of the status test (the compiler would otherwise sink
-        // them into the only branch that uses them); offsets are >= 0
-        if (o0 < 0) out[0] = cvt<P>(v[0]);
+        // consuming the info word and the first entry on this path keeps
+        // their loads ahead of the status test (the compiler would otherwise
+        // sink them into the only branch that uses them); info is >= 0
+        if (info < 0) out[0] = cvt<P>(v[0]);
         return;
     }
+    const int pos = info & 255;
+    // the same in every lane of the slice
+    const index_t o1 = o0 + __builtin_amdgcn_readfirstlane(info >> 8);
     const bool live = row >= 0;
     const index_t rr = live ? row : 0;
     // the backward sweep needs in_i only at the end: with several
@@ -334,15 +342,20 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
         P g[CH][W];
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
-            if (k0 + j >= o1) c[j] = kPadColumn;
-            const index_t cj = c[j] >= 0 ? c[j] : 0;
-            load_row<P, W>(out + size_t(cj) * W, g[j]);
+            if (k0 + j < o1) {
+                const index_t cj = c[j] >= 0 ? c[j] : 0;
+                load_row<P, W>(out + size_t(cj) * W, g[j]);
+            } else {
+                c[j] = kPadColumn;
+#pragma unroll
+                for (int r = 0; r < W; ++r) g[j][r] = zero<P>();
+            }
         }
-        // With several right-hand sides the gathered values fill the
-        // registers and a slice takes several chunks: the next chunk's
-        // entries load while this chunk's gathers are in flight (one memory
-        // round trip per chunk instead of two).  The guard is uniform over
-        // the slice.
+        // When a slice can take several chunks (several right-hand sides,
+        // or chunks of 8 complex values), the next chunk's entries load
+        // while this chunk's gathers are in flight (one memory round trip
+        // per chunk instead of two).  The guard is uniform over the slice.
+        constexpr bool kAhead = W > 1 || CH < 16;
         const index_t k1 = k0 + CH;
         index_t cn[CH];
         F vn[CH];
@@ -351,12 +364,14 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
             cn[j] = kPadColumn;
             vn[j] = zero<F>();
         }
-        if (W > 1 && k1 < o1) {
+        if (kAhead && k1 < o1) {
 #pragma unroll
             for (int j = 0; j < CH; ++j) {
-                const size_t at = (size_t(k1) + j) * 64 + lane;
-                cn[j] = col[at];
-                vn[j] = val[at];
+                if (k1 + j < o1) {
+                    const size_t at = (size_t(k1) + j) * 64 + lane;
+                    cn[j] = col[at];
+                    vn[j] = val[at];
+                }
             }
         }
 #pragma unroll
@@ -376,10 +391,10 @@ __device__ inline void sweep_slice(index_t s, const SweepArrays& A, const P* __r
         if (k0 >= o1) break;
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
-            if (W > 1) {
+            if (kAhead) {
                 c[j] = cn[j];
                 v[j] = vn[j];
-            } else {
+            } else if (k0 + j < o1) {
                 const size_t at = (size_t(k0) + j) * 64 + lane;
                 c[j] = col[at];
                 v[j] = val[at];
@@ -424,8 +439,10 @@ __device__ inline int all_stopped(const SolverState* __restrict__ st)
     return stopped;
 }
 
+// Slice s0 + k has its entries at slot slot0 + k * stride.
 template <typename P, typename F, int BR, bool FWD, int CH, int W>
-__global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, SweepArrays A,
+__global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, index_t slot0,
+                                                  index_t stride, SweepArrays A,
                                                   const P* __restrict__ in, P* out,
                                                   WalkExt walk,
                                                   const SolverState* __restrict__ st)
@@ -439,10 +456,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, SweepA
         walk_external<P, F, FWD, W>(walk, walk.t0 + size_t(lb - nreg) * kBlock + threadIdx.x, in, out);
         return;
     }
-    const index_t s = s0 + lb * kWavesPerBlock + (threadIdx.x >> 6);
+    const index_t wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const index_t s = s0 + lb * kWavesPerBlock + wave;
     if (s >= s1) return;                       // whole wavefront: no shuffle below
     const int stopped = all_stopped<W>(st + (A.slice_sys ? A.slice_sys[s] : 0));
-    sweep_slice<P, F, BR, FWD, CH, W>(s, A, in, out, stopped);
+    sweep_slice<P, F, BR, FWD, CH, W>(s, slot0 + (s - s0) * stride, stride, A, in, out, stopped);
 }
 
 // Several consecutive small colours in one workgroup of NT threads: the
@@ -456,6 +474,8 @@ constexpr int kTailMaxPhases = 16;
 struct TailPhases {
     int count;
     index_t slices[kTailMaxPhases + 1];
+    index_t slot[kTailMaxPhases];     // first entry slot of the phase's slices
+    index_t stride[kTailMaxPhases];   // entry slots from one slice to the next
 };
 
 template <typename P, typename F, int BR, bool FWD, int CH, int W>
@@ -464,11 +484,13 @@ __global__ __launch_bounds__((tail_threads<P, BR, W>())) void k_sweep_tail(TailP
                                                                      const SolverState* __restrict__ st)
 {
     if (!A.slice_sys && all_stopped<W>(st)) return;
-    const index_t w = index_t(threadIdx.x >> 6);
+    const index_t w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     constexpr index_t kWaves = tail_threads<P, BR, W>() / 64;
     for (int ph = 0; ph < T.count; ++ph) {
-        for (index_t s = T.slices[ph] + w; s < T.slices[ph + 1]; s += kWaves)
-            sweep_slice<P, F, BR, FWD, CH, W>(s, A, in, out, A.slice_sys ? st[A.slice_sys[s]].status : 0);
+        const index_t sp = T.slices[ph], stride = T.stride[ph];
+        for (index_t s = sp + w; s < T.slices[ph + 1]; s += kWaves)
+            sweep_slice<P, F, BR, FWD, CH, W>(s, T.slot[ph] + (s - sp) * stride, stride, A, in, out,
+                                              A.slice_sys ? st[A.slice_sys[s]].status : 0);
         __syncthreads();
     }
 }

@@ -11,7 +11,9 @@
 //    with the shuffle resolution of in-block entries, the external part and
 //    the step walk of oversized blocks read from the packed records (or the
 //    level fallback) -- and compares them with a sequential substitution
-//    over the host factor.
+//    over the host factor;
+//  * combines the system with a 2-D Laplacian into one batch layout and
+//    checks that the combined sweeps give each system's own results.
 //
 // usage: layout_check <matrix file> <complex 0|1> <block size> <colours> <use_abmc 0|1>
 // matrix file: int32 n, int32 row_ptr[n+1], int32 col_idx[nnz], values[nnz]
@@ -72,21 +74,31 @@ void emulate_sweep(const SweepLayout<T>& L, bool fwd, const std::vector<T>& invd
     for (index_t ph = 0; ph < L.nphases; ++ph) {
         // regular slices
         for (index_t s = L.phase_slices[ph]; s < L.phase_slices[ph + 1]; ++s) {
-            CHECK(size_t(s) + 1 < L.slice_slot.size());
-            const index_t o0 = L.slice_slot[s], o1 = L.slice_slot[s + 1];
-            CHECK(o1 >= o0);
-            // the kernels load whole batches of up to kSweepChunk slots
-            // without a guard
-            const index_t last = o0 + ((o1 - o0 + kSweepChunk - 1) / kSweepChunk) * kSweepChunk;
-            CHECK(size_t(std::max(last, o0 + kSweepChunk)) * kWave <= L.val.size());
+            // the kernels find the slice's slots from the phase's first
+            // slot and stride, its entry count K from the lane info words
+            // (the same in every lane), and load slots of [o0, o0 + stride)
+            // only: the first batch up to the stride, later ones up to K
+            CHECK(size_t(ph) < L.phase_slot.size() && size_t(ph) < L.phase_stride.size());
+            const index_t stride = L.phase_stride[ph];
+            const index_t o0 = L.phase_slot[ph] + (s - L.phase_slices[ph]) * stride;
+            CHECK(size_t(s) + 1 < L.slice_slot.size() && L.slice_slot[s] == o0);
+            CHECK(size_t(s) * kWave + kWave <= L.lane_info.size());
+            const index_t K = SweepLayout<T>::info_len(L.lane_info[size_t(s) * kWave]);
+            for (int l = 0; l < kWave; ++l)
+                CHECK(SweepLayout<T>::info_len(L.lane_info[size_t(s) * kWave + size_t(l)]) == K);
+            const index_t o1 = o0 + K;
+            CHECK(K >= 0 && K <= stride);
+            CHECK(size_t(o0 + stride) * kWave <= L.val.size());
             CHECK(L.val.size() == L.col.size());
+            for (index_t k = o1; k < o0 + stride; ++k)
+                for (int l = 0; l < kWave; ++l) CHECK(L.col[size_t(k) * kWave + size_t(l)] == kPadCol);
             std::vector<T> acc(kWave), res(kWave, T(0)), xin(kWave), dinv(kWave);
             std::vector<std::vector<T>> intv(kWave, std::vector<T>(size_t(BR), T(0)));
             std::vector<index_t> rowv(kWave), posv(kWave);
             for (int l = 0; l < kWave; ++l) {
                 const size_t li = size_t(s) * kWave + size_t(l);
-                CHECK(li < L.lane_row.size() && li < L.lane_pos.size());
-                const index_t row = L.lane_row[li], pos = L.lane_pos[li];
+                CHECK(li < L.lane_row.size() && li < L.lane_info.size());
+                const index_t row = L.lane_row[li], pos = SweepLayout<T>::info_pos(L.lane_info[li]);
                 rowv[l] = row;
                 posv[l] = pos;
                 CHECK(row < n);
@@ -330,11 +342,70 @@ int run(const char* path, int block, int colours, bool abmc)
     }
     const double e_fwd = rel(t, tdev), e_apply = rel(z, zdev);
     CHECK(e_fwd < 1e-12 && e_apply < 1e-12);
+
+    // Batches: the combined layouts of this system, a 2-D Laplacian (other
+    // colours, other strides) and this system again reproduce each
+    // system's own sweeps bit for bit.
+    const char* batch = "batch ok";
+    {
+        const int m = 23, n2 = m * m;
+        std::vector<int32_t> rp2(1, 0), ci2;
+        std::vector<T> v2;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                const int r = i * m + j;
+                const int nb[5] = {r - m, r - 1, r, r + 1, r + m};
+                const bool ok[5] = {i > 0, j > 0, true, j + 1 < m, i + 1 < m};
+                for (int q = 0; q < 5; ++q)
+                    if (ok[q]) {
+                        ci2.push_back(nb[q]);
+                        v2.push_back(T(q == 2 ? 4.0 : -1.0));
+                    }
+                rp2.push_back(int32_t(ci2.size()));
+            }
+        const int ns[3] = {n, n2, n};
+        const int* rps[3] = {rp.data(), rp2.data(), rp.data()};
+        const int* cis[3] = {ci.data(), ci2.data(), ci.data()};
+        const T* vs[3] = {v.data(), v2.data(), v.data()};
+        ssa_host::BatchPlan<T, T> bp;
+        bool built = true;
+        try {
+            ssa_host::build_batch<T, T>(bp, 3, ns, rps, cis, vs, st);
+        } catch (const std::runtime_error& e) {
+            // oversized blocks past the walk, or wide blocks, are refused
+            if (std::strncmp(e.what(), "batch:", 6) != 0) throw;
+            built = false;
+            batch = "batch refused";
+        }
+        if (built) {
+            ssa_host::HostPlan<T, T> p2;
+            ssa_host::build_plan<T, T>(p2, n2, rp2.data(), ci2.data(), v2.data(), st);
+            std::vector<T> x2(static_cast<size_t>(n2)), t2(x2.size(), T(0)), z2(x2.size(), T(0));
+            for (int i = 0; i < n2; ++i) x2[size_t(i)] = T(std::sin(0.37 * i + 0.11) + 0.5);
+            emulate_sweep(p2.fwd, true, p2.invd, x2, t2, n2);
+            emulate_sweep(p2.bwd, false, p2.invd, t2, z2, n2);
+            const int NB = bp.row0.back();
+            CHECK(bp.invd.size() == size_t(NB));
+            std::vector<T> xb(size_t(NB), T(0)), tb(xb.size(), T(0)), zb(xb.size(), T(0));
+            const std::vector<T>* xs[3] = {&x, &x2, &x};
+            const std::vector<T>* ts[3] = {&t, &t2, &t};
+            const std::vector<T>* zs[3] = {&z, &z2, &z};
+            for (int q = 0; q < 3; ++q)
+                for (int i = 0; i < ns[q]; ++i) xb[size_t(bp.row0[q] + i)] = (*xs[q])[size_t(i)];
+            emulate_sweep(bp.fwd, true, bp.invd, xb, tb, NB);
+            emulate_sweep(bp.bwd, false, bp.invd, tb, zb, NB);
+            for (int q = 0; q < 3; ++q)
+                for (int i = 0; i < ns[q]; ++i) {
+                    CHECK(tb[size_t(bp.row0[q] + i)] == (*ts[q])[size_t(i)]);
+                    CHECK(zb[size_t(bp.row0[q] + i)] == (*zs[q])[size_t(i)]);
+                }
+        }
+    }
     std::printf("ok n=%d phases=%d BR=%d big=%d big_rows=%d levels=%d walk_k=%d/%d "
-                "spmv %.1e forward %.1e apply %.1e\n",
+                "spmv %.1e forward %.1e apply %.1e %s\n",
                 n, int(plan.fwd.nphases), plan.fwd.BR, int(plan.fwd.phase_big.back()),
                 int(plan.fwd.max_big_rows), int(plan.fwd.max_levels), plan.fwd.walk_kmax,
-                plan.bwd.walk_kmax, e_spmv, e_fwd, e_apply);
+                plan.bwd.walk_kmax, e_spmv, e_fwd, e_apply, batch);
     return 0;
 }
 
