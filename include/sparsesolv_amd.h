@@ -82,6 +82,9 @@ typedef struct ssa_config {
     int precision;                    /* SSA_PRECISION_FP64 */
     int use_graph;                    /* 1: replay a captured iteration */
     int poll_interval;                /* iterations between host status polls (0: auto) */
+    int walk_overlap;                 /* -1 auto: the straggler walks run on a second stream beside
+                                         the colours that do not read them, where that pays; 0 off;
+                                         1 wherever the schedule allows (read at ssa_ic_create) */
 } ssa_config;
 
 /* sparsesolv::SolverResult (core/types.hpp:29-37) plus timing. */
@@ -111,6 +114,8 @@ typedef struct ssa_ic_info {
     size_t device_bytes;
     size_t factor_bytes;      /* bytes of one sweep's factor stream (values+indices) */
     size_t spmv_bytes;        /* bytes of the SpMV operator stream (values+indices) */
+    int walk_overlap;         /* -1 not decided yet (decided at the first apply), 0 serial sweeps,
+                                 1 straggler walks on a second stream */
 } ssa_ic_info;
 
 void ssa_config_default(ssa_config* cfg);

@@ -57,6 +57,9 @@ struct IcSettings {
     int abmc_block_size = 4;
     int abmc_num_colors = 4;
     bool abmc_use_rcm = false;
+    // device schedule, no counterpart in the reference: -1 auto, 0 off, 1 on
+    // (the straggler walks on a second stream)
+    int walk_overlap = -1;
 };
 
 template <typename T>

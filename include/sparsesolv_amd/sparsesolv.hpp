@@ -64,6 +64,7 @@ struct SolverConfig {
     // GPU fields
     int precision = SSA_PRECISION_FP64;
     bool use_graph = true;
+    int walk_overlap = -1;
     int device = 0;
 
     ssa_config to_c() const
@@ -96,6 +97,7 @@ struct SolverConfig {
         c.abmc_use_rcm = abmc_use_rcm;
         c.precision = precision;
         c.use_graph = use_graph;
+        c.walk_overlap = walk_overlap;
         return c;
     }
 };

@@ -136,6 +136,13 @@ struct DevSweep {
     bool walk_checked = false, walk_fits = false;
     index_t nphases = 0, max_big_rows = 0, max_levels = 0;
     size_t stream_bytes = 0;
+    // The straggler walk's place in the schedule: the one phase with
+    // oversized blocks (-1: none, several, or no walk), the first later
+    // phase whose slices read any of their rows (nphases: none), whether
+    // those rows have entries outside their blocks, and the walk's steps.
+    index_t walk_phase = -1, walk_reader = 0;
+    bool walk_ext = false;
+    size_t walk_steps = 0;
 
     template <typename F>
     void upload(const sparsesolv_amd::SweepLayout<F>& L, hipStream_t s)
@@ -173,6 +180,50 @@ struct DevSweep {
         max_levels = L.max_levels;
         stream_bytes = L.val.size() * sizeof(F) + L.col.size() * 4 + L.big_v.size() * sizeof(F) +
                        L.big_ci.size() * 4;
+        place_walk(L);
+    }
+
+    template <typename F>
+    void place_walk(const sparsesolv_amd::SweepLayout<F>& L)
+    {
+        using Lay = sparsesolv_amd::SweepLayout<F>;
+        walk_phase = -1;
+        walk_reader = L.nphases;
+        walk_ext = false;
+        walk_steps = 0;
+        for (index_t q = 0; q < L.nphases; ++q) {
+            if (L.phase_big[q + 1] == L.phase_big[q]) continue;
+            if (walk_phase >= 0) {
+                walk_phase = -1;   // several phases with oversized blocks
+                return;
+            }
+            walk_phase = q;
+        }
+        if (walk_phase < 0 || L.walk_kmax == 0) {
+            walk_phase = -1;
+            return;
+        }
+        const size_t t0 = size_t(L.walk_step_off[L.phase_big[walk_phase]]) * 64;
+        const size_t t1 = size_t(L.walk_step_off[L.phase_big[walk_phase + 1]]) * 64;
+        walk_steps = (t1 - t0) / 64;
+        walk_ext = L.walk_ext_rp[t1] != L.walk_ext_rp[t0];
+        index_t maxr = -1;
+        for (size_t t = t0; t < t1; ++t) maxr = std::max(maxr, L.walk_row[t]);
+        std::vector<char> big(size_t(maxr + 1), 0);
+        for (size_t t = t0; t < t1; ++t)
+            if (L.walk_row[t] >= 0) big[size_t(L.walk_row[t])] = 1;
+        for (index_t q = walk_phase + 1; q < L.nphases; ++q)
+            for (index_t sl = L.phase_slices[q]; sl < L.phase_slices[q + 1]; ++sl) {
+                const index_t K = Lay::info_len(L.lane_info[size_t(sl) * 64]);
+                const size_t a = size_t(L.slice_slot[sl]) * 64, e = a + size_t(K) * 64;
+                for (size_t k = a; k < e; ++k) {
+                    const index_t c = L.col[k];
+                    if (c >= 0 && c <= maxr && big[size_t(c)]) {
+                        walk_reader = q;
+                        return;
+                    }
+                }
+            }
     }
     // the walk's per-slot scratch for w right-hand sides
     void walk_scratch(int w, hipStream_t s)
@@ -238,6 +289,18 @@ struct IcImpl final : IcBase {
     // trip less often; complex<double> operands, and rows of 8 internal
     // entries, take too many registers for 16 (those walks prefetch 8)
     static constexpr int kWalkPrefetch = sizeof(P) > 8 ? 8 : 16;
+    // The straggler walks beside the colours that do not read them: a
+    // second stream (default priority) forks after the forward colour kernel
+    // of the walk's phase and joins before the first backward phase that
+    // reads the walk's rows.  A fork and join cost ~14 us on the main
+    // stream, so by default (walk_overlap = -1) it is used when the walks
+    // take at least kOverlapSteps steps (~0.13 us each).
+    static constexpr size_t kOverlapSteps = 160;
+    int overlap_setting = -1;
+    int overlap = -1;   // -1 undecided, 0 serial, 1 overlapped
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t sweep_stream = nullptr;   // where sweep kernels go (null: the context's)
     static_assert(sparsesolv_amd::kWalkDepth % kWalkPrefetch == 0, "walk padding");
     DevBuf lo_a, lo_b, lo_t;
     // diagonal_scaling: S = diag(1/sqrt|a_ii|) per device row and the
@@ -256,6 +319,7 @@ struct IcImpl final : IcBase {
            const int32_t* const* ci, const HT* const* v, const sparsesolv_amd::IcSettings& st)
         : ctx(c), count(count_), abmc_order(st.use_abmc), abmc_rcm(st.use_abmc && st.abmc_use_rcm)
     {
+        overlap_setting = st.walk_overlap;
         ssa_host::BatchPlan<HT, HF> plan;
         ssa_host::build_batch<HT, HF>(plan, count, ns, rp, ci, v, st);
         n = plan.row0.back();
@@ -370,6 +434,9 @@ struct IcImpl final : IcBase {
     {
         if (h_state) (void)hipHostFree(h_state);
         if (h_pass) (void)hipHostFree(h_pass);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
     }
 
     // ---- building blocks ---------------------------------------------------
@@ -473,10 +540,11 @@ struct IcImpl final : IcBase {
 
     // end of the run of small colours that starts at ph (ph itself when
     // fewer than two colours would share the tail kernel)
-    index_t tail_end(const DevSweep& D, index_t ph) const
+    index_t tail_end(const DevSweep& D, index_t ph, index_t limit = -1) const
     {
+        if (limit < 0) limit = D.nphases;
         index_t e = ph;
-        while (e < D.nphases && e - ph < ssa::kTailMaxPhases && small_phase(D, e)) ++e;
+        while (e < limit && e - ph < ssa::kTailMaxPhases && small_phase(D, e)) ++e;
         return e - ph >= 2 ? e : ph;
     }
 
@@ -494,21 +562,103 @@ struct IcImpl final : IcBase {
         return false;
     }
 
+    // what launch_phase enqueues of a phase: everything; the colour kernel
+    // with the external part of the oversized blocks but not their walk;
+    // the regular slices only; the external part only; the walk only
+    enum PhasePart { kWhole, kNoWalk, kRegularOnly, kExtOnly, kWalkOnly };
+
+    hipStream_t lstream() const { return sweep_stream ? sweep_stream : ctx->stream; }
+
     // all colours of one sweep; runs of small colours go to one workgroup
     template <bool FWD>
     void launch_sweep(DevSweep& D, const P* in, P* out, const SolverState* st)
     {
-        index_t ph = 0;
-        while (ph < D.nphases) {
-            const index_t e = tail_end(D, ph);
+        launch_phases<FWD>(D, 0, D.nphases, in, out, st);
+    }
+
+    // phases [a, b); phase `special` (never in a run of small colours: it
+    // has oversized blocks) enqueues only `part`
+    template <bool FWD>
+    void launch_phases(DevSweep& D, index_t a, index_t b, const P* in, P* out,
+                       const SolverState* st, index_t special = -1, PhasePart part = kWhole)
+    {
+        index_t ph = a;
+        while (ph < b) {
+            const index_t e = tail_end(D, ph, b);
             if (e > ph) {
                 launch_tail<FWD>(D, ph, e, in, out, st);
                 ph = e;
             } else {
-                launch_phase<FWD>(D, ph, in, out, st);
+                launch_phase<FWD>(D, ph, in, out, st, ph == special ? part : kWhole);
                 ++ph;
             }
         }
+    }
+
+    // launches of phases [a, b) as launch_phases groups them
+    index_t count_launches(const DevSweep& D, index_t a, index_t b) const
+    {
+        index_t k = 0;
+        for (index_t ph = a; ph < b; ++k) {
+            const index_t e = tail_end(D, ph, b);
+            ph = e > ph ? e : ph + 1;
+        }
+        return k;
+    }
+
+    // whether the walks run on the second stream; decided (and the stream
+    // made) at the first apply
+    bool use_overlap()
+    {
+        if (overlap >= 0) return overlap == 1;
+        overlap = 0;
+        const bool shape = overlap_setting != 0 && fwd.walk_phase >= 0 && bwd.walk_phase >= 0 &&
+                           fwd.walk_reader == fwd.nphases && !bwd.walk_ext &&
+                           walk_lds_ok(fwd) && walk_lds_ok(bwd);
+        if (!shape) return false;
+        const index_t window = count_launches(fwd, fwd.walk_phase + 1, fwd.nphases) +
+                               count_launches(bwd, 0, bwd.walk_reader);
+        if (overlap_setting < 0 && (fwd.walk_steps + bwd.walk_steps < kOverlapSteps || window < 3))
+            return false;
+        HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        overlap = 1;
+        return true;
+    }
+
+    // Both sweeps, out = M^-1 in through tbuf.  Overlapped: the forward
+    // walk reads the external sums its colour kernel wrote and writes rows
+    // no later forward colour reads; the backward walk's rows read no row
+    // outside their blocks (no external part), so its input -- the forward
+    // walk's rows of tbuf -- is all it waits for; the main stream joins
+    // before the first backward phase that reads the walk's rows.
+    void launch_sweeps(const P* in, P* out, P* tbuf, const SolverState* st)
+    {
+        if (!use_overlap()) {
+            launch_sweep<true>(fwd, in, tbuf, st);
+            launch_sweep<false>(bwd, tbuf, out, st);
+            return;
+        }
+        const index_t pf = fwd.walk_phase, pb = bwd.walk_phase, qb = bwd.walk_reader;
+        hipStream_t s = ctx->stream;
+        launch_phases<true>(fwd, 0, pf + 1, in, tbuf, st, pf, kNoWalk);
+        HIP_CHECK(hipEventRecord(ev_fork, s));
+        HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
+        struct Restore {
+            hipStream_t& v;
+            ~Restore() { v = nullptr; }
+        } restore{sweep_stream};
+        sweep_stream = side;
+        launch_phase<true>(fwd, pf, in, tbuf, st, kWalkOnly);
+        launch_phase<false>(bwd, pb, tbuf, out, st, kExtOnly);
+        launch_phase<false>(bwd, pb, tbuf, out, st, kWalkOnly);
+        sweep_stream = nullptr;
+        HIP_CHECK(hipEventRecord(ev_join, side));
+        launch_phases<true>(fwd, pf + 1, fwd.nphases, in, tbuf, st);
+        launch_phases<false>(bwd, 0, qb, tbuf, out, st, pb, kRegularOnly);
+        HIP_CHECK(hipStreamWaitEvent(s, ev_join, 0));
+        launch_phases<false>(bwd, qb, bwd.nphases, tbuf, out, st);
     }
 
     template <bool FWD>
@@ -542,7 +692,7 @@ struct IcImpl final : IcBase {
                 const SolverState* st)
     {
         constexpr int CH = WV == 1 ? 8 : sweep_chunk<P, WV>();
-        ssa::k_sweep_tail<P, F, BRV, FWD, CH, WV><<<1, ssa::tail_threads<P, BRV, WV>(), 0, ctx->stream>>>(
+        ssa::k_sweep_tail<P, F, BRV, FWD, CH, WV><<<1, ssa::tail_threads<P, BRV, WV>(), 0, lstream()>>>(
             tp, A, in, out, st);
     }
 
@@ -566,7 +716,7 @@ struct IcImpl final : IcBase {
                  const ssa::SweepArrays& A, const P* in, P* out, const ssa::WalkExt& we,
                  const SolverState* st)
     {
-        ssa::k_sweep<P, F, BRV, FWD, sweep_chunk<P, WV>(), WV><<<g, ssa::kBlock, 0, ctx->stream>>>(
+        ssa::k_sweep<P, F, BRV, FWD, sweep_chunk<P, WV>(), WV><<<g, ssa::kBlock, 0, lstream()>>>(
             s0, s1, slot0, stride, A, in, out, we, st);
     }
 
@@ -585,11 +735,12 @@ struct IcImpl final : IcBase {
     }
 
     template <bool FWD>
-    void launch_phase(DevSweep& D, index_t ph, const P* in, P* out, const SolverState* st)
+    void launch_phase(DevSweep& D, index_t ph, const P* in, P* out, const SolverState* st,
+                      PhasePart part = kWhole)
     {
         if (!st) st = idle.as<SolverState>();
-        hipStream_t s = ctx->stream;
-        const index_t s0 = D.phase_slices[ph], s1 = D.phase_slices[ph + 1];
+        hipStream_t s = lstream();
+        index_t s0 = D.phase_slices[ph], s1 = D.phase_slices[ph + 1];
         const index_t b0 = D.phase_big[ph], b1 = D.phase_big[ph + 1];
         const bool walk = b1 > b0 && D.walk_kmax > 0 && walk_lds_ok(D);
         ssa::WalkExt we{};
@@ -606,6 +757,8 @@ struct IcImpl final : IcBase {
             we.ws = D.walk_slots;
             gw = int((we.t1 - we.t0 + ssa::kBlock - 1) / ssa::kBlock);
         }
+        if (part == kWalkOnly || part == kExtOnly) s1 = s0;   // no regular slices
+        if (part == kWalkOnly || part == kRegularOnly) gw = 0;
         if (s1 > s0 || gw > 0) {
             const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
             const ssa::SweepArrays A = sweep_arrays(D);
@@ -622,7 +775,7 @@ struct IcImpl final : IcBase {
 #undef SSA_SWEEP
             HIP_CHECK(hipGetLastError());
         }
-        if (walk) {
+        if (walk && (part == kWhole || part == kWalkOnly)) {
             const size_t lds = (size_t(D.max_big_rows) + 2) * sizeof(P);
 #define SSA_WALK(KM, DEPTH)                                                                    \
     ssa::k_sweep_walk<P, F, FWD, KM, DEPTH><<<dim3(unsigned(b1 - b0), unsigned(W)), 64, lds, s>>>( \
@@ -636,7 +789,8 @@ struct IcImpl final : IcBase {
             }
 #undef SSA_WALK
             HIP_CHECK(hipGetLastError());
-        } else if (b1 > b0) {
+        } else if (!walk && b1 > b0) {
+            // level fallback (never split: the overlap needs the walk)
             const size_t lds = size_t(D.max_big_rows) * sizeof(P);
             const int use_lds = lds <= 64 * 1024 ? 1 : 0;
             ssa::k_sweep_big<P, F, FWD><<<dim3(unsigned(b1 - b0), unsigned(W)), 1024, use_lds ? lds : 0, s>>>(
@@ -677,13 +831,11 @@ struct IcImpl final : IcBase {
         if (scale_apply) {
             // S M^-1 S: the sweeps between two scalings
             launch_scale(in, sbuf.as<P>());
-            launch_sweep<true>(fwd, sbuf.as<P>(), tbuf, st);
-            launch_sweep<false>(bwd, tbuf, out, st);
+            launch_sweeps(sbuf.as<P>(), out, tbuf, st);
             launch_scale(out, out);
             return;
         }
-        launch_sweep<true>(fwd, in, tbuf, st);
-        launch_sweep<false>(bwd, tbuf, out, st);
+        launch_sweeps(in, out, tbuf, st);
     }
 
     void launch_scale(const P* in, P* out)
@@ -1413,6 +1565,7 @@ struct IcImpl final : IcBase {
                          invd.bytes + fwd.bytes() + bwd.bytes();
         o.factor_bytes = fwd.stream_bytes;
         o.spmv_bytes = spmv_bytes;
+        o.walk_overlap = ctx ? overlap : 0;
     }
 
     void one_system(const char* who) const

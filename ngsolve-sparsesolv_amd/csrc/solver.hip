@@ -79,6 +79,7 @@ void ssa_config_default(ssa_config* c)
     c->precision = SSA_PRECISION_FP64;
     c->use_graph = 1;
     c->poll_interval = 0;
+    c->walk_overlap = -1;
 }
 
 const char* ssa_last_error(void) { return g_last_error.c_str(); }
@@ -156,6 +157,7 @@ static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int c
     st.abmc_block_size = cfg->abmc_block_size;
     st.abmc_num_colors = cfg->abmc_num_colors;
     st.abmc_use_rcm = cfg->abmc_use_rcm != 0;
+    st.walk_overlap = cfg->walk_overlap < 0 ? -1 : cfg->walk_overlap > 0 ? 1 : 0;
     auto h = std::make_unique<ssa_ic>();
     h->dtype = dtype;
     // mixed: the factor (computed in FP64) is stored and applied in

@@ -62,7 +62,7 @@ class Config(C.Structure):
         ("conjugate", C.c_int), ("use_abmc", C.c_int), ("abmc_block_size", C.c_int),
         ("abmc_num_colors", C.c_int), ("abmc_reorder_spmv", C.c_int),
         ("abmc_use_rcm", C.c_int), ("precision", C.c_int), ("use_graph", C.c_int),
-        ("poll_interval", C.c_int),
+        ("poll_interval", C.c_int), ("walk_overlap", C.c_int),
     ]
 
 
@@ -80,7 +80,8 @@ class IcInfo(C.Structure):
                 ("max_big_rows", C.c_int), ("max_levels", C.c_int),
                 ("shift_trials", C.c_int), ("actual_shift", C.c_double),
                 ("setup_seconds", C.c_double), ("device_bytes", C.c_size_t),
-                ("factor_bytes", C.c_size_t), ("spmv_bytes", C.c_size_t)]
+                ("factor_bytes", C.c_size_t), ("spmv_bytes", C.c_size_t),
+                ("walk_overlap", C.c_int)]
 
 
 class SsaError(RuntimeError):
