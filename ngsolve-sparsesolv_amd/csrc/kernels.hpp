@@ -647,6 +647,12 @@ __global__ __launch_bounds__(64) void k_sweep_walk(
 
 // partial sums of conj?(a_i) b_i, accumulated in T (operands converted;
 // each product rounded, then added, as the block kernel adds them)
+// rows a thread of the vector kernels loads before it uses any: its rows of
+// a grid-stride loop come kRowsAhead at a time (one memory round trip for
+// up to 4 rows), and the first group is requested before the kernel's
+// control step sums the producer's partials
+constexpr int kRowsAhead = 4;
+
 template <typename T, typename TA, typename TB>
 __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const TA* __restrict__ a,
                                                const TB* __restrict__ b, int conj,
@@ -658,8 +664,21 @@ __global__ __launch_bounds__(kBlock) void k_dot(index_t n, const TA* __restrict_
     if (st && st[sp.sys].status) return;
     __shared__ T red[kWavesPerBlock];
     T d = zero<T>();
-    for (index_t i = sp.local * kBlock + threadIdx.x; i < sp.n; i += sp.G * kBlock)
-        d = add(d, mul(conj_if(cvt<T>(a[sp.row0 + i]), conj != 0), cvt<T>(b[sp.row0 + i])));
+    const index_t step = sp.G * kBlock;
+    for (index_t jb = sp.local * kBlock + threadIdx.x; jb < sp.n; jb += kRowsAhead * step) {
+        // a thread's next kRowsAhead rows load together, then add in order
+        TA av[kRowsAhead];
+        TB bv[kRowsAhead];
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            av[u] = j < sp.n ? a[sp.row0 + j] : TA{};
+            bv[u] = j < sp.n ? b[sp.row0 + j] : TB{};
+        }
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u)
+            if (jb + u * step < sp.n) d = add(d, mul(conj_if(cvt<T>(av[u]), conj != 0), cvt<T>(bv[u])));
+    }
     T t = block_sum(d, red);
     if (threadIdx.x == 0) partials[sp.slot] = t;
     if (check.part) {
@@ -755,18 +774,41 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(index_t n, T* __restrict__
 {
     const Span sp = vec_span(sy, n);
     if (st[sp.sys].status) return;
+    // the operands of this thread's first rows load before alpha is summed
+    const index_t step = sp.G * kBlock, j0 = sp.local * kBlock + threadIdx.x;
+    T xv[kRowsAhead], pv[kRowsAhead], av[kRowsAhead], rv[kRowsAhead];
+    auto load = [&](index_t jb) {
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            if (j >= sp.n) break;
+            const index_t i = sp.row0 + j;
+            xv[u] = x[i];
+            pv[u] = p[i];
+            av[u] = Ap[i];
+            rv[u] = r[i];
+        }
+    };
+    load(j0);
     __shared__ double red[kWavesPerBlock];
     __shared__ T redt[kWavesPerBlock];
     T alpha;
     if (!ctrl_alpha_all(ctrl_for(ctl, sy, sp.sys, sp.local), redt, alpha)) return;
     double s = 0.0;
-    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock) {
-        const index_t i = sp.row0 + j;
-        x[i] = fma_(alpha, p[i], x[i]);
-        const T ri = fms_(alpha, Ap[i], r[i]);
-        r[i] = ri;
-        if (r_low) r_low[i] = cvt<P>(ri);
-        s += norm2(ri);
+    for (index_t jb = j0; jb < sp.n;) {
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            if (j >= sp.n) break;
+            const index_t i = sp.row0 + j;
+            x[i] = fma_(alpha, pv[u], xv[u]);
+            const T ri = fms_(alpha, av[u], rv[u]);
+            r[i] = ri;
+            if (r_low) r_low[i] = cvt<P>(ri);
+            s += norm2(ri);
+        }
+        jb += kRowsAhead * step;
+        if (jb < sp.n) load(jb);
     }
     double t = block_sum(s, red);
     if (threadIdx.x == 0) partials[sp.slot] = t;
@@ -796,12 +838,33 @@ __global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const TZ* __restrict
     const Span sp = vec_span(sy, n);
     copy_if_improved(sp, x, best, st);
     if (st[sp.sys].status) return;
+    // the operands of this thread's first rows load before beta is summed
+    const index_t step = sp.G * kBlock, j0 = sp.local * kBlock + threadIdx.x;
+    T pv[kRowsAhead];
+    TZ zv[kRowsAhead];
+    auto load = [&](index_t jb) {
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            if (j >= sp.n) break;
+            const index_t i = sp.row0 + j;
+            pv[u] = p[i];
+            zv[u] = z[i];
+        }
+    };
+    load(j0);
     __shared__ T redt[kWavesPerBlock];
     T beta;
     if (!ctrl_beta_all(ctrl_for(ctl, sy, sp.sys, sp.local), redt, beta)) return;
-    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock) {
-        const index_t i = sp.row0 + j;
-        p[i] = fma_(beta, p[i], cvt<T>(z[i]));
+    for (index_t jb = j0; jb < sp.n;) {
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            if (j >= sp.n) break;
+            p[sp.row0 + j] = fma_(beta, pv[u], cvt<T>(zv[u]));
+        }
+        jb += kRowsAhead * step;
+        if (jb < sp.n) load(jb);
     }
 }
 
@@ -819,18 +882,44 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
 {
     const Span sp = vec_span(sy, n);
     if (st[sp.sys].status) return;
+    // the operands of this thread's first rows load before alpha is summed
+    const index_t step = sp.G * kBlock, j0 = sp.local * kBlock + threadIdx.x;
+    T xv[kRowsAhead], pv[kRowsAhead], rv[kRowsAhead], qv[kRowsAhead], rtv[kRowsAhead];
+    TQ qtv[kRowsAhead];
+    auto load = [&](index_t jb) {
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            if (j >= sp.n) break;
+            const index_t i = sp.row0 + j;
+            xv[u] = x[i];
+            pv[u] = p[i];
+            rv[u] = r[i];
+            qv[u] = q[i];
+            rtv[u] = rt[i];
+            qtv[u] = qt[i];
+        }
+    };
+    load(j0);
     __shared__ double red[kWavesPerBlock];
     __shared__ T redt[kWavesPerBlock];
     T alpha;
     if (!ctrl_alpha_all(ctrl_for(ctl, sy, sp.sys, sp.local), redt, alpha)) return;
     double s = 0.0;
-    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock) {
-        const index_t i = sp.row0 + j;
-        x[i] = fma_(alpha, p[i], x[i]);
-        const T ri = fms_(alpha, q[i], r[i]);
-        r[i] = ri;
-        rt[i] = fms_(alpha, cvt<T>(qt[i]), rt[i]);
-        s += norm2(ri);
+    for (index_t jb = j0; jb < sp.n;) {
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            if (j >= sp.n) break;
+            const index_t i = sp.row0 + j;
+            x[i] = fma_(alpha, pv[u], xv[u]);
+            const T ri = fms_(alpha, qv[u], rv[u]);
+            r[i] = ri;
+            rt[i] = fms_(alpha, cvt<T>(qtv[u]), rtv[u]);
+            s += norm2(ri);
+        }
+        jb += kRowsAhead * step;
+        if (jb < sp.n) load(jb);
     }
     double t = block_sum(s, red);
     if (threadIdx.x == 0) partials[sp.slot] = t;
@@ -849,15 +938,38 @@ __global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restri
     const Span sp = vec_span(sy, n);
     copy_if_improved(sp, x, best, st);
     if (st[sp.sys].status) return;
+    // the operands of this thread's first rows load before beta is summed
+    const index_t step = sp.G * kBlock, j0 = sp.local * kBlock + threadIdx.x;
+    T pv[kRowsAhead], rtv[kRowsAhead], qv[kRowsAhead], tv[kRowsAhead];
+    auto load = [&](index_t jb) {
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            if (j >= sp.n) break;
+            const index_t i = sp.row0 + j;
+            pv[u] = p[i];
+            rtv[u] = rt[i];
+            qv[u] = q[i];
+            tv[u] = t[i];
+        }
+    };
+    load(j0);
     __shared__ T redt[kWavesPerBlock];
     T beta;
     if (!ctrl_beta_all(ctrl_for(ctl, sy, sp.sys, sp.local), redt, beta)) return;
-    for (index_t j = sp.local * kBlock + threadIdx.x; j < sp.n; j += sp.G * kBlock) {
-        const index_t i = sp.row0 + j;
-        p[i] = fma_(beta, p[i], rt[i]);
-        const T qi = fma_(beta, q[i], t[i]);
-        q[i] = qi;
-        if (q_low) q_low[i] = cvt<P>(qi);
+    for (index_t jb = j0; jb < sp.n;) {
+#pragma unroll
+        for (int u = 0; u < kRowsAhead; ++u) {
+            const index_t j = jb + u * step;
+            if (j >= sp.n) break;
+            const index_t i = sp.row0 + j;
+            p[i] = fma_(beta, pv[u], rtv[u]);
+            const T qi = fma_(beta, qv[u], tv[u]);
+            q[i] = qi;
+            if (q_low) q_low[i] = cvt<P>(qi);
+        }
+        jb += kRowsAhead * step;
+        if (jb < sp.n) load(jb);
     }
 }
 
