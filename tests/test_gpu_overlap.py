@@ -91,3 +91,23 @@ def test_default_takes_the_overlap_for_long_walks_only():
     s = _lib.IcHandle(P.laplace7(24), _lib.default_config(use_abmc=1))
     s.apply(np.ones(24 ** 3))
     assert s.info()["walk_overlap"] == 0
+
+
+def test_batch_overlapped_equals_serial():
+    # two systems with the same pattern put their straggler blocks in the
+    # same colour: one walk launch per direction holds both, on the side
+    # stream; a 16^3 curl-curl beside them puts another straggler in another
+    # colour, and the batch stays serial
+    mats = [P.laplace7_varcoef(24, 10.0 ** P.uniform(s, 24 ** 3, -1.0, 1.0)) for s in (1, 2)]
+    bs = [P.uniform(40 + k, A.n, -1.0, 1.0) for k, A in enumerate(mats)]
+    for extra, expect in (([], 1), ([P.curlcurl(16).A], 0)):
+        ms = mats + extra
+        b = np.concatenate(bs + [P.uniform(50, A.n, -1.0, 1.0) for A in extra])
+        res = {}
+        for ov in (1, 0):
+            cfg = _lib.default_config(use_abmc=1, auto_shift=1, walk_overlap=ov)
+            h = _lib.BatchHandle(ms, cfg)
+            x, results = h.solve_batch(_lib.METHOD_ICCG, b, None, cfg)
+            res[ov] = (x, [r.iterations for r in results], h.info()["walk_overlap"])
+        assert res[1][2] == expect and res[0][2] == 0
+        assert np.array_equal(res[1][0], res[0][0]) and res[1][1] == res[0][1]
