@@ -151,6 +151,14 @@ int ssa_ic_factor(const ssa_ic* ic, int32_t* row_ptr, int32_t* col_idx, void* va
 int ssa_ic_apply(ssa_ic* ic, const void* x, void* y, int on_device);
 /* SparseMatrixView::multiply (core/sparse_matrix_view.hpp:139-147): y = A x on the GPU. */
 int ssa_spmv(ssa_ic* ic, const void* x, void* y, int on_device);
+/* The operator of the handle's solves and SpMV replaced by another matrix of the
+ * same order and dtype (any pattern); the preconditioner keeps its factor.  The
+ * reference pairs an operator with a preconditioner set up apart:
+ * IterativeSolver::solve(A, b, x, n, precond) (solvers/iterative_solver.hpp:58-80)
+ * and COCRSolverNGS(mat, pre) (ngsolve/sparsesolv_solvers.hpp:256-396).  One system
+ * per handle (SSA_ERR_UNSUPPORTED for a batch); the matrix is copied. */
+int ssa_ic_set_operator(ssa_ic* ic, int n, const int32_t* row_ptr, const int32_t* col_idx,
+                        const void* values);
 
 /* solve_iccg / solve_cocr (sparsesolv.hpp:43-105, :153-169), CGSolver / COCRSolver
  * without a preconditioner (ngsolve/sparsesolv_solvers.hpp:116-123), and the

@@ -190,6 +190,33 @@ public:
         ic_.reset(raw, [](ssa_ic* p) { ssa_ic_destroy(p); });
         n_ = A.rows();
         a_ = A.values();
+        // kept for solves of another operator with this factor
+        const index_t nnz = A.rows() > 0 ? A.row_ptr()[A.rows()] : 0;
+        m_rp_.assign(A.row_ptr(), A.row_ptr() + A.rows() + 1);
+        m_ci_.assign(A.col_idx(), A.col_idx() + nnz);
+        m_v_.assign(A.values(), A.values() + nnz);
+        op_.reset();
+    }
+
+    // The handle a solve of operator A runs on: this preconditioner's own when
+    // A is the matrix it was set up on; otherwise a second handle with the
+    // same factor (set up once on the kept copy of that matrix) whose
+    // operator is A, uploaded at every call (A's values may have changed) --
+    // the reference's solvers use a preconditioner as given, whatever matrix
+    // it was set up on (solvers/iterative_solver.hpp:58-80).
+    ssa_ic* handle_for(const SparseMatrixView<Scalar>& A) const
+    {
+        if (A.values() == a_) return ic_.get();
+        if (A.rows() != n_) throw std::invalid_argument("solve: the preconditioner's order differs");
+        if (!op_) {
+            ssa_config c = config_.to_c();
+            ssa_ic* raw = nullptr;
+            detail::check(ssa_ic_create(ctx_.get(), detail::dtype_of<Scalar>(), n_, m_rp_.data(),
+                                        m_ci_.data(), m_v_.data(), &c, &raw));
+            op_.reset(raw, [](ssa_ic* p) { ssa_ic_destroy(p); });
+        }
+        detail::check(ssa_ic_set_operator(op_.get(), A.rows(), A.row_ptr(), A.col_idx(), A.values()));
+        return op_.get();
     }
 
     void apply(const Scalar* x, Scalar* y, index_t size) const
@@ -225,6 +252,9 @@ private:
     std::shared_ptr<ssa_ic> ic_;
     index_t n_ = 0;
     const Scalar* a_ = nullptr;
+    std::vector<index_t> m_rp_, m_ci_;
+    std::vector<Scalar> m_v_;
+    mutable std::shared_ptr<ssa_ic> op_;
 };
 
 namespace detail {
@@ -235,21 +265,24 @@ SolverResult run(int method, const SparseMatrixView<Scalar>& A, const Scalar* b,
 {
     if (size != A.rows()) throw std::invalid_argument("solve: size mismatch");
     ICPreconditioner<Scalar> local;
-    const ICPreconditioner<Scalar>* use = pc;
-    if (!use || !use->is_ready() || use->matrix_values() != A.values()) {
-        // operator only (or a preconditioner of another matrix is not usable
-        // by the GPU loop): build the handle on A with the solver config
+    ssa_ic* h = nullptr;
+    if (pc && pc->is_ready()) {
+        // the preconditioner as given, on A's operator
+        h = pc->handle_for(A);
+    } else {
+        // operator only (or a preconditioner never set up): a handle on A
+        // with the solver config
         SolverConfig c = cfg;
         if (!pc) c.use_abmc = true;
         else c = pc->config();
         local.set_config(c);
         local.setup(A);
-        use = &local;
+        h = local.handle();
     }
     ssa_config c = cfg.to_c();
     std::vector<double> hist(static_cast<size_t>(std::max(cfg.max_iterations, 0)) + 4);
     ssa_result r;
-    check(ssa_solve(use->handle(), method, b, x, 0, &c, &r, hist.data()));
+    check(ssa_solve(h, method, b, x, 0, &c, &r, hist.data()));
     SolverResult out;
     out.converged = r.converged != 0;
     out.iterations = r.iterations;

@@ -36,6 +36,8 @@ struct IcBase {
     // nrhs vectors of info().n values each, back to back
     virtual void apply(const void* x, void* y, bool on_device, int nrhs) = 0;
     virtual void spmv(const void* x, void* y, bool on_device, int nrhs) = 0;
+    virtual void set_operator(int n, const int32_t* row_ptr, const int32_t* col_idx,
+                              const void* values) = 0;
     virtual void solve(int method, const void* b, void* x, bool on_device, const ssa_config& cfg,
                        ssa_result* results, double* history, int nrhs) = 0;
     virtual int system_count() const = 0;

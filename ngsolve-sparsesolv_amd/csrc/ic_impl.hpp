@@ -1547,6 +1547,30 @@ struct IcImpl final : IcBase {
         }
     }
 
+    // the operator of the solves and the SpMV: another matrix of the same
+    // order, in the device order (the factor and its layouts stay)
+    void set_operator(int nn, const int32_t* rp, const int32_t* ci, const void* vals) override
+    {
+        need_device();
+        if (count != 1)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "ssa_ic_set_operator: not available for a batched handle");
+        if (nn != n)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_set_operator: the operator must have the handle's order");
+        const auto B = sparsesolv_amd::permute_csr<HT>(nn, rp, ci, static_cast<const HT*>(vals), h_ordering);
+        sparsesolv_amd::SpmvLayout<HT> L;
+        L.build(B, h_ordering);
+        hipStream_t s = ctx->stream;
+        HIP_CHECK(hipStreamSynchronize(s));   // queued work may still read the old layout
+        A_off.upload(L.slice_off, s);
+        A_val.upload(L.val, s);
+        A_col.upload(L.col, s);
+        A_row.upload(L.row, s);
+        HIP_CHECK(hipStreamSynchronize(s));   // the host arrays end with this call
+        A_nslices = L.nslices;
+        spmv_bytes = L.val.size() * sizeof(HT) + L.col.size() * 4;
+        nnz = nn > 0 ? rp[nn] : 0;
+    }
+
     void info(ssa_ic_info& o) const override
     {
         o.n = tot;   // batches: all systems' rows (colours: the most of any system)

@@ -260,6 +260,16 @@ int ssa_spmv(ssa_ic* ic, const void* x, void* y, int on_device)
     });
 }
 
+int ssa_ic_set_operator(ssa_ic* ic, int n, const int32_t* row_ptr, const int32_t* col_idx,
+                        const void* values)
+{
+    return guarded([&] {
+        if (!ic) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_set_operator: null argument");
+        check_matrix("ssa_ic_set_operator", n, row_ptr, col_idx, values);
+        ic->impl->set_operator(n, row_ptr, col_idx, values);
+    });
+}
+
 int ssa_spmv_multi(ssa_ic* ic, int nrhs, const void* x, void* y, int on_device)
 {
     return guarded([&] {

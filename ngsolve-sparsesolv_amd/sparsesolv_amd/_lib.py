@@ -38,6 +38,7 @@ EXPORTED = (
     "ssa_ic_schedule", "ssa_ic_factor", "ssa_ic_apply", "ssa_spmv", "ssa_solve",
     "ssa_ic_profile", "ssa_batch_create", "ssa_batch_solve", "ssa_ic_systems",
     "ssa_solve_multi", "ssa_ic_apply_multi", "ssa_spmv_multi", "ssa_ic_profile_multi",
+    "ssa_ic_set_operator",
 )
 
 PROFILE_SPMV = 0
@@ -120,6 +121,7 @@ def load() -> C.CDLL:
         L.ssa_ic_factor.argtypes = [vp, vp, vp, vp, vp]
         L.ssa_ic_apply.argtypes = [vp, vp, vp, C.c_int]
         L.ssa_spmv.argtypes = [vp, vp, vp, C.c_int]
+        L.ssa_ic_set_operator.argtypes = [vp, C.c_int, i32p, i32p, vp]
         L.ssa_solve.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.POINTER(Config),
                                 C.POINTER(Result), vp]
         L.ssa_ic_profile.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -266,6 +268,16 @@ class IcHandle:
         y = np.empty_like(x)
         check(self.lib.ssa_spmv(self.h, x.ctypes.data, y.ctypes.data, 0))
         return y
+
+    def set_operator(self, A) -> None:
+        """The operator of this handle's solves and SpMV becomes A (same order
+        and dtype); the IC factor stays."""
+        if A.is_complex != self.is_complex:
+            raise TypeError("set_operator: the operator's dtype must match the handle's")
+        vals = np.ascontiguousarray(A.values, dtype=self.dtype)
+        rp = np.ascontiguousarray(A.row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(A.col_idx, dtype=np.int32)
+        check(self.lib.ssa_ic_set_operator(self.h, A.n, rp, ci, vals.ctypes.data))
 
     def apply_device(self, x_ptr: int, y_ptr: int) -> None:
         check(self.lib.ssa_ic_apply(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), 1))

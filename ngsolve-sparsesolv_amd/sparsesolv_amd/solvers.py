@@ -399,6 +399,8 @@ class _COCRSolver:
         self.tol = float(tol)
         self.printrates = bool(printrates)
         self._iterations = 0
+        self._own = None        # handle: pre's factor, this solver's operator
+        self._own_key = None
 
     @property
     def iterations(self) -> int:
@@ -407,6 +409,22 @@ class _COCRSolver:
     def GetIterations(self) -> int:
         return self._iterations
 
+    def _handle(self, fd) -> _lib.IcHandle:
+        """pre's handle when the operator is pre's matrix; otherwise a handle
+        of this solver's own with pre's factor (set up on pre's matrix, as pre
+        sets it up) and this solver's matrix as the operator
+        (ssa_ic_set_operator), made again when pre is updated."""
+        h = self.pre.handle
+        if self._same_operator():
+            return h
+        if self._own is None or self._own_key is not h:
+            own = _lib.IcHandle(_with_freedofs(self.pre.mat, self.pre.freedofs), self.pre.config())
+            own.set_operator(_with_freedofs(self.mat, fd))
+            if self._own is not None:
+                self._own.close()
+            self._own, self._own_key = own, h
+        return self._own
+
     def _same_operator(self) -> bool:
         a, b = self.mat, self.pre.mat
         return (a is b) or (a.n == b.n and np.array_equal(a.row_ptr, b.row_ptr)
@@ -414,12 +432,9 @@ class _COCRSolver:
                             and np.array_equal(a.values, b.values))
 
     def Mult(self, rhs, sol=None):
-        if not self._same_operator():
-            raise NotImplementedError("COCRSolver: the GPU path needs the preconditioner "
-                                      "to be built on the solver's matrix")
-        h = self.pre.handle
         b = np.asarray(rhs, dtype=self._dtype)
         fd = self.freedofs if self.freedofs is not None else self.pre.freedofs
+        h = self._handle(fd)
         if fd is not None:
             b = np.where(fd, b, 0)
         cfg = _lib.default_config(tolerance=self.tol, max_iterations=self.maxiter,

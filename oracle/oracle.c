@@ -416,6 +416,22 @@ int orc_solve_cocr(int dtype, int n, const int* row_ptr, const int* col_idx,
                        history, err, errlen);
 }
 
+int orc_solve_cocr_pre(int dtype, int n, const int* row_ptr, const int* col_idx,
+                       const void* values, const int* m_row_ptr, const int* m_col_idx,
+                       const void* m_values, const void* b, void* x,
+                       const orc_config* cfg, orc_result* res, double* history,
+                       char* err, int errlen)
+{
+    return dtype == 0
+        ? solve_cocr_pre_d(n, row_ptr, col_idx, (const double*)values, m_row_ptr, m_col_idx,
+                           (const double*)m_values, (const double*)b, (double*)x, cfg, res,
+                           history, err, errlen)
+        : solve_cocr_pre_z(n, row_ptr, col_idx, (const double _Complex*)values, m_row_ptr,
+                           m_col_idx, (const double _Complex*)m_values,
+                           (const double _Complex*)b, (double _Complex*)x, cfg, res, history,
+                           err, errlen);
+}
+
 int orc_solve_plain(int dtype, int method_cocr, int n, const int* row_ptr,
                     const int* col_idx, const void* values, const void* b,
                     void* x, const orc_config* cfg, orc_result* res,

@@ -109,6 +109,13 @@ int orc_solve_cocr(int dtype, int n, const int* row_ptr, const int* col_idx,
                    char* err, int errlen);
 /* Unpreconditioned CG / COCR (SparseSolvSolver method "CG"/"COCR",
  * ngsolve/sparsesolv_solvers.hpp:116-123). */
+/* COCR on the operator (row_ptr, col_idx, values) with the IC
+   preconditioner set up on another matrix m_* of the same order. */
+int orc_solve_cocr_pre(int dtype, int n, const int* row_ptr, const int* col_idx,
+                       const void* values, const int* m_row_ptr, const int* m_col_idx,
+                       const void* m_values, const void* b, void* x,
+                       const orc_config* cfg, orc_result* res, double* history,
+                       char* err, int errlen);
 int orc_solve_plain(int dtype, int method_cocr, int n, const int* row_ptr,
                     const int* col_idx, const void* values, const void* b,
                     void* x, const orc_config* cfg, orc_result* res,

@@ -84,6 +84,9 @@ class Oracle:
                  C.POINTER(OrcResult), vp, C.c_char_p, C.c_int]
         L.orc_solve_iccg.argtypes = sargs
         L.orc_solve_cocr.argtypes = sargs
+        L.orc_solve_cocr_pre.argtypes = [C.c_int, C.c_int, _i32p, _i32p, vp, _i32p, _i32p, vp,
+                                         vp, vp, C.POINTER(OrcConfig), C.POINTER(OrcResult), vp,
+                                         C.c_char_p, C.c_int]
         L.orc_solve_plain.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, vp, vp, vp,
                                       C.POINTER(OrcConfig), C.POINTER(OrcResult), vp,
                                       C.c_char_p, C.c_int]
@@ -148,6 +151,17 @@ class Oracle:
 
     def solve_cocr(self, A, b, cfg, x0=None):
         return self._solve(self.lib.orc_solve_cocr, A, b, cfg, x0)
+
+    def solve_cocr_pre(self, A, M, b, cfg, x0=None):
+        """COCR on A with the IC preconditioner set up on M (same order)."""
+        if M.n != A.n or M.is_complex != A.is_complex:
+            raise ValueError("the preconditioner's matrix must match the operator's order and type")
+        dt = np.complex128 if A.is_complex else np.float64
+        mv = np.ascontiguousarray(M.values, dtype=dt)
+        f = self.lib.orc_solve_cocr_pre
+        def fn(dtype, n, rp, ci, vals, *rest):
+            return f(dtype, n, rp, ci, vals, M.row_ptr, M.col_idx, mv.ctypes.data, *rest)
+        return self._solve(fn, A, b, cfg, x0)
 
     def solve_plain(self, A, b, cfg, method: str, x0=None):
         f = self.lib.orc_solve_plain

@@ -149,6 +149,31 @@ int solve_impl(int method, int n, const int* rp, const int* ci, const void* valu
     }
 }
 
+// COCRSolver::solve(A, b, x, n, &pc) with pc set up on another matrix M
+template <typename T>
+int solve_pre_impl(int n, const int* rp, const int* ci, const void* values, const int* mrp,
+                   const int* mci, const void* mvalues, const void* b, void* x,
+                   const orc_config* cfg, orc_result* res, double* hist, char* err, int errlen)
+{
+    try {
+        sparsesolv::SparseMatrixView<T> A(n, n, rp, ci, static_cast<const T*>(values));
+        sparsesolv::SparseMatrixView<T> M(n, n, mrp, mci, static_cast<const T*>(mvalues));
+        sparsesolv::SolverConfig c = to_config(cfg);
+        sparsesolv::ICPreconditioner<T> pc(c.shift_parameter);
+        pc.set_config(c);
+        pc.setup(M);
+        sparsesolv::COCRSolver<T> s;
+        s.set_config(c);
+        const sparsesolv::SolverResult r =
+            s.solve(A, static_cast<const T*>(b), static_cast<T*>(x), n, &pc);
+        fill_result<T>(r, res, hist, pc.actual_shift());
+        return 0;
+    } catch (const std::exception& e) {
+        put_err(err, errlen, e.what());
+        return -1;
+    }
+}
+
 template <typename T>
 double time_impl(int method, int n, const int* rp, const int* ci, const void* values,
                  const void* b, const orc_config* cfg, int iters, double* setup_s)
@@ -377,6 +402,17 @@ int orc_solve_cocr(int dtype, int n, const int* rp, const int* ci, const void* v
 {
     return dtype == 0 ? solve_impl<double>(1, n, rp, ci, values, b, x, cfg, res, hist, err, errlen)
                       : solve_impl<cplx>(1, n, rp, ci, values, b, x, cfg, res, hist, err, errlen);
+}
+
+int orc_solve_cocr_pre(int dtype, int n, const int* rp, const int* ci, const void* values,
+                       const int* mrp, const int* mci, const void* mvalues, const void* b,
+                       void* x, const orc_config* cfg, orc_result* res, double* hist,
+                       char* err, int errlen)
+{
+    return dtype == 0 ? solve_pre_impl<double>(n, rp, ci, values, mrp, mci, mvalues, b, x, cfg,
+                                               res, hist, err, errlen)
+                      : solve_pre_impl<cplx>(n, rp, ci, values, mrp, mci, mvalues, b, x, cfg,
+                                             res, hist, err, errlen);
 }
 
 int orc_solve_plain(int dtype, int method_cocr, int n, const int* rp, const int* ci,

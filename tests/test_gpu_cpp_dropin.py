@@ -25,3 +25,8 @@ def test_cpp_dropin_client():
     assert rows["cg_object"]["iterations"] == it["iterations"]
     assert rows["apply_before_setup"]["threw"] == 1
     assert rows["cocr"]["converged"] and rows["cocr"]["true_residual"] < 1e-9
+    # a preconditioner set up on another matrix is used as given: A's solve
+    # converges with M's factor, in another number of iterations
+    o = rows["cocr_other_pre"]
+    assert o["converged"] and o["true_residual"] < 1e-9
+    assert o["own_pre_iterations"] == rows["cocr"]["iterations"] and o["max_diff"] > 0.0
