@@ -60,7 +60,7 @@ typedef struct ssa_config {
     int norm_type;                    /* 0 RHS, 1 initial residual, 2 custom */
     double custom_norm;               /* 1.0 */
     double shift_parameter;           /* 1.05 */
-    int diagonal_scaling;             /* 0; 1: factor of S A S, S = diag(1/sqrt|a_ii|) (one system per handle) */
+    int diagonal_scaling;             /* 0; 1: factor of S A S, S = diag(1/sqrt|a_ii|) (per system in a batch) */
     int auto_shift;                   /* 0 */
     double shift_increment;           /* 0.05 */
     double max_shift_value;           /* 5.0 */

@@ -49,7 +49,7 @@ struct BatchPlan {
     sparsesolv_amd::SpmvLayout<T> spmv;
     sparsesolv_amd::SweepLayout<F> fwd, bwd;
     std::vector<F> invd;
-    std::vector<double> scale;      // diagonal_scaling (one system only)
+    std::vector<double> scale;      // diagonal_scaling: each system's S in its rows
     // per system: setup facts kept after the layouts are combined
     std::vector<double> actual_shift, seconds;
     std::vector<int> shift_trials, num_colors, num_blocks, nnz, nnz_l;

@@ -181,8 +181,6 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     if (count < 1) throw std::invalid_argument("batch: count must be >= 1");
-    if (count > 1 && settings.diagonal_scaling)
-        throw std::domain_error("batch: diagonal_scaling is not supported for several systems");
     bp = BatchPlan<T, F>{};
     bp.count = count;
     std::vector<HostPlan<T, F>> plans(static_cast<size_t>(count));
@@ -260,6 +258,12 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
                 bp.gather_in[bp.row0[s] + i] = bp.in_off[s] + S.reverse_ordering[i];
                 bp.gather_out[bp.in_off[s] + i] = bp.row0[s] + S.ordering[i];
             }
+        }
+        if (settings.diagonal_scaling) {
+            // each system's S in its rows; padding rows stay zero
+            bp.scale.assign(size_t(N), 0.0);
+            for (int s = 0; s < count; ++s)
+                std::copy(plans[s].scale.begin(), plans[s].scale.end(), bp.scale.begin() + bp.row0[s]);
         }
     }
     bp.wall_seconds = std::chrono::duration<double>(clk::now() - t0).count();

@@ -137,9 +137,6 @@ static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int c
     if (!cfg) throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": null argument");
     if (dtype != SSA_F64 && dtype != SSA_C128)
         throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": dtype must be SSA_F64 or SSA_C128");
-    if (cfg->diagonal_scaling && count > 1)
-        throw SsaError(SSA_ERR_UNSUPPORTED,
-                       std::string(who) + ": diagonal_scaling is not supported for several systems");
     if (cfg->precision != SSA_PRECISION_FP64 && cfg->precision != SSA_PRECISION_MIXED)
         throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": unknown precision");
     for (int q = 0; q < count; ++q) check_matrix(who, n[q], row_ptr[q], col_idx[q], values[q]);

@@ -108,3 +108,20 @@ def test_python_property_diagonal_scaling(oracle):
     x = P.uniform(7, A.n, -1.0, 1.0)
     yo = oracle.ic(A, oracle.config(use_abmc=0, diagonal_scaling=1)).apply(x)
     assert rel(pre * x, yo) <= 1e-12
+
+
+@pytest.mark.parametrize("use_abmc,path1", [(1, 1), (1, 0), (0, 0)])
+def test_batch_with_diagonal_scaling(use_abmc, path1):
+    # each system scaled by its own S: the batched solve gives each system's
+    # one-system result bit for bit (scaled applies in the Path 1 and level
+    # modes, the unscaled ABMC apply of the scaled factor in Path 3)
+    mats = [varcoef(9, seed=5), varcoef(7, seed=6), P.laplace7(8)]
+    bs = [P.uniform(70 + k, A.n, -1.0, 1.0) for k, A in enumerate(mats)]
+    cfg = _lib.default_config(use_abmc=use_abmc, abmc_reorder_spmv=path1, diagonal_scaling=1,
+                              max_iterations=2000)
+    hb = _lib.BatchHandle(mats, cfg)
+    x, results = hb.solve_batch(_lib.METHOD_ICCG, np.concatenate(bs), None, cfg)
+    for k, (A, b, xk) in enumerate(zip(mats, bs, hb.split(x))):
+        xs, rs, _ = _lib.IcHandle(A, cfg).solve(_lib.METHOD_ICCG, b, None, cfg)
+        assert rs.converged and results[k].iterations == rs.iterations
+        assert np.array_equal(xk, xs)
