@@ -1,62 +1,58 @@
 #!/usr/bin/env python3
 """Benchmark of the ABMC-IC(0) Krylov path on one MI355X.
 
-Primary workload (the `value` line): the real curl-curl system on a 64^3 hex
-grid, lowest-order edge unknowns, boundary edges eliminated (n = 762,048,
-nnz = 9,762,984): A = C^T diag(nu_f) C + 1e-6 diag(nu_e), nu = 1e-3 in a
-central iron cube, RHS = C^T g with Gaussian face currents (seed 1).
-Auto-shift ABMC-ICCG in FP64: shift 1.05, block 4, 4 colours requested,
-tol 1e-10, max_iterations 1000 (SparseSolvSolver's default).  IC(0)-CG does
-not reach 1e-10 on this near-singular operator within 1000 iterations (the
-reference does not either): a step is the solve exactly as the reference
-runs it, and the line reports the residual it reached.
+Headline workload (`value`): mixed-precision multi-RHS ABMC-COCR on the
+complex-symmetric eddy-current system of a 78^3 hex grid, lowest-order edge
+unknowns, boundary edges eliminated (n = 1,387,386, nnz = 17,821,650):
+A = K + j*s*M_e with K = C^T diag(nu_f) C + 1e-6 diag(nu_e) (nu = 1e-3 in a
+central iron cube of 20% volume), M_e the lumped edge mass and s = 10 on the
+edges of a central conductor box of 15% volume, 0 elsewhere.  A block of 16
+seeded complex-Gaussian right-hand sides (seed 2) is solved in one block
+solve: IC(0) factor (block 4, 8 colours, shift 1.05, unconjugated) stored and
+applied in complex<float>, the COCR iteration, its dots and the true
+residual in complex<double> with iterative refinement, tol 1e-10, at most
+1000 iterations per right-hand side.  The complex-Gaussian right-hand sides
+excite the gradient fields outside the conductor, which only the 1e-6 term
+touches: neither this path nor the reference's COCR reaches 1e-10 on this
+system in 1000 iterations, so a step is the solve exactly as configured and
+the line reports the residuals it reached, and each right-hand side's error
+against its FP64 block solution.
 
-One step = one solve from x = 0 with b, x, the operator and the factor
-resident in HBM (setup excluded; it is reported separately).  `value` =
-seconds per solve.
+One step = one block solve from X = 0 with B, X, the operator and the factor
+resident in HBM (setup excluded; reported apart).  `value` = seconds per
+block solve.
 
-Extra workloads (`extra`, skipped with --quick):
-  laplace48  7-point Laplacian on a 48^3 interior grid (n = 110,592), ABMC-ICCG
-             shift 1.05, block 4, 4 colours, tol 1e-10: converges, so this is
-             a true time-to-1e-10, with the error against the reference's
-             solution;
-  eddy78     complex-symmetric eddy-current system on 78^3 (n = 1,387,386),
-             A = K + j*10*M_e in a conductor box, ABMC-IC(0)-COCR in
-             complex<double>, block 4, 8 colours, tol 1e-10, 1000 iterations;
-  *-mixed    each of the three with the IC factor and both sweeps in float /
-             complex<float> inside the FP64 Krylov loop (SpMV, vectors, dots
-             and the final true-residual check in FP64);
-  *-level    curlcurl64 and laplace48 on the level-scheduled path
-             (use_abmc=False, the factories' default), with the reference's
-             own level path as the CPU baseline;
-  batch64    64 independent variable-coefficient Laplacians (16.2M DOFs)
-             in one batched solve, against the same systems solved one by
-             one on the GPU and the reference on a sample of them;
-  eddy78-multi16  eddy78 with a block of 16 seeded complex-Gaussian
-             right-hand sides in one mixed-precision block solve (the SpMV
-             and the colour sweeps load each matrix and factor entry once for
-             all 16), with each right-hand side's error against its FP64
-             solution, beside the FP64 block solve and the one-vector FP64
-             and mixed solves;
-  eddy78-lossy-multi16  the same on eddy78's grid with the conductor
-             filling the box (A = K + j*0.1*M_e on every edge): COCR reaches
-             1e-10 there, so the block solve is a time to tolerance and the
-             errors compare converged solutions (and the reference's, for the
-             first right-hand side).
+Beside it (`config`): the FP64 block solve of the same 16 right-hand sides
+and the plain FP64 one-vector solve of the first (the configured complex
+system run as the plain FP64 GPU path) with its time per right-hand side.
 
-The JSON line also carries
-  roofline      the dominant kernel by time, the ABMC colour kernel k_sweep:
-                algorithmic bytes per launch / mean launch time.  Every colour
-                launched on its own runs `--profile-reps` times back to back
-                between two HIP events on the library's stream; the figures
-                are means over those colours, against the 8 TB/s HBM peak.
-                `traffic` is the HBM bytes per launch that rocprofv3's
-                FETCH_SIZE + WRITE_SIZE counters read for the same launches,
-                taken from the counter summary under profiles/ (null when
-                there is none for the workload);
-  cpu_baseline  the reference (its own headers, OpenMP build) on this host's
-                cores, on the same system: the full solve when it fits the
-                time budget, else a bounded sample extrapolated.
+`roofline`: the kernel with the largest share of an iteration, from an extra
+solve of the same block per kernel class in which HIP events on the
+library's stream bracket every launch of that class (ssa_ic_trace; these
+solves run eagerly, cut to `--trace-iterations` iterations): algorithmic
+bytes per launch / mean launch time, against the 8 TB/s HBM peak.
+`traffic` is the HBM bytes per launch that rocprofv3's FETCH_SIZE +
+WRITE_SIZE counters read for the same kernel, from the counter summary under
+profiles/ (null when there is none for the kernel and workload).
+
+`cpu_baseline`: the reference's own COCR (its headers compiled with OpenMP,
+oracle/_ref/libssref_omp.so) on the first right-hand side of the same
+system, on this host's cores: set-up + a bounded number of iterations,
+extrapolated to 1000 iterations per right-hand side and the 16 right-hand
+sides of the block (the reference solves one vector per call, in FP64).
+
+Extra workloads (`extras`, one line of figures each; `--quick` skips them):
+  laplace48   7-point Laplacian, 48^3 interior grid, ABMC-ICCG fp64 (to 1e-10);
+  curlcurl64  real curl-curl, 64^3 hex grid, auto-shift ABMC-ICCG fp64, 1000
+              iterations, and the same in mixed precision;
+  eddy78      the headline system, one right-hand side, mixed precision;
+  batch64     64 independent variable-coefficient Laplacians (16.2M DOFs)
+              in one batched solve, against the same systems one by one;
+  eddy78-lossy-multi16  the headline block on eddy78's grid with the
+              conductor filling the box (A = K + j*0.1*M_e on every edge),
+              where COCR reaches 1e-10: a true time to tolerance.
+Every figure of every workload also goes to stderr as one JSON document, and
+to --detail PATH when given; stdout carries only the final line.
 """
 from __future__ import annotations
 
@@ -73,6 +69,8 @@ sys.path.insert(0, os.path.join(ROOT, "ngsolve-sparsesolv_amd"))
 
 PEAK_GBS = 8000.0
 METRIC = "time-to-1e-10 (s) & HBM GB/s: FP64 vs FP32-precond batched ABMC-ICCG/COCR"
+HEADLINE = "eddy78-multi16"
+MAX_LINE = 2000   # bytes of the final stdout line
 
 WORKLOADS = {
     "curlcurl64": dict(
@@ -90,16 +88,16 @@ WORKLOADS = {
                  tolerance=1e-10, max_iterations=1000)),
     "eddy78": dict(
         desc=("complex-symmetric eddy current, 78^3 hex grid (n=1,387,386, nnz=17,821,650), "
-              "ABMC-IC(0)-COCR complex<double>, shift 1.05, block 4, 8 colours, tol 1e-10, "
-              "max 1000 iterations"),
+              "A = K + j*10*M_e in a 15% conductor box, ABMC-IC(0)-COCR complex<double>, "
+              "shift 1.05, block 4, 8 colours, tol 1e-10, max 1000 iterations"),
         method="cocr",
         cfg=dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=8,
                  tolerance=1e-10, max_iterations=1000)),
     "eddy78-lossy": dict(
         desc=("complex-symmetric eddy current in a lossy medium, 78^3 hex grid (n=1,387,386, "
-              "nnz=17,821,650): eddy78's operator with the conductor filling the box, "
-              "A = K + j*0.1*M_e on every edge; ABMC-IC(0)-COCR complex<double>, shift 1.05, "
-              "block 4, 8 colours, tol 1e-10, max 1000 iterations"),
+              "nnz=17,821,650): the conductor fills the box, A = K + j*0.1*M_e on every edge; "
+              "ABMC-IC(0)-COCR complex<double>, shift 1.05, block 4, 8 colours, tol 1e-10, "
+              "max 1000 iterations"),
         method="cocr",
         cfg=dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=8,
                  tolerance=1e-10, max_iterations=1000)),
@@ -160,7 +158,63 @@ def bytes_per_iteration(n, nnz, nnz_l, s, cocr, s_low=None, nrhs=1):
     return S_A + 2 * S_L + (vec if nrhs == 1 else n * s_f + nrhs * vec_rhs)
 
 
-def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
+TRACE_NAMES = ("spmv", "colour", "update", "pq", "dot")
+TRACE_KERNELS = {
+    "spmv": "k_spmv (SpMV with the fused dot / convergence check)",
+    "colour": "k_sweep (ABMC colour kernel, forward and backward)",
+    "update": "x, r, rt update with the |r|^2 partials",
+    "pq": "p, q update with the best-x copy",
+    "dot": "dot partials after the preconditioner",
+}
+
+
+def trace_kernels(h, solve, iterations):
+    """Per kernel class: one solve (`solve()` returns the iterations it ran)
+    with every launch of that class bracketed by HIP events on the stream it
+    runs on.  Returns launches, mean us and algorithmic bytes per launch,
+    GB/s, fraction of the HBM peak and us per iteration."""
+    from sparsesolv_amd import _lib
+    out = {}
+    for name in TRACE_NAMES:
+        h.trace(getattr(_lib, "TRACE_" + name.upper()))
+        try:
+            its = max(1, solve())
+            k, us, by = h.trace_read()
+        finally:
+            h.trace(_lib.TRACE_OFF)
+        if k == 0 or us <= 0:
+            continue
+        out[name] = {"launches": int(k), "us": round(us / k, 2), "bytes": int(by / k),
+                     "GB_s": round(by / us / 1e3, 1), "frac": round(by / us / 1e3 / PEAK_GBS, 4),
+                     "us_per_iteration": round(us / its, 1), "iterations": int(its)}
+    log(f"  traced ({iterations} iterations): " +
+        ", ".join(f"{k}: {v['us']}us x{v['launches']} {v['GB_s']}GB/s" for k, v in out.items()))
+    return out
+
+
+def dominant(trace):
+    if not trace:
+        return None, None
+    name = max(trace, key=lambda k: trace[k]["us_per_iteration"])
+    return name, trace[name]
+
+
+def recorded_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3
+    counter summary kept beside this script (calibrated FETCH_SIZE +
+    WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def run_gpu(name, steps, warmup, budget=None, trace_iterations=0):
+    """One-vector workload: `steps` timed solves (device-resident vectors)."""
     import torch
     from sparsesolv_amd import _lib
     w = WORKLOADS[name]
@@ -213,19 +267,14 @@ def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
     mixed = kw.get("precision", 0) == 1
     model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], s, w["method"] == "cocr",
                                 s // 2 if mixed else None)
-    kern = {}
-    for what, label in ((_lib.PROFILE_SPMV, "spmv"), (_lib.PROFILE_FORWARD, "forward_sweep"),
-                        (_lib.PROFILE_BACKWARD, "backward_sweep"),
-                        (_lib.PROFILE_APPLY, "precond_apply"),
-                        (_lib.PROFILE_COLOUR_KERNELS, "colour_kernel")):
-        if profile_reps <= 0:
-            break
-        us, by = h.profile(what, profile_reps)
-        if us <= 0:
-            continue   # no colour launched on its own (all in small-colour kernels)
-        kern[label] = {"us": round(us, 2), "bytes": int(by), "GB_s": round(by / us / 1e3, 1),
-                       "frac": round(by / us / 1e3 / PEAK_GBS, 4)}
-        log(f"[{name}] {label}: {us:.2f} us  {by / 1e6:.2f} MB  {by / us / 1e3:.0f} GB/s")
+    trace = {}
+    if trace_iterations > 0:
+        tcfg = _lib.default_config(**dict(kw, max_iterations=trace_iterations))
+
+        def one():
+            x_d.zero_()
+            return h.solve_device(code, b_d.data_ptr(), x_d.data_ptr(), tcfg).iterations
+        trace = trace_kernels(h, one, trace_iterations)
     out = {
         "workload": name, "desc": w["desc"], "method": w["method"], "n": A.n, "nnz": A.nnz,
         "nnz_l": info["nnz_l"], "colours": info["num_colors"],
@@ -240,7 +289,7 @@ def run_gpu(name, steps, warmup, budget=None, profile_reps=20):
         "us_per_iteration": round(1e6 * sec / max(it, 1), 2),
         "bytes_per_iteration_model": int(model),
         "effective_GB_s": round(model * it / sec / 1e9, 1) if sec > 0 else None,
-        "kernels": kern, "times": [round(t, 5) for t in times],
+        "kernels_in_solve": trace, "times": [round(t, 5) for t in times],
     }
     return out, (A, b, kw, w["method"], x)
 
@@ -251,7 +300,7 @@ BATCH_DESC = ("64 independent 7-point variable-coefficient Laplacians, m ~ U{16.
               "in one batched solve with per-system stop status")
 
 
-def run_batch(steps, warmup, cpu=True, budget_s=25.0, profile_reps=10, tune=True):
+def run_batch(steps, warmup, cpu=True, budget_s=12.0, tune=True):
     """The 64-system batch: one batched solve per step (device-resident
     vectors), then the same systems solved one after another on the GPU, and
     the reference on the host on a bounded sample of the systems."""
@@ -304,7 +353,6 @@ def run_batch(steps, warmup, cpu=True, budget_s=25.0, profile_reps=10, tune=True
     torch.cuda.synchronize()
     seq_s = time.perf_counter() - t
     del singles, bufs
-    us, by = hb.profile(_lib.PROFILE_COLOUR_KERNELS, profile_reps) if profile_reps > 0 else (0.0, 0.0)
     out = {
         "workload": "batch64", "desc": BATCH_DESC, "systems": len(systems), "dofs": hb.n,
         "seconds": float(np.mean(times)), "times": [round(t, 5) for t in times],
@@ -313,8 +361,6 @@ def run_batch(steps, warmup, cpu=True, budget_s=25.0, profile_reps=10, tune=True
         "worst_true_rel_residual": worst, "setup_s": round(setup_s, 3),
         "sequential_single_solves_s": round(seq_s, 4),
         "batching_speedup": round(seq_s / float(np.mean(times)), 2),
-        "colour_kernel": None if us <= 0 else {"us": round(us, 2), "bytes": int(by),
-                                                "GB_s": round(by / us / 1e3, 1)},
     }
     if cpu:
         out["cpu_baseline"] = batch_cpu_baseline(systems, kw, iters, budget_s)
@@ -343,7 +389,8 @@ def run_batch(steps, warmup, cpu=True, budget_s=25.0, profile_reps=10, tune=True
             "all_converged": bool(all(q.converged for q in tres)),
             "trials": [[t.block_size, t.num_colors, round(t.seconds, 4), t.iterations, t.converged]
                        for t in r.trials]}
-        log(f"[batch64] tuned: {out['tuned']}")
+        log(f"[batch64] tuned: block {r.best.block_size}, colours {r.best.num_colors}, "
+            f"{out['tuned']['seconds']:.4f}s")
     return out
 
 
@@ -352,7 +399,7 @@ MULTI_DESC = ("{sys}; 16 seeded complex-Gaussian right-hand sides (seed 2; the f
               "complex<double> COCR with iterative refinement, all 16 in one block solve")
 
 
-def run_multi(system, steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_s=40.0):
+def run_multi(system, steps, warmup, nrhs=16, trace_iterations=0, budget=None):
     """`system` with a block of nrhs right-hand sides: the mixed-precision
     block solve per step (device-resident vectors), the FP64 block solve
     (each right-hand side's FP64 solution, for its error), and the one-vector
@@ -363,10 +410,14 @@ def run_multi(system, steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_
     from sparsesolv_amd import problems as P
     w = WORKLOADS[system]
     tag = f"[{system}-multi{nrhs}]"
+    t0 = time.perf_counter()
     sysm = make_system(system)
     A = sysm.A
     B = P.complex_gaussian_block(A.n, nrhs, 2)
+    build_s = time.perf_counter() - t0
     kw = dict(w["cfg"])
+    if budget:
+        kw["max_iterations"] = int(budget)
     cfg64 = _lib.default_config(**kw)
     cfgm = _lib.default_config(**dict(kw, precision=1))
     t0 = time.perf_counter()
@@ -374,6 +425,8 @@ def run_multi(system, steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_
     setup_s = time.perf_counter() - t0
     h64 = _lib.IcHandle(A, cfg64)
     info = hm.info()
+    log(f"{tag} n={A.n} nnz={A.nnz} colours={info['num_colors']} setup={setup_s:.2f}s "
+        f"(matrix build {build_s:.1f}s)")
     dev = torch.device("cuda:0")
     B_d = torch.from_numpy(B).to(dev)
     X_d = torch.zeros_like(B_d)
@@ -413,8 +466,8 @@ def run_multi(system, steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_
     log(f"{tag} fp64 block: {t64:.4f}s iterations max={max(r.iterations for r in res64)}")
     t1_64, r1_64 = single(h64, cfg64)
     t1_m, r1_m = single(hm, cfgm)
-    log(f"{tag} one vector: fp64 {t1_64:.4f}s ({r1_64.iterations} it), "
-        f"mixed {t1_m:.4f}s ({r1_m.iterations} it)")
+    log(f"{tag} one vector: fp64 {t1_64:.4f}s ({r1_64.iterations} it, residual "
+        f"{r1_64.final_residual:.3e}), mixed {t1_m:.4f}s ({r1_m.iterations} it)")
     As = A.to_scipy()
     err, trel = [], []
     for k in range(nrhs):
@@ -422,71 +475,56 @@ def run_multi(system, steps, warmup, cpu=True, profile_reps=10, nrhs=16, budget_
         trel.append(float(np.linalg.norm(B[k] - As @ Xm[k]) / np.linalg.norm(B[k])))
     sec = float(np.mean(times))
     its = [int(r.iterations) for r in res]
-    kern = {}
-    for what, label in ((_lib.PROFILE_SPMV, "spmv"), (_lib.PROFILE_FORWARD, "forward_sweep"),
-                        (_lib.PROFILE_BACKWARD, "backward_sweep"),
-                        (_lib.PROFILE_APPLY, "precond_apply"),
-                        (_lib.PROFILE_COLOUR_KERNELS, "colour_kernel")):
-        if profile_reps <= 0:
-            break
-        us, by = hm.profile_multi(what, nrhs, profile_reps)
-        if us <= 0:
-            continue
-        kern[label] = {"us": round(us, 2), "bytes": int(by), "GB_s": round(by / us / 1e3, 1),
-                       "frac": round(by / us / 1e3 / PEAK_GBS, 4)}
-        log(f"{tag} {label} (x{nrhs}): {us:.2f} us  {by / 1e6:.2f} MB  {by / us / 1e3:.0f} GB/s")
+    trace = {}
+    if trace_iterations > 0:
+        tcfg = _lib.default_config(**dict(kw, precision=1, max_iterations=trace_iterations))
+
+        def one():
+            X_d.zero_()
+            rr = hm.solve_multi_device(code, nrhs, B_d.data_ptr(), X_d.data_ptr(), tcfg)
+            return max(r.iterations for r in rr)
+        trace = trace_kernels(hm, one, trace_iterations)
     model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], 16, True, 8, nrhs)
     it_mean = float(np.mean(its))
     out = {
         "workload": f"{system}-multi{nrhs}", "desc": MULTI_DESC.format(sys=w["desc"]),
-        "nrhs": nrhs, "n": A.n,
+        "nrhs": nrhs, "n": A.n, "nnz": A.nnz,
         "seconds": sec, "times": [round(t, 5) for t in times],
         "seconds_per_rhs": round(sec / nrhs, 5),
-        "iterations": its, "all_converged": bool(all(r.converged for r in res)),
+        "iterations": its, "converged": [bool(r.converged) for r in res],
+        "all_converged": bool(all(r.converged for r in res)),
+        "final_residuals": [float(r.final_residual) for r in res],
         "max_rel_error_vs_fp64": max(err), "rel_error_vs_fp64": err,
         "max_true_rel_residual": max(trel),
         "fp64_block": {"seconds": round(t64, 5),
                        "iterations": [int(r.iterations) for r in res64],
-                       "all_converged": bool(all(r.converged for r in res64))},
+                       "all_converged": bool(all(r.converged for r in res64)),
+                       "final_residuals": [float(r.final_residual) for r in res64]},
         "fp64_one_vector": {"seconds": round(t1_64, 5), "iterations": int(r1_64.iterations),
-                            "converged": bool(r1_64.converged)},
-        "mixed_one_vector": {"seconds": round(t1_m, 5), "iterations": int(r1_m.iterations)},
+                            "converged": bool(r1_64.converged),
+                            "final_residual": float(r1_64.final_residual)},
+        "mixed_one_vector": {"seconds": round(t1_m, 5), "iterations": int(r1_m.iterations),
+                             "final_residual": float(r1_m.final_residual)},
         "speedup_vs_fp64_one_vector_each": round(nrhs * t1_64 / sec, 2),
         "speedup_vs_fp64_block": round(t64 / sec, 2),
         "setup_s": round(setup_s, 3),
         "bytes_per_iteration_model": int(model),
         "effective_GB_s": round(model * it_mean / sec / 1e9, 1),
-        "kernels": kern,
+        "us_per_iteration": round(1e6 * sec / max(max(its), 1), 1),
+        "kernels_in_solve": trace,
     }
-    if cpu:
-        out["cpu_reference_rhs0"] = multi_cpu_check(A, B[0], kw, Xm[0], X64[0], int(r1_64.iterations),
-                                                    budget_s)
-    return out
+    return out, (A, B, kw, Xm, X64)
 
 
-def multi_cpu_check(A, b, kw, xm, x64, gpu_iters, budget_s):
-    """The reference's FP64 COCR on the first right-hand side, when a full
-    solve fits the budget (estimated from 5 timed iterations and the GPU's
-    FP64 iteration count): the errors of the GPU's mixed and FP64 solutions
-    against it."""
+def _reference(kind_pref="ref_omp"):
+    # thread placement for the OpenMP runtime, set before the library loads
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    kind = "reference" if pyoracle.available("ref_omp") else "port"
-    orc = pyoracle.Oracle("ref_omp" if kind == "reference" else "oracle")
-    fields = dict(orc.config()._fields_)
-    cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
-    t_probe, setup = orc.time_iterations(A, b, cfg, 5, "COCR")
-    est = setup + t_probe / 5 * 1.2 * min(gpu_iters, kw.get("max_iterations", 1000))
-    if est > budget_s:
-        return {"skipped": f"estimated {est:.0f}s for a full reference solve > {budget_s:.0f}s"}
-    t = time.perf_counter()
-    x, r = orc.solve_cocr(A, b, cfg)
-    return {"kind": kind, "cores": orc.num_threads(), "seconds": round(time.perf_counter() - t, 3),
-            "iterations": int(r["iterations"]), "converged": bool(r["converged"]),
-            "mixed_rel_error": float(np.linalg.norm(xm - x) / np.linalg.norm(x)),
-            "fp64_rel_error": float(np.linalg.norm(x64 - x) / np.linalg.norm(x))}
+    kind = "reference" if pyoracle.available(kind_pref) else "port"
+    orc = pyoracle.Oracle(kind_pref if kind == "reference" else "oracle")
+    return pyoracle, orc, kind
 
 
 def batch_cpu_baseline(systems, kw, iters, budget_s):
@@ -494,12 +532,7 @@ def batch_cpu_baseline(systems, kw, iters, budget_s):
     systems taken in order until the time budget is spent, each timed for the
     GPU's iteration count of that system; the total is extrapolated by
     DOFs x iterations."""
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    kind = "reference" if pyoracle.available("ref_omp") else "port"
-    orc = pyoracle.Oracle("ref_omp" if kind == "reference" else "oracle")
+    _, orc, kind = _reference()
     fields = dict(orc.config()._fields_)
     cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
     spent, work_done, k = 0.0, 0.0, 0
@@ -528,19 +561,13 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0):
+def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0, single=True):
     """The reference's own ICCG / COCR (its headers compiled with OpenMP) on
     this host.  When it fits `budget_s`: one full solve (solution compared
     with the GPU's), then the same number of iterations timed three times
     (setup and iterations apart, the fastest run kept).  Otherwise setup + a
     bounded number of iterations, extrapolated to the GPU's count."""
-    # thread placement for the OpenMP runtime, set before the library loads
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    kind = "reference" if pyoracle.available("ref_omp") else "port"
-    orc = pyoracle.Oracle("ref_omp" if kind == "reference" else "oracle")
+    _, orc, kind = _reference()
     fields = dict(orc.config()._fields_)
     cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
     mname = "COCR" if method == "cocr" else "CG"
@@ -572,7 +599,7 @@ def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0):
                            f"({setup:.2f}s + {t_it:.2f}s), extrapolated to the GPU's {gpu_iters} "
                            f"iterations; value excludes setup"),
                    ms_per_iteration=round(per_it * 1e3, 3))
-    if kind == "reference" and out["cores"] > 1:
+    if single and kind == "reference" and out["cores"] > 1:
         out["single_thread"] = cpu_single_thread(orc, A, b, cfg, mname,
                                                  out.get("iterations", gpu_iters), out["cores"],
                                                  budget_s=0.4 * budget_s)
@@ -598,17 +625,53 @@ def cpu_single_thread(orc, A, b, cfg, mname, gpu_iters, threads, budget_s):
                        f"value excludes setup")}
 
 
-def recorded_traffic(workload):
-    """HBM bytes per colour-kernel launch from the rocprofv3 counter summary
-    kept beside this script (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
-    path = os.path.join(ROOT, "profiles", f"colour_kernel_{workload}_traffic.md")
-    if not os.path.exists(path):
-        return None, None
-    with open(path) as f:
-        for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1]), os.path.relpath(path, ROOT)
-    return None, None
+def multi_cpu_baseline(A, b, kw, nrhs, iterations, budget_s=20.0):
+    """The reference's FP64 COCR on the first right-hand side: setup + a
+    bounded number of iterations on this host's cores, extrapolated to
+    `iterations` per right-hand side and nrhs right-hand sides (the reference
+    solves one vector per call)."""
+    _, orc, kind = _reference()
+    fields = dict(orc.config()._fields_)
+    cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
+    t_probe, setup = orc.time_iterations(A, b, cfg, 3, "COCR")
+    per_it = max(t_probe / 3, 1e-6)
+    iters = int(max(3, min(iterations, (budget_s - setup) / per_it)))
+    t_it, setup = orc.time_iterations(A, b, cfg, iters, "COCR")
+    per_it = t_it / iters
+    return {"value": round(per_it * iterations * nrhs, 3), "unit": "s", "cores": orc.num_threads(),
+            "kind": kind, "cpu": cpu_model(),
+            "sample": (f"the reference's FP64 COCR on right-hand side 0 of the same system: setup "
+                       f"+ {iters} iterations timed ({setup:.2f}s + {t_it:.2f}s), extrapolated to "
+                       f"{iterations} iterations x {nrhs} right-hand sides, setup apart"),
+            "per_rhs_s": round(per_it * iterations, 3), "setup_s": round(setup, 3),
+            "ms_per_iteration": round(per_it * 1e3, 3)}
+
+
+def _g(v, d=4):
+    """Compact float for the summary line."""
+    return None if v is None else float(f"{v:.{d}g}")
+
+
+def summary(key, o):
+    """One compact object per extra workload for the final line."""
+    if key == "batch64":
+        s = {"s": _g(o["seconds"]), "it_max": o["iterations_max"], "conv": o["all_converged"],
+             "seq_s": _g(o["sequential_single_solves_s"]), "setup_s": _g(o["setup_s"], 3)}
+        if "cpu_baseline" in o:
+            s["cpu_s"] = _g(o["cpu_baseline"]["value"])
+        if "tuned" in o:
+            s["tuned_s"] = _g(o["tuned"]["seconds"])
+            s["tuned_bc"] = [o["tuned"]["abmc_block_size"], o["tuned"]["abmc_num_colors"]]
+        return s
+    if "nrhs" in o:
+        return {"s": _g(o["seconds"]), "it_max": max(o["iterations"]), "conv": o["all_converged"],
+                "err_vs_fp64": _g(o["max_rel_error_vs_fp64"], 2),
+                "fp64_block_s": _g(o["fp64_block"]["seconds"])}
+    s = {"s": _g(o["seconds"]), "it": o["iterations"], "conv": o["converged"],
+         "res": _g(o["final_residual"], 3)}
+    if "cpu_baseline" in o:
+        s["cpu_s"] = _g(o["cpu_baseline"]["value"])
+    return s
 
 
 def main():
@@ -616,15 +679,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="curlcurl64", choices=sorted(WORKLOADS))
-    ap.add_argument("--quick", action="store_true", help="primary workload only")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default=HEADLINE,
+                    choices=sorted(WORKLOADS) + [HEADLINE, "eddy78-lossy-multi16"])
+    ap.add_argument("--quick", action="store_true", help="headline workload only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--budget", type=int, default=None,
-                    help="fixed iteration budget per solve (per-iteration timing runs)")
-    ap.add_argument("--profile-reps", type=int, default=20,
-                    help="repetitions in the HIP-event kernel timing (0: skip)")
+                    help="iteration budget per solve instead of the workloads' own")
+    ap.add_argument("--trace-iterations", type=int, default=60,
+                    help="iterations of each traced solve (0: no in-solve kernel timing)")
     ap.add_argument("--extras", default=None,
                     help="comma-separated extra workloads to run (default: all of them)")
+    ap.add_argument("--detail", default=None, help="also write every figure as JSON to this path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB between two device buffers (a copy kernel of "
                          "known size for scaling hardware byte counters)")
@@ -643,41 +708,86 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 1073741824 bytes read, 1073741824 bytes written")
         del src, dst
-    main_out, (A, b, kw, method, x_gpu) = run_gpu(args.workload, args.steps, args.warmup,
-                                                 args.budget, args.profile_reps)
-    extra = {"primary": main_out}
-    names = ("curlcurl64-mixed", "curlcurl64-level", "laplace48", "laplace48-mixed",
-             "laplace48-level", "eddy78", "eddy78-mixed", "batch64", "eddy78-multi16",
+    detail = {}
+    cpu = None
+    if args.workload.endswith("-multi16"):
+        system = args.workload[:-len("-multi16")]
+        main_out, (A, B, kw, Xm, X64) = run_multi(system, args.steps, args.warmup,
+                                                  trace_iterations=args.trace_iterations,
+                                                  budget=args.budget)
+        if not args.no_cpu:
+            log("cpu baseline ...")
+            cpu = multi_cpu_baseline(A, B[0], kw, 16, max(main_out["iterations"]))
+            log(f"cpu baseline: {cpu}")
+        del Xm, X64
+        cfg_line = {
+            "workload": f"{args.workload}: {WORKLOADS[system]['desc']}; 16 complex-Gaussian "
+                        f"right-hand sides (seed 2) in one block solve, complex<float> IC factor "
+                        f"and sweeps inside complex<double> COCR",
+            "n": A.n, "nnz": A.nnz, "nrhs": 16, "precision": "mixed",
+            "iterations_max": max(main_out["iterations"]),
+            "converged": f"{sum(main_out['converged'])}/16",
+            "final_residual_max": _g(max(main_out["final_residuals"]), 4),
+            "err_vs_fp64_block_max": _g(main_out["max_rel_error_vs_fp64"], 3),
+            "fp64_block_s": main_out["fp64_block"]["seconds"],
+            "fp64_one_vector_s": main_out["fp64_one_vector"]["seconds"],
+            "fp64_one_vector_residual": _g(main_out["fp64_one_vector"]["final_residual"], 4),
+            "speedup_vs_fp64_block": main_out["speedup_vs_fp64_block"],
+            "speedup_vs_fp64_one_vector_each": main_out["speedup_vs_fp64_one_vector_each"],
+            "us_per_iteration": main_out["us_per_iteration"],
+            "effective_GB_s": main_out["effective_GB_s"], "parallelism": "single-gpu"}
+        dtype = "c128 (c64 preconditioner)"
+    else:
+        main_out, (A, b, kw, method, x_gpu) = run_gpu(args.workload, args.steps, args.warmup,
+                                                     args.budget, args.trace_iterations)
+        if not args.no_cpu:
+            log("cpu baseline ...")
+            cpu = cpu_baseline(A, b, kw, method, main_out["iterations"], x_gpu)
+            log(f"cpu baseline: {cpu}")
+        cfg_line = {"workload": f"{args.workload}: {WORKLOADS[args.workload]['desc']}",
+                    "n": A.n, "nnz": A.nnz, "iterations": main_out["iterations"],
+                    "converged": main_out["converged"],
+                    "final_residual": _g(main_out["final_residual"], 4),
+                    "parallelism": "single-gpu"}
+        mixed = WORKLOADS[args.workload]["cfg"].get("precision", 0) == 1
+        base = "c128" if A.is_complex else "f64"
+        dtype = base + ((" (c64 preconditioner)" if A.is_complex else " (f32 preconditioner)")
+                        if mixed else "")
+    detail[args.workload] = main_out
+    kname, k = dominant(main_out["kernels_in_solve"])
+    roofline = None
+    if k is not None:
+        traffic, traffic_src = recorded_traffic(kname, args.workload)
+        roofline = {"bound": "hbm", "kernel": TRACE_KERNELS[kname], "achieved": k["GB_s"],
+                    "peak": PEAK_GBS, "unit": "GB/s", "frac": k["frac"], "traffic": traffic,
+                    "bytes_per_launch": k["bytes"], "us_per_launch": k["us"],
+                    "share_us_per_iteration": k["us_per_iteration"],
+                    "source": "HIP events around each in-solve launch"}
+        if traffic_src:
+            roofline["traffic_source"] = traffic_src
+
+    names = ("laplace48", "curlcurl64", "curlcurl64-mixed", "eddy78-mixed", "batch64",
              "eddy78-lossy-multi16")
-    wanted = set(names if args.extras is None else args.extras.split(","))
+    wanted = set(names if args.extras is None else [e for e in args.extras.split(",") if e])
+    extras = {}
     if not args.quick:
-        for other in names[:7]:
+        esteps = max(1, min(args.steps, 3))
+        for other in names:
             if other == args.workload or other not in wanted:
                 continue
-            o, (A2, b2, kw2, m2, x2) = run_gpu(other, max(1, args.steps), 1, args.budget,
-                                               args.profile_reps)
-            if not args.no_cpu and other in ("laplace48", "laplace48-level", "curlcurl64-level"):
-                log(f"cpu baseline ({other}) ...")
-                o["cpu_baseline"] = cpu_baseline(A2, b2, kw2, m2, o["iterations"], x2)
-                log(f"cpu baseline ({other}): {o['cpu_baseline']}")
-            extra[other] = o
-    if not args.quick and "batch64" in wanted:
-        extra["batch64"] = run_batch(max(1, args.steps), 1, cpu=not args.no_cpu,
-                                     profile_reps=args.profile_reps)
-        log(f"[batch64] {json.dumps({k: v for k, v in extra['batch64'].items() if k != 'desc'})}")
-    for system in ("eddy78", "eddy78-lossy"):
-        key = f"{system}-multi16"
-        if not args.quick and key in wanted:
-            extra[key] = run_multi(system, max(1, args.steps), 1, cpu=not args.no_cpu,
-                                   profile_reps=args.profile_reps)
-            log(f"[{key}] {json.dumps({k: v for k, v in extra[key].items() if k != 'desc'})}")
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(A, b, kw, method, main_out["iterations"], x_gpu)
-        log(f"cpu baseline: {cpu}")
-    ck = main_out["kernels"].get("colour_kernel")
-    traffic, traffic_src = recorded_traffic(args.workload)
+            if other == "batch64":
+                o = run_batch(esteps, 1, cpu=not args.no_cpu)
+            elif other.endswith("-multi16"):
+                o, _ = run_multi(other[:-len("-multi16")], esteps, 1, budget=args.budget)
+            else:
+                o, (A2, b2, kw2, m2, x2) = run_gpu(other, esteps, 1, args.budget)
+                if not args.no_cpu and other in ("laplace48", "curlcurl64"):
+                    log(f"cpu baseline ({other}) ...")
+                    o["cpu_baseline"] = cpu_baseline(A2, b2, kw2, m2, o["iterations"], x2,
+                                                     budget_s=12.0, single=False)
+            detail[other] = o
+            extras[other] = summary(other, o)
+            log(f"[{other}] {json.dumps(extras[other])}")
     line = {
         "metric": METRIC,
         "value": round(main_out["seconds"], 6),
@@ -689,22 +799,35 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128" if A.is_complex else "f64",
+        "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]['desc']}",
-                   "n": A.n, "nnz": A.nnz, "iterations": main_out["iterations"],
-                   "converged": main_out["converged"],
-                   "final_residual": main_out["final_residual"],
-                   "parallelism": "single-gpu"},
-        "roofline": None if ck is None else {
-            "bound": "hbm", "kernel": "k_sweep (ABMC colour kernel, forward and backward)",
-            "achieved": ck["GB_s"], "peak": PEAK_GBS, "unit": "GB/s", "frac": ck["frac"],
-            "traffic": traffic, "traffic_source": traffic_src,
-            "bytes_per_launch": ck["bytes"], "us_per_launch": ck["us"]},
-        "cpu_baseline": cpu,
-        "extra": extra,
+        "config": cfg_line,
+        "roofline": roofline,
+        "cpu_baseline": None if cpu is None else {
+            k2: cpu[k2] for k2 in ("value", "unit", "cores", "kind", "sample", "per_rhs_s",
+                                   "setup_s", "ms_per_iteration") if k2 in cpu},
+        "extras": extras,
     }
-    print(json.dumps(line), flush=True)
+    full = dict(line, detail=detail)
+    text = json.dumps(full)
+    log("detail: " + text)
+    if args.detail:
+        with open(args.detail, "w") as f:
+            f.write(text + "\n")
+    print(final_line(line), flush=True)
+
+
+def final_line(line):
+    """The stdout line: compact JSON of at most MAX_LINE bytes (the extras'
+    figures are cut to their times first, then dropped)."""
+    out = json.dumps(line, separators=(",", ":"))
+    if len(out) > MAX_LINE:
+        line = dict(line, extras={k: {"s": v.get("s")} for k, v in line["extras"].items()})
+        out = json.dumps(line, separators=(",", ":"))
+    if len(out) > MAX_LINE:
+        line = dict(line, extras={})
+        out = json.dumps(line, separators=(",", ":"))
+    return out
 
 
 if __name__ == "__main__":

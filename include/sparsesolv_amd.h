@@ -231,6 +231,30 @@ int ssa_ic_profile(ssa_ic* ic, int what, int reps, double* avg_us, double* bytes
  * side and the matrix and factor entries once. */
 int ssa_ic_profile_multi(ssa_ic* ic, int what, int nrhs, int reps, double* avg_us, double* bytes);
 
+/* Kernel timing inside solves (no reference counterpart: measurement hook for
+ * the roofline report).  ssa_ic_trace(ic, what) selects one kernel class and
+ * clears the totals; while a class is selected, ssa_solve / ssa_solve_multi /
+ * ssa_batch_solve run their iterations without graph capture and bracket every
+ * launch of that class in the iteration loop with two HIP events on the stream
+ * it runs on.  ssa_ic_trace_read returns the number of timed launches, the sum
+ * of their durations (us) and of their algorithmic bytes (every operand once,
+ * as for ssa_ic_profile; the best-x copy is not counted).  SSA_TRACE_OFF turns
+ * it off and keeps the totals.  Classes:
+ *   SSA_TRACE_SPMV    the SpMV (with its fused dot or convergence check);
+ *   SSA_TRACE_COLOUR  the colour kernel k_sweep of colours without oversized
+ *                     blocks (both sweeps);
+ *   SSA_TRACE_UPDATE  x, r (COCR: and rt) update with the |r|^2 partials;
+ *   SSA_TRACE_PQ      the p (COCR: and q) update with the best-x copy;
+ *   SSA_TRACE_DOT     the dot partials after the preconditioner. */
+#define SSA_TRACE_OFF (-1)
+#define SSA_TRACE_SPMV 0
+#define SSA_TRACE_COLOUR 1
+#define SSA_TRACE_UPDATE 2
+#define SSA_TRACE_PQ 3
+#define SSA_TRACE_DOT 4
+int ssa_ic_trace(ssa_ic* ic, int what);
+int ssa_ic_trace_read(const ssa_ic* ic, long long* launches, double* total_us, double* total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

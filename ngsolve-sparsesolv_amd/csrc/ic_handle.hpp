@@ -46,6 +46,9 @@ struct IcBase {
     virtual void schedule(int32_t* co, int32_t* bo) const = 0;
     virtual void factor(int32_t* rp, int32_t* ci, void* vals, void* invd) const = 0;
     virtual void profile(int what, int reps, int nrhs, double& avg_us, double& bytes) = 0;
+    // kernel class timed inside solves (-1: none) and the totals so far
+    virtual void trace(int what) = 0;
+    virtual void trace_read(long long& launches, double& us, double& bytes) const = 0;
 };
 
 // one handle implementation per (vector, factor) precision pair; values[s]

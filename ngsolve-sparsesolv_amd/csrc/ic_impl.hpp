@@ -315,6 +315,93 @@ struct IcImpl final : IcBase {
 
     SolverState* h_state = nullptr;  // pinned, one per state
 
+    // Kernel timing inside solves (ssa_ic_trace): while a kernel class is
+    // selected, solves run without graph capture and each launch of that
+    // class in the iteration loop is bracketed by two events on the stream
+    // it runs on; the pairs are read after every status poll (the stream
+    // has drained by then) and summed with the launch's algorithmic bytes.
+    int trace_cls = -1;
+    bool trace_armed = false;   // inside a solve's iteration loop
+    struct TracePair {
+        hipEvent_t a = nullptr, b = nullptr;
+        double bytes = 0.0;
+    };
+    std::vector<TracePair> trace_pool;
+    size_t trace_used = 0;
+    long long trace_launches = 0;
+    double trace_us = 0.0, trace_bytes = 0.0;
+
+    template <typename Fn>
+    void traced(int cls, hipStream_t s, double bytes, Fn&& launch)
+    {
+        if (!trace_armed || cls != trace_cls) {
+            launch();
+            return;
+        }
+        if (trace_used == trace_pool.size()) {
+            TracePair tp;
+            HIP_CHECK(hipEventCreate(&tp.a));
+            if (hipEventCreate(&tp.b) != hipSuccess) {
+                (void)hipEventDestroy(tp.a);
+                throw SsaError(SSA_ERR_DEVICE, "ssa_ic_trace: hipEventCreate failed");
+            }
+            trace_pool.push_back(tp);
+        }
+        TracePair& tp = trace_pool[trace_used++];
+        tp.bytes = bytes;
+        HIP_CHECK(hipEventRecord(tp.a, s));
+        launch();
+        HIP_CHECK(hipEventRecord(tp.b, s));
+    }
+
+    // the recorded launches into the totals (the streams have drained)
+    void trace_harvest()
+    {
+        for (size_t i = 0; i < trace_used; ++i) {
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, trace_pool[i].a, trace_pool[i].b));
+            trace_us += 1000.0 * double(ms);
+            trace_bytes += trace_pool[i].bytes;
+            ++trace_launches;
+        }
+        trace_used = 0;
+    }
+
+    void trace(int what) override
+    {
+        need_device();
+        if (what < -1 || what > SSA_TRACE_DOT) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_trace: unknown kernel class");
+        trace_cls = what;
+        trace_used = 0;
+        if (what >= 0) {   // a new selection starts from zero; off keeps the totals
+            trace_launches = 0;
+            trace_us = trace_bytes = 0.0;
+        }
+    }
+
+    void trace_read(long long& launches, double& us, double& bytes) const override
+    {
+        launches = trace_launches;
+        us = trace_us;
+        bytes = trace_bytes;
+    }
+
+    // algorithmic bytes of one launch of a vector kernel class: every
+    // operand once, per right-hand side (the best-x copy is not counted)
+    double vector_bytes(int cls, bool cocr) const
+    {
+        const double e = double(n) * double(W), sv = double(sizeof(T)), sp = double(sizeof(P));
+        const double low = kMixed ? sp : 0.0;   // the narrow copy a mixed kernel also writes
+        switch (cls) {
+        case SSA_TRACE_UPDATE:   // COCR: x, p, r, q, rt, qt read; x, r, rt written
+            return cocr ? e * (8.0 * sv + (kMixed ? sp : sv)) : e * (6.0 * sv + low);
+        case SSA_TRACE_PQ:       // COCR: rt, t, p, q read; p, q written; CG: z, p read, p written
+            return cocr ? e * (6.0 * sv + low) : e * (2.0 * sv + (kMixed ? sp : sv));
+        default:                 // dot: two operands
+            return e * (sv + (kMixed ? sp : sv));
+        }
+    }
+
     IcImpl(ssa_context* c, int count_, const int* ns, const int32_t* const* rp,
            const int32_t* const* ci, const HT* const* v, const sparsesolv_amd::IcSettings& st)
         : ctx(c), count(count_), abmc_order(st.use_abmc), abmc_rcm(st.use_abmc && st.abmc_use_rcm)
@@ -437,6 +524,10 @@ struct IcImpl final : IcBase {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (side) (void)hipStreamDestroy(side);
+        for (auto& tp : trace_pool) {
+            (void)hipEventDestroy(tp.a);
+            (void)hipEventDestroy(tp.b);
+        }
     }
 
     // ---- building blocks ---------------------------------------------------
@@ -509,12 +600,17 @@ struct IcImpl final : IcBase {
         const index_t* col = A_col.as<index_t>();
         const index_t* rows = A_row.as<index_t>();
         const T* val = A_val.as<T>();
-        if (dot == 0)
-            ssa::k_spmv<T, 0, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
-        else if (dot == 1)
-            ssa::k_spmv<T, 1, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
-        else
-            ssa::k_spmv<T, 2, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+        // read x once, write y, the matrix entries and row pointers once
+        const double bytes = double(nnz) * (sizeof(T) + 4.0) + 4.0 * (double(n) + 1.0) +
+                             2.0 * double(n) * WV * sizeof(T);
+        traced(SSA_TRACE_SPMV, s, bytes, [&] {
+            if (dot == 0)
+                ssa::k_spmv<T, 0, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+            else if (dot == 1)
+                ssa::k_spmv<T, 1, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+            else
+                ssa::k_spmv<T, 2, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+        });
     }
 
     ssa::SweepArrays sweep_arrays(DevSweep& D)
@@ -762,16 +858,22 @@ struct IcImpl final : IcBase {
         if (s1 > s0 || gw > 0) {
             const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
             const ssa::SweepArrays A = sweep_arrays(D);
+            // timed as the colour kernel: colours without oversized blocks
+            const int cls = b1 == b0 ? SSA_TRACE_COLOUR : -2;
+            const double bytes = double(D.phase_entries[ph]) * (sizeof(F) + 4.0) +
+                                 double(D.phase_rows[ph]) * (4.0 + 2.0 * W * sizeof(P));
 #define SSA_SWEEP(BRV) \
     sweep_br<FWD, BRV>(g, s0, s1, D.phase_slot[ph], D.phase_stride[ph], A, in, out, we, st)
-            switch (D.BR == 0 ? 1 : D.BR) {
-            case 1: SSA_SWEEP(1); break;
-            case 2: SSA_SWEEP(2); break;
-            case 4: SSA_SWEEP(4); break;
-            case 8: SSA_SWEEP(8); break;
-            case 16: SSA_SWEEP(16); break;
-            default: throw SsaError(SSA_ERR_UNSUPPORTED, "sweep: unsupported rows per lane");
-            }
+            traced(cls, s, bytes, [&] {
+                switch (D.BR == 0 ? 1 : D.BR) {
+                case 1: SSA_SWEEP(1); break;
+                case 2: SSA_SWEEP(2); break;
+                case 4: SSA_SWEEP(4); break;
+                case 8: SSA_SWEEP(8); break;
+                case 16: SSA_SWEEP(16); break;
+                default: throw SsaError(SSA_ERR_UNSUPPORTED, "sweep: unsupported rows per lane");
+                }
+            });
 #undef SSA_SWEEP
             HIP_CHECK(hipGetLastError());
         }
@@ -1074,25 +1176,37 @@ struct IcImpl final : IcBase {
             // kernel (after the preconditioner, which cannot change x or r),
             // beta and the best-x copy in the p update
             launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
-            ssa::k_cg_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv, true), low_copy(), y);
+            traced(SSA_TRACE_UPDATE, s, vector_bytes(SSA_TRACE_UPDATE, false), [&] {
+                ssa::k_cg_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, ctl(part_a.p, np_spmv, true), low_copy(), y);
+            });
             if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
             else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), st, y);
-            ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G, false), y);
-            ssa::k_xpby<T, P><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G, false), y);
+            traced(SSA_TRACE_DOT, s, vector_bytes(SSA_TRACE_DOT, false), [&] {
+                ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>(), st, ctl(part_b.p, G, false), y);
+            });
+            traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, false), [&] {
+                ssa::k_xpby<T, P><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), p.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, G, false), y);
+            });
         } else {
             // COCRSolver::do_iterate body (cocr_solver.hpp:96-162); z holds rt,
             // Ap holds q.  alpha in the update kernel, the check in the
             // t = A rt kernel, beta and the best-x copy in the p, q update
             if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
             else ssa::k_copy<T, T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), qt.as<T>(), st, y);
-            if (kMixed)   // (q, qt) with the narrow operand second
-                ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
-            else
-                ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
-            ssa::k_cocr_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
+            traced(SSA_TRACE_DOT, s, vector_bytes(SSA_TRACE_DOT, true), [&] {
+                if (kMixed)   // (q, qt) with the narrow operand second
+                    ssa::k_dot<T, T, P><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
+                else
+                    ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
+            });
+            traced(SSA_TRACE_UPDATE, s, vector_bytes(SSA_TRACE_UPDATE, true), [&] {
+                ssa::k_cocr_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
+            });
             if (kMixed && pre && replace) replace_rt(st);
             launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G, false));
-            ssa::k_cocr_pq<T, P><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
+            traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, true), [&] {
+                ssa::k_cocr_pq<T, P><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
+            });
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -1121,27 +1235,39 @@ struct IcImpl final : IcBase {
             if (!c) {
                 launch_spmv(p.as<T>(), Ap.as<T>(), prm.conjugate ? 2 : 1, part_a.as<T>(), st);
                 ssa::k_ctrl_alpha_blk<T><<<WV, CT, 0, s>>>(part_a.as<T>(), np_spmv, st, prm, H, hs);
-                ssa::k_cg_update_blk<T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, low_copy());
+                traced(SSA_TRACE_UPDATE, s, vector_bytes(SSA_TRACE_UPDATE, false), [&] {
+                    ssa::k_cg_update_blk<T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), p.as<T>(), Ap.as<T>(), part_b.as<double>(), st, low_copy());
+                });
                 ssa::k_ctrl_check_blk<<<WV, CT, 0, s>>>(part_b.as<double>(), G, st, prm, H, hs);
                 if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
                 else ssa::k_copy_blk<T, T, WV><<<FG, ssa::kBlock, 0, s>>>(ne, r.as<T>(), z.as<T>(), st);
-                ssa::k_dot_blk<T, T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>());
+                traced(SSA_TRACE_DOT, s, vector_bytes(SSA_TRACE_DOT, false), [&] {
+                    ssa::k_dot_blk<T, T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), pre_out(c), prm.conjugate, part_c.as<T>());
+                });
                 ssa::k_ctrl_beta_blk<T><<<WV, CT, 0, s>>>(part_c.as<T>(), G, st, prm);
-                ssa::k_xpby_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, pre_out(c), p.as<T>(), x.as<T>(), bx, st);
+                traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, false), [&] {
+                    ssa::k_xpby_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, pre_out(c), p.as<T>(), x.as<T>(), bx, st);
+                });
             } else {
                 if (pre) launch_precond(pre_in(c), pre_out(c), pre_tmp(), st);
                 else ssa::k_copy_blk<T, T, WV><<<FG, ssa::kBlock, 0, s>>>(ne, Ap.as<T>(), qt.as<T>(), st);
-                if (kMixed)   // (q, qt) with the narrow operand second
-                    ssa::k_dot_blk<T, T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>());
-                else
-                    ssa::k_dot_blk<T, P, T, WV><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>());
+                traced(SSA_TRACE_DOT, s, vector_bytes(SSA_TRACE_DOT, true), [&] {
+                    if (kMixed)   // (q, qt) with the narrow operand second
+                        ssa::k_dot_blk<T, T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), pre_out(c), 0, part_a.as<T>());
+                    else
+                        ssa::k_dot_blk<T, P, T, WV><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>());
+                });
                 ssa::k_ctrl_alpha_blk<T><<<WV, CT, 0, s>>>(part_a.as<T>(), G, st, prm, H, hs);
-                ssa::k_cocr_update_blk<T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st);
+                traced(SSA_TRACE_UPDATE, s, vector_bytes(SSA_TRACE_UPDATE, true), [&] {
+                    ssa::k_cocr_update_blk<T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st);
+                });
                 ssa::k_ctrl_check_blk<<<WV, CT, 0, s>>>(part_b.as<double>(), G, st, prm, H, hs);
                 if (kMixed && pre && replace) replace_rt(st);
                 launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st);
                 ssa::k_ctrl_beta_blk<T><<<WV, CT, 0, s>>>(part_c.as<T>(), np_spmv, st, prm);
-                ssa::k_cocr_pq_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, low_copy());
+                traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, true), [&] {
+                    ssa::k_cocr_pq_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, low_copy());
+                });
             }
         });
         HIP_CHECK(hipGetLastError());
@@ -1236,11 +1362,13 @@ struct IcImpl final : IcBase {
         // status between groups.  The control kernels stop each state at its
         // iteration budget themselves and every kernel returns at once after
         // a stop, so iterations enqueued past the stop cost only launches.
-        const int unroll = cfg.use_graph ? std::min(kGraphIterations, maxit + 1) : 1;
+        // timed launches (ssa_ic_trace) need the eager loop
+        const bool graph_on = cfg.use_graph && trace_cls < 0;
+        const int unroll = graph_on ? std::min(kGraphIterations, maxit + 1) : 1;
         int poll = std::max(cfg.poll_interval > 0 ? cfg.poll_interval : 8, unroll);
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        if (cfg.use_graph) {
+        if (graph_on) {
             HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int u = 0; u < unroll; ++u) enqueue_iteration(method, prm, replace_at(u));
             HIP_CHECK(hipStreamEndCapture(s, &graph));
@@ -1251,18 +1379,25 @@ struct IcImpl final : IcBase {
         auto running = [&] {
             HIP_CHECK(hipMemcpyAsync(h_state, st, sizeof(SolverState) * S, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
+            trace_harvest();
             for (int q = 0; q < S; ++q)
                 if (h_state[q].status == ssa::kRunning) return true;
             return false;
         };
         bool more = running();
+        struct Disarm {
+            bool& v;
+            ~Disarm() { v = false; }
+        } disarm{trace_armed};
         while (more && done <= maxit) {
             const int k = std::min(poll, maxit + 1 - done);
             const int groups = (k + unroll - 1) / unroll;
+            trace_armed = trace_cls >= 0;
             for (int i = 0; i < groups; ++i) {
                 if (exec) HIP_CHECK(hipGraphLaunch(exec, s));
                 else enqueue_iteration(method, prm, replace_at(enqueued++));
             }
+            trace_armed = false;
             done += groups * unroll;
             more = running();
             if (poll < 64) poll *= 2;

@@ -338,4 +338,21 @@ int ssa_ic_profile_multi(ssa_ic* ic, int what, int nrhs, int reps, double* avg_u
     });
 }
 
+int ssa_ic_trace(ssa_ic* ic, int what)
+{
+    return guarded([&] {
+        if (!ic) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_trace: null argument");
+        ic->impl->trace(what);
+    });
+}
+
+int ssa_ic_trace_read(const ssa_ic* ic, long long* launches, double* total_us, double* total_bytes)
+{
+    return guarded([&] {
+        if (!ic || !launches || !total_us || !total_bytes)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_trace_read: null argument");
+        ic->impl->trace_read(*launches, *total_us, *total_bytes);
+    });
+}
+
 }  // extern "C"

@@ -38,7 +38,7 @@ EXPORTED = (
     "ssa_ic_schedule", "ssa_ic_factor", "ssa_ic_apply", "ssa_spmv", "ssa_solve",
     "ssa_ic_profile", "ssa_batch_create", "ssa_batch_solve", "ssa_ic_systems",
     "ssa_solve_multi", "ssa_ic_apply_multi", "ssa_spmv_multi", "ssa_ic_profile_multi",
-    "ssa_ic_set_operator",
+    "ssa_ic_set_operator", "ssa_ic_trace", "ssa_ic_trace_read",
 )
 
 PROFILE_SPMV = 0
@@ -46,6 +46,13 @@ PROFILE_FORWARD = 1
 PROFILE_BACKWARD = 2
 PROFILE_APPLY = 3
 PROFILE_COLOUR_KERNELS = 4
+
+TRACE_OFF = -1
+TRACE_SPMV = 0
+TRACE_COLOUR = 1
+TRACE_UPDATE = 2
+TRACE_PQ = 3
+TRACE_DOT = 4
 
 
 class Config(C.Structure):
@@ -137,6 +144,9 @@ def load() -> C.CDLL:
         L.ssa_spmv_multi.argtypes = [vp, C.c_int, vp, vp, C.c_int]
         L.ssa_ic_profile_multi.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                            C.POINTER(C.c_double)]
+        L.ssa_ic_trace.argtypes = [vp, C.c_int]
+        L.ssa_ic_trace_read.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
         for name in EXPORTED:
             getattr(L, name).restype = C.c_char_p if name == "ssa_last_error" else (
                 None if name == "ssa_config_default" else C.c_int)
@@ -348,6 +358,17 @@ class IcHandle:
         us, by = C.c_double(), C.c_double()
         check(self.lib.ssa_ic_profile_multi(self.h, what, nrhs, reps, C.byref(us), C.byref(by)))
         return us.value, by.value
+
+    def trace(self, what: int) -> None:
+        """ssa_ic_trace: time every launch of one kernel class inside the
+        following solves (TRACE_OFF stops); the totals restart from zero."""
+        check(self.lib.ssa_ic_trace(self.h, what))
+
+    def trace_read(self):
+        """(launches, total microseconds, total algorithmic bytes) timed so far."""
+        k, us, by = C.c_longlong(), C.c_double(), C.c_double()
+        check(self.lib.ssa_ic_trace_read(self.h, C.byref(k), C.byref(us), C.byref(by)))
+        return k.value, us.value, by.value
 
 
 class BatchHandle(IcHandle):

@@ -44,3 +44,64 @@ def test_cpu_baseline_reports_one_thread_beside_all_threads():
     if out["cores"] > 1:
         one = out["single_thread"]
         assert one["cores"] == 1 and one["value"] > 0 and one["ms_per_iteration"] > 0
+
+
+def _fake_outputs():
+    """Figures shaped like the bench's own (the longest strings it writes)."""
+    one = {"seconds": 0.2374, "iterations": 1000, "converged": False, "final_residual": 0.2371,
+           "cpu_baseline": {"value": 2.27}}
+    multi = {"nrhs": 16, "seconds": 1.6, "iterations": [517] * 16, "all_converged": True,
+             "max_rel_error_vs_fp64": 8.8e-11, "fp64_block": {"seconds": 1.67}}
+    batch = {"seconds": 0.53, "iterations_max": 309, "all_converged": True,
+             "sequential_single_solves_s": 1.71, "setup_s": 3.2, "cpu_baseline": {"value": 8.1},
+             "tuned": {"seconds": 0.52, "abmc_block_size": 2, "abmc_num_colors": 4}}
+    extras = {k: bench.summary(k, v) for k, v in (
+        ("laplace48", one), ("curlcurl64", one), ("curlcurl64-mixed", one),
+        ("eddy78-mixed", one), ("batch64", batch), ("eddy78-lossy-multi16", multi))}
+    return extras
+
+
+def test_final_line_is_compact_json():
+    import json
+    desc = bench.WORKLOADS["eddy78"]["desc"]
+    line = {
+        "metric": bench.METRIC, "value": 2.881234, "unit": "s", "n_gpus": 1, "steps": 20,
+        "warmup": 5, "ms_per_step": 2881.234, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128 (c64 preconditioner)", "data": "synthetic",
+        "config": {"workload": "eddy78-multi16: " + desc + "; 16 complex-Gaussian right-hand "
+                   "sides (seed 2) in one block solve, complex<float> IC factor and sweeps "
+                   "inside complex<double> COCR", "n": 1387386, "nnz": 17821650, "nrhs": 16,
+                   "precision": "mixed", "iterations_max": 1000, "converged": "0/16",
+                   "final_residual_max": 0.9123, "err_vs_fp64_block_max": 0.512,
+                   "fp64_block_s": 3.36123, "fp64_one_vector_s": 0.47512,
+                   "fp64_one_vector_residual": 0.8851, "speedup_vs_fp64_block": 1.17,
+                   "speedup_vs_fp64_one_vector_each": 2.64, "us_per_iteration": 2881.2,
+                   "effective_GB_s": 2810.5, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "kernel": bench.TRACE_KERNELS["colour"], "achieved": 1234.5,
+                     "peak": 8000.0, "unit": "GB/s", "frac": 0.1543, "traffic": 85123456,
+                     "bytes_per_launch": 23456789, "us_per_launch": 28.12,
+                     "share_us_per_iteration": 634.1,
+                     "source": "HIP events around each in-solve launch",
+                     "traffic_source": "profiles/colour_eddy78-multi16_traffic.md"},
+        "cpu_baseline": {"value": 1634.123, "unit": "s", "cores": 16, "kind": "reference",
+                         "sample": "the reference's FP64 COCR on right-hand side 0 of the same "
+                                   "system: setup + 180 iterations timed (1.23s + 18.77s), "
+                                   "extrapolated to 1000 iterations x 16 right-hand sides, "
+                                   "setup apart", "per_rhs_s": 102.1, "setup_s": 1.23,
+                         "ms_per_iteration": 102.1},
+        "extras": _fake_outputs(),
+    }
+    out = bench.final_line(line)
+    assert len(out) <= bench.MAX_LINE, len(out)
+    back = json.loads(out)
+    assert back["value"] == line["value"] and back["roofline"]["frac"] == 0.1543
+    assert "\n" not in out
+    # every extra workload keeps at least its time
+    assert set(back["extras"]) == set(line["extras"])
+
+
+def test_dominant_kernel_is_the_largest_share():
+    tr = {"spmv": {"us_per_iteration": 590.0}, "colour": {"us_per_iteration": 640.0},
+          "pq": {"us_per_iteration": 630.0}}
+    assert bench.dominant(tr)[0] == "colour"
+    assert bench.dominant({}) == (None, None)
