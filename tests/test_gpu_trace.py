@@ -14,13 +14,15 @@ CLASSES = (_lib.TRACE_SPMV, _lib.TRACE_COLOUR, _lib.TRACE_UPDATE, _lib.TRACE_PQ,
 
 @pytest.mark.parametrize("case", ["iccg", "cocr-block"])
 def test_trace_keeps_results_and_counts_launches(case):
+    # sizes with colours large enough to run as colour kernels of their own
+    # (smaller colours share the small-colour kernel)
     if case == "iccg":
-        s = P.poisson(20)
+        s = P.poisson(48)
         A, method, nrhs = s.A, _lib.METHOD_ICCG, 1
         B = s.b[None, :]
         cfg = _lib.default_config(use_abmc=1, max_iterations=300)
     else:
-        A = P.eddy(6, cond_frac=1.0, sigma=0.1).A
+        A = P.eddy(16, cond_frac=1.0, sigma=0.1).A
         method, nrhs = _lib.METHOD_ICCOCR, 4
         B = P.complex_gaussian_block(A.n, nrhs, 2)
         cfg = _lib.default_config(use_abmc=1, abmc_num_colors=8, max_iterations=300,
