@@ -149,6 +149,22 @@ int ssa_ic_factor(const ssa_ic* ic, int32_t* row_ptr, int32_t* col_idx, void* va
 
 /* ICPreconditioner::apply (ic_preconditioner.hpp:215-225): y = M^{-1} x, run on the GPU. */
 int ssa_ic_apply(ssa_ic* ic, const void* x, void* y, int on_device);
+
+/* No reference counterpart: ssa_ic_apply for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  x and y are device pointers to one vector each in
+ * the caller's ordering; the call makes no device allocation (no hipMalloc,
+ * hipFree, stream or event creation) and returns y bit for bit as
+ * ssa_ic_apply does.  Its work vectors live in one fixed device array of
+ * SSA_NOALLOC_SCRATCH_BYTES per GPU, held from the first ssa_context_create
+ * on that GPU until the process ends: a system needs 256-byte-rounded
+ * (one stop state per system + n scalars + 2 n preconditioner scalars, plus
+ * n more in mixed precision and n more for the scaled level-path apply), and
+ * a larger one is rejected with SSA_ERR_ARGUMENT.  All calls on one GPU share
+ * the array: calls that overlap on one GPU (several threads, several
+ * contexts) race, so the caller serialises them. */
+#define SSA_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int ssa_ic_apply_noalloc(ssa_ic* ic, const void* x, void* y);
 /* SparseMatrixView::multiply (core/sparse_matrix_view.hpp:139-147): y = A x on the GPU. */
 int ssa_spmv(ssa_ic* ic, const void* x, void* y, int on_device);
 /* The operator of the handle's solves and SpMV replaced by another matrix of the

@@ -15,6 +15,10 @@
 struct ssa_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the device address of ssa_ic_apply_noalloc's scratch array on this
+    // context's GPU (resolved once per device by ssa_context_create)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
 };
 
 namespace ssa_impl {
@@ -35,6 +39,9 @@ struct IcBase {
     virtual ~IcBase() = default;
     // nrhs vectors of info().n values each, back to back
     virtual void apply(const void* x, void* y, bool on_device, int nrhs) = 0;
+    // apply() of one vector (device pointers) with every work vector in the
+    // context's scratch array: no device allocation, no stream or event
+    virtual void apply_noalloc(const void* x, void* y) = 0;
     virtual void spmv(const void* x, void* y, bool on_device, int nrhs) = 0;
     virtual void set_operator(int n, const int32_t* row_ptr, const int32_t* col_idx,
                               const void* values) = 0;

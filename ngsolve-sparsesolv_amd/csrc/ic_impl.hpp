@@ -301,6 +301,7 @@ struct IcImpl final : IcBase {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipStream_t sweep_stream = nullptr;   // where sweep kernels go (null: the context's)
+    bool serial_sweeps = false;           // no walk overlap (no second stream is made)
     static_assert(sparsesolv_amd::kWalkDepth % kWalkPrefetch == 0, "walk padding");
     DevBuf lo_a, lo_b, lo_t;
     // diagonal_scaling: S = diag(1/sqrt|a_ii|) per device row and the
@@ -731,7 +732,7 @@ struct IcImpl final : IcBase {
     // before the first backward phase that reads the walk's rows.
     void launch_sweeps(const P* in, P* out, P* tbuf, const SolverState* st)
     {
-        if (!use_overlap()) {
+        if (serial_sweeps || !use_overlap()) {
             launch_sweep<true>(fwd, in, tbuf, st);
             launch_sweep<false>(bwd, tbuf, out, st);
             return;
@@ -927,13 +928,15 @@ struct IcImpl final : IcBase {
         return D.walk_fits;
     }
 
-    // out = M^{-1} in (ABMC order); t is the forward result
-    void launch_precond(const P* in, P* out, P* tbuf, const SolverState* st)
+    // out = M^{-1} in (ABMC order); t is the forward result; sb: the scaled
+    // input's buffer (null: the handle's)
+    void launch_precond(const P* in, P* out, P* tbuf, const SolverState* st, P* sb = nullptr)
     {
         if (scale_apply) {
             // S M^-1 S: the sweeps between two scalings
-            launch_scale(in, sbuf.as<P>());
-            launch_sweeps(sbuf.as<P>(), out, tbuf, st);
+            P* sv = sb ? sb : sbuf.as<P>();
+            launch_scale(in, sv);
+            launch_sweeps(sv, out, tbuf, st);
             launch_scale(out, out);
             return;
         }
@@ -1129,6 +1132,79 @@ struct IcImpl final : IcBase {
         T* dst = on_device ? yout : tmp2.as<T>();
         scatter_out(z.as<T>(), dst, m);
         if (!on_device) HIP_CHECK(hipMemcpyAsync(yout, tmp2.p, vb, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // bytes of scratch apply_noalloc carves: the stop states, the gathered
+    // input (T), its preconditioner-precision copy (mixed), the forward
+    // result, the output, the scaled input (level path with scaling), each
+    // rounded up to 256 bytes
+    size_t noalloc_bytes() const
+    {
+        auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+        const size_t vn = size_t(n);
+        size_t b = up(sizeof(SolverState) * size_t(count)) + up(vn * sizeof(T)) + 2 * up(vn * sizeof(P));
+        if (kMixed) b += up(vn * sizeof(P));
+        if (has_scale && !abmc_order) b += up(vn * sizeof(P));
+        return b;
+    }
+
+    void apply_noalloc(const void* xin, void* yout) override
+    {
+        need_device();
+        const size_t need = noalloc_bytes();
+        if (!ctx->scratch || need > ctx->scratch_bytes)
+            throw SsaError(SSA_ERR_ARGUMENT,
+                           "ssa_ic_apply_noalloc: the system needs " + std::to_string(need) +
+                               " bytes of scratch, more than the " + std::to_string(ctx->scratch_bytes) +
+                               " the library holds per GPU (use ssa_ic_apply)");
+        hipStream_t s = ctx->stream;
+        char* q = static_cast<char*>(ctx->scratch);
+        auto take = [&](size_t b) {
+            char* r0 = q;
+            q += (b + 255) & ~size_t(255);
+            return r0;
+        };
+        const size_t vn = size_t(n);
+        SolverState* st = reinterpret_cast<SolverState*>(take(sizeof(SolverState) * size_t(count)));
+        T* a = reinterpret_cast<T*>(take(vn * sizeof(T)));
+        P* pa = kMixed ? reinterpret_cast<P*>(take(vn * sizeof(P))) : reinterpret_cast<P*>(a);
+        P* tb = reinterpret_cast<P*>(take(vn * sizeof(P)));
+        P* ob = reinterpret_cast<P*>(take(vn * sizeof(P)));
+        P* sb = (has_scale && !abmc_order) ? reinterpret_cast<P*>(take(vn * sizeof(P))) : nullptr;
+        // one right-hand side, serial sweeps; the handle's own settings
+        // come back on return
+        struct Keep {
+            IcImpl& h;
+            int W;
+            bool serial, scale;
+            ~Keep()
+            {
+                h.W = W;
+                h.serial_sweeps = serial;
+                h.scale_apply = scale;
+            }
+        } keep{*this, W, serial_sweeps, scale_apply};
+        W = 1;
+        serial_sweeps = true;
+        scale_apply = sb != nullptr;   // the preconditioner's own apply: scaled on the level path
+        // the stop states start at zero (the array keeps the last call's contents)
+        const size_t words = sizeof(SolverState) * size_t(count) / sizeof(unsigned);
+        ssa::k_zero_words<<<1, ssa::kBlock, 0, s>>>(reinterpret_cast<unsigned*>(st), words);
+        HIP_CHECK(hipGetLastError());
+        launch_gather(rev.as<index_t>(), static_cast<const T*>(xin), a, n);
+        if (kMixed) {
+            ssa::k_copy<T, P><<<vgrid(), ssa::kBlock, 0, s>>>(n, a, pa, nullptr, systems());
+            HIP_CHECK(hipGetLastError());
+        }
+        launch_precond(pa, ob, tb, st, sb);
+        const T* res = reinterpret_cast<const T*>(ob);
+        if (kMixed) {
+            ssa::k_copy<P, T><<<vgrid(), ssa::kBlock, 0, s>>>(n, ob, a, nullptr, systems());
+            HIP_CHECK(hipGetLastError());
+            res = a;
+        }
+        launch_gather(perm.as<index_t>(), res, static_cast<T*>(yout), tot);
         HIP_CHECK(hipStreamSynchronize(s));
     }
 

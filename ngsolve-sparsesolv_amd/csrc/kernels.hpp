@@ -750,6 +750,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(index_t n, const index_t* __r
     }
 }
 
+// p[0 .. words) = 0: clears device state with a kernel (no memset)
+static __global__ __launch_bounds__(kBlock) void k_zero_words(unsigned* __restrict__ p, size_t words)
+{
+    for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < words; e += size_t(gridDim.x) * kBlock)
+        p[e] = 0u;
+}
+
 // dst = src, converted to the destination's precision
 template <typename TS, typename TD>
 __global__ __launch_bounds__(kBlock) void k_copy(index_t n, const TS* __restrict__ src,

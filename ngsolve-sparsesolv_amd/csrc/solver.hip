@@ -6,6 +6,7 @@
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,32 @@ int guarded(Fn&& fn)
         g_last_error = e.what();
         return SSA_ERR_RUNTIME;
     }
+}
+
+// Work space of ssa_ic_apply_noalloc: a file-scope device array, one copy
+// per GPU for the life of the process.  Host code never uses the array's
+// name as an address: ssa_context_create resolves the device address once
+// per GPU (with that GPU current) and every use goes through it.
+static __device__ __align__(256) unsigned char scratch[SSA_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the scratch array's address on `device` (the current device)
+void* scratch_address(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        throw SsaError(SSA_ERR_ARGUMENT, "ssa_context_create: device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw SsaError(SSA_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
 }
 
 }  // namespace
@@ -93,8 +120,11 @@ int ssa_context_create(int device, ssa_context** out)
         if (device < 0 || device >= count)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_context_create: no such device " + std::to_string(device));
         HIP_CHECK(hipSetDevice(device));
+        void* scr = scratch_address(device);
         auto* c = new ssa_context();
         c->device = device;
+        c->scratch = scr;
+        c->scratch_bytes = SSA_NOALLOC_SCRATCH_BYTES;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete c;
@@ -238,6 +268,14 @@ int ssa_ic_apply(ssa_ic* ic, const void* x, void* y, int on_device)
     return guarded([&] {
         if (!ic || !x || !y) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_apply: null argument");
         ic->impl->apply(x, y, on_device != 0, 1);
+    });
+}
+
+int ssa_ic_apply_noalloc(ssa_ic* ic, const void* x, void* y)
+{
+    return guarded([&] {
+        if (!ic || !x || !y) throw SsaError(SSA_ERR_ARGUMENT, "ssa_ic_apply_noalloc: null argument");
+        ic->impl->apply_noalloc(x, y);
     });
 }
 

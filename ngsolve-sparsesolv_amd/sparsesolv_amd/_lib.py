@@ -38,7 +38,7 @@ EXPORTED = (
     "ssa_ic_schedule", "ssa_ic_factor", "ssa_ic_apply", "ssa_spmv", "ssa_solve",
     "ssa_ic_profile", "ssa_batch_create", "ssa_batch_solve", "ssa_ic_systems",
     "ssa_solve_multi", "ssa_ic_apply_multi", "ssa_spmv_multi", "ssa_ic_profile_multi",
-    "ssa_ic_set_operator", "ssa_ic_trace", "ssa_ic_trace_read",
+    "ssa_ic_set_operator", "ssa_ic_trace", "ssa_ic_trace_read", "ssa_ic_apply_noalloc",
 )
 
 PROFILE_SPMV = 0
@@ -144,6 +144,7 @@ def load() -> C.CDLL:
         L.ssa_spmv_multi.argtypes = [vp, C.c_int, vp, vp, C.c_int]
         L.ssa_ic_profile_multi.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                            C.POINTER(C.c_double)]
+        L.ssa_ic_apply_noalloc.argtypes = [vp, vp, vp]
         L.ssa_ic_trace.argtypes = [vp, C.c_int]
         L.ssa_ic_trace_read.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]
