@@ -64,6 +64,12 @@ constexpr int walk_record_words(int kmax, int fs) { return ((kmax + 1) * fs + (2
 // entries a sweep kernel loads per batch; the entry arrays end with this
 // many padding slots so a batch never needs a bounds guard
 constexpr int kSweepChunk = 16;
+// blocks a colour's longest-row sort reorders among: 16 blocks (64 rows of
+// 4-row blocks) keep the slices that run together spatially close
+// (64^3 curl-curl ICCG 72.0 -> 69.0 ms per 300 iterations, 48^3 Laplacian
+// 11.5 -> 10.9 ms against a whole-colour sort; 16 right-hand sides
+// unchanged within 1%)
+constexpr size_t kSortWindow = 16;
 
 template <typename F>
 struct SpmvLayout {
@@ -278,8 +284,13 @@ struct SweepLayout {
             std::vector<std::pair<index_t, index_t>> keyed;
             keyed.reserve(regular.size());
             for (index_t b : regular) keyed.emplace_back(longest(b), b);
-            std::stable_sort(keyed.begin(), keyed.end(),
-                             [](const auto& x, const auto& y) { return x.first > y.first; });
+            // sorted within windows of kSortWindow consecutive blocks: the
+            // slices that run together keep the blocks' (breadth-first,
+            // hence spatial) order, so their gathers stay close
+            for (size_t w0 = 0; w0 < keyed.size(); w0 += kSortWindow)
+                std::stable_sort(keyed.begin() + w0,
+                                 keyed.begin() + std::min(keyed.size(), w0 + kSortWindow),
+                                 [](const auto& x, const auto& y) { return x.first > y.first; });
             // fill slices with whole blocks, then lay them out stride slots
             // apart
             struct Slice {

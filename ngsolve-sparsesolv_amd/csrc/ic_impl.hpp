@@ -340,9 +340,12 @@ struct IcImpl final : IcBase {
             return;
         }
         if (trace_used == trace_pool.size()) {
+            // timing-only events: no system-scope fence (cache writeback and
+            // invalidation) at record, which would lengthen the bracketed
+            // launch; they are read after the stream has been synchronised
             TracePair tp;
-            HIP_CHECK(hipEventCreate(&tp.a));
-            if (hipEventCreate(&tp.b) != hipSuccess) {
+            HIP_CHECK(hipEventCreateWithFlags(&tp.a, hipEventDisableSystemFence));
+            if (hipEventCreateWithFlags(&tp.b, hipEventDisableSystemFence) != hipSuccess) {
                 (void)hipEventDestroy(tp.a);
                 throw SsaError(SSA_ERR_DEVICE, "ssa_ic_trace: hipEventCreate failed");
             }
