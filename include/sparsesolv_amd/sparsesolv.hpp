@@ -6,7 +6,9 @@
 //   SolverConfig        core/solver_config.hpp:33-159
 //   SolverResult        core/types.hpp:29-37
 //   SparseMatrixView    core/sparse_matrix_view.hpp:58-173 (non-owning CSR)
+//   Preconditioner      core/preconditioner.hpp:22-72
 //   ICPreconditioner    preconditioners/ic_preconditioner.hpp:52-1543
+//   IterativeSolver     solvers/iterative_solver.hpp:44-322
 //   CGSolver, COCRSolver solvers/cg_solver.hpp:59-147, solvers/cocr_solver.hpp:58-187
 //   solve_iccg, solve_cocr sparsesolv.hpp:43-105, :153-169
 // Errors surface as the reference's exception types: std::runtime_error for
@@ -20,10 +22,12 @@
 #include <algorithm>
 #include <complex>
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../sparsesolv_amd.h"
@@ -144,43 +148,157 @@ inline std::shared_ptr<ssa_context> context(int device)
 
 }  // namespace detail
 
-template <typename Scalar>
+// Non-owning CSR view (core/sparse_matrix_view.hpp:58-173): the caller keeps
+// the arrays alive.  multiply() is the view's host matrix-vector product, as
+// in the reference (client code uses it for residual checks); the solvers
+// below run their products on the GPU.
+template <typename Scalar = double>
 class SparseMatrixView {
 public:
+    SparseMatrixView() = default;
     SparseMatrixView(index_t rows, index_t cols, const index_t* row_ptr, const index_t* col_idx,
                      const Scalar* values)
         : rows_(rows), cols_(cols), row_ptr_(row_ptr), col_idx_(col_idx), values_(values) {}
     index_t rows() const { return rows_; }
     index_t cols() const { return cols_; }
-    index_t nnz() const { return row_ptr_[rows_]; }
+    index_t nnz() const { return row_ptr_ ? row_ptr_[rows_] : 0; }
     const index_t* row_ptr() const { return row_ptr_; }
     const index_t* col_idx() const { return col_idx_; }
     const Scalar* values() const { return values_; }
 
+    // [start, end) of row i in col_idx / values
+    std::pair<index_t, index_t> row_range(index_t i) const
+    {
+        if (i < 0 || i >= rows_) throw std::out_of_range("SparseMatrixView::row_range: row out of range");
+        return {row_ptr_[i], row_ptr_[i + 1]};
+    }
+
+    // A(i, j) by binary search of the row's sorted columns; 0 when absent
+    Scalar operator()(index_t i, index_t j) const
+    {
+        const auto r = row_range(i);
+        const index_t* it = std::lower_bound(col_idx_ + r.first, col_idx_ + r.second, j);
+        if (it != col_idx_ + r.second && *it == j) return values_[it - col_idx_];
+        return Scalar(0);
+    }
+
+    Scalar diagonal(index_t i) const { return (*this)(i, i); }
+
+    // y = A x (x: cols entries, y: rows entries)
+    void multiply(const Scalar* x, Scalar* y) const
+    {
+        for (index_t i = 0; i < rows_; ++i) {
+            Scalar sum = Scalar(0);
+            for (index_t k = row_ptr_[i]; k < row_ptr_[i + 1]; ++k) sum += values_[k] * x[col_idx_[k]];
+            y[i] = sum;
+        }
+    }
+
+    void multiply(const std::vector<Scalar>& x, std::vector<Scalar>& y) const
+    {
+        if (static_cast<index_t>(x.size()) < cols_ || static_cast<index_t>(y.size()) < rows_)
+            throw std::invalid_argument("SparseMatrixView::multiply: vector too short");
+        multiply(x.data(), y.data());
+    }
+
 private:
-    index_t rows_, cols_;
-    const index_t* row_ptr_;
-    const index_t* col_idx_;
-    const Scalar* values_;
+    index_t rows_ = 0, cols_ = 0;
+    const index_t* row_ptr_ = nullptr;
+    const index_t* col_idx_ = nullptr;
+    const Scalar* values_ = nullptr;
 };
 
-// IC(0) preconditioner whose factor lives on the GPU.  setup() runs the ABMC
+namespace detail {
+
+// r . y (conj(r) . y when conjugate), summed in index order
+template <typename Scalar>
+Scalar dot(const Scalar* r, const Scalar* y, index_t size, bool conjugate)
+{
+    Scalar s = Scalar(0);
+    for (index_t i = 0; i < size; ++i) {
+        if constexpr (std::is_same_v<Scalar, double>) {
+            (void)conjugate;
+            s += r[i] * y[i];
+        } else {
+            s += (conjugate ? std::conj(r[i]) : r[i]) * y[i];
+        }
+    }
+    return s;
+}
+
+// a fingerprint of a matrix's values (whether they changed in place)
+template <typename Scalar>
+std::uint64_t fingerprint(const Scalar* v, index_t nnz)
+{
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(v);
+    const size_t bytes = static_cast<size_t>(nnz) * sizeof(Scalar);
+    std::uint64_t h = 0x9E3779B97F4A7C15ull;
+    for (size_t k = 0; k + 8 <= bytes; k += 8) {
+        std::uint64_t w;
+        std::memcpy(&w, p + k, 8);
+        h = (h ^ w) * 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 31;
+    }
+    return h;
+}
+
+}  // namespace detail
+
+// Abstract preconditioner (core/preconditioner.hpp:22-72): setup(A), then
+// apply(x, y) = M^{-1} x.  device_handle_for() is this path's addition: the
+// library handle that runs the preconditioner inside a GPU solve of A, or
+// null for a preconditioner with no device implementation (the GPU solvers
+// reject those with std::invalid_argument).
+template <typename Scalar = double>
+class Preconditioner {
+public:
+    virtual ~Preconditioner() = default;
+    virtual void setup(const SparseMatrixView<Scalar>& A) = 0;
+    virtual void apply(const Scalar* x, Scalar* y, index_t size) const = 0;
+
+    void apply(const std::vector<Scalar>& x, std::vector<Scalar>& y) const
+    {
+        if (x.size() != y.size()) throw std::invalid_argument("Preconditioner::apply: size mismatch");
+        apply(x.data(), y.data(), static_cast<index_t>(x.size()));
+    }
+
+    // y = M^{-1} x and dot(r_for_dot, y) (conj(r_for_dot) . y when
+    // conjugate): apply, then the dot (core/preconditioner.hpp:51-65)
+    virtual Scalar apply_fused_dot(const Scalar* r_for_dot, const Scalar* x, Scalar* y, index_t size,
+                                   bool conjugate) const
+    {
+        apply(x, y, size);
+        return detail::dot(r_for_dot, y, size, conjugate);
+    }
+
+    virtual std::string name() const = 0;
+    virtual bool is_ready() const { return is_setup_; }
+
+    virtual ssa_ic* device_handle_for(const SparseMatrixView<Scalar>& A) const
+    {
+        (void)A;
+        return nullptr;
+    }
+
+protected:
+    bool is_setup_ = false;
+};
+
+// IC(0) preconditioner whose factor lives on the GPU
+// (preconditioners/ic_preconditioner.hpp:52-1543).  setup() runs the ABMC
 // schedule and factorisation on the host and uploads them; apply() runs the
 // triangular sweeps on the GPU (host vectors in, host vectors out).
 template <typename Scalar = double>
-class ICPreconditioner {
+class ICPreconditioner : public Preconditioner<Scalar> {
 public:
     explicit ICPreconditioner(double shift = 1.05) { config_.shift_parameter = shift; }
 
-    void set_config(const SolverConfig& c)
-    {
-        config_ = c;
-    }
+    void set_config(const SolverConfig& c) { config_ = c; }
     const SolverConfig& config() const { return config_; }
     double shift_parameter() const { return config_.shift_parameter; }
     void set_shift_parameter(double s) { config_.shift_parameter = s; }
 
-    void setup(const SparseMatrixView<Scalar>& A)
+    void setup(const SparseMatrixView<Scalar>& A) override
     {
         ctx_ = detail::context(config_.device);
         ssa_config c = config_.to_c();
@@ -195,18 +313,31 @@ public:
         m_rp_.assign(A.row_ptr(), A.row_ptr() + A.rows() + 1);
         m_ci_.assign(A.col_idx(), A.col_idx() + nnz);
         m_v_.assign(A.values(), A.values() + nnz);
+        a_print_ = detail::fingerprint(A.values(), nnz);
         op_.reset();
+        this->is_setup_ = true;
     }
 
-    // The handle a solve of operator A runs on: this preconditioner's own when
-    // A is the matrix it was set up on; otherwise a second handle with the
-    // same factor (set up once on the kept copy of that matrix) whose
-    // operator is A, uploaded at every call (A's values may have changed) --
-    // the reference's solvers use a preconditioner as given, whatever matrix
-    // it was set up on (solvers/iterative_solver.hpp:58-80).
-    ssa_ic* handle_for(const SparseMatrixView<Scalar>& A) const
+    // The handle a solve of operator A runs on.  The reference's solvers
+    // multiply by the live A at every solve and use the preconditioner as
+    // given, whatever matrix it was set up on (solvers/iterative_solver.hpp:
+    // 58-80).  A the matrix this preconditioner was set up on: its own
+    // handle, whose operator is uploaded again when A's values have changed
+    // in place since (the factor stays the set-up one); another matrix of
+    // the same order: a second handle with the same factor (set up once on
+    // the kept copy of the set-up matrix) whose operator A is uploaded at
+    // every call.
+    ssa_ic* device_handle_for(const SparseMatrixView<Scalar>& A) const override
     {
-        if (A.values() == a_) return ic_.get();
+        if (!ic_) throw std::runtime_error("ICPreconditioner::apply called before setup");
+        if (A.values() == a_) {
+            const std::uint64_t now = detail::fingerprint(A.values(), A.nnz());
+            if (now != a_print_) {
+                detail::check(ssa_ic_set_operator(ic_.get(), A.rows(), A.row_ptr(), A.col_idx(), A.values()));
+                a_print_ = now;
+            }
+            return ic_.get();
+        }
         if (A.rows() != n_) throw std::invalid_argument("solve: the preconditioner's order differs");
         if (!op_) {
             ssa_config c = config_.to_c();
@@ -219,15 +350,24 @@ public:
         return op_.get();
     }
 
-    void apply(const Scalar* x, Scalar* y, index_t size) const
+    void apply(const Scalar* x, Scalar* y, index_t size) const override
     {
         if (!ic_) throw std::runtime_error("ICPreconditioner::apply called before setup");
         if (size != n_) throw std::invalid_argument("ICPreconditioner::apply: size mismatch");
         detail::check(ssa_ic_apply(ic_.get(), x, y, 0));
     }
+    using Preconditioner<Scalar>::apply;
 
-    std::string name() const { return "IC"; }
-    bool is_ready() const { return static_cast<bool>(ic_); }
+    // ic_preconditioner.hpp:516-532: the GPU apply, then r . y on the host
+    Scalar apply_fused_dot(const Scalar* r_for_dot, const Scalar* x, Scalar* y, index_t size,
+                           bool conjugate) const override
+    {
+        if (!ic_) throw std::runtime_error("ICPreconditioner::apply_fused_dot called before setup");
+        apply(x, y, size);
+        return detail::dot(r_for_dot, y, size, conjugate);
+    }
+
+    std::string name() const override { return "IC"; }
 
     double actual_shift() const
     {
@@ -252,80 +392,90 @@ private:
     std::shared_ptr<ssa_ic> ic_;
     index_t n_ = 0;
     const Scalar* a_ = nullptr;
+    mutable std::uint64_t a_print_ = 0;
     std::vector<index_t> m_rp_, m_ci_;
     std::vector<Scalar> m_v_;
     mutable std::shared_ptr<ssa_ic> op_;
 };
 
-namespace detail {
-
-template <typename Scalar>
-SolverResult run(int method, const SparseMatrixView<Scalar>& A, const Scalar* b, Scalar* x,
-                 index_t size, const ICPreconditioner<Scalar>* pc, const SolverConfig& cfg)
-{
-    if (size != A.rows()) throw std::invalid_argument("solve: size mismatch");
-    ICPreconditioner<Scalar> local;
-    ssa_ic* h = nullptr;
-    if (pc && pc->is_ready()) {
-        // the preconditioner as given, on A's operator
-        h = pc->handle_for(A);
-    } else {
-        // operator only (or a preconditioner never set up): a handle on A
-        // with the solver config
-        SolverConfig c = cfg;
-        if (!pc) c.use_abmc = true;
-        else c = pc->config();
-        local.set_config(c);
-        local.setup(A);
-        h = local.handle();
-    }
-    ssa_config c = cfg.to_c();
-    std::vector<double> hist(static_cast<size_t>(std::max(cfg.max_iterations, 0)) + 4);
-    ssa_result r;
-    check(ssa_solve(h, method, b, x, 0, &c, &r, hist.data()));
-    SolverResult out;
-    out.converged = r.converged != 0;
-    out.iterations = r.iterations;
-    out.final_residual = r.final_residual;
-    if (cfg.save_residual_history) out.residual_history.assign(hist.begin(), hist.begin() + r.history_len);
-    return out;
-}
-
-}  // namespace detail
-
+// Abstract iterative solver (solvers/iterative_solver.hpp:44-322): solve(A, b,
+// x, n, precond) with x the initial guess on entry.  The whole iteration runs
+// on the GPU; method() names the library's loop for this solver with (true)
+// or without a preconditioner.
 template <typename Scalar = double>
-class CGSolver {
+class IterativeSolver {
 public:
-    std::string name() const { return "CG"; }
+    virtual ~IterativeSolver() = default;
+
+    SolverResult solve(const SparseMatrixView<Scalar>& A, const Scalar* b, Scalar* x, index_t size,
+                       const Preconditioner<Scalar>* precond = nullptr)
+    {
+        if (size != A.rows()) throw std::invalid_argument("solve: size mismatch");
+        ICPreconditioner<Scalar> local;
+        ssa_ic* h = nullptr;
+        if (precond) {
+            if (!precond->is_ready())
+                throw std::runtime_error(precond->name() + " preconditioner used before setup");
+            h = precond->device_handle_for(A);
+            if (!h)
+                throw std::invalid_argument("IterativeSolver::solve: the " + precond->name() +
+                                            " preconditioner has no GPU implementation on this path");
+        } else {
+            // operator only: a handle on A with the solver's settings
+            SolverConfig c = config_;
+            c.use_abmc = true;
+            local.set_config(c);
+            local.setup(A);
+            h = local.handle();
+        }
+        ssa_config c = config_.to_c();
+        std::vector<double> hist(static_cast<size_t>(std::max(config_.max_iterations, 0)) + 4);
+        ssa_result r;
+        detail::check(ssa_solve(h, method(precond != nullptr), b, x, 0, &c, &r, hist.data()));
+        SolverResult out;
+        out.converged = r.converged != 0;
+        out.iterations = r.iterations;
+        out.final_residual = r.final_residual;
+        if (config_.save_residual_history)
+            out.residual_history.assign(hist.begin(), hist.begin() + r.history_len);
+        return out;
+    }
+
+    SolverResult solve(const SparseMatrixView<Scalar>& A, const std::vector<Scalar>& b,
+                       std::vector<Scalar>& x, const Preconditioner<Scalar>* precond = nullptr)
+    {
+        if (x.size() != b.size()) x.resize(b.size());
+        return solve(A, b.data(), x.data(), static_cast<index_t>(b.size()), precond);
+    }
+
     void set_config(const SolverConfig& c) { config_ = c; }
     const SolverConfig& config() const { return config_; }
     SolverConfig& config() { return config_; }
-    SolverResult solve(const SparseMatrixView<Scalar>& A, const Scalar* b, Scalar* x, index_t size,
-                       const ICPreconditioner<Scalar>* precond = nullptr)
-    {
-        return detail::run(precond ? SSA_METHOD_ICCG : SSA_METHOD_CG, A, b, x, size, precond, config_);
-    }
+    virtual std::string name() const = 0;
 
-private:
+protected:
+    virtual int method(bool preconditioned) const = 0;
     SolverConfig config_;
 };
 
+// CG (COCG for complex symmetric, conjugate = false) (solvers/cg_solver.hpp:59-147)
 template <typename Scalar = double>
-class COCRSolver {
+class CGSolver : public IterativeSolver<Scalar> {
 public:
-    std::string name() const { return "COCR"; }
-    void set_config(const SolverConfig& c) { config_ = c; }
-    const SolverConfig& config() const { return config_; }
-    SolverConfig& config() { return config_; }
-    SolverResult solve(const SparseMatrixView<Scalar>& A, const Scalar* b, Scalar* x, index_t size,
-                       const ICPreconditioner<Scalar>* precond = nullptr)
-    {
-        return detail::run(precond ? SSA_METHOD_ICCOCR : SSA_METHOD_COCR, A, b, x, size, precond,
-                           config_);
-    }
+    std::string name() const override { return "CG"; }
 
-private:
-    SolverConfig config_;
+protected:
+    int method(bool pre) const override { return pre ? SSA_METHOD_ICCG : SSA_METHOD_CG; }
+};
+
+// COCR (solvers/cocr_solver.hpp:58-187)
+template <typename Scalar = double>
+class COCRSolver : public IterativeSolver<Scalar> {
+public:
+    std::string name() const override { return "COCR"; }
+
+protected:
+    int method(bool pre) const override { return pre ? SSA_METHOD_ICCOCR : SSA_METHOD_COCR; }
 };
 
 template <typename Scalar = double>

@@ -48,6 +48,21 @@ def _as_csr(mat) -> CSR:
     raise TypeError("expected a sparse matrix (scipy.sparse or problems.CSR)")
 
 
+def _words_hash(a: np.ndarray) -> int:
+    """A position-weighted sum of an array's 32-bit words (wrapping 64-bit
+    arithmetic)."""
+    w = np.ascontiguousarray(a).view(np.uint32).astype(np.uint64)
+    k = np.arange(1, w.size + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        return int(np.sum((w + np.uint64(0x632BE59BD9B4E019)) * k, dtype=np.uint64))
+
+
+def _fingerprint(A: CSR) -> tuple:
+    """Content identity of a matrix (pattern and values): whether an operator
+    is the one a handle holds, or changed in place since."""
+    return (A.n, _words_hash(A.row_ptr), _words_hash(A.col_idx), _words_hash(A.values))
+
+
 def _with_freedofs(A: CSR, freedofs) -> CSR:
     """BuildSparseMatrixView (ngsolve/sparsesolv_precond.hpp:35-81): constrained
     rows become identity rows, constrained columns of free rows are zeroed;
@@ -117,7 +132,9 @@ class _ICPreconditioner:
         """Rebuild the factor with the current properties (Update(), :188-191)."""
         if self._handle is not None:
             self._handle.close()
-        self._handle = _lib.IcHandle(_with_freedofs(self.mat, self.freedofs), self.config())
+        M = _with_freedofs(self.mat, self.freedofs)
+        self._handle = _lib.IcHandle(M, self.config())
+        self._operator = _fingerprint(M)   # the operator the handle holds
         self._stale = False
 
     @property
@@ -399,8 +416,6 @@ class _COCRSolver:
         self.tol = float(tol)
         self.printrates = bool(printrates)
         self._iterations = 0
-        self._own = None        # handle: pre's factor, this solver's operator
-        self._own_key = None
 
     @property
     def iterations(self) -> int:
@@ -410,30 +425,32 @@ class _COCRSolver:
         return self._iterations
 
     def _handle(self, fd) -> _lib.IcHandle:
-        """pre's handle when the operator is pre's matrix; otherwise a handle
-        of this solver's own with pre's factor (set up on pre's matrix, as pre
-        sets it up) and this solver's matrix as the operator
-        (ssa_ic_set_operator), made again when pre is updated."""
+        """pre's handle with this solve's operator (this solver's matrix with
+        its freedofs) as the handle's operator: COCRSolverNGS multiplies by
+        the live matrix at every Mult (ngsolve/sparsesolv_solvers.hpp:273-369)
+        and applies pre's factor as set up.  The operator is uploaded
+        (ssa_ic_set_operator) only when it differs from the one the handle
+        holds (another matrix, or values changed in place); pre's own
+        applies never use it."""
         h = self.pre.handle
-        if self._same_operator():
-            return h
-        if self._own is None or self._own_key is not h:
-            own = _lib.IcHandle(_with_freedofs(self.pre.mat, self.pre.freedofs), self.pre.config())
-            own.set_operator(_with_freedofs(self.mat, fd))
-            if self._own is not None:
-                self._own.close()
-            self._own, self._own_key = own, h
-        return self._own
-
-    def _same_operator(self) -> bool:
-        a, b = self.mat, self.pre.mat
-        return (a is b) or (a.n == b.n and np.array_equal(a.row_ptr, b.row_ptr)
-                            and np.array_equal(a.col_idx, b.col_idx)
-                            and np.array_equal(a.values, b.values))
+        op = _with_freedofs(self.mat, fd)
+        fp = _fingerprint(op)
+        if fp != self.pre._operator:
+            h.set_operator(op)
+            self.pre._operator = fp
+        return h
 
     def Mult(self, rhs, sol=None):
+        """sol = A^{-1} rhs from a zero guess; vectors are projected with this
+        solver's freedofs only, as COCRSolverNGS does (:379-384)."""
         b = np.asarray(rhs, dtype=self._dtype)
-        fd = self.freedofs if self.freedofs is not None else self.pre.freedofs
+        fd = self.freedofs
+        if fd is None and self.pre.freedofs is not None:
+            # the reference then projects nothing in the loop while its
+            # preconditioner zeroes the constrained outputs; the GPU loop
+            # applies the factor without that mask
+            raise ValueError("COCRSolver: the preconditioner has freedofs; pass the same "
+                             "freedofs to COCRSolver")
         h = self._handle(fd)
         if fd is not None:
             b = np.where(fd, b, 0)

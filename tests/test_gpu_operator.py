@@ -81,3 +81,49 @@ def test_set_operator_errors():
     with pytest.raises(_lib.SsaError) as e:
         _lib.IcHandle.set_operator(hb, P.laplace7(4))
     assert e.value.code == _lib.SSA_ERR_UNSUPPORTED
+
+
+def test_cocr_solver_sees_values_changed_in_place(oracle):
+    # the reference's COCRSolverNGS multiplies by the live matrix and applies
+    # its preconditioner as set up: after an in-place change of A's values
+    # the solve is COCR on the new A with the old factor
+    import sparsesolv_amd as ss
+    A, b = complex_shifted_laplace(9)
+    A_old = P.CSR(A.n, A.row_ptr.copy(), A.col_idx.copy(), A.values.copy())
+    pre = ss.ICPreconditioner(A)
+    pre.use_abmc = True
+    pre.abmc_num_colors = 8
+    solver = ss.COCRSolver(A, pre, maxiter=500, tol=1e-10)
+    ocfg = oracle.config(use_abmc=1, abmc_num_colors=8, tolerance=1e-10, max_iterations=500,
+                         save_best_result=0)
+    x0 = solver * b
+    xo, _ = oracle.solve_cocr_pre(A, A, b, ocfg)
+    assert rel(x0, xo) < 1e-9
+    diag = A.col_idx == np.repeat(np.arange(A.n), np.diff(A.row_ptr))
+    A.values[diag] += 0.4 + 0.3j
+    x1 = solver * b
+    xo1, ro1 = oracle.solve_cocr_pre(A, A_old, b, ocfg)
+    assert ro1["converged"]
+    assert abs(solver.GetIterations() - ro1["iterations"]) <= max(1, math.ceil(0.02 * ro1["iterations"]))
+    assert rel(x1, xo1) < 1e-9
+    # a second solver on another matrix shares pre: each solve uses its own
+    M2 = shifted(A_old, 0.5)
+    s2 = ss.COCRSolver(M2, pre, maxiter=500, tol=1e-10)
+    x2 = s2 * b
+    assert rel(x2, oracle.solve_cocr_pre(M2, A_old, b, ocfg)[0]) < 1e-9
+    assert rel(solver * b, xo1) < 1e-9
+
+
+def test_cocr_solver_projects_with_its_own_freedofs():
+    import sparsesolv_amd as ss
+    A, b = complex_shifted_laplace(7)
+    free = np.ones(A.n, dtype=bool)
+    free[::5] = False
+    pre = ss.ICPreconditioner(A, freedofs=free)
+    pre.use_abmc = True
+    with pytest.raises(ValueError):
+        ss.COCRSolver(A, pre) * b          # the preconditioner's freedofs only
+    x = ss.COCRSolver(A, pre, freedofs=free, tol=1e-10) * b
+    assert np.all(x[~free] == 0)
+    r = np.where(free, b - A.to_scipy() @ x, 0)
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(np.where(free, b, 0))
