@@ -308,11 +308,12 @@ class _SparseSolvSolver:
         result = SparseSolvResult(res.converged, res.iterations, res.final_residual,
                                   hist if cfg.save_residual_history else [])
         if self.printrates and result.residual_history:
+            # doubles as std::cout writes them: 6 significant digits, %g
             for i, r in enumerate(result.residual_history):
-                print(f"{self.method} iteration {i}, residual = {r}")
+                print(f"{self.method} iteration {i}, residual = {r:g}")
             state = "converged in" if result.converged else "NOT converged after"
             print(f"{self.method} {state} {result.iterations} iterations, "
-                  f"residual = {result.final_residual}")
+                  f"residual = {result.final_residual:g}")
         self._last_result = result
         return result
 
@@ -461,7 +462,7 @@ class _COCRSolver:
         self._iterations = res.iterations
         if self.printrates:
             for i, r in enumerate(hist):
-                print(f"COCR iter {i}: res = {r}")
+                print(f"COCR iter {i}: res = {r:g}")
         if fd is not None:
             x[~fd] = 0
         if sol is not None:

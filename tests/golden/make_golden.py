@@ -7,7 +7,7 @@ build on small generated systems and stores inputs' identities and outputs
 (ABMC arrays, IC factor, solver results) as .npz.  The inputs themselves are
 regenerated by sparsesolv_amd.problems from their seeds.
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [--all]
 """
 import os
 import sys
@@ -22,6 +22,22 @@ sys.path.insert(0, os.path.join(ROOT, "ngsolve-sparsesolv_amd"))
 import pyoracle  # noqa: E402
 from sparsesolv_amd import problems as P  # noqa: E402
 
+
+def varcoef6():
+    """7-point variable-coefficient Laplacian, 6^3 cells, coefficients
+    log-uniform in [1e-2, 1e2] (seed 13), uniform RHS (seed 14)."""
+    A = P.laplace7_varcoef(6, 10.0 ** P.uniform(13, 216, -2.0, 2.0))
+    return P.System(A, P.uniform(14, 216, -1.0, 1.0), "varcoef6", {})
+
+
+def zvarcoef6():
+    """varcoef6 + 0.3j I (complex symmetric), complex Gaussian RHS (seed 15)."""
+    import scipy.sparse as sp
+    A = varcoef6().A.to_scipy().astype(np.complex128) + sp.identity(216) * 0.3j
+    z = P.gaussian(15, 2 * 216)
+    return P.System(P.CSR.from_scipy(A.tocsr()), z[0::2] + 1j * z[1::2], "zvarcoef6", {})
+
+
 # name -> (system builder, config overrides, solver)
 CASES = {
     "lap6_abmc": (lambda: P.poisson(6), dict(use_abmc=1), "iccg"),
@@ -35,6 +51,22 @@ CASES = {
     "eddy4_cocg": (lambda: P.eddy(4), dict(use_abmc=1), "iccg"),
     "eddy4_plaincocr": (lambda: P.eddy(4), dict(max_iterations=400), "plain_cocr"),
     "lap6_plaincg": (lambda: P.poisson(6), dict(), "plain_cg"),
+    # diagonal_scaling: the factor of S A S; the ABMC apply (Path 3, COCR)
+    # runs it unscaled, Path 1 and the level path scale around it
+    "vc6_scaled_abmc": (varcoef6, dict(use_abmc=1, diagonal_scaling=1), "iccg"),
+    "vc6_scaled_path1": (varcoef6, dict(use_abmc=1, diagonal_scaling=1, abmc_reorder_spmv=1), "iccg"),
+    "vc6_scaled_level": (varcoef6, dict(use_abmc=0, diagonal_scaling=1), "iccg"),
+    "zvc6_scaled_cocr": (zvarcoef6, dict(use_abmc=1, abmc_num_colors=8, diagonal_scaling=1), "cocr"),
+    # divergence_check = StagnationCount: a stop after progress (residual
+    # >= 2x the best for 5 iterations running), and threshold 1.0, where
+    # every improving iteration counts (the best is updated first)
+    "cc6_divergence": (lambda: P.curlcurl(6), dict(use_abmc=1, auto_shift=1, divergence_check=1,
+                                                  divergence_threshold=2.0, divergence_count=5), "iccg"),
+    "eddy6_divergence": (lambda: P.eddy(6), dict(use_abmc=1, abmc_num_colors=8, divergence_check=1,
+                                                divergence_threshold=2.0, divergence_count=5), "cocr"),
+    "lap6_divergence_thr1": (lambda: P.poisson(6), dict(use_abmc=1, divergence_check=1,
+                                                       divergence_threshold=1.0, divergence_count=4),
+                             "iccg"),
 }
 
 
@@ -75,8 +107,11 @@ def run(ref, name):
 
 
 def main():
+    # existing fixtures are kept unless --all is given
     ref = pyoracle.Oracle("ref")
     for name in CASES:
+        if "--all" not in sys.argv and os.path.exists(os.path.join(HERE, name + ".npz")):
+            continue
         data = run(ref, name)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **data)
         print(f"{name}: n={int(data['n'])} iterations={int(data['iterations'])} "

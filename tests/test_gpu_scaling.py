@@ -10,7 +10,9 @@ commutes with diagonal scaling (the factor of S A S is S L), so the scaled
 modes equal the unscaled preconditioner up to rounding, while the unscaled
 ABMC apply of the scaled factor is another preconditioner: on a
 variable-coefficient Laplacian it costs Path 3 ~600 iterations against 40.
-Each mode is compared with the oracle's restatement."""
+Each mode is compared with the oracle's restatement, and with golden
+vectors the reference itself wrote (tests/golden/*scaled*: Path 3, Path 1,
+the level path, COCR)."""
 import math
 
 import numpy as np
@@ -18,7 +20,10 @@ import pytest
 
 from sparsesolv_amd import _lib
 from sparsesolv_amd import problems as P
+from conftest import load_golden
 from test_gpu_parity import complex_shifted_laplace
+
+import make_golden as G
 
 pytestmark = pytest.mark.gpu
 
@@ -125,3 +130,33 @@ def test_batch_with_diagonal_scaling(use_abmc, path1):
         xs, rs, _ = _lib.IcHandle(A, cfg).solve(_lib.METHOD_ICCG, b, None, cfg)
         assert rs.converged and results[k].iterations == rs.iterations
         assert np.array_equal(xk, xs)
+
+
+FACTOR_CASES = [n for n, c in sorted(G.CASES.items()) if c[2] in ("iccg", "cocr")]
+
+
+@pytest.mark.parametrize("name", FACTOR_CASES)
+def test_apply_matches_reference_golden(name):
+    # the preconditioner's own apply (ICPreconditioner::apply) against the
+    # reference's on the fixture's vector: unscaled on the ABMC path,
+    # S M^-1 S on the level path
+    g = load_golden(name)
+    build, kw, _ = G.CASES[name]
+    A = build().A
+    h = _lib.IcHandle(A, _lib.default_config(**kw))
+    assert rel(h.apply(g["apply_x"]), g["apply_y"]) < 1e-12
+
+
+@pytest.mark.parametrize("name", [n for n in FACTOR_CASES if G.CASES[n][1].get("use_abmc", 0) == 0])
+def test_level_solve_matches_reference_golden(name):
+    g = load_golden(name)
+    build, kw, solver = G.CASES[name]
+    s = build()
+    cfg = _lib.default_config(save_residual_history=1, **kw)
+    code = _lib.METHOD_ICCG if solver == "iccg" else _lib.METHOD_ICCOCR
+    x, res, hist = _lib.IcHandle(s.A, cfg).solve(code, s.b, None, cfg)
+    n_ref = int(g["iterations"])
+    assert bool(res.converged) == bool(g["converged"])
+    assert abs(res.iterations - n_ref) <= max(1, math.ceil(0.02 * n_ref))
+    assert rel(x, g["x"]) < 1e-9
+    np.testing.assert_allclose(hist[:5], g["history"][:5], rtol=1e-9)
