@@ -259,8 +259,11 @@ int ssa_ic_profile_multi(ssa_ic* ic, int what, int nrhs, int reps, double* avg_u
  *   SSA_TRACE_SPMV    the SpMV (with its fused dot or convergence check);
  *   SSA_TRACE_COLOUR  the colour kernel k_sweep of colours without oversized
  *                     blocks (both sweeps);
- *   SSA_TRACE_UPDATE  x, r (COCR: and rt) update with the |r|^2 partials;
- *   SSA_TRACE_PQ      the p (COCR: and q) update with the best-x copy;
+ *   SSA_TRACE_UPDATE  x, r (COCR: and rt; COCR on a block of right-hand sides:
+ *                     also the previous iteration's p) update with the |r|^2
+ *                     partials;
+ *   SSA_TRACE_PQ      the p (COCR: and q; COCR on a block: q only) update with
+ *                     the best-x copy;
  *   SSA_TRACE_DOT     the dot partials after the preconditioner. */
 #define SSA_TRACE_OFF (-1)
 #define SSA_TRACE_SPMV 0

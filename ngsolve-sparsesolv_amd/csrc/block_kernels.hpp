@@ -157,12 +157,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_blk(index_t n, T* __restri
     if (threadIdx.x < W) partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
 
-// COCR: x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt may
-// be in the preconditioner's precision)
+// COCR: the previous iteration's direction update p = rt + beta p (beta
+// from the state; 0 in a pass's first iteration, where p holds rt), then
+// x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt may be in
+// the preconditioner's precision).  p is read here anyway (x += alpha p),
+// so the p, q update after the SpMV leaves p alone: the same operations on
+// the same operands as the one-vector loop (p updated there, read here),
+// two vectors fewer through HBM.
 template <typename T, typename TQ, int W>
 __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __restrict__ x,
                                                            T* __restrict__ r, T* __restrict__ rt,
-                                                           const T* __restrict__ p,
+                                                           T* __restrict__ p,
                                                            const T* __restrict__ q,
                                                            const TQ* __restrict__ qt,
                                                            double* __restrict__ partials,
@@ -172,13 +177,14 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
     __shared__ double red[kWavesPerBlock * W];
     constexpr int CP = pass_cols<double, W>();
     __shared__ double tile[kBlock * (CP + 1)];
-    T alpha[W / CP];
+    T alpha[W / CP], beta[W / CP];
     bool run[W / CP];
 #pragma unroll
     for (int h = 0; h < W / CP; ++h) {
         const int c = pass_column<double, W>(h);
         run[h] = st[c].status == 0;
         alpha[h] = from_pair<T>(st[c].alpha);
+        beta[h] = from_pair<T>(st[c].beta);
     }
     double s[W];
 #pragma unroll
@@ -187,10 +193,13 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
         const index_t rows = n - base < kBlock ? n - base : kBlock;
         tile_reduce<double, W>(base, rows, tile, s, [&](size_t e, int h) {
             if (!run[h]) return 0.0;
-            x[e] = fma_(alpha[h], p[e], x[e]);
+            const T rte = rt[e];
+            const T pe = fma_(beta[h], p[e], rte);
+            p[e] = pe;
+            x[e] = fma_(alpha[h], pe, x[e]);
             const T ri = fms_(alpha[h], q[e], r[e]);
             r[e] = ri;
-            rt[e] = fms_(alpha[h], cvt<T>(qt[e]), rt[e]);
+            rt[e] = fms_(alpha[h], cvt<T>(qt[e]), rte);
             return norm2(ri);
         });
     }
@@ -234,15 +243,15 @@ __global__ __launch_bounds__(kBlock) void k_xpby_blk(size_t total, const TZ* __r
     }
 }
 
-// COCR: best <- x when improved; p = rt + beta p; q = t + beta q (q also in
-// the preconditioner's precision when q_low is given)
+// COCR: best <- x when improved; q = t + beta q (q also in the
+// preconditioner's precision when q_low is given).  p = rt + beta p is left
+// to the next iteration's k_cocr_update_blk.
 template <typename T, typename P, int W>
-__global__ __launch_bounds__(kBlock) void k_cocr_pq_blk(size_t total, const T* __restrict__ rt,
-                                                       const T* __restrict__ t, T* __restrict__ p,
-                                                       T* __restrict__ q, const T* __restrict__ x,
-                                                       T* __restrict__ best,
-                                                       const SolverState* __restrict__ st,
-                                                       P* __restrict__ q_low)
+__global__ __launch_bounds__(kBlock) void k_cocr_q_blk(size_t total, const T* __restrict__ t,
+                                                      T* __restrict__ q, const T* __restrict__ x,
+                                                      T* __restrict__ best,
+                                                      const SolverState* __restrict__ st,
+                                                      P* __restrict__ q_low)
 {
     const int k = int(threadIdx.x) & (W - 1);
     const bool copy = best && st[k].improved;
@@ -253,7 +262,6 @@ __global__ __launch_bounds__(kBlock) void k_cocr_pq_blk(size_t total, const T* _
          e += size_t(gridDim.x) * kBlock) {
         if (copy) best[e] = x[e];
         if (run) {
-            p[e] = fma_(beta, p[e], rt[e]);
             const T qi = fma_(beta, q[e], t[e]);
             q[e] = qi;
             if (q_low) q_low[e] = cvt<P>(qi);

@@ -396,11 +396,16 @@ struct IcImpl final : IcBase {
     {
         const double e = double(n) * double(W), sv = double(sizeof(T)), sp = double(sizeof(P));
         const double low = kMixed ? sp : 0.0;   // the narrow copy a mixed kernel also writes
+        const double sq = kMixed ? sp : sv;    // qt, z: the preconditioner's output
         switch (cls) {
-        case SSA_TRACE_UPDATE:   // COCR: x, p, r, q, rt, qt read; x, r, rt written
-            return cocr ? e * (8.0 * sv + (kMixed ? sp : sv)) : e * (6.0 * sv + low);
-        case SSA_TRACE_PQ:       // COCR: rt, t, p, q read; p, q written; CG: z, p read, p written
-            return cocr ? e * (6.0 * sv + low) : e * (2.0 * sv + (kMixed ? sp : sv));
+        case SSA_TRACE_UPDATE:
+            // COCR: x, p, r, q, rt, qt read; x, r, rt (blocks: and p) written
+            if (cocr) return e * ((W > 1 ? 9.0 : 8.0) * sv + sq);
+            return e * (6.0 * sv + low);       // CG: x, p, r, Ap read; x, r written
+        case SSA_TRACE_PQ:
+            // COCR: t, q read, q written (one vector: also rt, p read, p written)
+            if (cocr) return e * ((W > 1 ? 3.0 : 6.0) * sv + low);
+            return e * (2.0 * sv + sq);        // CG: z, p read; p written
         default:                 // dot: two operands
             return e * (sv + (kMixed ? sp : sv));
         }
@@ -1345,7 +1350,7 @@ struct IcImpl final : IcBase {
                 launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st);
                 ssa::k_ctrl_beta_blk<T><<<WV, CT, 0, s>>>(part_c.as<T>(), np_spmv, st, prm);
                 traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, true), [&] {
-                    ssa::k_cocr_pq_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, low_copy());
+                    ssa::k_cocr_q_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, t.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, low_copy());
                 });
             }
         });
