@@ -285,6 +285,8 @@ static __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_init(const double*
     st->improved = 0;
     st->iter = 0;
     st->hist_len = 0;
+    st->replace_norm = init;
+    st->replace_iter = 0;
     st->status = kRunning;
     st->converged = 0;
     st->tol = pp ? pp[s].tolerance : prm.tolerance;
@@ -326,6 +328,30 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl_store(const T* __restrict
     double* dst = slot == 0 ? st->rz : st->denom;
     dst[0] = pr[0];
     dst[1] = pr[1];
+}
+
+// Mixed-precision COCR, residual replacement: whether state q recomputes
+// rt = M^{-1} r now -- when ||r|| has fallen below `drop` times its value at
+// the last replacement (or the pass start), or `gap` iterations have passed
+// since.  rst[q].status is kRunning for the states that replace and nonzero
+// for the others (the replacement's kernels read only the status: the
+// others, and stopped states, skip them).
+static __global__ __launch_bounds__(64) void k_ctrl_replace(SolverState* __restrict__ st,
+                                                            SolverState* __restrict__ rst,
+                                                            int count, double drop, int gap)
+{
+    const int q = int(blockIdx.x) * 64 + int(threadIdx.x);
+    if (q >= count) return;
+    SolverState* s = st + q;
+    int go = 0;
+    if (s->status == kRunning) {
+        go = s->norm_r <= s->replace_norm * drop || s->iter - s->replace_iter >= gap;
+        if (go) {
+            s->replace_norm = s->norm_r;
+            s->replace_iter = s->iter;
+        }
+    }
+    rst[q].status = go ? kRunning : kMaxIter;
 }
 
 }  // namespace ssa

@@ -99,10 +99,21 @@ inline int rhs_width(int nrhs) { return nrhs <= 1 ? 1 : nrhs <= 4 ? 4 : nrhs <= 
 static_assert(sparsesolv_amd::kSweepChunk >= 16, "layout padding shorter than a batch");
 // Krylov iterations per captured graph
 constexpr int kGraphIterations = 8;
-// mixed precision: refinement passes per solve, and the iterations between
-// two residual replacements of mixed-precision COCR (one per captured group)
+// mixed precision: refinement passes per solve.  Mixed-precision COCR
+// decides at the end of every captured group (kGraphIterations) whether to
+// recompute rt = M^{-1} r (residual replacement): when ||r|| has fallen
+// kReplaceDrop-fold since the last replacement, or kReplaceGap iterations
+// have passed.  A fixed interval was measured first on the 16-RHS 78^3 eddy
+// blocks: every 8 / 16 / 32 / 64 iterations the lossy system converges in
+// 471-517 / 475-499 / 470-500 / 471-510 iterations (1.547 / 1.474 / 1.456 /
+// 1.459 s), the configured one runs 1000 iterations in 2.749 / 2.691 /
+// 2.658 / 2.640 s -- but every 32 leaves a fast-converging system (a
+// complex-shifted 12^3 Laplacian) unconverged: its residual falls by
+// orders of magnitude between replacements, and the recurrence's float
+// error grows against the shrinking rt.
 constexpr int kRefinePasses = 8;
-constexpr int kReplaceEvery = kGraphIterations;
+constexpr int kReplaceGap = 32;
+constexpr double kReplaceDrop = 1e-2;
 
 inline int vec_grid(index_t n)
 {
@@ -273,6 +284,7 @@ struct IcImpl final : IcBase {
     // device: work space
     DevBuf r, z, p, Ap, t, qt, x, best, bvec, tmp, tmp2;
     DevBuf part_a, part_b, part_c, part_d, state, hist;
+    DevBuf rstate;   // mixed COCR: which states recompute rt (status 0)
     DevBuf idle;   // states that never stop: sweeps outside a solve
     // right-hand sides of the current operation (every vector holds W blocks
     // of n rows; W > 1 for one system only) and the width the work space
@@ -570,6 +582,7 @@ struct IcImpl final : IcBase {
                           size_t(W) * sizeof(ssa::c128);
         for (DevBuf* b : {&part_a, &part_b, &part_c, &part_d}) b->reserve(np);
         state.reserve(sizeof(SolverState) * nst);
+        rstate.reserve(sizeof(SolverState) * nst);
         pass.reserve(sizeof(ssa::PassParams) * nst);
         if (kMixed)
             for (DevBuf* b : {&lo_a, &lo_b, &lo_t})
@@ -1219,18 +1232,25 @@ struct IcImpl final : IcBase {
     // Mixed-precision COCR carries rt = M^{-1} r by the recurrence
     // rt -= alpha qt, and qt comes from the float preconditioner: the error
     // of the recurrence grows until the iteration stalls (near 1e-7 of the
-    // starting residual).  Every kReplaceEvery iterations rt is therefore
-    // recomputed from the FP64 residual, rt = M^{-1} r (residual
-    // replacement); t = A rt follows, so q = A p stays exact.  qt (lo_b) is
-    // consumed by then, lo_t is scratch.
-    void replace_rt(const SolverState* st)
+    // starting residual).  rt is therefore recomputed from the FP64
+    // residual, rt = M^{-1} r (residual replacement), for the states whose
+    // k_ctrl_replace decision says so (rst: status 0 for those); t = A rt
+    // follows, so q = A p stays exact.  qt (lo_b) is consumed by then, lo_t
+    // is scratch.
+    void replace_rt(SolverState* st)
     {
+        SolverState* rst = rstate.as<SolverState>();
+        const int S = nstates();
+        ssa::k_ctrl_replace<<<(S + 63) / 64, 64, 0, ctx->stream>>>(st, rst, S, kReplaceDrop, kReplaceGap);
+        HIP_CHECK(hipGetLastError());
+        st = rst;
         convert(r.as<T>(), lo_b.as<P>(), st);
         launch_precond(lo_b.as<P>(), lo_b.as<P>(), lo_t.as<P>(), st);
         convert(lo_b.as<P>(), z.as<T>(), st);
     }
     // whether iteration k of a pass (counted from 0) replaces rt
-    static bool replace_at(int k) { return k % kReplaceEvery == kReplaceEvery - 1; }
+    // whether iteration k of a pass (counted from 0) decides on a replacement
+    static bool replace_at(int k) { return k % kGraphIterations == kGraphIterations - 1; }
 
     // one Krylov iteration (kernels only; no host synchronisation)
     void enqueue_iteration(int method, const SolveParams& prm, bool replace)

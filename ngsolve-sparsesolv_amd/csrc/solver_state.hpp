@@ -38,6 +38,11 @@ struct SolverState {
     int bad_count;
     int hist_len;
     int maxit;             // iteration budget of this pass
+    // mixed-precision COCR: ||r|| and the loop index at the last
+    // recomputation of rt = M^{-1} r (or the pass start)
+    double replace_norm;
+    int replace_iter;
+    int replace_pad_;
 };
 
 // Per-system settings of one pass (set by the host before the pass): the
