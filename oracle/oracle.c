@@ -445,6 +445,44 @@ int orc_solve_plain(int dtype, int method_cocr, int n, const int* row_ptr,
                         (double _Complex*)x, cfg, res, history, err, errlen);
 }
 
+void orc_sgs_apply(int dtype, int n, const int* row_ptr, const int* col_idx,
+                   const void* values, const void* x, void* y)
+{
+    if (dtype == 0) {
+        sgs_d* g = sgs_setup_d(n, row_ptr, col_idx, (const double*)values);
+        sgs_apply_d(g, (const double*)x, (double*)y);
+        sgs_free_d(g);
+    } else {
+        sgs_z* g = sgs_setup_z(n, row_ptr, col_idx, (const double _Complex*)values);
+        sgs_apply_z(g, (const double _Complex*)x, (double _Complex*)y);
+        sgs_free_z(g);
+    }
+}
+
+int orc_solve_sgs(int dtype, int method_cocr, int n, const int* row_ptr, const int* col_idx,
+                  const void* values, const void* b, void* x, const orc_config* cfg,
+                  orc_result* res, double* history, char* err, int errlen)
+{
+    return dtype == 0
+        ? solve_sgs_pre_d(method_cocr ? 1 : 0, n, row_ptr, col_idx, (const double*)values,
+                          (const double*)b, (double*)x, cfg, res, history, err, errlen)
+        : solve_sgs_pre_z(method_cocr ? 1 : 0, n, row_ptr, col_idx,
+                          (const double _Complex*)values, (const double _Complex*)b,
+                          (double _Complex*)x, cfg, res, history, err, errlen);
+}
+
+int orc_solve_sgsmrtr(int dtype, int n, const int* row_ptr, const int* col_idx,
+                      const void* values, const void* b, void* x, const orc_config* cfg,
+                      orc_result* res, double* history, char* err, int errlen)
+{
+    return dtype == 0
+        ? solve_sgsmrtr_d(n, row_ptr, col_idx, (const double*)values, (const double*)b,
+                          (double*)x, cfg, res, history, err, errlen)
+        : solve_sgsmrtr_z(n, row_ptr, col_idx, (const double _Complex*)values,
+                          (const double _Complex*)b, (double _Complex*)x, cfg, res, history,
+                          err, errlen);
+}
+
 double orc_time_iterations(int dtype, int method, int n, const int* row_ptr,
                            const int* col_idx, const void* values,
                            const void* b, const orc_config* cfg, int iters,

@@ -121,6 +121,21 @@ int orc_solve_plain(int dtype, int method_cocr, int n, const int* row_ptr,
                     void* x, const orc_config* cfg, orc_result* res,
                     double* history, char* err, int errlen);
 
+/* ---- SGS preconditioner and SGS-MRTR --------------------------------- */
+/* y = M^{-1} x of the SGSPreconditioner set up on the matrix
+ * (preconditioners/sgs_preconditioner.hpp:71-131). */
+void orc_sgs_apply(int dtype, int n, const int* row_ptr, const int* col_idx,
+                   const void* values, const void* x, void* y);
+/* CGSolver (method_cocr 0) / COCRSolver (1) with an SGSPreconditioner set up
+ * on the operator (iterative_solver.hpp:58-80). */
+int orc_solve_sgs(int dtype, int method_cocr, int n, const int* row_ptr, const int* col_idx,
+                  const void* values, const void* b, void* x, const orc_config* cfg,
+                  orc_result* res, double* history, char* err, int errlen);
+/* solve_sgsmrtr (sparsesolv.hpp:124-137): SGSMRTRSolver. */
+int orc_solve_sgsmrtr(int dtype, int n, const int* row_ptr, const int* col_idx,
+                      const void* values, const void* b, void* x, const orc_config* cfg,
+                      orc_result* res, double* history, char* err, int errlen);
+
 /* Bounded timing helper for the CPU baseline: runs exactly `iters`
  * preconditioned iterations (no convergence stop) of ICCG (method 0) or
  * COCR (method 1) after setup and returns the wall seconds spent in the

@@ -90,6 +90,11 @@ class Oracle:
         L.orc_solve_plain.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, vp, vp, vp,
                                       C.POINTER(OrcConfig), C.POINTER(OrcResult), vp,
                                       C.c_char_p, C.c_int]
+        L.orc_sgs_apply.argtypes = [C.c_int, C.c_int, _i32p, _i32p, vp, vp, vp]
+        L.orc_solve_sgs.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, vp, vp, vp,
+                                    C.POINTER(OrcConfig), C.POINTER(OrcResult), vp,
+                                    C.c_char_p, C.c_int]
+        L.orc_solve_sgsmrtr.argtypes = sargs
         L.orc_time_iterations.restype = C.c_double
         L.orc_time_iterations.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, vp, vp,
                                           C.POINTER(OrcConfig), C.c_int,
@@ -168,6 +173,26 @@ class Oracle:
         def fn(dtype, *rest):
             return f(dtype, 1 if method.upper() == "COCR" else 0, *rest)
         return self._solve(fn, A, b, cfg, x0)
+
+    def sgs_apply(self, A, x):
+        """y = M^{-1} x of the SGS preconditioner set up on A."""
+        dt = np.complex128 if A.is_complex else np.float64
+        vals = np.ascontiguousarray(A.values, dtype=dt)
+        x = np.ascontiguousarray(x, dtype=dt)
+        y = np.empty_like(x)
+        self.lib.orc_sgs_apply(1 if A.is_complex else 0, A.n, A.row_ptr, A.col_idx,
+                               vals.ctypes.data, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def solve_sgs(self, A, b, cfg, method: str = "CG", x0=None):
+        """CG / COCR on A preconditioned by the SGS preconditioner of A."""
+        f = self.lib.orc_solve_sgs
+        def fn(dtype, *rest):
+            return f(dtype, 1 if method.upper() == "COCR" else 0, *rest)
+        return self._solve(fn, A, b, cfg, x0)
+
+    def solve_sgsmrtr(self, A, b, cfg, x0=None):
+        return self._solve(self.lib.orc_solve_sgsmrtr, A, b, cfg, x0)
 
     def time_iterations(self, A, b, cfg, iters: int, method: str = "CG"):
         dt = np.complex128 if A.is_complex else np.float64

@@ -136,10 +136,25 @@ int solve_impl(int method, int n, const int* rp, const int* ci, const void* valu
             sparsesolv::CGSolver<T> s;
             s.set_config(c);
             r = s.solve(A, bb, xx, n, nullptr);
-        } else {
+        } else if (method == 3) {
             sparsesolv::COCRSolver<T> s;
             s.set_config(c);
             r = s.solve(A, bb, xx, n, nullptr);
+        } else if (method == 4 || method == 5) {
+            // CGSolver / COCRSolver with an SGSPreconditioner set up on A
+            sparsesolv::SGSPreconditioner<T> pc;
+            pc.setup(A);
+            if (method == 4) {
+                sparsesolv::CGSolver<T> s;
+                s.set_config(c);
+                r = s.solve(A, bb, xx, n, &pc);
+            } else {
+                sparsesolv::COCRSolver<T> s;
+                s.set_config(c);
+                r = s.solve(A, bb, xx, n, &pc);
+            }
+        } else {
+            r = sparsesolv::solve_sgsmrtr<T>(A, bb, xx, n, c);
         }
         fill_result<T>(r, res, hist, shift);
         return 0;
@@ -147,6 +162,16 @@ int solve_impl(int method, int n, const int* rp, const int* ci, const void* valu
         put_err(err, errlen, e.what());
         return -1;
     }
+}
+
+// SGSPreconditioner::setup + apply
+template <typename T>
+void sgs_apply_impl(int n, const int* rp, const int* ci, const void* values, const void* x, void* y)
+{
+    sparsesolv::SparseMatrixView<T> A(n, n, rp, ci, static_cast<const T*>(values));
+    sparsesolv::SGSPreconditioner<T> pc;
+    pc.setup(A);
+    pc.apply(static_cast<const T*>(x), static_cast<T*>(y), n);
 }
 
 // COCRSolver::solve(A, b, x, n, &pc) with pc set up on another matrix M
@@ -422,6 +447,30 @@ int orc_solve_plain(int dtype, int method_cocr, int n, const int* rp, const int*
     const int m = method_cocr ? 3 : 2;
     return dtype == 0 ? solve_impl<double>(m, n, rp, ci, values, b, x, cfg, res, hist, err, errlen)
                       : solve_impl<cplx>(m, n, rp, ci, values, b, x, cfg, res, hist, err, errlen);
+}
+
+void orc_sgs_apply(int dtype, int n, const int* rp, const int* ci, const void* values,
+                   const void* x, void* y)
+{
+    if (dtype == 0) sgs_apply_impl<double>(n, rp, ci, values, x, y);
+    else sgs_apply_impl<cplx>(n, rp, ci, values, x, y);
+}
+
+int orc_solve_sgs(int dtype, int method_cocr, int n, const int* rp, const int* ci,
+                  const void* values, const void* b, void* x, const orc_config* cfg,
+                  orc_result* res, double* hist, char* err, int errlen)
+{
+    const int m = method_cocr ? 5 : 4;
+    return dtype == 0 ? solve_impl<double>(m, n, rp, ci, values, b, x, cfg, res, hist, err, errlen)
+                      : solve_impl<cplx>(m, n, rp, ci, values, b, x, cfg, res, hist, err, errlen);
+}
+
+int orc_solve_sgsmrtr(int dtype, int n, const int* rp, const int* ci, const void* values,
+                      const void* b, void* x, const orc_config* cfg, orc_result* res,
+                      double* hist, char* err, int errlen)
+{
+    return dtype == 0 ? solve_impl<double>(6, n, rp, ci, values, b, x, cfg, res, hist, err, errlen)
+                      : solve_impl<cplx>(6, n, rp, ci, values, b, x, cfg, res, hist, err, errlen);
 }
 
 double orc_time_iterations(int dtype, int method, int n, const int* rp, const int* ci,

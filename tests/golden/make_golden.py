@@ -67,6 +67,13 @@ CASES = {
     "lap6_divergence_thr1": (lambda: P.poisson(6), dict(use_abmc=1, divergence_check=1,
                                                        divergence_threshold=1.0, divergence_count=4),
                              "iccg"),
+    # SGS preconditioner (with CG / COCR) and SGS-MRTR (solve_sgsmrtr)
+    "vc6_sgsmrtr": (varcoef6, dict(), "sgsmrtr"),
+    "zvc6_sgsmrtr": (zvarcoef6, dict(), "sgsmrtr"),
+    "cc5_sgsmrtr_maxiter": (lambda: P.curlcurl(5), dict(max_iterations=7), "sgsmrtr"),
+    "eddy5_sgsmrtr": (lambda: P.eddy(5), dict(), "sgsmrtr"),
+    "vc6_sgs_cg": (varcoef6, dict(), "sgs_cg"),
+    "eddy5_sgs_cocr": (lambda: P.eddy(5), dict(), "sgs_cocr"),
 }
 
 
@@ -90,10 +97,18 @@ def run(ref, name):
         xr = np.linspace(-1.0, 1.0, A.n) * (1.0 + (0.5j if A.is_complex else 0.0))
         out["apply_x"] = xr
         out["apply_y"] = ic.apply(xr)
+    if solver.startswith("sgs"):
+        xr = np.linspace(-1.0, 1.0, A.n) * (1.0 + (0.5j if A.is_complex else 0.0))
+        out["apply_x"] = xr
+        out["apply_y"] = ref.sgs_apply(A, xr)
     if solver == "iccg":
         x, r = ref.solve_iccg(A, b, cfg)
     elif solver == "cocr":
         x, r = ref.solve_cocr(A, b, cfg)
+    elif solver == "sgsmrtr":
+        x, r = ref.solve_sgsmrtr(A, b, cfg)
+    elif solver in ("sgs_cg", "sgs_cocr"):
+        x, r = ref.solve_sgs(A, b, cfg, "COCR" if solver == "sgs_cocr" else "CG")
     elif solver == "plain_cg":
         x, r = ref.solve_plain(A, b, cfg, "CG")
     else:

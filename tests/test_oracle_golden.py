@@ -36,10 +36,16 @@ def test_oracle_matches_reference_golden(oracle, name):
                 np.testing.assert_array_equal(f[k], g[k])
         y = oracle.ic(A, cfg).apply(g["apply_x"])
         np.testing.assert_array_equal(y, g["apply_y"])
+    if solver.startswith("sgs"):
+        np.testing.assert_array_equal(oracle.sgs_apply(A, g["apply_x"]), g["apply_y"])
     if solver == "iccg":
         x, r = oracle.solve_iccg(A, b, cfg)
     elif solver == "cocr":
         x, r = oracle.solve_cocr(A, b, cfg)
+    elif solver == "sgsmrtr":
+        x, r = oracle.solve_sgsmrtr(A, b, cfg)
+    elif solver in ("sgs_cg", "sgs_cocr"):
+        x, r = oracle.solve_sgs(A, b, cfg, "COCR" if solver == "sgs_cocr" else "CG")
     elif solver == "plain_cg":
         x, r = oracle.solve_plain(A, b, cfg, "CG")
     else:
@@ -88,3 +94,39 @@ def test_oracle_solvers_match_reference_build(oracle, ref, case):
     assert ro["iterations"] == rr["iterations"]
     np.testing.assert_array_equal(ro["residual_history"], rr["residual_history"])
     np.testing.assert_array_equal(xo, xr)
+
+
+@pytest.mark.parametrize("case", ["lap9", "zlap7", "eddy6", "cc6"])
+def test_oracle_sgs_matches_reference_build(oracle, ref, case):
+    # SGS apply, CG / COCR with SGS and SGS-MRTR, bit for bit, on further
+    # systems (complex conjugated and unconjugated dots included)
+    import scipy.sparse as sp
+    if case == "lap9":
+        s = P.poisson(9)
+        A, b = s.A, s.b
+    elif case == "zlap7":
+        A = P.CSR.from_scipy((P.laplace7(7).to_scipy().astype(complex) + 0.4j * sp.identity(343)).tocsr())
+        z = P.gaussian(4, 2 * 343)
+        b = z[0::2] + 1j * z[1::2]
+    else:
+        s = P.eddy(6) if case == "eddy6" else P.curlcurl(6)
+        A, b = s.A, s.b
+    x = P.uniform(8, A.n, -1.0, 1.0) * (1.0 + (0.3j if A.is_complex else 0.0))
+    np.testing.assert_array_equal(oracle.sgs_apply(A, x), ref.sgs_apply(A, x))
+    variants = [dict(), dict(max_iterations=9), dict(divergence_check=1, divergence_threshold=1.5,
+                                                     divergence_count=3)]
+    if A.is_complex:
+        variants.append(dict(conjugate=1, max_iterations=300))
+    for kw in variants:
+        kw = dict(dict(save_residual_history=1, max_iterations=3000), **kw)
+        xo, ro = oracle.solve_sgsmrtr(A, b, oracle.config(**kw))
+        xr, rr = ref.solve_sgsmrtr(A, b, ref.config(**kw))
+        assert ro["iterations"] == rr["iterations"] and ro["converged"] == rr["converged"], kw
+        np.testing.assert_array_equal(ro["residual_history"], rr["residual_history"])
+        np.testing.assert_array_equal(xo, xr)
+    for method in ("CG", "COCR") if A.is_complex else ("CG",):
+        cfg = dict(save_residual_history=1, max_iterations=3000)
+        xo, ro = oracle.solve_sgs(A, b, oracle.config(**cfg), method)
+        xr, rr = ref.solve_sgs(A, b, ref.config(**cfg), method)
+        assert ro["iterations"] == rr["iterations"]
+        np.testing.assert_array_equal(xo, xr)

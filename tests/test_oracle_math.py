@@ -276,3 +276,43 @@ def test_zero_rhs(oracle):
     s = P.poisson(5)
     x, r = oracle.solve_iccg(s.A, np.zeros(s.A.n), oracle.config(use_abmc=1))
     assert r["converged"] and np.all(x == 0)
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_sgs_apply_is_the_dense_sgs_inverse(oracle, complex_):
+    # M = (D + L) D^-1 (D + U) with A = L + D + U (sgs_preconditioner.hpp:22-40)
+    import scipy.sparse as sp
+    A = P.laplace7_varcoef(4, 10.0 ** P.uniform(3, 64, -1.0, 1.0))
+    if complex_:
+        A = P.CSR.from_scipy((A.to_scipy().astype(complex) + 0.2j * sp.identity(64)).tocsr())
+    Ad = A.to_scipy().toarray()
+    D = np.diag(np.diag(Ad))
+    M = (D + np.tril(Ad, -1)) @ np.linalg.inv(D) @ (D + np.triu(Ad, 1))
+    x = P.uniform(5, A.n, -1.0, 1.0) * (1.0 + (0.7j if complex_ else 0.0))
+    y = oracle.sgs_apply(A, x)
+    assert np.linalg.norm(y - np.linalg.solve(M, x)) <= 1e-12 * np.linalg.norm(y)
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_sgsmrtr_reaches_the_dense_solution(oracle, complex_):
+    # The split formula ARd = u + L^-1 (rd - u) (sgs_mrtr_solver.hpp:167-176)
+    # is exact when D A D has a unit diagonal: real positive diagonals.  The
+    # complex case keeps a real diagonal (complex symmetric off-diagonals);
+    # with a complex diagonal the reference's iteration stops at another
+    # vector, which the oracle reproduces (test_oracle_golden.py).
+    A = P.laplace7_varcoef(5, 10.0 ** P.uniform(6, 125, -1.0, 1.0))
+    b = P.uniform(7, 125, -1.0, 1.0)
+    if complex_:
+        As = A.to_scipy().tocoo()
+        off = As.row != As.col
+        vals = As.data.astype(complex)
+        w = P.uniform(11, int(off.sum()), -0.3, 0.3)
+        vals[off] += 1j * w * np.abs(As.data[off])
+        M = sp.coo_matrix((vals, (As.row, As.col)), shape=As.shape).tocsr()
+        M = (M + M.T) * 0.5                              # complex symmetric
+        A = P.CSR.from_scipy(M.tocsr())
+        b = b + 1j * P.uniform(9, 125, -1.0, 1.0)
+    x, r = oracle.solve_sgsmrtr(A, b, oracle.config(tolerance=1e-12, max_iterations=500))
+    assert r["converged"]
+    xd = np.linalg.solve(A.to_scipy().toarray(), b)
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
