@@ -47,6 +47,13 @@ extern "C" {
 #define SSA_METHOD_CG 2          /* CG without preconditioner */
 #define SSA_METHOD_COCR 3        /* COCR without preconditioner */
 #define SSA_METHOD_NGS_ICCOCR 4  /* COCRSolverNGS loop with the IC preconditioner */
+#define SSA_METHOD_SGSMRTR 5     /* SGSMRTRSolver (solve_sgsmrtr); needs an SSA_PRECOND_SGSMRTR handle */
+
+/* What a handle's factor is (ssa_config.preconditioner).  SSA_METHOD_ICCG /
+ * SSA_METHOD_ICCOCR run CG / COCR preconditioned by it. */
+#define SSA_PRECOND_IC 0         /* IC(0) (ICPreconditioner) */
+#define SSA_PRECOND_SGS 1        /* symmetric Gauss-Seidel (SGSPreconditioner) */
+#define SSA_PRECOND_SGSMRTR 2    /* the split factor of SGSMRTRSolver: lower(D A D) */
 
 typedef struct ssa_context ssa_context;
 typedef struct ssa_ic ssa_ic;
@@ -85,6 +92,11 @@ typedef struct ssa_config {
     int walk_overlap;                 /* -1 auto: the straggler walks run on a second stream beside
                                          the colours that do not read them, where that pays; 0 off;
                                          1 wherever the schedule allows (read at ssa_ic_create) */
+    int preconditioner;               /* SSA_PRECOND_IC; SSA_PRECOND_SGS and SSA_PRECOND_SGSMRTR are
+                                         built in the natural row order on the level schedule
+                                         (use_abmc, diagonal_scaling and the shift settings do
+                                         not apply); SSA_PRECOND_SGSMRTR: FP64, one system per
+                                         handle (read at ssa_ic_create) */
 } ssa_config;
 
 /* sparsesolv::SolverResult (core/types.hpp:29-37) plus timing. */
@@ -147,7 +159,10 @@ int ssa_ic_schedule(const ssa_ic* ic, int32_t* color_offsets, int32_t* block_off
 int ssa_ic_factor(const ssa_ic* ic, int32_t* row_ptr, int32_t* col_idx, void* values,
                   void* inv_diag);
 
-/* ICPreconditioner::apply (ic_preconditioner.hpp:215-225): y = M^{-1} x, run on the GPU. */
+/* ICPreconditioner::apply (ic_preconditioner.hpp:215-225): y = M^{-1} x, run on the GPU.
+ * On an SSA_PRECOND_SGS handle SGSPreconditioner::apply (preconditioners/
+ * sgs_preconditioner.hpp:113-131); an SSA_PRECOND_SGSMRTR handle has no apply
+ * (SSA_ERR_UNSUPPORTED). */
 int ssa_ic_apply(ssa_ic* ic, const void* x, void* y, int on_device);
 
 /* No reference counterpart: ssa_ic_apply for callers that cannot allocate
@@ -177,9 +192,12 @@ int ssa_ic_set_operator(ssa_ic* ic, int n, const int32_t* row_ptr, const int32_t
                         const void* values);
 
 /* solve_iccg / solve_cocr (sparsesolv.hpp:43-105, :153-169), CGSolver / COCRSolver
- * without a preconditioner (ngsolve/sparsesolv_solvers.hpp:116-123), and the
+ * without a preconditioner (ngsolve/sparsesolv_solvers.hpp:116-123), the
  * COCRSolverNGS loop (ngsolve/sparsesolv_solvers.hpp:273-369) with this IC
- * preconditioner.  x holds the initial guess on entry (zeroed for
+ * preconditioner, CGSolver / COCRSolver with an SGSPreconditioner (an
+ * SSA_PRECOND_SGS handle), and solve_sgsmrtr (sparsesolv.hpp:123-137, an
+ * SSA_PRECOND_SGSMRTR handle, method SSA_METHOD_SGSMRTR, one right-hand side).
+ * x holds the initial guess on entry (zeroed for
  * SSA_METHOD_NGS_ICCOCR) and the solution on exit.  `history` receives
  * result->history_len residuals when cfg->save_residual_history (may be NULL;
  * needs max_iterations+2 entries). */

@@ -60,7 +60,19 @@ struct IcSettings {
     // device schedule, no counterpart in the reference: -1 auto, 0 off, 1 on
     // (the straggler walks on a second stream)
     int walk_overlap = -1;
+    // what the factor is: kFactorIc the IC(0) factorisation; kFactorSgs
+    // A's lower triangle with the pivots diag(A), whose sweeps apply the SGS
+    // preconditioner (D + L) D^-1 (D + U) (preconditioners/
+    // sgs_preconditioner.hpp:22-40); kFactorSgsMrtr the lower triangle of
+    // D A D, D = diag(1/sqrt|a_ii|), that SGS-MRTR solves with
+    // (solvers/sgs_mrtr_solver.hpp:285-303).  Both split factors are in the
+    // natural order (the level path).
+    int factor_kind = 0;
 };
+
+constexpr int kFactorIc = 0;
+constexpr int kFactorSgs = 1;
+constexpr int kFactorSgsMrtr = 2;
 
 template <typename T>
 inline double abs_of(const T& v) { return std::abs(v); }
@@ -150,6 +162,9 @@ struct IcFactorHost {
     CsrHost<T> U;                       // L^T, diagonal first
     std::vector<T> inv_diag;
     std::vector<double> scaling;        // 1/sqrt|a_ii| if diagonal_scaling
+    // kFactorSgsMrtr: D_i = 1/sqrt|a_ii| and sqrt|a_ii| (1 where |a_ii| <=
+    // 1e-15), natural order
+    std::vector<T> split_d, split_id;
     double actual_shift = 0.0;
     int shift_trials = 0;
     CsrHost<T> A_perm;                  // A in ABMC order (Krylov operator)
@@ -158,6 +173,10 @@ struct IcFactorHost {
                const IcSettings& cfg, bool keep_permuted_matrix)
     {
         n = n_;
+        if (cfg.factor_kind != kFactorIc) {
+            build_split(rp, ci, v, cfg.factor_kind, keep_permuted_matrix);
+            return;
+        }
         if (cfg.use_abmc) {
             if (cfg.abmc_use_rcm) {
                 // ABMC of the RCM-reordered matrix (ic_preconditioner.hpp:97-113):
@@ -193,6 +212,72 @@ struct IcFactorHost {
     }
 
 private:
+    // The SGS and SGS-MRTR factors (no factorisation), natural order.  Every
+    // row of L ends with its diagonal entry (a zero one where A stores none)
+    // so the sweeps find the strictly lower part in front of it.
+    void build_split(const index_t* rp, const index_t* ci, const T* v, int kind,
+                     bool keep_permuted_matrix)
+    {
+        L.n = n;
+        L.rp.assign(size_t(n) + 1, 0);
+        std::vector<T> diag(static_cast<size_t>(n), T(0));
+        std::vector<char> has(static_cast<size_t>(n), 0);
+        for (index_t i = 0; i < n; ++i) {
+            index_t c = 0;
+            for (index_t k = rp[i]; k < rp[i + 1]; ++k) {
+                if (ci[k] < i) ++c;
+                if (ci[k] == i) { has[i] = 1; diag[i] = v[k]; }
+            }
+            L.rp[i + 1] = L.rp[i] + c + 1;
+        }
+        if (kind == kFactorSgsMrtr) {
+            // sgs_mrtr_solver.hpp:88-98
+            split_d.assign(static_cast<size_t>(n), T(1));
+            split_id.assign(static_cast<size_t>(n), T(1));
+            for (index_t i = 0; i < n; ++i) {
+                if (!has[i])
+                    throw std::runtime_error("SGS-MRTR: missing diagonal at row " + std::to_string(i));
+                const double a = abs_of(diag[i]);
+                if (a > 1e-15) {
+                    const double sq = std::sqrt(a);
+                    split_d[i] = T(1.0 / sq);
+                    split_id[i] = T(sq);
+                }
+            }
+        }
+        L.ci.resize(L.rp[n]);
+        L.v.resize(L.rp[n]);
+        inv_diag.assign(static_cast<size_t>(n), T(1));
+        for (index_t i = 0; i < n; ++i) {
+            index_t pos = L.rp[i];
+            for (index_t k = rp[i]; k < rp[i + 1]; ++k) {
+                if (ci[k] >= i) continue;
+                L.ci[pos] = ci[k];
+                // L of D A D: (D_i a_ij) D_j (:292)
+                L.v[pos] = kind == kFactorSgsMrtr ? split_d[i] * v[k] * split_d[ci[k]] : v[k];
+                ++pos;
+            }
+            L.ci[pos] = i;
+            if (kind == kFactorSgsMrtr) {
+                L.v[pos] = split_d[i] * diag[i] * split_d[i];
+                inv_diag[i] = T(1) / L.v[pos];      // the reference divides by L_ii
+            } else {
+                // sgs_preconditioner.hpp:80-87
+                L.v[pos] = diag[i];
+                if (abs_of(diag[i]) > 1e-15) inv_diag[i] = T(1) / diag[i];
+            }
+        }
+        if (keep_permuted_matrix) {
+            A_perm.n = n;
+            A_perm.rp.assign(rp, rp + n + 1);
+            A_perm.ci.assign(ci, ci + rp[n]);
+            A_perm.v.assign(v, v + rp[n]);
+        }
+        actual_shift = 0.0;
+        shift_trials = 0;
+        transpose();
+    }
+
     void extract_lower(index_t nn, const index_t* rp, const index_t* ci, const T* v)
     {
         L.n = nn;

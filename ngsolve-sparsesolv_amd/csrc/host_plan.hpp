@@ -21,6 +21,10 @@ struct HostPlan {
     sparsesolv_amd::SweepLayout<F> fwd, bwd;
     std::vector<F> invd;                   // inv_diag in the factor precision
     std::vector<double> scale;             // diagonal_scaling: 1/sqrt|a_ii| per device row
+    // SGS-MRTR (factor kind kFactorSgsMrtr), in device order: L = lower(D A D)
+    // as CSR (its product gives the normaliser) and D, 1/D
+    sparsesolv_amd::CsrHost<T> lcsr;
+    std::vector<T> split_d, split_id;
     double seconds = 0.0;                  // host setup wall time
 };
 
@@ -50,6 +54,9 @@ struct BatchPlan {
     sparsesolv_amd::SweepLayout<F> fwd, bwd;
     std::vector<F> invd;
     std::vector<double> scale;      // diagonal_scaling: each system's S in its rows
+    // SGS-MRTR (one system only): see HostPlan
+    sparsesolv_amd::CsrHost<T> lcsr;
+    std::vector<T> split_d, split_id;
     // per system: setup facts kept after the layouts are combined
     std::vector<double> actual_shift, seconds;
     std::vector<int> shift_trials, num_colors, num_blocks, nnz, nnz_l;

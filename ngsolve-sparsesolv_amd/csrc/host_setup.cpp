@@ -113,6 +113,15 @@ void build_plan(HostPlan<T, F>& plan, int n, const int* row_ptr, const int* col_
         Ldev = &Lp;
         Udev = &Up;
         block_size = 1;
+        if (settings.factor_kind == sparsesolv_amd::kFactorSgsMrtr) {
+            plan.lcsr = Lp;
+            plan.split_d.resize(size_t(n));
+            plan.split_id.resize(size_t(n));
+            for (int i = 0; i < n; ++i) {
+                plan.split_d[size_t(ord[i])] = plan.fac.split_d[size_t(i)];
+                plan.split_id[size_t(ord[i])] = plan.fac.split_id[size_t(i)];
+            }
+        }
     }
     // SpMV slices in the original row order (ordering[i]: device row of
     // original row i)
@@ -181,6 +190,8 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     if (count < 1) throw std::invalid_argument("batch: count must be >= 1");
+    if (count > 1 && settings.factor_kind == sparsesolv_amd::kFactorSgsMrtr)
+        throw std::invalid_argument("batch: SGS-MRTR runs one system per handle");
     bp = BatchPlan<T, F>{};
     bp.count = count;
     std::vector<HostPlan<T, F>> plans(static_cast<size_t>(count));
@@ -235,6 +246,9 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
         bp.bwd = std::move(P.bwd);
         bp.invd = std::move(P.invd);
         bp.scale = std::move(P.scale);
+        bp.lcsr = std::move(P.lcsr);
+        bp.split_d = std::move(P.split_d);
+        bp.split_id = std::move(P.split_id);
         bp.single = std::move(P);
     } else {
         std::vector<const sparsesolv_amd::SpmvLayout<T>*> sp;

@@ -32,6 +32,7 @@
 #include "host_plan.hpp"
 #include "block_kernels.hpp"
 #include "kernels.hpp"
+#include "mrtr_kernels.hpp"
 #include "ic_handle.hpp"
 #include "sparsesolv_amd.h"
 
@@ -325,6 +326,11 @@ struct IcImpl final : IcBase {
     // current operation's choice.
     DevBuf scale_d, sbuf;
     bool has_scale = false, abmc_order = true, abmc_rcm = false, scale_apply = false;
+    // what the factor is (sparsesolv_amd::kFactorIc / kFactorSgs /
+    // kFactorSgsMrtr); SGS-MRTR also keeps L = lower(D A D) as CSR (for its
+    // normaliser |L D b|) and D, D^-1, all in device order
+    int factor_kind = sparsesolv_amd::kFactorIc;
+    DevBuf mr_rp, mr_ci, mr_v, mr_d, mr_id;
 
     SolverState* h_state = nullptr;  // pinned, one per state
 
@@ -428,6 +434,7 @@ struct IcImpl final : IcBase {
         : ctx(c), count(count_), abmc_order(st.use_abmc), abmc_rcm(st.use_abmc && st.abmc_use_rcm)
     {
         overlap_setting = st.walk_overlap;
+        factor_kind = st.factor_kind;
         ssa_host::BatchPlan<HT, HF> plan;
         ssa_host::build_batch<HT, HF>(plan, count, ns, rp, ci, v, st);
         n = plan.row0.back();
@@ -466,6 +473,13 @@ struct IcImpl final : IcBase {
         perm.upload(plan.gather_out, s);
         rev.upload(plan.gather_in, s);
         invd.upload(plan.invd, s);
+        if (factor_kind == sparsesolv_amd::kFactorSgsMrtr) {
+            mr_rp.upload(plan.lcsr.rp, s);
+            mr_ci.upload(plan.lcsr.ci, s);
+            mr_v.upload(plan.lcsr.v, s);
+            mr_d.upload(plan.split_d, s);
+            mr_id.upload(plan.split_id, s);
+        }
         if (!plan.scale.empty()) {
             scale_d.upload(plan.scale, s);
             has_scale = true;
@@ -1132,6 +1146,7 @@ struct IcImpl final : IcBase {
     // y = M^{-1} x (precond) or y = A x for m right-hand sides
     void operate(const T* xin, T* yout, bool on_device, int m, bool precond)
     {
+        if (precond) no_split_apply("ssa_ic_apply");
         ensure_workspace(rhs_width(m));
         // the preconditioner's own apply: scaled on the level path only
         scale_apply = has_scale && !abmc_order;
@@ -1170,9 +1185,18 @@ struct IcImpl final : IcBase {
         return b;
     }
 
+    // the SGS-MRTR split factor is no preconditioner of its own
+    void no_split_apply(const char* who) const
+    {
+        if (factor_kind == sparsesolv_amd::kFactorSgsMrtr)
+            throw SsaError(SSA_ERR_UNSUPPORTED,
+                           std::string(who) + ": an SGS-MRTR handle has no preconditioner apply");
+    }
+
     void apply_noalloc(const void* xin, void* yout) override
     {
         need_device();
+        no_split_apply("ssa_ic_apply_noalloc");
         const size_t need = noalloc_bytes();
         if (!ctx->scratch || need > ctx->scratch_bytes)
             throw SsaError(SSA_ERR_ARGUMENT,
@@ -1462,10 +1486,21 @@ struct IcImpl final : IcBase {
         }
         HIP_CHECK(hipGetLastError());
 
-        // iterations: replay a captured group of iterations, poll the stop
-        // status between groups.  The control kernels stop each state at its
-        // iteration budget themselves and every kernel returns at once after
-        // a stop, so iterations enqueued past the stop cost only launches.
+        iterate(maxit, cfg, [&](int k) { enqueue_iteration(method, prm, replace_at(k)); });
+    }
+
+    // The iteration loop: replay a captured group of iterations, poll the
+    // stop status between groups.  The control kernels stop each state at
+    // its iteration budget themselves and every kernel returns at once after
+    // a stop, so iterations enqueued past the stop cost only launches.
+    // enqueue(k) enqueues iteration k of the pass (counted from 0); leaves
+    // the final states in h_state.
+    template <typename Fn>
+    void iterate(int maxit, const ssa_config& cfg, Fn&& enqueue)
+    {
+        hipStream_t s = ctx->stream;
+        const int S = nstates();
+        SolverState* st = state.as<SolverState>();
         // timed launches (ssa_ic_trace) need the eager loop
         const bool graph_on = cfg.use_graph && trace_cls < 0;
         const int unroll = graph_on ? std::min(kGraphIterations, maxit + 1) : 1;
@@ -1474,7 +1509,7 @@ struct IcImpl final : IcBase {
         hipGraphExec_t exec = nullptr;
         if (graph_on) {
             HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int u = 0; u < unroll; ++u) enqueue_iteration(method, prm, replace_at(u));
+            for (int u = 0; u < unroll; ++u) enqueue(u);
             HIP_CHECK(hipStreamEndCapture(s, &graph));
             HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         }
@@ -1499,7 +1534,7 @@ struct IcImpl final : IcBase {
             trace_armed = trace_cls >= 0;
             for (int i = 0; i < groups; ++i) {
                 if (exec) HIP_CHECK(hipGraphLaunch(exec, s));
-                else enqueue_iteration(method, prm, replace_at(enqueued++));
+                else enqueue(enqueued++);
             }
             trace_armed = false;
             done += groups * unroll;
@@ -1508,6 +1543,56 @@ struct IcImpl final : IcBase {
         }
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+    }
+
+    // solve_sgsmrtr (sparsesolv.hpp:123-137): SGSMRTRSolver's
+    // prepare_iteration and do_iterate (solvers/sgs_mrtr_solver.hpp:82-260)
+    // in device order.  bvec holds b and x the initial guess on entry; on
+    // return x holds x2 = D^-1 x (the iterate in the scaled space: the
+    // caller takes the best one when the solve did not converge, then
+    // x = D x2) and h_state the final state.  Vectors: r = rd, z = u,
+    // qt = the forward input (r = D (b - A x) first, then rd - u), Ap = A x
+    // first, then w and Ard, t = y, tmp = rd / L_ii, p = p, bvec = b2.
+    void run_mrtr(const SolveParams& prm, const ssa_config& cfg)
+    {
+        if constexpr (kMixed) {
+            throw SsaError(SSA_ERR_UNSUPPORTED, "SGS-MRTR runs in FP64");
+        } else {
+            hipStream_t s = ctx->stream;
+            const int G = vgrid();
+            SolverState* st = state.as<SolverState>();
+            HIP_CHECK(hipMemsetAsync(state.p, 0, sizeof(SolverState), s));
+            HIP_CHECK(hipMemsetAsync(p.p, 0, size_t(n) * sizeof(T), s));
+            launch_spmv(x.as<T>(), Ap.as<T>(), 0, nullptr, nullptr);
+            ssa::k_mrtr_prep<T><<<G, ssa::kBlock, 0, s>>>(n, mr_d.as<T>(), mr_id.as<T>(), bvec.as<T>(), Ap.as<T>(), x.as<T>(), qt.as<T>(), part_a.as<double>(), part_b.as<double>());
+            ssa::k_mrtr_lnorm<T><<<G, ssa::kBlock, 0, s>>>(n, mr_rp.as<index_t>(), mr_ci.as<index_t>(), mr_v.as<T>(), bvec.as<T>(), part_c.as<double>());
+            HIP_CHECK(hipGetLastError());
+            launch_sweep<true>(fwd, qt.as<T>(), r.as<T>(), st);   // rd = L^-1 r
+            ssa::k_mrtr_start<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), invd.as<T>(), t.as<T>(), tmp.as<T>(), part_d.as<double>());
+            ssa::k_mrtr_init<<<1, ssa::kCtrlThreads, 0, s>>>(part_a.as<double>(), part_b.as<double>(), part_c.as<double>(), part_d.as<double>(), G, st, prm, hist.as<double>());
+            HIP_CHECK(hipGetLastError());
+            iterate(prm.max_iterations, cfg, [&](int) { enqueue_mrtr_iteration(prm); });
+        }
+    }
+
+    // one SGS-MRTR iteration (sgs_mrtr_solver.hpp:166-240)
+    void enqueue_mrtr_iteration(const SolveParams& prm)
+    {
+        if constexpr (!kMixed) {
+            hipStream_t s = ctx->stream;
+            SolverState* st = state.as<SolverState>();
+            const int G = vgrid();
+            constexpr int CT = ssa::kCtrlThreads;
+            launch_sweep<false>(bwd, tmp.as<T>(), z.as<T>(), st);   // u = L^-T rd
+            ssa::k_mrtr_sub<T><<<G, ssa::kBlock, 0, s>>>(n, r.as<T>(), z.as<T>(), qt.as<T>(), st);
+            launch_sweep<true>(fwd, qt.as<T>(), Ap.as<T>(), st);    // w = L^-1 (rd - u)
+            ssa::k_mrtr_dots<T><<<G, ssa::kBlock, 0, s>>>(n, Ap.as<T>(), z.as<T>(), r.as<T>(), t.as<T>(), prm.conjugate, part_a.as<T>(), part_b.as<T>(), part_c.as<T>(), st);
+            ssa::k_mrtr_coef<T><<<1, CT, 0, s>>>(part_a.as<T>(), part_b.as<T>(), part_c.as<T>(), G, st);
+            ssa::k_mrtr_update<T><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), Ap.as<T>(), p.as<T>(), x.as<T>(), t.as<T>(), r.as<T>(), tmp.as<T>(), invd.as<T>(), part_d.as<double>(), st);
+            ssa::k_mrtr_check<<<1, CT, 0, s>>>(part_d.as<double>(), G, st, prm, hist.as<double>());
+            if (prm.save_best) ssa::k_mrtr_best<T><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), best.as<T>(), st);
+            HIP_CHECK(hipGetLastError());
+        }
     }
 
     void solve(int method, const void* bin, void* xio, bool on_device, const ssa_config& cfg,
@@ -1564,6 +1649,14 @@ struct IcImpl final : IcBase {
 
         if (S > 1 && prm.ngs)
             throw SsaError(SSA_ERR_UNSUPPORTED, "COCRSolverNGS runs one system and one right-hand side");
+        const bool mrtr = method == SSA_METHOD_SGSMRTR;
+        if (mrtr != (factor_kind == sparsesolv_amd::kFactorSgsMrtr))
+            throw SsaError(SSA_ERR_ARGUMENT,
+                           mrtr ? "SSA_METHOD_SGSMRTR needs a handle created with preconditioner = "
+                                  "SSA_PRECOND_SGSMRTR"
+                                : "an SSA_PRECOND_SGSMRTR handle solves with SSA_METHOD_SGSMRTR only");
+        if (mrtr && S > 1)
+            throw SsaError(SSA_ERR_UNSUPPORTED, "SGS-MRTR runs one right-hand side");
         if (kMixed && !pre)
             throw SsaError(SSA_ERR_UNSUPPORTED,
                            "ssa_solve: mixed precision needs the IC preconditioner");
@@ -1592,7 +1685,8 @@ struct IcImpl final : IcBase {
         if (!kMixed || prm.ngs) {
             for (int q = 0; q < S; ++q)
                 h_pass[q] = ssa::PassParams{cfg.tolerance, cfg.max_iterations, active[size_t(q)]};
-            run_pass(method, prm, pre, cfg);
+            if (mrtr) run_mrtr(prm, cfg);
+            else run_pass(method, prm, pre, cfg);
             for (int q = 0; q < S; ++q) {
                 fin[size_t(q)] = h_state[q];
                 iters[size_t(q)] = h_state[q].result_iters;
@@ -1673,6 +1767,10 @@ struct IcImpl final : IcBase {
                 copy_rows(x, best, q);
                 res.final_residual = Sq.best_residual;
             }
+        }
+        if (mrtr) {
+            ssa::k_mrtr_unscale<T><<<vgrid(), ssa::kBlock, 0, s>>>(n, mr_d.as<T>(), x.as<T>());
+            HIP_CHECK(hipGetLastError());
         }
         T* xdst = on_device ? xio : tmp2.as<T>();
         scatter_out(x.as<T>(), xdst, m);

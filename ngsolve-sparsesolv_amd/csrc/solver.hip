@@ -107,6 +107,7 @@ void ssa_config_default(ssa_config* c)
     c->use_graph = 1;
     c->poll_interval = 0;
     c->walk_overlap = -1;
+    c->preconditioner = SSA_PRECOND_IC;
 }
 
 const char* ssa_last_error(void) { return g_last_error.c_str(); }
@@ -169,6 +170,13 @@ static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int c
         throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": dtype must be SSA_F64 or SSA_C128");
     if (cfg->precision != SSA_PRECISION_FP64 && cfg->precision != SSA_PRECISION_MIXED)
         throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": unknown precision");
+    const int kind = cfg->preconditioner;
+    if (kind != SSA_PRECOND_IC && kind != SSA_PRECOND_SGS && kind != SSA_PRECOND_SGSMRTR)
+        throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": unknown preconditioner");
+    if (kind == SSA_PRECOND_SGSMRTR && cfg->precision != SSA_PRECISION_FP64)
+        throw SsaError(SSA_ERR_UNSUPPORTED, std::string(who) + ": SGS-MRTR runs in FP64");
+    if (kind == SSA_PRECOND_SGSMRTR && count > 1)
+        throw SsaError(SSA_ERR_UNSUPPORTED, std::string(who) + ": SGS-MRTR runs one system per handle");
     for (int q = 0; q < count; ++q) check_matrix(who, n[q], row_ptr[q], col_idx[q], values[q]);
     if (ctx) HIP_CHECK(hipSetDevice(ctx->device));
     sparsesolv_amd::IcSettings st;
@@ -185,6 +193,15 @@ static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int c
     st.abmc_num_colors = cfg->abmc_num_colors;
     st.abmc_use_rcm = cfg->abmc_use_rcm != 0;
     st.walk_overlap = cfg->walk_overlap < 0 ? -1 : cfg->walk_overlap > 0 ? 1 : 0;
+    if (kind != SSA_PRECOND_IC) {
+        // the split factors: natural order on the level schedule, no
+        // scaling or shift (SGSPreconditioner::setup, SGSMRTRSolver)
+        st.factor_kind = kind == SSA_PRECOND_SGS ? sparsesolv_amd::kFactorSgs : sparsesolv_amd::kFactorSgsMrtr;
+        st.use_abmc = false;
+        st.abmc_use_rcm = false;
+        st.diagonal_scaling = false;
+        st.auto_shift = false;
+    }
     auto h = std::make_unique<ssa_ic>();
     h->dtype = dtype;
     // mixed: the factor (computed in FP64) is stored and applied in
@@ -318,7 +335,7 @@ int ssa_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device, con
 {
     return guarded([&] {
         if (!ic || !b || !x || !cfg || !result) throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve: null argument");
-        if (method < SSA_METHOD_ICCG || method > SSA_METHOD_NGS_ICCOCR)
+        if (method < SSA_METHOD_ICCG || method > SSA_METHOD_SGSMRTR)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve: unknown method");
         if (ic->impl->system_count() != 1)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve: a batched handle is solved with ssa_batch_solve");

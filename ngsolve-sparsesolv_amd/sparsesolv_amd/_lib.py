@@ -31,6 +31,11 @@ METHOD_ICCOCR = 1
 METHOD_CG = 2
 METHOD_COCR = 3
 METHOD_NGS_ICCOCR = 4
+METHOD_SGSMRTR = 5
+
+PRECOND_IC = 0
+PRECOND_SGS = 1
+PRECOND_SGSMRTR = 2
 
 EXPORTED = (
     "ssa_config_default", "ssa_last_error", "ssa_context_create", "ssa_context_destroy",
@@ -71,6 +76,7 @@ class Config(C.Structure):
         ("abmc_num_colors", C.c_int), ("abmc_reorder_spmv", C.c_int),
         ("abmc_use_rcm", C.c_int), ("precision", C.c_int), ("use_graph", C.c_int),
         ("poll_interval", C.c_int), ("walk_overlap", C.c_int),
+        ("preconditioner", C.c_int),
     ]
 
 
