@@ -699,8 +699,11 @@ def main():
               f"(got --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
+    import torch
+    # torch's own HIP runtime takes the GPU before the library's does
+    torch.zeros(1, device="cuda:0")
+    torch.cuda.synchronize()
     if args.copy_calibration:
-        import torch
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")   # 1 GiB
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
