@@ -1,4 +1,4 @@
-// sparsesolv.hpp -- header-only C++17 drop-in for the ICCG / COCR path of
+// sparsesolv.hpp -- header-only C++17 drop-in for the ICCG / COCR / SGS path of
 // ksugahar/ngsolve-sparsesolv (include/sparsesolv/sparsesolv.hpp), running on
 // one MI355X through the C ABI of libsparsesolv_amd.so.
 //
@@ -8,9 +8,11 @@
 //   SparseMatrixView    core/sparse_matrix_view.hpp:58-173 (non-owning CSR)
 //   Preconditioner      core/preconditioner.hpp:22-72
 //   ICPreconditioner    preconditioners/ic_preconditioner.hpp:52-1543
+//   SGSPreconditioner   preconditioners/sgs_preconditioner.hpp:47-203
 //   IterativeSolver     solvers/iterative_solver.hpp:44-322
 //   CGSolver, COCRSolver solvers/cg_solver.hpp:59-147, solvers/cocr_solver.hpp:58-187
-//   solve_iccg, solve_cocr sparsesolv.hpp:43-105, :153-169
+//   SGSMRTRSolver       solvers/sgs_mrtr_solver.hpp:53-365
+//   solve_iccg, solve_cocr, solve_sgsmrtr sparsesolv.hpp:43-105, :153-169, :123-150
 // Errors surface as the reference's exception types: std::runtime_error for
 // setup failures (missing diagonal) and apply-before-setup,
 // std::invalid_argument for bad arguments.
@@ -284,24 +286,19 @@ protected:
     bool is_setup_ = false;
 };
 
-// IC(0) preconditioner whose factor lives on the GPU
-// (preconditioners/ic_preconditioner.hpp:52-1543).  setup() runs the ABMC
-// schedule and factorisation on the host and uploads them; apply() runs the
-// triangular sweeps on the GPU (host vectors in, host vectors out).
-template <typename Scalar = double>
-class ICPreconditioner : public Preconditioner<Scalar> {
+namespace detail {
+
+// A preconditioner whose factor lives on the GPU: setup() builds a library
+// handle on A (schedule and factor on the host, uploaded once) from
+// handle_config(); apply() runs its sweeps on the GPU (host vectors in, host
+// vectors out).
+template <typename Scalar>
+class DeviceFactor : public Preconditioner<Scalar> {
 public:
-    explicit ICPreconditioner(double shift = 1.05) { config_.shift_parameter = shift; }
-
-    void set_config(const SolverConfig& c) { config_ = c; }
-    const SolverConfig& config() const { return config_; }
-    double shift_parameter() const { return config_.shift_parameter; }
-    void set_shift_parameter(double s) { config_.shift_parameter = s; }
-
     void setup(const SparseMatrixView<Scalar>& A) override
     {
-        ctx_ = detail::context(config_.device);
-        ssa_config c = config_.to_c();
+        ctx_ = detail::context(device());
+        ssa_config c = handle_config();
         ssa_ic* raw = nullptr;
         detail::check(ssa_ic_create(ctx_.get(), detail::dtype_of<Scalar>(), A.rows(), A.row_ptr(),
                                     A.col_idx(), A.values(), &c, &raw));
@@ -329,7 +326,7 @@ public:
     // every call.
     ssa_ic* device_handle_for(const SparseMatrixView<Scalar>& A) const override
     {
-        if (!ic_) throw std::runtime_error("ICPreconditioner::apply called before setup");
+        if (!ic_) throw std::runtime_error(this->name() + "Preconditioner::apply called before setup");
         if (A.values() == a_) {
             const std::uint64_t now = detail::fingerprint(A.values(), A.nnz());
             if (now != a_print_) {
@@ -340,7 +337,7 @@ public:
         }
         if (A.rows() != n_) throw std::invalid_argument("solve: the preconditioner's order differs");
         if (!op_) {
-            ssa_config c = config_.to_c();
+            ssa_config c = handle_config();
             ssa_ic* raw = nullptr;
             detail::check(ssa_ic_create(ctx_.get(), detail::dtype_of<Scalar>(), n_, m_rp_.data(),
                                         m_ci_.data(), m_v_.data(), &c, &raw));
@@ -352,50 +349,120 @@ public:
 
     void apply(const Scalar* x, Scalar* y, index_t size) const override
     {
-        if (!ic_) throw std::runtime_error("ICPreconditioner::apply called before setup");
-        if (size != n_) throw std::invalid_argument("ICPreconditioner::apply: size mismatch");
+        if (!ic_) throw std::runtime_error(this->name() + "Preconditioner::apply called before setup");
+        if (size != n_) throw std::invalid_argument(this->name() + "Preconditioner::apply: size mismatch");
         detail::check(ssa_ic_apply(ic_.get(), x, y, 0));
     }
     using Preconditioner<Scalar>::apply;
 
-    // ic_preconditioner.hpp:516-532: the GPU apply, then r . y on the host
+    // the GPU apply, then r . y on the host (ic_preconditioner.hpp:516-532)
     Scalar apply_fused_dot(const Scalar* r_for_dot, const Scalar* x, Scalar* y, index_t size,
                            bool conjugate) const override
     {
-        if (!ic_) throw std::runtime_error("ICPreconditioner::apply_fused_dot called before setup");
+        if (!ic_)
+            throw std::runtime_error(this->name() + "Preconditioner::apply_fused_dot called before setup");
         apply(x, y, size);
         return detail::dot(r_for_dot, y, size, conjugate);
     }
+
+    ssa_ic* handle() const { return ic_.get(); }
+    const Scalar* matrix_values() const { return a_; }
+
+protected:
+    virtual ssa_config handle_config() const = 0;
+    virtual int device() const { return 0; }
+
+    std::shared_ptr<ssa_context> ctx_;
+    std::shared_ptr<ssa_ic> ic_;
+    index_t n_ = 0;
+
+private:
+    const Scalar* a_ = nullptr;
+    mutable std::uint64_t a_print_ = 0;
+    std::vector<index_t> m_rp_, m_ci_;
+    std::vector<Scalar> m_v_;
+    mutable std::shared_ptr<ssa_ic> op_;
+};
+
+// the handle of a solve without a preconditioner (the operator, and for
+// SGS-MRTR the solver's own split factor)
+template <typename Scalar>
+class OperatorHandle final : public DeviceFactor<Scalar> {
+public:
+    OperatorHandle(const ssa_config& c, int device) : c_(c), device_(device) {}
+    std::string name() const override { return "Operator"; }
+
+protected:
+    ssa_config handle_config() const override { return c_; }
+    int device() const override { return device_; }
+
+private:
+    ssa_config c_;
+    int device_;
+};
+
+}  // namespace detail
+
+// IC(0) preconditioner whose factor lives on the GPU
+// (preconditioners/ic_preconditioner.hpp:52-1543): the ABMC schedule and the
+// factorisation on the host at setup(), the triangular sweeps on the GPU.
+template <typename Scalar = double>
+class ICPreconditioner : public detail::DeviceFactor<Scalar> {
+public:
+    explicit ICPreconditioner(double shift = 1.05) { config_.shift_parameter = shift; }
+
+    void set_config(const SolverConfig& c) { config_ = c; }
+    const SolverConfig& config() const { return config_; }
+    double shift_parameter() const { return config_.shift_parameter; }
+    void set_shift_parameter(double s) { config_.shift_parameter = s; }
 
     std::string name() const override { return "IC"; }
 
     double actual_shift() const
     {
         ssa_ic_info info;
-        detail::check(ssa_ic_info_get(ic_.get(), &info));
+        detail::check(ssa_ic_info_get(this->ic_.get(), &info));
         return info.actual_shift;
     }
 
     std::vector<index_t> abmc_ordering() const
     {
-        std::vector<index_t> o(n_);
-        detail::check(ssa_ic_ordering(ic_.get(), o.data()));
+        std::vector<index_t> o(this->n_);
+        detail::check(ssa_ic_ordering(this->ic_.get(), o.data()));
         return o;
     }
 
-    ssa_ic* handle() const { return ic_.get(); }
-    const Scalar* matrix_values() const { return a_; }
+protected:
+    ssa_config handle_config() const override { return config_.to_c(); }
+    int device() const override { return config_.device; }
 
 private:
     SolverConfig config_;
-    std::shared_ptr<ssa_context> ctx_;
-    std::shared_ptr<ssa_ic> ic_;
-    index_t n_ = 0;
-    const Scalar* a_ = nullptr;
-    mutable std::uint64_t a_print_ = 0;
-    std::vector<index_t> m_rp_, m_ci_;
-    std::vector<Scalar> m_v_;
-    mutable std::shared_ptr<ssa_ic> op_;
+};
+
+// Symmetric Gauss-Seidel preconditioner (preconditioners/
+// sgs_preconditioner.hpp:47-203): M = (D + L) D^-1 (D + U) from A's own
+// triangles and diagonal, applied as two level-schedule sweeps on the GPU.
+// No settings (the reference's has none); device 0 unless set.
+template <typename Scalar = double>
+class SGSPreconditioner : public detail::DeviceFactor<Scalar> {
+public:
+    SGSPreconditioner() = default;
+    void set_device(int d) { device_ = d; }
+    std::string name() const override { return "SGS"; }
+
+protected:
+    ssa_config handle_config() const override
+    {
+        ssa_config c;
+        ssa_config_default(&c);
+        c.preconditioner = SSA_PRECOND_SGS;
+        return c;
+    }
+    int device() const override { return device_; }
+
+private:
+    int device_ = 0;
 };
 
 // Abstract iterative solver (solvers/iterative_solver.hpp:44-322): solve(A, b,
@@ -411,7 +478,12 @@ public:
                        const Preconditioner<Scalar>* precond = nullptr)
     {
         if (size != A.rows()) throw std::invalid_argument("solve: size mismatch");
-        ICPreconditioner<Scalar> local;
+        if (!takes_preconditioner()) precond = nullptr;
+        // operator only: a handle on A with the solver's settings
+        ssa_config oc = config_.to_c();
+        oc.use_abmc = 1;
+        oc.preconditioner = own_factor();
+        detail::OperatorHandle<Scalar> local(oc, config_.device);
         ssa_ic* h = nullptr;
         if (precond) {
             if (!precond->is_ready())
@@ -421,10 +493,6 @@ public:
                 throw std::invalid_argument("IterativeSolver::solve: the " + precond->name() +
                                             " preconditioner has no GPU implementation on this path");
         } else {
-            // operator only: a handle on A with the solver's settings
-            SolverConfig c = config_;
-            c.use_abmc = true;
-            local.set_config(c);
             local.setup(A);
             h = local.handle();
         }
@@ -455,6 +523,9 @@ public:
 
 protected:
     virtual int method(bool preconditioned) const = 0;
+    // what the handle of a solve without a preconditioner holds besides A
+    virtual int own_factor() const { return SSA_PRECOND_IC; }
+    virtual bool takes_preconditioner() const { return true; }
     SolverConfig config_;
 };
 
@@ -476,6 +547,21 @@ public:
 
 protected:
     int method(bool pre) const override { return pre ? SSA_METHOD_ICCOCR : SSA_METHOD_COCR; }
+};
+
+// SGS-MRTR (solvers/sgs_mrtr_solver.hpp:53-365): MRTR in the scaled space of
+// D A D with the split SGS formula of L = lower(D A D), its own factor (a
+// preconditioner passed to solve() is ignored, as in the reference).  FP64,
+// one right-hand side.
+template <typename Scalar = double>
+class SGSMRTRSolver : public IterativeSolver<Scalar> {
+public:
+    std::string name() const override { return "SGS-MRTR"; }
+
+protected:
+    int method(bool) const override { return SSA_METHOD_SGSMRTR; }
+    int own_factor() const override { return SSA_PRECOND_SGSMRTR; }
+    bool takes_preconditioner() const override { return false; }
 };
 
 template <typename Scalar = double>
@@ -520,6 +606,26 @@ template <typename Scalar = double>
 {
     if (x.size() != b.size()) x.resize(b.size());
     return solve_cocr(A, b.data(), x.data(), static_cast<index_t>(b.size()), config);
+}
+
+// solve_sgsmrtr (sparsesolv.hpp:123-150)
+template <typename Scalar = double>
+[[nodiscard]] inline SolverResult solve_sgsmrtr(const SparseMatrixView<Scalar>& A, const Scalar* b,
+                                                Scalar* x, index_t size,
+                                                const SolverConfig& config = SolverConfig())
+{
+    SGSMRTRSolver<Scalar> solver;
+    solver.set_config(config);
+    return solver.solve(A, b, x, size);
+}
+
+template <typename Scalar = double>
+[[nodiscard]] inline SolverResult solve_sgsmrtr(const SparseMatrixView<Scalar>& A,
+                                                const std::vector<Scalar>& b, std::vector<Scalar>& x,
+                                                const SolverConfig& config = SolverConfig())
+{
+    if (x.size() != b.size()) x.resize(b.size());
+    return solve_sgsmrtr(A, b.data(), x.data(), static_cast<index_t>(b.size()), config);
 }
 
 }  // namespace sparsesolv_amd

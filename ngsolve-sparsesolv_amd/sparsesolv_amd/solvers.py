@@ -1,7 +1,8 @@
 """Python drop-in for the sparsesolv_ngsolve factories on this path.
 
 Mirrors ngsolve/sparsesolv_python_export.hpp: ``SparseSolvSolver`` (:242-329),
-``ICPreconditioner`` (:176-207) and ``COCRSolver`` (:577-632), with the same
+``ICPreconditioner`` (:176-207), ``SGSPreconditioner`` (:82-88, :209-237) and
+``COCRSolver`` (:577-632), with the same
 keyword arguments, defaults, properties and ``SparseSolvResult`` fields.  The
 matrices are CSR (``problems.CSR`` or any scipy sparse matrix) instead of
 NGSolve ``SparseMatrix``; ``freedofs`` is a boolean array instead of a
@@ -20,9 +21,10 @@ import scipy.sparse as sp
 from . import _lib
 from .problems import CSR
 
-__all__ = ["SparseSolvSolver", "ICPreconditioner", "COCRSolver", "SparseSolvResult",
-           "SparseSolvSolverD", "SparseSolvSolverC", "ICPreconditionerD",
-           "ICPreconditionerC", "COCRSolverD", "COCRSolverC"]
+__all__ = ["SparseSolvSolver", "ICPreconditioner", "SGSPreconditioner", "COCRSolver",
+           "SparseSolvResult", "SparseSolvSolverD", "SparseSolvSolverC", "ICPreconditionerD",
+           "ICPreconditionerC", "SGSPreconditionerD", "SGSPreconditionerC", "COCRSolverD",
+           "COCRSolverC"]
 
 
 class SparseSolvResult:
@@ -92,41 +94,22 @@ class _GpuConfigMixin:
 
 
 # ---------------------------------------------------------------------------
-# ICPreconditioner
+# Preconditioners
 # ---------------------------------------------------------------------------
-class _ICPreconditioner:
-    """SparseSolvICPreconditioner (ngsolve/sparsesolv_precond.hpp:178-224)."""
+class _DevicePreconditioner:
+    """SparseSolvPrecondBase (ngsolve/sparsesolv_precond.hpp:88-176): the
+    matrix with its freedofs, Update(), Mult / MultAdd; the factor lives in a
+    device handle built from config()."""
     _dtype = np.float64
 
-    def __init__(self, mat, freedofs=None, shift: float = 1.05):
+    def __init__(self, mat, freedofs=None):
         self.mat = _as_csr(mat)
         self.freedofs = None if freedofs is None else np.array(freedofs, dtype=bool)
-        self._shift = float(shift)
-        self._use_abmc = False
-        self._abmc_block_size = 4
-        self._abmc_num_colors = 4
-        self._diagonal_scaling = False
         self._handle = None
         self._stale = True
 
-    # properties (python_export.hpp:63-80)
-    shift = property(lambda s: s._shift, lambda s, v: s._set("_shift", float(v)))
-    use_abmc = property(lambda s: s._use_abmc, lambda s, v: s._set("_use_abmc", bool(v)))
-    abmc_block_size = property(lambda s: s._abmc_block_size,
-                               lambda s, v: s._set("_abmc_block_size", int(v)))
-    abmc_num_colors = property(lambda s: s._abmc_num_colors,
-                               lambda s, v: s._set("_abmc_num_colors", int(v)))
-    diagonal_scaling = property(lambda s: s._diagonal_scaling,
-                                lambda s, v: s._set("_diagonal_scaling", bool(v)))
-
-    def _set(self, name, value):
-        setattr(self, name, value)
-
     def config(self) -> _lib.Config:
-        return _lib.default_config(shift_parameter=self._shift, use_abmc=self._use_abmc,
-                                   abmc_block_size=self._abmc_block_size,
-                                   abmc_num_colors=self._abmc_num_colors,
-                                   diagonal_scaling=self._diagonal_scaling)
+        raise NotImplementedError
 
     def Update(self):
         """Rebuild the factor with the current properties (Update(), :188-191)."""
@@ -178,6 +161,37 @@ class _ICPreconditioner:
         return self.mat.n
 
 
+class _ICPreconditioner(_DevicePreconditioner):
+    """SparseSolvICPreconditioner (ngsolve/sparsesolv_precond.hpp:178-224)."""
+
+    def __init__(self, mat, freedofs=None, shift: float = 1.05):
+        super().__init__(mat, freedofs)
+        self._shift = float(shift)
+        self._use_abmc = False
+        self._abmc_block_size = 4
+        self._abmc_num_colors = 4
+        self._diagonal_scaling = False
+
+    # properties (python_export.hpp:63-80)
+    shift = property(lambda s: s._shift, lambda s, v: s._set("_shift", float(v)))
+    use_abmc = property(lambda s: s._use_abmc, lambda s, v: s._set("_use_abmc", bool(v)))
+    abmc_block_size = property(lambda s: s._abmc_block_size,
+                               lambda s, v: s._set("_abmc_block_size", int(v)))
+    abmc_num_colors = property(lambda s: s._abmc_num_colors,
+                               lambda s, v: s._set("_abmc_num_colors", int(v)))
+    diagonal_scaling = property(lambda s: s._diagonal_scaling,
+                                lambda s, v: s._set("_diagonal_scaling", bool(v)))
+
+    def _set(self, name, value):
+        setattr(self, name, value)
+
+    def config(self) -> _lib.Config:
+        return _lib.default_config(shift_parameter=self._shift, use_abmc=self._use_abmc,
+                                   abmc_block_size=self._abmc_block_size,
+                                   abmc_num_colors=self._abmc_num_colors,
+                                   diagonal_scaling=self._diagonal_scaling)
+
+
 class ICPreconditionerD(_ICPreconditioner):
     _dtype = np.float64
 
@@ -193,13 +207,41 @@ def ICPreconditioner(mat, freedofs=None, shift: float = 1.05):
     return cls(A, freedofs, shift)
 
 
+class _SGSPreconditioner(_DevicePreconditioner):
+    """SparseSolvSGSPreconditioner (ngsolve/sparsesolv_precond.hpp:229-250):
+    SGSPreconditioner (preconditioners/sgs_preconditioner.hpp:47-203), whose
+    apply (D + L) D^-1 (D + U) runs as the two level-schedule sweeps of A's
+    triangles in natural order."""
+
+    def config(self) -> _lib.Config:
+        return _lib.default_config(preconditioner=_lib.PRECOND_SGS)
+
+
+class SGSPreconditionerD(_SGSPreconditioner):
+    _dtype = np.float64
+
+
+class SGSPreconditionerC(_SGSPreconditioner):
+    _dtype = np.complex128
+
+
+def SGSPreconditioner(mat, freedofs=None):
+    """Symmetric Gauss-Seidel (SGS) preconditioner; real/complex auto-detected,
+    set up at once (the factory calls Update, python_export.hpp:209-230)."""
+    A = _as_csr(mat)
+    cls = SGSPreconditionerC if A.is_complex else SGSPreconditionerD
+    p = cls(A, freedofs)
+    p.Update()
+    return p
+
+
 # ---------------------------------------------------------------------------
 # SparseSolvSolver
 # ---------------------------------------------------------------------------
 _METHODS = {"ICCG": _lib.METHOD_ICCG, "iccg": _lib.METHOD_ICCG,
             "CG": _lib.METHOD_CG, "cg": _lib.METHOD_CG,
             "COCR": _lib.METHOD_COCR, "cocr": _lib.METHOD_COCR,
-            "SGSMRTR": None, "sgsmrtr": None}
+            "SGSMRTR": _lib.METHOD_SGSMRTR, "sgsmrtr": _lib.METHOD_SGSMRTR}
 
 
 class _SparseSolvSolver:
@@ -262,23 +304,24 @@ class _SparseSolvSolver:
             use_abmc=self.use_abmc, abmc_block_size=self.abmc_block_size,
             abmc_num_colors=self.abmc_num_colors, abmc_reorder_spmv=self.abmc_reorder_spmv,
             abmc_use_rcm=self.abmc_use_rcm, precision=self.precision,
-            use_graph=1 if self.use_graph else 0)
+            use_graph=1 if self.use_graph else 0,
+            # solve_sgsmrtr builds its own split factor (sparsesolv.hpp:123-137)
+            preconditioner=_lib.PRECOND_SGSMRTR if _METHODS.get(self.method) == _lib.METHOD_SGSMRTR
+            else _lib.PRECOND_IC)
 
     def _method_code(self) -> int:
         if self.method not in _METHODS:
             raise RuntimeError(f"SparseSolvSolver: Unknown method '{self.method}'. "
                                "Available: ICCG, SGSMRTR, CG, COCR")
-        code = _METHODS[self.method]
-        if code is None:
-            raise RuntimeError("SparseSolvSolver: method SGSMRTR is not provided by this GPU path")
-        return code
+        return _METHODS[self.method]
 
     def _get_handle(self, cfg: _lib.Config) -> _lib.IcHandle:
         # the reference rebuilds the preconditioner on every Solve
         # (sparsesolv.hpp:52-54); the factor depends only on the matrix and
         # these fields, so the GPU copy is reused while they are unchanged
         key = (cfg.shift_parameter, cfg.auto_shift, cfg.diagonal_scaling, cfg.use_abmc,
-               cfg.abmc_block_size, cfg.abmc_num_colors, cfg.abmc_use_rcm, cfg.precision)
+               cfg.abmc_block_size, cfg.abmc_num_colors, cfg.abmc_use_rcm, cfg.precision,
+               cfg.preconditioner)
         if self._handle is None or key != self._cache_key:
             if self._handle is not None:
                 self._handle.close()
@@ -385,7 +428,7 @@ def SparseSolvSolver(mat, method="ICCG", freedofs=None, tol=1e-10, maxiter=1000,
                      save_best_result=True, save_residual_history=False, printrates=False,
                      conjugate=False, use_abmc=False, abmc_block_size=4, abmc_num_colors=4,
                      abmc_reorder_spmv=False, abmc_use_rcm=False):
-    """Iterative solver (ICCG / CG / COCR); real/complex auto-detected."""
+    """Iterative solver (ICCG / SGSMRTR / CG / COCR); real/complex auto-detected."""
     A = _as_csr(mat)
     cls = SparseSolvSolverC if A.is_complex else SparseSolvSolverD
     s = cls(A, method, freedofs, tol, maxiter, shift, save_best_result,
@@ -408,9 +451,9 @@ class _COCRSolver:
 
     def __init__(self, mat, pre, freedofs=None, maxiter=500, tol=1e-8, printrates=False):
         self.mat = _as_csr(mat)
-        if not isinstance(pre, _ICPreconditioner):
+        if not isinstance(pre, _DevicePreconditioner):
             raise TypeError("COCRSolver: on the GPU the preconditioner must be an "
-                            "ICPreconditioner from this module")
+                            "ICPreconditioner or SGSPreconditioner from this module")
         self.pre = pre
         self.freedofs = None if freedofs is None else np.array(freedofs, dtype=bool)
         self.maxiter = int(maxiter)

@@ -1,7 +1,8 @@
 // A client of the header-only C++ drop-in written against the reference's
-// signatures (namespace sparsesolv; SolverConfig; solve_iccg / solve_cocr;
-// IterativeSolver and Preconditioner used through base-class pointers;
-// apply_fused_dot; the SparseMatrixView accessors and product).  It prints
+// signatures (namespace sparsesolv; SolverConfig; solve_iccg / solve_cocr /
+// solve_sgsmrtr; IterativeSolver and Preconditioner used through base-class
+// pointers; ICPreconditioner and SGSPreconditioner; apply_fused_dot; the
+// SparseMatrixView accessors and product).  It prints
 // one JSON line per case and writes every matrix, right-hand side and
 // solution it used as raw little-endian arrays into the directory given as
 // argv[1], where the GPU test compares them with the oracle.
@@ -186,6 +187,34 @@ int main(int argc, char** argv)
         }
         std::printf("{\"case\": \"view\", \"bad\": %d, \"multiply_max_diff\": %.3e, \"nnz\": %d}\n", bad,
                     vmax, int(A.nnz()));
+
+        // the SGS preconditioner through the base classes, and SGS-MRTR
+        std::unique_ptr<sparsesolv::Preconditioner<double>> sgs(new sparsesolv::SGSPreconditioner<double>());
+        sgs->setup(A);
+        std::vector<double> ys(n), xs(n, 0.0);
+        sgs->apply(b.data(), ys.data(), n);
+        dump("sgs_y", ys);
+        auto rs = cg->solve(A, b.data(), xs.data(), n, sgs.get());
+        dump("sgs_cg_x", xs);
+        std::printf("{\"case\": \"sgs_cg\", \"precond\": \"%s\", \"converged\": %d, \"iterations\": %d, "
+                    "\"true_residual\": %.3e}\n",
+                    sgs->name().c_str(), int(rs.converged), rs.iterations, resid(A, xs, b));
+        std::vector<double> xm(n, 0.0);
+        sparsesolv::SolverConfig mc;
+        mc.save_residual_history = true;
+        auto rm = sparsesolv::solve_sgsmrtr(A, b, xm, mc);
+        dump("sgsmrtr_x", xm);
+        dump("sgsmrtr_hist", rm.residual_history);
+        // SGSMRTRSolver ignores a preconditioner passed to solve()
+        sparsesolv::SGSMRTRSolver<double> mrtr;
+        mrtr.set_config(mc);
+        std::vector<double> xm2(n, 0.0);
+        auto rm2 = mrtr.solve(A, b.data(), xm2.data(), n, sgs.get());
+        double dm = 0;
+        for (index_t i = 0; i < n; ++i) dm = std::max(dm, std::abs(xm2[i] - xm[i]));
+        std::printf("{\"case\": \"sgsmrtr\", \"solver\": \"%s\", \"converged\": %d, \"iterations\": %d, "
+                    "\"true_residual\": %.3e, \"with_precond_iterations\": %d, \"with_precond_max_diff\": %.3e}\n",
+                    mrtr.name().c_str(), int(rm.converged), rm.iterations, resid(A, xm, b), rm2.iterations, dm);
     }
     {
         std::vector<index_t> rp, ci;
@@ -242,6 +271,18 @@ int main(int argc, char** argv)
         std::printf("{\"case\": \"cocr_changed_values\", \"converged\": %d, \"iterations\": %d, "
                     "\"true_residual\": %.3e}\n",
                     int(rn.converged), rn.iterations, resid(A, xn, b));
+
+        // COCR with the SGS preconditioner, and SGS-MRTR, on the changed A
+        sparsesolv::SGSPreconditioner<C> sgs;
+        sgs.setup(A);
+        std::vector<C> xs(n, C(0)), xg(n, C(0));
+        auto rs = solver.solve(A, b.data(), xs.data(), n, &sgs);
+        dump("cplx_sgs_cocr_x", xs);
+        auto rg = sparsesolv::solve_sgsmrtr(A, b, xg, cfg);
+        dump("cplx_sgsmrtr_x", xg);
+        std::printf("{\"case\": \"cplx_sgs\", \"cocr_converged\": %d, \"cocr_iterations\": %d, "
+                    "\"cocr_true_residual\": %.3e, \"mrtr_converged\": %d, \"mrtr_iterations\": %d}\n",
+                    int(rs.converged), rs.iterations, resid(A, xs, b), int(rg.converged), rg.iterations);
     }
     return 0;
 }

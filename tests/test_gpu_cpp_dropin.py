@@ -135,3 +135,45 @@ def test_cocr_after_values_changed_in_place(run, oracle):
     assert r["converged"] and ro["converged"] and r["true_residual"] < 1e-9
     assert abs(r["iterations"] - ro["iterations"]) <= it_tol(ro["iterations"])
     assert rel(load("cocr_changed_x", np.complex128), xo) < 1e-9
+
+
+def test_sgs_preconditioner_and_sgsmrtr(run, oracle):
+    # SGSPreconditioner through the base class (apply, CGSolver with it) and
+    # solve_sgsmrtr / SGSMRTRSolver (which ignores a preconditioner)
+    rows, load = run
+    A = mat(load, "real", np.float64)
+    b = load("real_b", np.float64)
+    assert rel(load("sgs_y", np.float64), oracle.sgs_apply(A, b)) < 1e-12
+    r = rows["sgs_cg"]
+    xo, ro = oracle.solve_sgs(A, b, oracle.config(), "CG")
+    assert r["precond"] == "SGS" and r["converged"] and ro["converged"]
+    assert abs(r["iterations"] - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(load("sgs_cg_x", np.float64), xo) < 1e-9
+    m = rows["sgsmrtr"]
+    xo, ro = oracle.solve_sgsmrtr(A, b, oracle.config(save_residual_history=1))
+    assert m["solver"] == "SGS-MRTR" and m["converged"] and ro["converged"]
+    assert abs(m["iterations"] - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(load("sgsmrtr_x", np.float64), xo) < 1e-9
+    hist = load("sgsmrtr_hist", np.float64)
+    assert len(hist) == m["iterations"] + 1
+    np.testing.assert_allclose(hist[:5], ro["residual_history"][:5], rtol=1e-9)
+    assert m["with_precond_iterations"] == m["iterations"] and m["with_precond_max_diff"] == 0.0
+
+
+def test_complex_sgs_cocr_and_sgsmrtr(run, oracle):
+    # on the complex operator after its in-place change; its diagonal is
+    # complex, where the reference's SGS-MRTR split formula is not exact and
+    # its iterate is followed all the same
+    rows, load = run
+    A = mat(load, "cplx", np.complex128, values="cplx_v_new")
+    b = load("cplx_b", np.complex128)
+    cfg = oracle.config(use_abmc=1, abmc_num_colors=8)
+    r = rows["cplx_sgs"]
+    xo, ro = oracle.solve_sgs(A, b, cfg, "COCR")
+    assert r["cocr_converged"] and ro["converged"] and r["cocr_true_residual"] < 1e-9
+    assert abs(r["cocr_iterations"] - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(load("cplx_sgs_cocr_x", np.complex128), xo) < 1e-9
+    xo, ro = oracle.solve_sgsmrtr(A, b, cfg)
+    assert bool(r["mrtr_converged"]) == bool(ro["converged"])
+    assert abs(r["mrtr_iterations"] - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(load("cplx_sgsmrtr_x", np.complex128), xo) < 1e-9

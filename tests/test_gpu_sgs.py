@@ -174,3 +174,121 @@ def test_split_handles_refuse_what_they_cannot_do():
     with pytest.raises(_lib.SsaError):
         _lib.IcHandle(A, _lib.default_config(preconditioner=_lib.PRECOND_SGSMRTR,
                                              precision=_lib.PRECISION_MIXED))
+
+
+# ---------------------------------------------------------------------------
+# Python drop-in: SGSPreconditioner and SparseSolvSolver(method="SGSMRTR"),
+# the cases of the reference's tests/test_sparsesolv.py (SGSMRTR with
+# freedofs, SGS with an outside CG, complex auto-dispatch, Hermitian systems
+# with conjugate=True, printrates)
+# ---------------------------------------------------------------------------
+def test_sparsesolv_solver_sgsmrtr_with_freedofs(oracle):
+    import sparsesolv_amd as ss
+    from sparsesolv_amd.solvers import _with_freedofs
+    s = P.poisson(10)
+    free = np.ones(s.A.n, dtype=bool)
+    free[::7] = False
+    solver = ss.SparseSolvSolver(s.A, method="SGSMRTR", freedofs=free, tol=1e-10, maxiter=2000)
+    assert solver.method == "SGSMRTR"
+    sol = np.full(s.A.n, 3.0)
+    res = solver.Solve(s.b, sol)
+    assert res.converged and solver.last_result.iterations == res.iterations
+    assert np.all(sol[~free] == 3.0)          # constrained entries untouched
+    M = _with_freedofs(s.A, free)
+    xo, ro = oracle.solve_sgsmrtr(M, np.where(free, s.b, 0), oracle.config(max_iterations=2000),
+                                  x0=np.where(free, 3.0, 0.0))
+    assert abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(sol[free], xo[free]) < 1e-9
+    # Mult: zero initial guess, the solution on the free entries
+    y = solver * s.b
+    r = (s.A.to_scipy() @ y - s.b)[free]
+    assert np.linalg.norm(r) / np.linalg.norm(s.b[free]) < 1e-8
+
+
+def test_sgs_preconditioner_with_an_outside_cg():
+    # the reference hands its SGSPreconditioner to NGSolve's CGSolver; here
+    # scipy's CG calls Mult through a LinearOperator
+    import scipy.sparse.linalg as sla
+    import sparsesolv_amd as ss
+    A = varcoef(12)
+    b = P.uniform(2, A.n, -1.0, 1.0)
+    pre = ss.SGSPreconditioner(A)
+    assert isinstance(pre, ss.SGSPreconditionerD)
+    M = sla.LinearOperator((A.n, A.n), matvec=lambda v: pre.Mult(np.ascontiguousarray(v)))
+    its = []
+    x, info = sla.cg(A.to_scipy(), b, M=M, rtol=1e-10, maxiter=500, callback=lambda _: its.append(1))
+    assert info == 0
+    assert np.linalg.norm(b - A.to_scipy() @ x) / np.linalg.norm(b) < 1e-9
+    # the same preconditioner in the GPU's own CG: the same count
+    h, cfg = handle(A, _lib.PRECOND_SGS, tolerance=1e-10)
+    _, res, _ = h.solve(_lib.METHOD_ICCG, b, None, cfg)
+    assert res.converged and abs(res.iterations - len(its)) <= 2
+
+
+def test_sgs_factory_complex_dispatch_and_cocr_solver():
+    import sparsesolv_amd as ss
+    A, b = complex_laplace(10)
+    pre = ss.SGSPreconditioner(A)
+    assert isinstance(pre, ss.SGSPreconditionerC)
+    x = pre * b
+    assert x.dtype == np.complex128 and np.linalg.norm(x) > 0
+    inv = ss.COCRSolver(A, pre, maxiter=2000, tol=1e-10)
+    y = inv * b
+    r = b - A.to_scipy() @ y
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-9
+
+
+def complex_laplace(m):
+    """7-point Laplacian + 0.3j I: complex symmetric, real-dominant diagonal."""
+    import scipy.sparse as sp
+    L = P.laplace7(m).to_scipy().astype(np.complex128)
+    A = P.CSR.from_scipy((L + 0.3j * sp.identity(L.shape[0])).tocsr())
+    z = P.gaussian(7, 2 * A.n)
+    return A, z[0::2] + 1j * z[1::2]
+
+
+def hermitian_laplace(m):
+    """A Hermitian system in a complex space: real coefficients, complex RHS."""
+    L = P.laplace7(m).to_scipy().astype(np.complex128)
+    A = P.CSR.from_scipy(L.tocsr())
+    z = P.gaussian(8, 2 * A.n)
+    return A, z[0::2] + 1j * z[1::2]
+
+
+@pytest.mark.parametrize("method", ["ICCG", "SGSMRTR"])
+def test_hermitian_system_with_conjugate(oracle, method):
+    import sparsesolv_amd as ss
+    A, b = hermitian_laplace(10)
+    solver = ss.SparseSolvSolver(A, method=method, tol=1e-10, maxiter=2000, conjugate=True)
+    assert isinstance(solver, ss.SparseSolvSolverC)
+    x = solver * b
+    res = solver.last_result
+    assert res.converged
+    assert np.linalg.norm(b - A.to_scipy() @ x) / np.linalg.norm(b) < 1e-8
+    if method == "SGSMRTR":
+        xo, ro = oracle.solve_sgsmrtr(A, b, oracle.config(conjugate=1, max_iterations=2000))
+        assert abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+        assert rel(x, xo) < 1e-9
+
+
+def test_sgsmrtr_printrates_and_refusals(capsys):
+    import sparsesolv_amd as ss
+    g = load_golden("vc6_sgsmrtr")
+    s = G.varcoef6()
+    solver = ss.SparseSolvSolver(s.A, method="SGSMRTR", printrates=True)
+    x = solver * s.b
+    res = solver.last_result
+    assert res.converged and res.iterations == int(g["iterations"])
+    assert rel(x, g["x"]) < 1e-9
+    lines = capsys.readouterr().out.splitlines()
+    assert len(lines) == len(res.residual_history) + 1
+    assert lines[0] == f"SGSMRTR iteration 0, residual = {g['history'][0]:g}"
+    assert lines[-1] == (f"SGSMRTR converged in {res.iterations} iterations, "
+                         f"residual = {res.final_residual:g}")
+    # SGS-MRTR runs in FP64 and one vector at a time
+    solver.mixed_precision = True
+    with pytest.raises(_lib.SsaError):
+        solver * s.b
+    solver.mixed_precision = False
+    with pytest.raises(RuntimeError):
+        solver.SolveBlock(np.stack([s.b, s.b]), np.zeros((2, s.A.n)))
