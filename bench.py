@@ -50,7 +50,9 @@ Extra workloads (`extras`, one line of figures each; `--quick` skips them):
               in one batched solve, against the same systems one by one;
   eddy78-lossy-multi16  the headline block on eddy78's grid with the
               conductor filling the box (A = K + j*0.1*M_e on every edge),
-              where COCR reaches 1e-10: a true time to tolerance.
+              where COCR reaches 1e-10: a true time to tolerance;
+  laplace48-sgsmrtr  SGS-MRTR (solve_sgsmrtr) on laplace48's system, against
+              the reference's own solve_sgsmrtr.
 Every figure of every workload also goes to stderr as one JSON document, and
 to --detail PATH when given; stdout carries only the final line.
 """
@@ -101,6 +103,12 @@ WORKLOADS = {
         method="cocr",
         cfg=dict(use_abmc=1, shift_parameter=1.05, abmc_block_size=4, abmc_num_colors=8,
                  tolerance=1e-10, max_iterations=1000)),
+    "laplace48-sgsmrtr": dict(
+        desc=("7-point Laplacian, 48^3 interior grid (n=110,592, nnz=760,320), SGS-MRTR fp64 "
+              "(MRTR on D A D with the split SGS sweeps of L = lower(D A D), level schedule in "
+              "natural order), tol 1e-10, max 1000 iterations"),
+        method="sgsmrtr",
+        cfg=dict(preconditioner=2, tolerance=1e-10, max_iterations=1000)),
 }
 
 
@@ -128,7 +136,7 @@ _SYSTEMS = {}
 
 def make_system(name):
     from sparsesolv_amd import problems as P
-    base = name.replace("-mixed", "").replace("-level", "")
+    base = name.replace("-mixed", "").replace("-level", "").replace("-sgsmrtr", "")
     if base not in _SYSTEMS:
         if base == "curlcurl64":
             _SYSTEMS[base] = P.curlcurl(64)
@@ -156,6 +164,16 @@ def bytes_per_iteration(n, nnz, nnz_l, s, cocr, s_low=None, nrhs=1):
     vec = n * ((20 if cocr else 14) * s + 2 * s_p) + n * s_f
     vec_rhs = n * ((20 if cocr else 14) * s + 2 * s_p)
     return S_A + 2 * S_L + (vec if nrhs == 1 else n * s_f + nrhs * vec_rhs)
+
+
+def mrtr_bytes_per_iteration(n, nnz_l, s):
+    """Compulsory bytes of one SGS-MRTR iteration: the factor L = lower(D A D)
+    read by both sweeps (value + index per entry, a row pointer per row), and
+    the vector operands once each: the sweeps' inputs and outputs (4), rd - u
+    (3), Ard and its three dots (5), the p, x2, y, rd, rd / L_ii update (11)
+    and the best-x copy (2)."""
+    S_L = nnz_l * (s + 4) + 4 * (n + 1)
+    return 2 * S_L + 25 * n * s
 
 
 TRACE_NAMES = ("spmv", "colour", "update", "pq", "dot")
@@ -237,7 +255,8 @@ def run_gpu(name, steps, warmup, budget=None, trace_iterations=0):
     dev = torch.device("cuda:0")
     b_d = torch.from_numpy(np.ascontiguousarray(b)).to(dev)
     x_d = torch.zeros(A.n, dtype=dt, device=dev)
-    code = _lib.METHOD_ICCG if w["method"] == "iccg" else _lib.METHOD_ICCOCR
+    code = {"iccg": _lib.METHOD_ICCG, "cocr": _lib.METHOD_ICCOCR,
+            "sgsmrtr": _lib.METHOD_SGSMRTR}[w["method"]]
     res = None
     for _ in range(warmup):
         x_d.zero_()
@@ -265,8 +284,11 @@ def run_gpu(name, steps, warmup, budget=None, trace_iterations=0):
     it = int(np.median(iters))
     s = 16 if A.is_complex else 8
     mixed = kw.get("precision", 0) == 1
-    model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], s, w["method"] == "cocr",
-                                s // 2 if mixed else None)
+    if w["method"] == "sgsmrtr":
+        model = mrtr_bytes_per_iteration(A.n, info["nnz_l"], s)
+    else:
+        model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], s, w["method"] == "cocr",
+                                    s // 2 if mixed else None)
     trace = {}
     if trace_iterations > 0:
         tcfg = _lib.default_config(**dict(kw, max_iterations=trace_iterations))
@@ -606,6 +628,30 @@ def cpu_baseline(A, b, kw, method, gpu_iters, x_gpu=None, budget_s=25.0, single=
     return out
 
 
+def mrtr_cpu_baseline(A, b, kw, x_gpu=None):
+    """The reference's own solve_sgsmrtr (its headers compiled with OpenMP)
+    on this host: the whole solve (its setup of D and L included), fastest
+    of three runs, its solution compared with the GPU's."""
+    _, orc, kind = _reference()
+    fields = dict(orc.config()._fields_)
+    cfg = orc.config(**{k: v for k, v in kw.items() if k in fields})
+    runs = []
+    for _ in range(3):
+        t = time.perf_counter()
+        x, r = orc.solve_sgsmrtr(A, b, cfg)
+        runs.append(time.perf_counter() - t)
+    its = int(r["iterations"])
+    out = {"value": round(min(runs), 4), "unit": "s", "cores": orc.num_threads(), "kind": kind,
+           "cpu": cpu_model(), "iterations": its,
+           "ms_per_iteration": round(1e3 * min(runs) / max(its, 1), 3),
+           "sample": (f"the reference's full solve_sgsmrtr of the same system ({its} iterations, "
+                      f"converged={r['converged']}, residual {r['final_residual']:.3e}), setup "
+                      f"of D and L included, fastest of 3 runs")}
+    if x_gpu is not None:
+        out["gpu_vs_cpu_rel_diff"] = float(np.linalg.norm(x_gpu - x) / max(np.linalg.norm(x), 1e-300))
+    return out
+
+
 def cpu_single_thread(orc, A, b, cfg, mname, gpu_iters, threads, budget_s):
     """The same reference build on one thread: setup + a bounded number of
     iterations, extrapolated to the GPU's iteration count."""
@@ -745,7 +791,10 @@ def main():
                                                      args.budget, args.trace_iterations)
         if not args.no_cpu:
             log("cpu baseline ...")
-            cpu = cpu_baseline(A, b, kw, method, main_out["iterations"], x_gpu)
+            if method == "sgsmrtr":
+                cpu = mrtr_cpu_baseline(A, b, kw, x_gpu)
+            else:
+                cpu = cpu_baseline(A, b, kw, method, main_out["iterations"], x_gpu)
             log(f"cpu baseline: {cpu}")
         cfg_line = {"workload": f"{args.workload}: {WORKLOADS[args.workload]['desc']}",
                     "n": A.n, "nnz": A.nnz, "iterations": main_out["iterations"],
@@ -770,7 +819,7 @@ def main():
             roofline["traffic_source"] = traffic_src
 
     names = ("laplace48", "curlcurl64", "curlcurl64-mixed", "eddy78-mixed", "batch64",
-             "eddy78-lossy-multi16")
+             "eddy78-lossy-multi16", "laplace48-sgsmrtr")
     wanted = set(names if args.extras is None else [e for e in args.extras.split(",") if e])
     extras = {}
     if not args.quick:
@@ -788,6 +837,9 @@ def main():
                     log(f"cpu baseline ({other}) ...")
                     o["cpu_baseline"] = cpu_baseline(A2, b2, kw2, m2, o["iterations"], x2,
                                                      budget_s=12.0, single=False)
+                elif not args.no_cpu and m2 == "sgsmrtr":
+                    log(f"cpu baseline ({other}) ...")
+                    o["cpu_baseline"] = mrtr_cpu_baseline(A2, b2, kw2, x2)
             detail[other] = o
             extras[other] = summary(other, o)
             log(f"[{other}] {json.dumps(extras[other])}")
