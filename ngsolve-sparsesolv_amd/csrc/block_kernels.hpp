@@ -159,14 +159,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_blk(index_t n, T* __restri
 
 // COCR: the previous iteration's direction update p = rt + beta p (beta
 // from the state; 0 in a pass's first iteration, where p holds rt), then
-// x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt may be in
-// the preconditioner's precision).  p is read here anyway (x += alpha p),
-// so the p, q update after the SpMV leaves p alone: the same operations on
-// the same operands as the one-vector loop (p updated there, read here),
-// two vectors fewer through HBM.
-template <typename T, typename TQ, int W>
+// x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt and rt may
+// be in the preconditioner's precision, rt computed wide, stored narrow).
+// p is read here anyway (x += alpha p), so the p, q update after the SpMV
+// leaves p alone: the same operations on the same operands as the
+// one-vector loop (p updated there, read here), two vectors fewer through
+// HBM.
+template <typename T, typename TQ, int W, typename TR = T>
 __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __restrict__ x,
-                                                           T* __restrict__ r, T* __restrict__ rt,
+                                                           T* __restrict__ r, TR* __restrict__ rt,
                                                            T* __restrict__ p,
                                                            const T* __restrict__ q,
                                                            const TQ* __restrict__ qt,
@@ -193,13 +194,13 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
         const index_t rows = n - base < kBlock ? n - base : kBlock;
         tile_reduce<double, W>(base, rows, tile, s, [&](size_t e, int h) {
             if (!run[h]) return 0.0;
-            const T rte = rt[e];
+            const T rte = cvt<T>(rt[e]);
             const T pe = fma_(beta[h], p[e], rte);
             p[e] = pe;
             x[e] = fma_(alpha[h], pe, x[e]);
             const T ri = fms_(alpha[h], q[e], r[e]);
             r[e] = ri;
-            rt[e] = fms_(alpha[h], cvt<T>(qt[e]), rte);
+            rt[e] = cvt<TR>(fms_(alpha[h], cvt<T>(qt[e]), rte));
             return norm2(ri);
         });
     }

@@ -316,7 +316,12 @@ struct IcImpl final : IcBase {
     hipStream_t sweep_stream = nullptr;   // where sweep kernels go (null: the context's)
     bool serial_sweeps = false;           // no walk overlap (no second stream is made)
     static_assert(sparsesolv_amd::kWalkDepth % kWalkPrefetch == 0, "walk padding");
-    DevBuf lo_a, lo_b, lo_t;
+    // mixed: q (lo_a, the sweeps' input), qt (lo_b) and the sweeps'
+    // intermediate (lo_t) in P, and COCR's rt = M^-1 r (lo_r): rt comes from
+    // the P preconditioner, is updated by rt -= alpha qt with qt in P and is
+    // the SpMV's input t = A rt, so it is kept in P (computed in T, stored
+    // rounded): the SpMV gathers and the update move half the bytes for it
+    DevBuf lo_a, lo_b, lo_t, lo_r;
     // diagonal_scaling: S = diag(1/sqrt|a_ii|) per device row and the
     // preconditioner's scaled input.  The reference applies S M^-1 S on the
     // level path and in ICCG's reordered-space apply (Path 1,
@@ -414,15 +419,15 @@ struct IcImpl final : IcBase {
     {
         const double e = double(n) * double(W), sv = double(sizeof(T)), sp = double(sizeof(P));
         const double low = kMixed ? sp : 0.0;   // the narrow copy a mixed kernel also writes
-        const double sq = kMixed ? sp : sv;    // qt, z: the preconditioner's output
+        const double sq = kMixed ? sp : sv;    // qt, z, COCR's rt: the preconditioner's precision
         switch (cls) {
         case SSA_TRACE_UPDATE:
             // COCR: x, p, r, q, rt, qt read; x, r, rt (blocks: and p) written
-            if (cocr) return e * ((W > 1 ? 9.0 : 8.0) * sv + sq);
+            if (cocr) return e * ((W > 1 ? 7.0 : 6.0) * sv + 3.0 * sq);
             return e * (6.0 * sv + low);       // CG: x, p, r, Ap read; x, r written
         case SSA_TRACE_PQ:
             // COCR: t, q read, q written (one vector: also rt, p read, p written)
-            if (cocr) return e * ((W > 1 ? 3.0 : 6.0) * sv + low);
+            if (cocr) return e * (W > 1 ? 3.0 * sv + low : 5.0 * sv + sq + low);
             return e * (2.0 * sv + sq);        // CG: z, p read; p written
         default:                 // dot: two operands
             return e * (sv + (kMixed ? sp : sv));
@@ -599,7 +604,7 @@ struct IcImpl final : IcBase {
         rstate.reserve(sizeof(SolverState) * nst);
         pass.reserve(sizeof(ssa::PassParams) * nst);
         if (kMixed)
-            for (DevBuf* b : {&lo_a, &lo_b, &lo_t})
+            for (DevBuf* b : {&lo_a, &lo_b, &lo_t, &lo_r})
                 if (b->reserve(size_t(n) * size_t(W) * sizeof(P)))
                     HIP_CHECK(hipMemsetAsync(b->p, 0, b->bytes, s));
         if (has_scale) sbuf.reserve(size_t(n) * size_t(W) * sizeof(P));
@@ -626,8 +631,22 @@ struct IcImpl final : IcBase {
         HIP_CHECK(hipGetLastError());
     }
 
-    template <int WV>
-    void spmv_w(const T* xin, T* y, int dot, T* part, const SolverState* st, const ssa::Ctrl& ctrl)
+    // y = A x with x in the preconditioner's precision (widened exactly)
+    void launch_spmv_low(const P* xin, T* y, int dot, T* part, const SolverState* st,
+                         const ssa::Ctrl& ctrl = ssa::Ctrl{})
+    {
+        switch (W) {
+        case 1: spmv_w<1, P>(xin, y, dot, part, st, ctrl); break;
+        case 4: spmv_w<4, P>(xin, y, dot, part, st, ctrl); break;
+        case 8: spmv_w<8, P>(xin, y, dot, part, st, ctrl); break;
+        case 16: spmv_w<16, P>(xin, y, dot, part, st, ctrl); break;
+        default: throw SsaError(SSA_ERR_ARGUMENT, "unsupported block width");
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    template <int WV, typename TX = T>
+    void spmv_w(const TX* xin, T* y, int dot, T* part, const SolverState* st, const ssa::Ctrl& ctrl)
     {
         const int g = std::max(1, spmv_grid());
         hipStream_t s = ctx->stream;
@@ -638,14 +657,14 @@ struct IcImpl final : IcBase {
         const T* val = A_val.as<T>();
         // read x once, write y, the matrix entries and row pointers once
         const double bytes = double(nnz) * (sizeof(T) + 4.0) + 4.0 * (double(n) + 1.0) +
-                             2.0 * double(n) * WV * sizeof(T);
+                             double(n) * WV * (sizeof(TX) + sizeof(T));
         traced(SSA_TRACE_SPMV, s, bytes, [&] {
             if (dot == 0)
-                ssa::k_spmv<T, 0, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+                ssa::k_spmv<T, 0, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
             else if (dot == 1)
-                ssa::k_spmv<T, 1, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+                ssa::k_spmv<T, 1, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
             else
-                ssa::k_spmv<T, 2, WV><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+                ssa::k_spmv<T, 2, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
         });
     }
 
@@ -1049,6 +1068,8 @@ struct IcImpl final : IcBase {
     P* pre_tmp() { return kMixed ? lo_t.as<P>() : t.as<P>(); }
     // the P copy a mixed-precision kernel also writes (none in FP64)
     P* low_copy() { return kMixed ? lo_a.as<P>() : nullptr; }
+    // COCR's rt (P: the Krylov vector itself in FP64, the narrow copy in mixed)
+    P* rt_vec() { return kMixed ? lo_r.as<P>() : z.as<P>(); }
 
     // dst = src converted (states that have stopped are left alone)
     template <typename S, typename D>
@@ -1269,8 +1290,14 @@ struct IcImpl final : IcBase {
         HIP_CHECK(hipGetLastError());
         st = rst;
         convert(r.as<T>(), lo_b.as<P>(), st);
+        if (W == 1) {
+            launch_precond(lo_b.as<P>(), rt_vec(), lo_t.as<P>(), st);
+            return;
+        }
+        // a block's sweeps write every right-hand side of a row while any
+        // runs: they go to scratch, and only the replacing ones are copied
         launch_precond(lo_b.as<P>(), lo_b.as<P>(), lo_t.as<P>(), st);
-        convert(lo_b.as<P>(), z.as<T>(), st);
+        convert(lo_b.as<P>(), rt_vec(), st);
     }
     // whether iteration k of a pass (counted from 0) replaces rt
     // whether iteration k of a pass (counted from 0) decides on a replacement
@@ -1328,12 +1355,12 @@ struct IcImpl final : IcBase {
                     ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
             });
             traced(SSA_TRACE_UPDATE, s, vector_bytes(SSA_TRACE_UPDATE, true), [&] {
-                ssa::k_cocr_update<T, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
+                ssa::k_cocr_update<T, P, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), rt_vec(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
             });
             if (kMixed && pre && replace) replace_rt(st);
-            launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G, false));
+            launch_spmv_low(rt_vec(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G, false));
             traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, true), [&] {
-                ssa::k_cocr_pq<T, P><<<G, ssa::kBlock, 0, s>>>(n, z.as<T>(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
+                ssa::k_cocr_pq<T, P, P><<<G, ssa::kBlock, 0, s>>>(n, rt_vec(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
             });
         }
         HIP_CHECK(hipGetLastError());
@@ -1387,11 +1414,11 @@ struct IcImpl final : IcBase {
                 });
                 ssa::k_ctrl_alpha_blk<T><<<WV, CT, 0, s>>>(part_a.as<T>(), G, st, prm, H, hs);
                 traced(SSA_TRACE_UPDATE, s, vector_bytes(SSA_TRACE_UPDATE, true), [&] {
-                    ssa::k_cocr_update_blk<T, P, WV><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), z.as<T>(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st);
+                    ssa::k_cocr_update_blk<T, P, WV, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), rt_vec(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st);
                 });
                 ssa::k_ctrl_check_blk<<<WV, CT, 0, s>>>(part_b.as<double>(), G, st, prm, H, hs);
                 if (kMixed && pre && replace) replace_rt(st);
-                launch_spmv(z.as<T>(), t.as<T>(), 1, part_c.as<T>(), st);
+                launch_spmv_low(rt_vec(), t.as<T>(), 1, part_c.as<T>(), st);
                 ssa::k_ctrl_beta_blk<T><<<WV, CT, 0, s>>>(part_c.as<T>(), np_spmv, st, prm);
                 traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, true), [&] {
                     ssa::k_cocr_q_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, t.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, low_copy());
@@ -1473,13 +1500,11 @@ struct IcImpl final : IcBase {
             // rt = M^{-1} r; p = rt; q = A p; rho = rt^T q (cocr_solver.hpp:78-88)
             if (pre) {
                 if (kMixed) convert(r.as<T>(), pre_in(c), nullptr);
-                launch_precond(kMixed ? pre_in(c) : r.as<P>(), kMixed ? pre_out(c) : z.as<P>(),
-                               pre_tmp(), nullptr);
-                if (kMixed) convert(pre_out(c), z.as<T>(), nullptr);
+                launch_precond(kMixed ? pre_in(c) : r.as<P>(), rt_vec(), pre_tmp(), nullptr);
             } else {
                 convert(r.as<T>(), z.as<T>(), nullptr);
             }
-            convert(z.as<T>(), p.as<T>(), nullptr);
+            convert(rt_vec(), p.as<T>(), nullptr);
             launch_spmv(p.as<T>(), Ap.as<T>(), 1, part_c.as<T>(), nullptr);
             ssa::k_ctrl_store<T><<<S, ssa::kBlock, 0, s>>>(part_c.as<T>(), std::max(1, spmv_grid()), st, 0, y, 1);
             if (kMixed) convert(Ap.as<T>(), pre_in(c), nullptr);

@@ -109,13 +109,15 @@ __device__ inline T block_sum_multi(const T (&v)[W], T* scratch)
 // and applies it to all W vectors; the partials of right-hand side r are
 // [r*gridDim.x, (r+1)*gridDim.x).  Right-hand sides that have stopped are
 // still multiplied (their results are not read).  The fused convergence
-// check (`check`) is for one vector only.
+// check (`check`) is for one vector only.  x may be stored narrower (TX:
+// the preconditioner's precision); its values are widened exactly before
+// they are multiplied, so the products are those of the widened vector.
 // ---------------------------------------------------------------------------
-template <typename T, int DOT, int W>
+template <typename T, int DOT, int W, typename TX = T>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const index_t* __restrict__ slice_off, const T* __restrict__ val,
     const index_t* __restrict__ col, const index_t* __restrict__ rows, index_t nslices,
-    const T* __restrict__ x, T* __restrict__ y, T* __restrict__ partials,
+    const TX* __restrict__ x, T* __restrict__ y, T* __restrict__ partials,
     const SolverState* __restrict__ st, Ctrl check, Systems sy)
 {
     const unsigned lb = W == 1 ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -147,10 +149,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
             for (index_t k = o0; k < o1; ++k) {
                 const size_t at = size_t(k) * 64 + lane;
                 const T v = val[at];
-                T g[W];
-                load_row<T, W>(x + size_t(col[at]) * W, g);
+                TX g[W];
+                load_row<TX, W>(x + size_t(col[at]) * W, g);
 #pragma unroll
-                for (int r = 0; r < W; ++r) acc[r] = fma_(v, g[r], acc[r]);
+                for (int r = 0; r < W; ++r) acc[r] = fma_(v, cvt<T>(g[r]), acc[r]);
             }
         } else {
             // one entry's W gathered values fill the registers: the next
@@ -171,19 +173,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
                     cn = col[at];
                     vn = val[at];
                 }
-                T g[W];
-                load_row<T, W>(x + size_t(c) * W, g);
+                TX g[W];
+                load_row<TX, W>(x + size_t(c) * W, g);
 #pragma unroll
-                for (int r = 0; r < W; ++r) acc[r] = fma_(v, g[r], acc[r]);
+                for (int r = 0; r < W; ++r) acc[r] = fma_(v, cvt<T>(g[r]), acc[r]);
             }
         }
         if (row >= 0) {
             store_row<T, W>(y + size_t(row) * W, acc);
             if (DOT) {
-                T xr[W];
-                load_row<T, W>(x + size_t(row) * W, xr);
+                TX xr[W];
+                load_row<TX, W>(x + size_t(row) * W, xr);
 #pragma unroll
-                for (int r = 0; r < W; ++r) d[r] = mul(conj_if(xr[r], DOT == 2), acc[r]);
+                for (int r = 0; r < W; ++r) d[r] = mul(conj_if(cvt<T>(xr[r]), DOT == 2), acc[r]);
             }
         }
     }
@@ -875,11 +877,12 @@ __global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const TZ* __restrict
     }
 }
 
-// COCR: x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt may
-// be in the preconditioner's precision)
-template <typename T, typename TQ>
+// COCR: x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt and
+// rt may be in the preconditioner's precision: rt is then computed wide and
+// stored narrow)
+template <typename T, typename TQ, typename TR = T>
 __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict__ x,
-                                                       T* __restrict__ r, T* __restrict__ rt,
+                                                       T* __restrict__ r, TR* __restrict__ rt,
                                                        const T* __restrict__ p,
                                                        const T* __restrict__ q,
                                                        const TQ* __restrict__ qt,
@@ -891,7 +894,8 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
     if (st[sp.sys].status) return;
     // the operands of this thread's first rows load before alpha is summed
     const index_t step = sp.G * kBlock, j0 = sp.local * kBlock + threadIdx.x;
-    T xv[kRowsAhead], pv[kRowsAhead], rv[kRowsAhead], qv[kRowsAhead], rtv[kRowsAhead];
+    T xv[kRowsAhead], pv[kRowsAhead], rv[kRowsAhead], qv[kRowsAhead];
+    TR rtv[kRowsAhead];
     TQ qtv[kRowsAhead];
     auto load = [&](index_t jb) {
 #pragma unroll
@@ -922,7 +926,7 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
             x[i] = fma_(alpha, pv[u], xv[u]);
             const T ri = fms_(alpha, qv[u], rv[u]);
             r[i] = ri;
-            rt[i] = fms_(alpha, cvt<T>(qtv[u]), rtv[u]);
+            rt[i] = cvt<TR>(fms_(alpha, cvt<T>(qtv[u]), cvt<T>(rtv[u])));
             s += norm2(ri);
         }
         jb += kRowsAhead * step;
@@ -933,9 +937,10 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
 }
 
 // COCR: p = rt + beta p; q = t + beta q (and the best-x copy); q also in
-// the preconditioner's precision when q_low is given
-template <typename T, typename P>
-__global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restrict__ rt,
+// the preconditioner's precision when q_low is given; rt may be stored in
+// it (TR)
+template <typename T, typename P, typename TR = T>
+__global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const TR* __restrict__ rt,
                                                    const T* __restrict__ t, T* __restrict__ p,
                                                    T* __restrict__ q, const T* __restrict__ x,
                                                    T* __restrict__ best,
@@ -947,7 +952,8 @@ __global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restri
     if (st[sp.sys].status) return;
     // the operands of this thread's first rows load before beta is summed
     const index_t step = sp.G * kBlock, j0 = sp.local * kBlock + threadIdx.x;
-    T pv[kRowsAhead], rtv[kRowsAhead], qv[kRowsAhead], tv[kRowsAhead];
+    T pv[kRowsAhead], qv[kRowsAhead], tv[kRowsAhead];
+    TR rtv[kRowsAhead];
     auto load = [&](index_t jb) {
 #pragma unroll
         for (int u = 0; u < kRowsAhead; ++u) {
@@ -970,7 +976,7 @@ __global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const T* __restri
             const index_t j = jb + u * step;
             if (j >= sp.n) break;
             const index_t i = sp.row0 + j;
-            p[i] = fma_(beta, pv[u], rtv[u]);
+            p[i] = fma_(beta, pv[u], cvt<T>(rtv[u]));
             const T qi = fma_(beta, qv[u], tv[u]);
             q[i] = qi;
             if (q_low) q_low[i] = cvt<P>(qi);
