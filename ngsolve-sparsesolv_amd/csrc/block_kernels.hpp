@@ -165,6 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_blk(index_t n, T* __restri
 // leaves p alone: the same operations on the same operands as the
 // one-vector loop (p updated there, read here), two vectors fewer through
 // HBM.
+// best <- x (before its update) for the right-hand sides whose last check
+// improved, as the one-vector k_cocr_update does.
 template <typename T, typename TQ, int W, typename TR = T>
 __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __restrict__ x,
                                                            T* __restrict__ r, TR* __restrict__ rt,
@@ -172,18 +174,20 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
                                                            const T* __restrict__ q,
                                                            const TQ* __restrict__ qt,
                                                            double* __restrict__ partials,
-                                                           const SolverState* __restrict__ st)
+                                                           const SolverState* __restrict__ st,
+                                                           T* __restrict__ best)
 {
     if (all_stopped<W>(st)) return;
     __shared__ double red[kWavesPerBlock * W];
     constexpr int CP = pass_cols<double, W>();
     __shared__ double tile[kBlock * (CP + 1)];
     T alpha[W / CP], beta[W / CP];
-    bool run[W / CP];
+    bool run[W / CP], copy[W / CP];
 #pragma unroll
     for (int h = 0; h < W / CP; ++h) {
         const int c = pass_column<double, W>(h);
         run[h] = st[c].status == 0;
+        copy[h] = best && st[c].improved;
         alpha[h] = from_pair<T>(st[c].alpha);
         beta[h] = from_pair<T>(st[c].beta);
     }
@@ -197,7 +201,9 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
             const T rte = cvt<T>(rt[e]);
             const T pe = fma_(beta[h], p[e], rte);
             p[e] = pe;
-            x[e] = fma_(alpha[h], pe, x[e]);
+            const T xe = x[e];
+            if (copy[h]) best[e] = xe;
+            x[e] = fma_(alpha[h], pe, xe);
             const T ri = fms_(alpha[h], q[e], r[e]);
             r[e] = ri;
             rt[e] = cvt<TR>(fms_(alpha[h], cvt<T>(qt[e]), rte));
@@ -244,29 +250,23 @@ __global__ __launch_bounds__(kBlock) void k_xpby_blk(size_t total, const TZ* __r
     }
 }
 
-// COCR: best <- x when improved; q = t + beta q (q also in the
-// preconditioner's precision when q_low is given).  p = rt + beta p is left
-// to the next iteration's k_cocr_update_blk.
+// COCR: q = t + beta q (q also in the preconditioner's precision when q_low
+// is given).  p = rt + beta p and the best-x copy are left to the next
+// iteration's k_cocr_update_blk.
 template <typename T, typename P, int W>
 __global__ __launch_bounds__(kBlock) void k_cocr_q_blk(size_t total, const T* __restrict__ t,
-                                                      T* __restrict__ q, const T* __restrict__ x,
-                                                      T* __restrict__ best,
+                                                      T* __restrict__ q,
                                                       const SolverState* __restrict__ st,
                                                       P* __restrict__ q_low)
 {
     const int k = int(threadIdx.x) & (W - 1);
-    const bool copy = best && st[k].improved;
-    const bool run = st[k].status == 0;
-    if (!copy && !run) return;
+    if (st[k].status != 0) return;
     const T beta = from_pair<T>(st[k].beta);
     for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < total;
          e += size_t(gridDim.x) * kBlock) {
-        if (copy) best[e] = x[e];
-        if (run) {
-            const T qi = fma_(beta, q[e], t[e]);
-            q[e] = qi;
-            if (q_low) q_low[e] = cvt<P>(qi);
-        }
+        const T qi = fma_(beta, q[e], t[e]);
+        q[e] = qi;
+        if (q_low) q_low[e] = cvt<P>(qi);
     }
 }
 

@@ -34,12 +34,16 @@ __device__ inline c128 ddiv(c128 a, c128 b)
 // IterativeSolver::check_convergence (solvers/iterative_solver.hpp:217-249);
 // with prm.ngs the COCRSolverNGS test res/rhs_norm <= tol
 // (ngsolve/sparsesolv_solvers.hpp:338-347).
+// `recheck`: the check of a breakdown, on the residual the last check saw;
+// it leaves `improved` alone unless it improves (only before any check),
+// so a kernel that reads `improved` while its workgroup 0 runs this step
+// reads an unchanging value.
 __device__ inline bool check_convergence(SolverState* s, const SolveParams& prm, double nr,
-                                         double* hist)
+                                         double* hist, bool recheck = false)
 {
     const double rel = nr / s->normalizer;
     if (prm.save_history) hist[s->hist_len++] = rel;
-    s->improved = 0;
+    if (!recheck) s->improved = 0;
     if (prm.save_best && rel < s->best_residual) {
         s->best_residual = rel;
         s->improved = 1;
@@ -73,7 +77,7 @@ __device__ inline void ctrl_alpha_op(SolverState* st, const SolveParams& prm, do
             stop(st, kBreakdown, 0, st->iter, nr);
             return;
         }
-        const bool conv = check_convergence(st, prm, nr, hist);
+        const bool conv = check_convergence(st, prm, nr, hist, /*recheck=*/true);
         stop(st, kBreakdown, conv ? 1 : 0, conv ? st->iter + 1 : st->iter, nr);
         return;
     }

@@ -1355,12 +1355,12 @@ struct IcImpl final : IcBase {
                     ssa::k_dot<T, P, T><<<G, ssa::kBlock, 0, s>>>(n, pre_out(c), Ap.as<T>(), 0, part_a.as<T>(), st, ssa::Ctrl{}, y);
             });
             traced(SSA_TRACE_UPDATE, s, vector_bytes(SSA_TRACE_UPDATE, true), [&] {
-                ssa::k_cocr_update<T, P, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), rt_vec(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y);
+                ssa::k_cocr_update<T, P, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), rt_vec(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, ctl(part_a.p, G, false), y, bx);
             });
             if (kMixed && pre && replace) replace_rt(st);
             launch_spmv_low(rt_vec(), t.as<T>(), 1, part_c.as<T>(), st, ctl(part_b.p, G, false));
             traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, true), [&] {
-                ssa::k_cocr_pq<T, P, P><<<G, ssa::kBlock, 0, s>>>(n, rt_vec(), t.as<T>(), p.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, ctl(part_c.p, np_spmv, true), low_copy(), y);
+                ssa::k_cocr_pq<T, P, P><<<G, ssa::kBlock, 0, s>>>(n, rt_vec(), t.as<T>(), p.as<T>(), Ap.as<T>(), st, ctl(part_c.p, np_spmv, true), low_copy(), y);
             });
         }
         HIP_CHECK(hipGetLastError());
@@ -1414,14 +1414,14 @@ struct IcImpl final : IcBase {
                 });
                 ssa::k_ctrl_alpha_blk<T><<<WV, CT, 0, s>>>(part_a.as<T>(), G, st, prm, H, hs);
                 traced(SSA_TRACE_UPDATE, s, vector_bytes(SSA_TRACE_UPDATE, true), [&] {
-                    ssa::k_cocr_update_blk<T, P, WV, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), rt_vec(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st);
+                    ssa::k_cocr_update_blk<T, P, WV, P><<<G, ssa::kBlock, 0, s>>>(n, x.as<T>(), r.as<T>(), rt_vec(), p.as<T>(), Ap.as<T>(), pre_out(c), part_b.as<double>(), st, bx);
                 });
                 ssa::k_ctrl_check_blk<<<WV, CT, 0, s>>>(part_b.as<double>(), G, st, prm, H, hs);
                 if (kMixed && pre && replace) replace_rt(st);
                 launch_spmv_low(rt_vec(), t.as<T>(), 1, part_c.as<T>(), st);
                 ssa::k_ctrl_beta_blk<T><<<WV, CT, 0, s>>>(part_c.as<T>(), np_spmv, st, prm);
                 traced(SSA_TRACE_PQ, s, vector_bytes(SSA_TRACE_PQ, true), [&] {
-                    ssa::k_cocr_q_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, t.as<T>(), Ap.as<T>(), x.as<T>(), bx, st, low_copy());
+                    ssa::k_cocr_q_blk<T, P, WV><<<FG, ssa::kBlock, 0, s>>>(ne, t.as<T>(), Ap.as<T>(), st, low_copy());
                 });
             }
         });
@@ -1751,7 +1751,10 @@ struct IcImpl final : IcBase {
                     if (!run[u]) continue;
                     fin[u] = h_state[q];
                     iters[u] += h_state[q].result_iters;
-                    if (!h_state[q].converged && prm.save_best) copy_rows(x, best, q);
+                    // the best iterate: x itself when the last check improved
+                    // (the copy into best follows an improving check)
+                    if (!h_state[q].converged && prm.save_best && !h_state[q].improved)
+                        copy_rows(x, best, q);
                 }
                 const std::vector<double> res = true_residuals();
                 for (int q = 0; q < S; ++q) {
@@ -1789,7 +1792,9 @@ struct IcImpl final : IcBase {
             res.history_len = prm.save_history && S == 1 ? Sq.hist_len : 0;
             res.actual_shift = pre ? actual_shift : 0.0;
             if (true_rel[u] < 0.0 && !Sq.converged && prm.save_best) {
-                copy_rows(x, best, q);
+                // x is the best iterate when the last check improved; best
+                // holds it otherwise (copied after its check)
+                if (!Sq.improved) copy_rows(x, best, q);
                 res.final_residual = Sq.best_residual;
             }
         }

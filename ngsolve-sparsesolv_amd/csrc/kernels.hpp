@@ -879,7 +879,10 @@ __global__ __launch_bounds__(kBlock) void k_xpby(index_t n, const TZ* __restrict
 
 // COCR: x += alpha p; r -= alpha q; rt -= alpha qt; partial |r|^2 (qt and
 // rt may be in the preconditioner's precision: rt is then computed wide and
-// stored narrow)
+// stored narrow).  best <- x (before its update) when the last check
+// improved the best residual: x is read here anyway, so the copy costs one
+// store.  A solve that stops at an improving check ends with x itself as its
+// best iterate (the caller takes x, not best, when `improved` is set).
 template <typename T, typename TQ, typename TR = T>
 __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict__ x,
                                                        T* __restrict__ r, TR* __restrict__ rt,
@@ -888,10 +891,12 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
                                                        const TQ* __restrict__ qt,
                                                        double* __restrict__ partials,
                                                        const SolverState* __restrict__ st,
-                                                       Ctrl ctl, Systems sy)
+                                                       Ctrl ctl, Systems sy, T* __restrict__ best)
 {
     const Span sp = vec_span(sy, n);
     if (st[sp.sys].status) return;
+    // set by the last check; this kernel's control step does not change it
+    const bool copy = best && st[sp.sys].improved;
     // the operands of this thread's first rows load before alpha is summed
     const index_t step = sp.G * kBlock, j0 = sp.local * kBlock + threadIdx.x;
     T xv[kRowsAhead], pv[kRowsAhead], rv[kRowsAhead], qv[kRowsAhead];
@@ -923,6 +928,7 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
             const index_t j = jb + u * step;
             if (j >= sp.n) break;
             const index_t i = sp.row0 + j;
+            if (copy) best[i] = xv[u];
             x[i] = fma_(alpha, pv[u], xv[u]);
             const T ri = fms_(alpha, qv[u], rv[u]);
             r[i] = ri;
@@ -936,19 +942,17 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update(index_t n, T* __restrict
     if (threadIdx.x == 0) partials[sp.slot] = t;
 }
 
-// COCR: p = rt + beta p; q = t + beta q (and the best-x copy); q also in
-// the preconditioner's precision when q_low is given; rt may be stored in
-// it (TR)
+// COCR: p = rt + beta p; q = t + beta q; q also in the preconditioner's
+// precision when q_low is given; rt may be stored in it (TR).  (The best-x
+// copy is the next k_cocr_update's.)
 template <typename T, typename P, typename TR = T>
 __global__ __launch_bounds__(kBlock) void k_cocr_pq(index_t n, const TR* __restrict__ rt,
                                                    const T* __restrict__ t, T* __restrict__ p,
-                                                   T* __restrict__ q, const T* __restrict__ x,
-                                                   T* __restrict__ best,
+                                                   T* __restrict__ q,
                                                    const SolverState* __restrict__ st, Ctrl ctl,
                                                    P* __restrict__ q_low, Systems sy)
 {
     const Span sp = vec_span(sy, n);
-    copy_if_improved(sp, x, best, st);
     if (st[sp.sys].status) return;
     // the operands of this thread's first rows load before beta is summed
     const index_t step = sp.G * kBlock, j0 = sp.local * kBlock + threadIdx.x;
