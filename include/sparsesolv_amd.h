@@ -253,7 +253,8 @@ int ssa_spmv_multi(ssa_ic* ic, int nrhs, const void* x, void* y, int on_device);
  * avg_us = mean wall time of one call (colour kernels: of one launch);
  * bytes = algorithmic bytes one call (launch) must move: every operand once,
  * factor entries with the diagonal at value + int32 index, one int32 row
- * pointer and the input and output entry per row. */
+ * pointer and the input and output entry per row, and (colour kernels) each
+ * distinct row of earlier colours the launch reads. */
 #define SSA_PROFILE_SPMV 0
 #define SSA_PROFILE_FORWARD 1
 #define SSA_PROFILE_BACKWARD 2

@@ -177,10 +177,12 @@ struct SweepLayout {
     std::vector<index_t> phase_stride;  // nphases: entry slots from one slice to the next
     std::vector<F> val;                 // slots*64
     std::vector<index_t> col;
-    // per colour, rows of the regular slices and their factor entries
-    // counting the diagonal (the algorithmic size of one colour kernel)
+    // per colour, rows of the regular slices, their factor entries counting
+    // the diagonal, and the distinct rows of earlier colours their entries
+    // read (the algorithmic size of one colour kernel: every operand once)
     std::vector<index_t> phase_rows;
     std::vector<int64_t> phase_entries;
+    std::vector<int64_t> phase_ext_rows;
 
     // big blocks (level scheduled), each owned by one workgroup
     std::vector<index_t> phase_big;     // nphases+1: big-block ranges
@@ -413,14 +415,27 @@ struct SweepLayout {
         }
         phase_rows.assign(size_t(nphases), 0);
         phase_entries.assign(size_t(nphases), 0);
+        phase_ext_rows.assign(size_t(nphases), 0);
+        std::vector<index_t> seen(size_t(M.n), -1);
         for (index_t p = 0; p < nphases; ++p)
-            for (index_t sl = phase_slices[p]; sl < phase_slices[p + 1]; ++sl)
+            for (index_t sl = phase_slices[p]; sl < phase_slices[p + 1]; ++sl) {
                 for (int lane = 0; lane < kWave; ++lane) {
                     const index_t row = lane_row[size_t(sl) * kWave + lane];
                     if (row < 0) continue;
                     ++phase_rows[p];
                     phase_entries[p] += M.rp[row + 1] - M.rp[row];   // diagonal included
                 }
+                // external entries (columns >= 0 among the slice's K slots)
+                const index_t K = info_len(lane_info[size_t(sl) * kWave]);
+                const size_t at = size_t(slice_slot[sl]) * kWave;
+                for (size_t q = 0; q < size_t(K) * kWave; ++q) {
+                    const index_t c = col[at + q];
+                    if (c >= 0 && seen[size_t(c)] != p) {
+                        seen[size_t(c)] = p;
+                        ++phase_ext_rows[p];
+                    }
+                }
+            }
         build_walk(M, forward, invd);
         val.resize(val.size() + size_t(kSweepChunk) * kWave, F(0));
         col.resize(col.size() + size_t(kSweepChunk) * kWave, kPadCol);
@@ -589,6 +604,7 @@ public:
         out.walk_ext_rp.assign(1, 0);
         out.phase_rows.assign(size_t(out.nphases), 0);
         out.phase_entries.assign(size_t(out.nphases), 0);
+        out.phase_ext_rows.assign(size_t(out.nphases), 0);
         for (index_t p = 0; p < out.nphases; ++p) {
             // the phase's stride: the longest slice of any system
             index_t stride = 0, nsl = 0;
@@ -611,6 +627,7 @@ public:
                 const index_t r0 = row0[si];
                 out.phase_rows[p] += P.phase_rows[p];
                 out.phase_entries[p] += P.phase_entries[p];
+                out.phase_ext_rows[p] += P.phase_ext_rows[p];
                 for (index_t sl = P.phase_slices[p]; sl < P.phase_slices[p + 1]; ++sl, ++k) {
                     const index_t Ks = info_len(P.lane_info[size_t(sl) * kWave]);
                     const size_t src = size_t(P.slice_slot[sl]) * kWave;

@@ -140,7 +140,7 @@ struct DevSweep {
     DevBuf big_pos, big_lvl_off, big_lvl, big_rows, big_rp, big_ci, big_v;
     DevBuf w_step_off, w_row, w_ext_rp, w_ext_ci, w_ext_v, w_rec, w_sa, w_sb;
     std::vector<index_t> phase_slices, phase_big, phase_rows, phase_slot, phase_stride;
-    std::vector<int64_t> phase_entries;
+    std::vector<int64_t> phase_entries, phase_ext_rows;
     int BR = 1;
     int walk_kmax = 0;
     std::vector<index_t> h_step_off;
@@ -186,6 +186,7 @@ struct DevSweep {
         phase_big = L.phase_big;
         phase_rows = L.phase_rows;
         phase_entries = L.phase_entries;
+        phase_ext_rows = L.phase_ext_rows;
         BR = L.BR;
         nphases = L.nphases;
         max_big_rows = L.max_big_rows;
@@ -885,6 +886,17 @@ struct IcImpl final : IcBase {
         }
     }
 
+    // algorithmic bytes of colour ph's kernel: its factor entries (value and
+    // index, the diagonal counted), a row pointer, the input and output
+    // entry of its rows, and each distinct row of earlier colours it reads,
+    // once (W values per row)
+    double colour_bytes(const DevSweep& D, index_t ph) const
+    {
+        return double(D.phase_entries[ph]) * (sizeof(F) + 4.0) +
+               double(D.phase_rows[ph]) * (4.0 + 2.0 * W * sizeof(P)) +
+               double(D.phase_ext_rows[ph]) * W * sizeof(P);
+    }
+
     template <bool FWD>
     void launch_phase(DevSweep& D, index_t ph, const P* in, P* out, const SolverState* st,
                       PhasePart part = kWhole)
@@ -915,8 +927,7 @@ struct IcImpl final : IcBase {
             const ssa::SweepArrays A = sweep_arrays(D);
             // timed as the colour kernel: colours without oversized blocks
             const int cls = b1 == b0 ? SSA_TRACE_COLOUR : -2;
-            const double bytes = double(D.phase_entries[ph]) * (sizeof(F) + 4.0) +
-                                 double(D.phase_rows[ph]) * (4.0 + 2.0 * W * sizeof(P));
+            const double bytes = colour_bytes(D, ph);
 #define SSA_SWEEP(BRV) \
     sweep_br<FWD, BRV>(g, s0, s1, D.phase_slot[ph], D.phase_stride[ph], A, in, out, we, st)
             traced(cls, s, bytes, [&] {
@@ -1845,8 +1856,7 @@ struct IcImpl final : IcBase {
                 float ms = 0.f;
                 HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
                 total_us += 1000.0 * double(ms) / reps;
-                total_bytes += double(D.phase_entries[ph]) * (sizeof(F) + 4.0) +
-                               double(D.phase_rows[ph]) * (4.0 + 2.0 * W * sizeof(P));
+                total_bytes += colour_bytes(D, ph);
                 ++count;
             }
         }

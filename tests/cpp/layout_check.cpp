@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -72,6 +73,8 @@ void emulate_sweep(const SweepLayout<T>& L, bool fwd, const std::vector<T>& invd
     CHECK(BR >= 0 && BR <= 16 && (BR & (BR - 1)) == 0);
     CHECK(BR > 0 || L.nslices() == 0);
     for (index_t ph = 0; ph < L.nphases; ++ph) {
+        // the distinct rows of earlier colours the phase's slices read
+        std::set<index_t> ext;
         // regular slices
         for (index_t s = L.phase_slices[ph]; s < L.phase_slices[ph + 1]; ++s) {
             // the kernels find the slice's slots from the phase's first
@@ -112,6 +115,7 @@ void emulate_sweep(const SweepLayout<T>& L, bool fwd, const std::vector<T>& invd
                     const index_t c = L.col[at];
                     if (c >= 0) {
                         CHECK(c < n);
+                        ext.insert(c);
                         acc[l] -= L.val[at] * out[size_t(c)];
                     } else if (c != kPadCol) {
                         const index_t q = -1 - c;
@@ -139,6 +143,7 @@ void emulate_sweep(const SweepLayout<T>& L, bool fwd, const std::vector<T>& invd
                 }
             }
         }
+        CHECK(size_t(ph) < L.phase_ext_rows.size() && L.phase_ext_rows[ph] == int64_t(ext.size()));
         // oversized blocks of this phase
         const index_t b0 = L.phase_big[ph], b1 = L.phase_big[ph + 1];
         if (b1 == b0) continue;
