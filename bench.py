@@ -387,14 +387,18 @@ def run_batch(steps, warmup, cpu=True, budget_s=12.0, tune=True):
     if cpu:
         out["cpu_baseline"] = batch_cpu_baseline(systems, kw, iters, budget_s)
     if tune:
-        # autotuned ABMC parameters, chosen on the largest system
+        # autotuned ABMC parameters per size class (three classes of 21-22
+        # systems, each tuned on its largest system), one batch with each
+        # system's choice
         from sparsesolv_amd import autotune as at
-        big = max(systems, key=lambda q: q.A.n)
-        r = at.autotune(big.A, big.b, tolerance=1e-10, max_iterations=2000, shift_parameter=1.05)
-        tkw = r.config(**kw)
-        tcfg = _lib.default_config(**tkw)
+        t0 = time.perf_counter()
+        r = at.autotune_batch(systems, classes=3, tolerance=1e-10, max_iterations=2000,
+                              shift_parameter=1.05)
+        tune_s = time.perf_counter() - t0
+        tcfg = cfg
         del hb
-        ht = _lib.BatchHandle([q.A for q in systems], tcfg)
+        ht = _lib.BatchHandle([q.A for q in systems], tcfg, block_sizes=r.block_sizes,
+                              num_colors=r.num_colors)
         ht.solve_batch_device(_lib.METHOD_ICCG, b_d.data_ptr(), x_d.data_ptr(), tcfg)
         tt = []
         for _ in range(max(1, steps)):
@@ -404,15 +408,23 @@ def run_batch(steps, warmup, cpu=True, budget_s=12.0, tune=True):
             tres = ht.solve_batch_device(_lib.METHOD_ICCG, b_d.data_ptr(), x_d.data_ptr(), tcfg)
             torch.cuda.synchronize()
             tt.append(time.perf_counter() - t)
+        classes = []
+        for idx, rep, tr in r.classes:
+            classes.append({
+                "systems": len(idx), "n_range": [min(systems[i].A.n for i in idx),
+                                                 max(systems[i].A.n for i in idx)],
+                "tuned_on_n": systems[rep].A.n, "abmc_block_size": tr.best.block_size,
+                "abmc_num_colors": tr.best.num_colors,
+                "trials": [[t.block_size, t.num_colors, round(t.seconds, 4), t.iterations,
+                            t.converged] for t in tr.trials]})
         out["tuned"] = {
-            "abmc_block_size": r.best.block_size, "abmc_num_colors": r.best.num_colors,
-            "tuned_on": f"the largest system (n={big.A.n})", "seconds": float(np.mean(tt)),
+            "per_class": classes, "tune_s": round(tune_s, 2), "seconds": float(np.mean(tt)),
             "iterations_max": max(int(q.iterations) for q in tres),
-            "all_converged": bool(all(q.converged for q in tres)),
-            "trials": [[t.block_size, t.num_colors, round(t.seconds, 4), t.iterations, t.converged]
-                       for t in r.trials]}
-        log(f"[batch64] tuned: block {r.best.block_size}, colours {r.best.num_colors}, "
-            f"{out['tuned']['seconds']:.4f}s")
+            "all_converged": bool(all(q.converged for q in tres))}
+        log(f"[batch64] tuned per class: " +
+            ", ".join(f"n<={c['n_range'][1]}: block {c['abmc_block_size']} colours "
+                      f"{c['abmc_num_colors']}" for c in classes) +
+            f"; {out['tuned']['seconds']:.4f}s (tuning {tune_s:.1f}s)")
     return out
 
 
@@ -707,7 +719,7 @@ def summary(key, o):
             s["cpu_s"] = _g(o["cpu_baseline"]["value"])
         if "tuned" in o:
             s["tuned_s"] = _g(o["tuned"]["seconds"])
-            s["tuned_bc"] = [o["tuned"]["abmc_block_size"], o["tuned"]["abmc_num_colors"]]
+            s["tuned_bc"] = [[c["abmc_block_size"], c["abmc_num_colors"]] for c in o["tuned"]["per_class"]]
         return s
     if "nrhs" in o:
         return {"s": _g(o["seconds"]), "it_max": max(o["iterations"]), "conv": o["all_converged"],

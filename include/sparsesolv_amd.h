@@ -97,6 +97,11 @@ typedef struct ssa_config {
                                          (use_abmc, diagonal_scaling and the shift settings do
                                          not apply); SSA_PRECOND_SGSMRTR: FP64, one system per
                                          handle (read at ssa_ic_create) */
+    int rhs_block;                    /* 0: 16; 1, 4, 8 or 16: right-hand sides per launch of the
+                                         several-right-hand-side calls (a tuning knob: results
+                                         are the same bits for every width).  Solves read the
+                                         solve's cfg (0 there: the handle's), apply / spmv the
+                                         handle's (read at ssa_ic_create) */
 } ssa_config;
 
 /* sparsesolv::SolverResult (core/types.hpp:29-37) plus timing. */
@@ -215,6 +220,14 @@ int ssa_solve(ssa_ic* ic, int method, const void* b, void* x, int on_device,
 int ssa_batch_create(ssa_context* ctx, int dtype, int count, const int32_t* n,
                      const int32_t* const* row_ptr, const int32_t* const* col_idx,
                      const void* const* values, const ssa_config* cfg, ssa_ic** out);
+/* ssa_batch_create with an ABMC block size and colour count of its own for
+ * each system (block_size[s], num_colors[s]; either may be NULL for cfg's
+ * abmc_block_size / abmc_num_colors).  No reference counterpart: a batch
+ * whose parameters are chosen per size class (sparsesolv_amd.autotune). */
+int ssa_batch_create_each(ssa_context* ctx, int dtype, int count, const int32_t* n,
+                          const int32_t* const* row_ptr, const int32_t* const* col_idx,
+                          const void* const* values, const ssa_config* cfg,
+                          const int32_t* block_size, const int32_t* num_colors, ssa_ic** out);
 /* solve_iccg / solve_cocr (or unpreconditioned CG / COCR) for every system
  * of a handle at once: one sequence of launches covers all systems, each
  * with its own scalars and stop status, and results[s] (count entries) is

@@ -60,6 +60,13 @@ struct IcSettings {
     // device schedule, no counterpart in the reference: -1 auto, 0 off, 1 on
     // (the straggler walks on a second stream)
     int walk_overlap = -1;
+    // device schedule, no counterpart in the reference: right-hand sides per
+    // launch of the several-right-hand-side calls (0: 16)
+    int rhs_block = 0;
+    // batches: per-system ABMC block size and colour count (null: the
+    // fields above for every system)
+    const int* batch_block_size = nullptr;
+    const int* batch_num_colors = nullptr;
     // what the factor is: kFactorIc the IC(0) factorisation; kFactorSgs
     // A's lower triangle with the pivots diag(A), whose sweeps apply the SGS
     // preconditioner (D + L) D^-1 (D + U) (preconditioners/

@@ -71,6 +71,7 @@ struct SolverConfig {
     int precision = SSA_PRECISION_FP64;
     bool use_graph = true;
     int walk_overlap = -1;
+    int rhs_block = 0;
     int device = 0;
 
     ssa_config to_c() const
@@ -104,6 +105,7 @@ struct SolverConfig {
         c.precision = precision;
         c.use_graph = use_graph;
         c.walk_overlap = walk_overlap;
+        c.rhs_block = rhs_block;
         return c;
     }
 };

@@ -206,7 +206,11 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
         if (count > 1) cap = 1;
         for (int s; (s = next.fetch_add(1)) < count;) {
             try {
-                build_plan<T, F>(plans[s], n[s], row_ptr[s], col_idx[s], values[s], settings);
+                sparsesolv_amd::IcSettings own = settings;
+                if (settings.batch_block_size) own.abmc_block_size = settings.batch_block_size[s];
+                if (settings.batch_num_colors) own.abmc_num_colors = settings.batch_num_colors[s];
+                own.batch_block_size = own.batch_num_colors = nullptr;
+                build_plan<T, F>(plans[s], n[s], row_ptr[s], col_idx[s], values[s], own);
             } catch (const std::exception& e) {
                 errors[s] = e.what();
             }

@@ -336,6 +336,7 @@ struct IcImpl final : IcBase {
     // kFactorSgsMrtr); SGS-MRTR also keeps L = lower(D A D) as CSR (for its
     // normaliser |L D b|) and D, D^-1, all in device order
     int factor_kind = sparsesolv_amd::kFactorIc;
+    int rhs_block = 0;   // right-hand sides per launch (0: kMaxRhs)
     DevBuf mr_rp, mr_ci, mr_v, mr_d, mr_id;
 
     SolverState* h_state = nullptr;  // pinned, one per state
@@ -441,6 +442,7 @@ struct IcImpl final : IcBase {
     {
         overlap_setting = st.walk_overlap;
         factor_kind = st.factor_kind;
+        rhs_block = st.rhs_block;
         ssa_host::BatchPlan<HT, HF> plan;
         ssa_host::build_batch<HT, HF>(plan, count, ns, rp, ci, v, st);
         n = plan.row0.back();
@@ -1150,12 +1152,14 @@ struct IcImpl final : IcBase {
     // the caller's nrhs vectors (tot values each), in blocks of at most
     // kMaxRhs right-hand sides
     template <typename Fn>
-    void in_blocks(int nrhs, Fn&& fn)
+    void in_blocks(int nrhs, Fn&& fn, int width = 0)
     {
         if (nrhs < 1) throw SsaError(SSA_ERR_ARGUMENT, "nrhs must be at least 1");
         if (nrhs > 1 && count > 1)
             throw SsaError(SSA_ERR_UNSUPPORTED, "several right-hand sides need a one-system handle");
-        for (int r0 = 0; r0 < nrhs; r0 += kMaxRhs) fn(r0, std::min(kMaxRhs, nrhs - r0));
+        // right-hand sides per launch: the call's width, else the handle's
+        const int blk = width > 0 ? width : rhs_block > 0 ? rhs_block : kMaxRhs;
+        for (int r0 = 0; r0 < nrhs; r0 += blk) fn(r0, std::min(blk, nrhs - r0));
     }
 
     // ---- public operations -------------------------------------------------
@@ -1638,7 +1642,7 @@ struct IcImpl final : IcBase {
             const size_t off = size_t(r0) * size_t(tot);
             solve_block(method, static_cast<const T*>(bin) + off, static_cast<T*>(xio) + off, on_device,
                         cfg, results + r0, nrhs == 1 ? history : nullptr, m);
-        });
+        }, cfg.rhs_block);
     }
 
     // one block of m right-hand sides (m = 1 for a batch: its systems)

@@ -108,6 +108,7 @@ void ssa_config_default(ssa_config* c)
     c->poll_interval = 0;
     c->walk_overlap = -1;
     c->preconditioner = SSA_PRECOND_IC;
+    c->rhs_block = 0;
 }
 
 const char* ssa_last_error(void) { return g_last_error.c_str(); }
@@ -160,10 +161,17 @@ static void check_matrix(const char* who, int n, const int32_t* row_ptr, const i
             throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": column index out of range");
 }
 
+static void check_rhs_block(const char* who, int b)
+{
+    if (b != 0 && b != 1 && b != 4 && b != 8 && b != 16)
+        throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": rhs_block must be 0, 1, 4, 8 or 16");
+}
+
 static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int count,
                              const int32_t* n, const int32_t* const* row_ptr,
                              const int32_t* const* col_idx, const void* const* values,
-                             const ssa_config* cfg)
+                             const ssa_config* cfg, const int32_t* block_size = nullptr,
+                             const int32_t* num_colors = nullptr)
 {
     if (!cfg) throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": null argument");
     if (dtype != SSA_F64 && dtype != SSA_C128)
@@ -193,6 +201,14 @@ static ssa_ic* create_handle(const char* who, ssa_context* ctx, int dtype, int c
     st.abmc_num_colors = cfg->abmc_num_colors;
     st.abmc_use_rcm = cfg->abmc_use_rcm != 0;
     st.walk_overlap = cfg->walk_overlap < 0 ? -1 : cfg->walk_overlap > 0 ? 1 : 0;
+    check_rhs_block(who, cfg->rhs_block);
+    st.rhs_block = cfg->rhs_block;
+    for (int q = 0; q < count; ++q) {
+        if ((block_size && block_size[q] < 1) || (num_colors && num_colors[q] < 1))
+            throw SsaError(SSA_ERR_ARGUMENT, std::string(who) + ": block sizes and colour counts must be >= 1");
+    }
+    st.batch_block_size = block_size;
+    st.batch_num_colors = num_colors;
     if (kind != SSA_PRECOND_IC) {
         // the split factors: natural order on the level schedule, no
         // scaling or shift (SGSPreconditioner::setup, SGSMRTRSolver)
@@ -231,6 +247,19 @@ int ssa_batch_create(ssa_context* ctx, int dtype, int count, const int32_t* n,
         if (!out || !n || !row_ptr || !col_idx || !values || count < 1)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_batch_create: bad argument");
         *out = create_handle("ssa_batch_create", ctx, dtype, count, n, row_ptr, col_idx, values, cfg);
+    });
+}
+
+int ssa_batch_create_each(ssa_context* ctx, int dtype, int count, const int32_t* n,
+                          const int32_t* const* row_ptr, const int32_t* const* col_idx,
+                          const void* const* values, const ssa_config* cfg,
+                          const int32_t* block_size, const int32_t* num_colors, ssa_ic** out)
+{
+    return guarded([&] {
+        if (!out || !n || !row_ptr || !col_idx || !values || count < 1)
+            throw SsaError(SSA_ERR_ARGUMENT, "ssa_batch_create_each: bad argument");
+        *out = create_handle("ssa_batch_create_each", ctx, dtype, count, n, row_ptr, col_idx, values,
+                             cfg, block_size, num_colors);
     });
 }
 
@@ -355,6 +384,7 @@ int ssa_solve_multi(ssa_ic* ic, int method, int nrhs, const void* b, void* x, in
         if (nrhs < 1) throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve_multi: nrhs must be at least 1");
         if (ic->impl->system_count() != 1)
             throw SsaError(SSA_ERR_ARGUMENT, "ssa_solve_multi: a batched handle is solved with ssa_batch_solve");
+        check_rhs_block("ssa_solve_multi", cfg->rhs_block);
         std::memset(results, 0, sizeof(ssa_result) * size_t(nrhs));
         ic->impl->solve(method, b, x, on_device != 0, *cfg, results, nullptr, nrhs);
     });

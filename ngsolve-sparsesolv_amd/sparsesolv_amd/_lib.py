@@ -43,7 +43,7 @@ EXPORTED = (
     "ssa_ic_schedule", "ssa_ic_factor", "ssa_ic_apply", "ssa_spmv", "ssa_solve",
     "ssa_ic_profile", "ssa_batch_create", "ssa_batch_solve", "ssa_ic_systems",
     "ssa_solve_multi", "ssa_ic_apply_multi", "ssa_spmv_multi", "ssa_ic_profile_multi",
-    "ssa_ic_set_operator", "ssa_ic_trace", "ssa_ic_trace_read", "ssa_ic_apply_noalloc",
+    "ssa_ic_set_operator", "ssa_ic_trace", "ssa_ic_trace_read", "ssa_ic_apply_noalloc", "ssa_batch_create_each",
 )
 
 PROFILE_SPMV = 0
@@ -76,7 +76,7 @@ class Config(C.Structure):
         ("abmc_num_colors", C.c_int), ("abmc_reorder_spmv", C.c_int),
         ("abmc_use_rcm", C.c_int), ("precision", C.c_int), ("use_graph", C.c_int),
         ("poll_interval", C.c_int), ("walk_overlap", C.c_int),
-        ("preconditioner", C.c_int),
+        ("preconditioner", C.c_int), ("rhs_block", C.c_int),
     ]
 
 
@@ -139,6 +139,8 @@ def load() -> C.CDLL:
                                 C.POINTER(Result), vp]
         L.ssa_ic_profile.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
+        L.ssa_batch_create_each.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(Config),
+                                            vp, vp, C.POINTER(vp)]
         L.ssa_batch_create.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(Config),
                                        C.POINTER(vp)]
         L.ssa_batch_solve.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.POINTER(Config),
@@ -382,7 +384,10 @@ class BatchHandle(IcHandle):
     """Several independent systems in one handle (ssa_batch_create).  Vectors
     hold the systems back to back, each in its own ordering."""
 
-    def __init__(self, systems, cfg: Config, ctx: Context | None = None, host_only: bool = False):
+    def __init__(self, systems, cfg: Config, ctx: Context | None = None, host_only: bool = False,
+                 block_sizes=None, num_colors=None):
+        """block_sizes / num_colors: an ABMC block size / colour count per
+        system (ssa_batch_create_each) instead of cfg's."""
         self.ctx = None if host_only else (ctx or context(0))
         self.lib = load()
         mats = list(systems)
@@ -409,10 +414,25 @@ class BatchHandle(IcHandle):
         ns = np.array(self.sizes, dtype=np.int32)
         arr = C.c_void_p * self.count
         h = C.c_void_p()
-        check(self.lib.ssa_batch_create(self.ctx.h if self.ctx else None,
-                                        SSA_C128 if self.is_complex else SSA_F64, self.count,
-                                        ns.ctypes.data, arr(*rps), arr(*cis), arr(*vals),
-                                        C.byref(cfg), C.byref(h)))
+        if block_sizes is None and num_colors is None:
+            check(self.lib.ssa_batch_create(self.ctx.h if self.ctx else None,
+                                            SSA_C128 if self.is_complex else SSA_F64, self.count,
+                                            ns.ctypes.data, arr(*rps), arr(*cis), arr(*vals),
+                                            C.byref(cfg), C.byref(h)))
+        else:
+            def per_system(v):
+                if v is None:
+                    return None, None
+                a = np.ascontiguousarray(v, dtype=np.int32)
+                if a.shape != (self.count,):
+                    raise ValueError("one block size / colour count per system expected")
+                return a, a.ctypes.data
+            bs, bsp = per_system(block_sizes)
+            nc, ncp = per_system(num_colors)
+            check(self.lib.ssa_batch_create_each(self.ctx.h if self.ctx else None,
+                                                 SSA_C128 if self.is_complex else SSA_F64, self.count,
+                                                 ns.ctypes.data, arr(*rps), arr(*cis), arr(*vals),
+                                                 C.byref(cfg), bsp, ncp, C.byref(h)))
         self.h = h
         self._keep = []
 

@@ -54,7 +54,7 @@ def _fake_outputs():
              "max_rel_error_vs_fp64": 8.8e-11, "fp64_block": {"seconds": 1.67}}
     batch = {"seconds": 0.53, "iterations_max": 309, "all_converged": True,
              "sequential_single_solves_s": 1.71, "setup_s": 3.2, "cpu_baseline": {"value": 8.1},
-             "tuned": {"seconds": 0.52, "abmc_block_size": 2, "abmc_num_colors": 4}}
+             "tuned": {"seconds": 0.52, "per_class": [{"abmc_block_size": 2, "abmc_num_colors": 4}, {"abmc_block_size": 4, "abmc_num_colors": 8}]}}
     extras = {k: bench.summary(k, v) for k, v in (
         ("laplace48", one), ("curlcurl64", one), ("curlcurl64-mixed", one),
         ("eddy78-mixed", one), ("batch64", batch), ("eddy78-lossy-multi16", multi))}

@@ -130,9 +130,54 @@ def test_batch_mixed_precision_matches_single_solves():
 def test_autotune_on_gpu():
     from sparsesolv_amd import autotune as at
     A, b = varcoef(12, 40)
-    r = at.autotune(A, b, block_sizes=(2, 4), colours=(4, 8), reps=1, max_iterations=1000)
-    assert len(r.trials) == 4 and all(t.converged for t in r.trials)
+    # block sizes x colours 4, 8, then the colour counts within 3 of the best
+    r = at.autotune(A, b, block_sizes=(2, 4), colours=tuple(range(4, 10)), reps=1, max_iterations=1000)
+    stage1 = [(t.block_size, t.num_colors) for t in r.trials[:4]]
+    assert stage1 == [(2, 4), (2, 8), (4, 4), (4, 8)]
+    assert len(r.trials) > 4 and all(t.converged for t in r.trials)
     assert r.best.seconds == min(t.seconds for t in r.trials)
     cfg = _lib.default_config(**r.config(max_iterations=1000))
     x, res, _ = _lib.IcHandle(A, cfg).solve(_lib.METHOD_ICCG, b, None, cfg)
     assert res.converged and np.linalg.norm(b - A.to_scipy() @ x) < 1e-9 * np.linalg.norm(b)
+
+
+def test_width_tuning_keeps_the_results():
+    # the right-hand sides per launch change only the launch shape: every
+    # width gives the same bits
+    from sparsesolv_amd import autotune as at
+    A, _ = varcoef(10, 41)
+    B = np.stack([P.uniform(50 + k, A.n, -1.0, 1.0) for k in range(10)])
+    kw = dict(use_abmc=1, max_iterations=1000)
+    h = _lib.IcHandle(A, _lib.default_config(**kw))
+    w = at.autotune_width(h, B, kw, method=_lib.METHOD_ICCG)
+    assert set(w.seconds) == {4, 8, 16} and w.width in w.seconds
+    X = {}
+    for width in (1, 4, 8, 16):
+        X[width], res = h.solve_multi(_lib.METHOD_ICCG, B, None, _lib.default_config(rhs_block=width, **kw))
+        assert all(r.converged for r in res)
+    for width in (4, 8, 16):
+        np.testing.assert_array_equal(X[width], X[1])
+    with pytest.raises(_lib.SsaError):
+        h.solve_multi(_lib.METHOD_ICCG, B, None, _lib.default_config(rhs_block=3, **kw))
+
+
+def test_batch_tuned_per_size_class():
+    # per-system block sizes and colour counts (ssa_batch_create_each): each
+    # system's batched solve is its own solve with its parameters, bit for bit
+    from sparsesolv_amd import autotune as at
+    from types import SimpleNamespace
+    systems = [SimpleNamespace(A=A, b=b) for A, b in real_batch()]
+    kw = dict(use_abmc=1, max_iterations=1000)
+    t = at.autotune_batch(systems, classes=2, block_sizes=(2, 4), colours=(4, 6, 8), refine=0, **kw)
+    assert len(t.classes) == 2
+    pairs = list(zip(t.block_sizes, t.num_colors))
+    cfg = _lib.default_config(**kw)
+    hb = _lib.BatchHandle([s.A for s in systems], cfg, block_sizes=t.block_sizes, num_colors=t.num_colors)
+    b = np.concatenate([s.b for s in systems])
+    x, results = hb.solve_batch(_lib.METHOD_ICCG, b, None, cfg)
+    for k, s in enumerate(systems):
+        bs, nc = pairs[k]
+        c1 = _lib.default_config(abmc_block_size=bs, abmc_num_colors=nc, **kw)
+        xk, rk, _ = _lib.IcHandle(s.A, c1).solve(_lib.METHOD_ICCG, s.b, None, c1)
+        assert results[k].converged and results[k].iterations == rk.iterations
+        np.testing.assert_array_equal(hb.split(x)[k], xk)
