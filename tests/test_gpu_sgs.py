@@ -292,3 +292,47 @@ def test_sgsmrtr_printrates_and_refusals(capsys):
     solver.mixed_precision = False
     with pytest.raises(RuntimeError):
         solver.SolveBlock(np.stack([s.b, s.b]), np.zeros((2, s.A.n)))
+
+
+def test_sgs_blocks_and_batches_match_single_solves():
+    # the SGS factor through the block (several right-hand sides) and batch
+    # paths: each right-hand side / system is its one-vector solve, bit for bit
+    A = varcoef(10)
+    B = np.stack([P.uniform(60 + k, A.n, -1.0, 1.0) for k in range(6)])
+    h, cfg = handle(A, _lib.PRECOND_SGS, max_iterations=1000)
+    X, res = h.solve_multi(_lib.METHOD_ICCG, B, None, cfg)
+    Y = h.apply_multi(B)
+    for k in range(B.shape[0]):
+        x1, r1, _ = h.solve(_lib.METHOD_ICCG, B[k], None, cfg)
+        np.testing.assert_array_equal(X[k], x1)
+        assert res[k].iterations == r1.iterations and res[k].converged
+        np.testing.assert_array_equal(Y[k], h.apply(B[k]))
+    mats = [varcoef(m, 70 + m) for m in (7, 9, 6)]
+    bs = [P.uniform(80 + k, M.n, -1.0, 1.0) for k, M in enumerate(mats)]
+    hb = _lib.BatchHandle(mats, cfg)
+    xb, rb = hb.solve_batch(_lib.METHOD_ICCG, np.concatenate(bs), None, cfg)
+    for k, (M, bk) in enumerate(zip(mats, bs)):
+        xk, rk, _ = _lib.IcHandle(M, cfg).solve(_lib.METHOD_ICCG, bk, None, cfg)
+        np.testing.assert_array_equal(hb.split(xb)[k], xk)
+        assert rb[k].iterations == rk.iterations
+    with pytest.raises(_lib.SsaError):      # SGS-MRTR: one system per handle
+        _lib.BatchHandle(mats, _lib.default_config(preconditioner=_lib.PRECOND_SGSMRTR))
+
+
+def test_sgs_mixed_precision_against_fp64_oracle(oracle):
+    # the SGS factor and sweeps in float inside the FP64 CG / COCR with
+    # refinement: the FP64 oracle's solution to 1e-8
+    A = varcoef(12)
+    b = P.uniform(5, A.n, -1.0, 1.0)
+    h, cfg = handle(A, _lib.PRECOND_SGS, precision=_lib.PRECISION_MIXED, max_iterations=2000)
+    x, res, _ = h.solve(_lib.METHOD_ICCG, b, None, cfg)
+    xo, ro = oracle.solve_sgs(A, b, oracle.config(max_iterations=2000), "CG")
+    assert res.converged and ro["converged"]
+    assert np.linalg.norm(b - A.to_scipy() @ x) / np.linalg.norm(b) < 1e-10
+    assert rel(x, xo) < 1e-8
+    assert res.iterations <= int(1.6 * ro["iterations"]) + 3
+    Z, zb = complex_laplace(8)
+    hz, cz = handle(Z, _lib.PRECOND_SGS, precision=_lib.PRECISION_MIXED, max_iterations=2000)
+    xz, rz, _ = hz.solve(_lib.METHOD_ICCOCR, zb, None, cz)
+    xzo, rzo = oracle.solve_sgs(Z, zb, oracle.config(max_iterations=2000), "COCR")
+    assert rz.converged and rzo["converged"] and rel(xz, xzo) < 1e-8
