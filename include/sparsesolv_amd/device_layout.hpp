@@ -123,44 +123,55 @@ struct SpmvLayout {
     }
 };
 
+// The first row of every system in combine_spmv's layout, and the end.
+template <typename F>
+std::vector<index_t> combined_rows(const std::vector<const SpmvLayout<F>*>& parts, index_t align)
+{
+    std::vector<index_t> row0(1, 0);
+    for (const SpmvLayout<F>* P : parts)
+        row0.push_back(row0.back() + (P->nslices + align - 1) / align * align * kWave);
+    return row0;
+}
+
 // Several systems in one SELL-64 layout: system s occupies the rows
 // [row0[s], row0[s+1]); each system's slice count is padded to a multiple of
 // `align` slices (empty slices) so no group of `align` slices spans two
-// systems.  Columns are shifted by the system's first row.
+// systems.  Columns are shifted by the system's first row.  The sizes are
+// summed first, then the systems are copied side by side.
 template <typename F>
 SpmvLayout<F> combine_spmv(const std::vector<const SpmvLayout<F>*>& parts, index_t align,
                            std::vector<index_t>& row0)
 {
     SpmvLayout<F> out;
-    row0.assign(1, 0);
-    out.slice_off.assign(1, 0);
-    for (const SpmvLayout<F>* P : parts) {
-        const index_t r0 = row0.back();
-        for (index_t s = 0; s < P->nslices; ++s) {
-            const index_t K = P->slice_off[s + 1] - P->slice_off[s];
-            const size_t src = size_t(P->slice_off[s]) * kWave;
-            const size_t dst = size_t(out.slice_off.back()) * kWave;
-            out.val.insert(out.val.end(), P->val.begin() + src, P->val.begin() + src + size_t(K) * kWave);
-            out.col.resize(dst + size_t(K) * kWave);
-            for (size_t q = 0; q < size_t(K) * kWave; ++q) out.col[dst + q] = P->col[src + q] + r0;
-            out.slice_off.push_back(out.slice_off.back() + K);
-            for (int l = 0; l < kWave; ++l) {
-                const index_t r = P->row[size_t(s) * kWave + l];
-                out.row.push_back(r >= 0 ? r + r0 : -1);
-            }
-        }
-        index_t ns = P->nslices;
-        while (ns % align) {
-            out.slice_off.push_back(out.slice_off.back());
-            out.row.insert(out.row.end(), kWave, -1);
-            ++ns;
-        }
-        row0.push_back(r0 + ns * kWave);
+    const size_t np = parts.size();
+    std::vector<index_t> slice0(np + 1, 0);     // first combined slice of each system
+    std::vector<size_t> slot0(np + 1, 0);       // its first entry slot
+    row0 = combined_rows(parts, align);
+    for (size_t i = 0; i < np; ++i) {
+        const SpmvLayout<F>& P = *parts[i];
+        slice0[i + 1] = slice0[i] + (P.nslices + align - 1) / align * align;
+        slot0[i + 1] = slot0[i] + size_t(P.slice_off[P.nslices]);
     }
-    out.nslices = index_t(out.slice_off.size()) - 1;
+    out.nslices = slice0[np];
     out.n = row0.back();
-    if (out.val.empty()) { out.val.assign(kWave, F(0)); out.col.assign(kWave, 0); }
-    if (out.row.empty()) out.row.assign(kWave, -1);
+    out.slice_off.assign(size_t(out.nslices) + 1, 0);
+    out.row.assign(std::max<size_t>(size_t(out.nslices) * kWave, kWave), -1);
+    out.val.resize(std::max<size_t>(slot0[np] * kWave, kWave));
+    out.col.resize(out.val.size());
+    if (slot0[np] == 0) { out.val.assign(kWave, F(0)); out.col.assign(kWave, 0); }
+    parallel_items(index_t(np), [&](index_t i) {
+        const SpmvLayout<F>& P = *parts[size_t(i)];
+        const index_t r0 = row0[size_t(i)];
+        const size_t base = slot0[size_t(i)] * kWave;
+        const size_t len = size_t(P.slice_off[P.nslices]) * kWave;
+        std::copy(P.val.begin(), P.val.begin() + len, out.val.begin() + base);
+        for (size_t q = 0; q < len; ++q) out.col[base + q] = P.col[q] + r0;
+        index_t* so = out.slice_off.data() + slice0[size_t(i)];
+        for (index_t s = slice0[size_t(i)]; s < slice0[size_t(i) + 1]; ++s, ++so)
+            so[1] = index_t(slot0[size_t(i)]) + P.slice_off[std::min(s - slice0[size_t(i)] + 1, P.nslices)];
+        index_t* rw = out.row.data() + size_t(slice0[size_t(i)]) * kWave;
+        for (size_t q = 0; q < size_t(P.nslices) * kWave; ++q) rw[q] = P.row[q] >= 0 ? P.row[q] + r0 : -1;
+    });
     return out;
 }
 
@@ -595,99 +606,133 @@ public:
             }
         }
         const int K = out.walk_kmax;
+        const size_t np = parts.size();
+        const index_t nph = out.nphases;
         out.phase_slices.assign(1, 0);
         out.phase_big.assign(1, 0);
         out.slice_slot.assign(1, 0);
-        out.phase_slot.assign(size_t(out.nphases), 0);
-        out.phase_stride.assign(size_t(out.nphases), 0);
+        out.phase_slot.assign(size_t(nph), 0);
+        out.phase_stride.assign(size_t(nph), 0);
         out.walk_step_off.assign(1, 0);
-        out.walk_ext_rp.assign(1, 0);
-        out.phase_rows.assign(size_t(out.nphases), 0);
-        out.phase_entries.assign(size_t(out.nphases), 0);
-        out.phase_ext_rows.assign(size_t(out.nphases), 0);
-        for (index_t p = 0; p < out.nphases; ++p) {
-            // the phase's stride: the longest slice of any system
+        out.phase_rows.assign(size_t(nph), 0);
+        out.phase_entries.assign(size_t(nph), 0);
+        out.phase_ext_rows.assign(size_t(nph), 0);
+        // Sizes first: phase p holds, system by system, the slices of every
+        // system's phase p (one stride per phase: its longest slice), then
+        // the walk steps of their oversized blocks.  Per (phase, system):
+        // its first combined slice, walk step and external walk entry.
+        std::vector<index_t> first_slice(size_t(nph) * np, 0), first_step(size_t(nph) * np, 0);
+        std::vector<int64_t> first_ext(size_t(nph) * np, 0);
+        index_t nsl_all = 0, steps_all = 0;
+        int64_t ext_all = 0;
+        for (index_t p = 0; p < nph; ++p) {
             index_t stride = 0, nsl = 0;
-            for (const SweepLayout* P : parts) {
-                if (p >= P->nphases) continue;
-                for (index_t sl = P->phase_slices[p]; sl < P->phase_slices[p + 1]; ++sl, ++nsl)
-                    stride = std::max(stride, info_len(P->lane_info[size_t(sl) * kWave]));
+            for (size_t si = 0; si < np; ++si) {
+                const SweepLayout& P = *parts[si];
+                first_slice[size_t(p) * np + si] = nsl_all + nsl;
+                if (p >= P.nphases) continue;
+                for (index_t sl = P.phase_slices[p]; sl < P.phase_slices[p + 1]; ++sl, ++nsl) {
+                    stride = std::max(stride, info_len(P.lane_info[size_t(sl) * kWave]));
+                    slice_sys.push_back(int(si));
+                }
+                out.phase_rows[p] += P.phase_rows[p];
+                out.phase_entries[p] += P.phase_entries[p];
+                out.phase_ext_rows[p] += P.phase_ext_rows[p];
             }
             const index_t first = out.slice_slot.back();
             out.phase_slot[p] = first;
             out.phase_stride[p] = stride;
             out.slice_slot.pop_back();
             for (index_t k = 0; k <= nsl; ++k) out.slice_slot.push_back(first + k * stride);
-            out.val.resize(size_t(first + nsl * stride) * kWave, F(0));
-            out.col.resize(size_t(first + nsl * stride) * kWave, kPadCol);
-            index_t k = 0;
-            for (size_t si = 0; si < parts.size(); ++si) {
+            nsl_all += nsl;
+            out.phase_slices.push_back(nsl_all);
+            for (size_t si = 0; si < np; ++si) {
                 const SweepLayout& P = *parts[si];
+                first_step[size_t(p) * np + si] = steps_all;
+                first_ext[size_t(p) * np + si] = ext_all;
                 if (p >= P.nphases) continue;
-                const index_t r0 = row0[si];
-                out.phase_rows[p] += P.phase_rows[p];
-                out.phase_entries[p] += P.phase_entries[p];
-                out.phase_ext_rows[p] += P.phase_ext_rows[p];
+                for (index_t bb = P.phase_big[p]; bb < P.phase_big[p + 1]; ++bb) {
+                    const index_t st0 = P.walk_step_off[bb], st1 = P.walk_step_off[bb + 1];
+                    steps_all += st1 - st0;
+                    ext_all += P.walk_ext_rp[size_t(st1) * kWave] - P.walk_ext_rp[size_t(st0) * kWave];
+                    out.walk_step_off.push_back(out.walk_step_off.back() + (st1 - st0));
+                    walk_sys.push_back(int(si));
+                }
+            }
+            out.phase_big.push_back(index_t(walk_sys.size()));
+        }
+        const size_t slots = size_t(out.slice_slot.back());
+        out.lane_row.assign(size_t(nsl_all) * kWave, -1);
+        out.lane_info.assign(size_t(nsl_all) * kWave, 0);
+        out.val.assign((slots + kSweepChunk) * kWave, F(0));
+        out.col.assign((slots + kSweepChunk) * kWave, kPadCol);
+        const size_t wslots = size_t(steps_all) * kWave;
+        out.walk_row.assign(wslots, -1);
+        out.walk_pos.assign(wslots, -1);
+        out.walk_invd.assign(wslots, F(0));
+        out.walk_ext_rp.assign(wslots + 1, 0);
+        out.walk_ext_ci.assign(size_t(ext_all), 0);
+        out.walk_ext_v.assign(size_t(ext_all), F(0));
+        out.walk_v.assign(wslots * size_t(K), F(0));
+        out.walk_ix.assign(wslots * size_t(K), out.max_big_rows);
+        // then every system's part, the systems side by side
+        parallel_items(index_t(np), [&](index_t sidx) {
+            const size_t si = size_t(sidx);
+            const SweepLayout& P = *parts[si];
+            const index_t r0 = row0[si];
+            for (index_t p = 0; p < std::min(nph, P.nphases); ++p) {
+                index_t k = first_slice[size_t(p) * np + si];
                 for (index_t sl = P.phase_slices[p]; sl < P.phase_slices[p + 1]; ++sl, ++k) {
                     const index_t Ks = info_len(P.lane_info[size_t(sl) * kWave]);
                     const size_t src = size_t(P.slice_slot[sl]) * kWave;
-                    const size_t dst = size_t(first + k * stride) * kWave;
+                    const size_t dst = size_t(out.slice_slot[k]) * kWave;
                     for (size_t q = 0; q < size_t(Ks) * kWave; ++q) {
                         const index_t c = P.col[src + q];
                         out.col[dst + q] = c >= 0 ? c + r0 : c;
                         out.val[dst + q] = P.val[src + q];
                     }
                     for (int l = 0; l < kWave; ++l) {
-                        const index_t r = P.lane_row[size_t(sl) * kWave + l];
-                        out.lane_row.push_back(r >= 0 ? r + r0 : r);
-                        out.lane_info.push_back(P.lane_info[size_t(sl) * kWave + l]);
+                        const size_t a = size_t(sl) * kWave + l, b = size_t(k) * kWave + l;
+                        const index_t r = P.lane_row[a];
+                        out.lane_row[b] = r >= 0 ? r + r0 : r;
+                        out.lane_info[b] = P.lane_info[a];
                     }
-                    slice_sys.push_back(int(si));
                 }
-            }
-            out.phase_slices.push_back(index_t(slice_sys.size()));
-            for (size_t si = 0; si < parts.size(); ++si) {
-                const SweepLayout& P = *parts[si];
-                if (p >= P.nphases) continue;
-                const index_t r0 = row0[si];
-                for (index_t bb = P.phase_big[p]; bb < P.phase_big[p + 1]; ++bb) {
-                    for (index_t st = P.walk_step_off[bb]; st < P.walk_step_off[bb + 1]; ++st) {
+                if (P.phase_big[p + 1] == P.phase_big[p]) continue;
+                const index_t st0 = P.walk_step_off[P.phase_big[p]];
+                const index_t st1 = P.walk_step_off[P.phase_big[p + 1]];
+                const size_t t0 = size_t(st0) * kWave;
+                const size_t g0 = size_t(first_step[size_t(p) * np + si]) * kWave;
+                const int64_t e0 = P.walk_ext_rp[t0];
+                const int64_t ge = first_ext[size_t(p) * np + si];
+                for (size_t t = t0; t < size_t(st1) * kWave; ++t) {
+                    const size_t g = g0 + (t - t0);
+                    const index_t r = P.walk_row[t];
+                    out.walk_row[g] = r >= 0 ? r + r0 : r;
+                    out.walk_pos[g] = P.walk_pos[t];
+                    out.walk_invd[g] = P.walk_invd[t];
+                    for (index_t e = P.walk_ext_rp[t]; e < P.walk_ext_rp[t + 1]; ++e) {
+                        out.walk_ext_ci[size_t(ge + (e - e0))] = P.walk_ext_ci[e] + r0;
+                        out.walk_ext_v[size_t(ge + (e - e0))] = P.walk_ext_v[e];
+                    }
+                    out.walk_ext_rp[g + 1] = index_t(ge + (P.walk_ext_rp[t + 1] - e0));
+                }
+                for (index_t st = st0; st < st1; ++st) {
+                    const size_t vbase = (size_t(first_step[size_t(p) * np + si] + (st - st0))) * size_t(K) * kWave;
+                    for (int k = 0; k < P.walk_kmax; ++k)
                         for (int l = 0; l < kWave; ++l) {
-                            const size_t t = size_t(st) * kWave + l;
-                            const index_t r = P.walk_row[t];
-                            out.walk_row.push_back(r >= 0 ? r + r0 : r);
-                            out.walk_pos.push_back(P.walk_pos[t]);
-                            out.walk_invd.push_back(P.walk_invd[t]);
-                            for (index_t e = P.walk_ext_rp[t]; e < P.walk_ext_rp[t + 1]; ++e) {
-                                out.walk_ext_ci.push_back(P.walk_ext_ci[e] + r0);
-                                out.walk_ext_v.push_back(P.walk_ext_v[e]);
-                            }
-                            out.walk_ext_rp.push_back(index_t(out.walk_ext_ci.size()));
+                            const size_t a = (size_t(st) * P.walk_kmax + k) * kWave + l;
+                            const index_t ix = P.walk_ix[a];
+                            out.walk_v[vbase + size_t(k) * kWave + l] = P.walk_v[a];
+                            // the system's zero slot becomes the combined one
+                            out.walk_ix[vbase + size_t(k) * kWave + l] =
+                                ix == P.max_big_rows ? out.max_big_rows : ix;
                         }
-                        const size_t vbase = out.walk_v.size();
-                        out.walk_v.resize(vbase + size_t(K) * kWave, F(0));
-                        out.walk_ix.resize(vbase + size_t(K) * kWave, out.max_big_rows);
-                        for (int k = 0; k < P.walk_kmax; ++k)
-                            for (int l = 0; l < kWave; ++l) {
-                                const size_t a = (size_t(st) * P.walk_kmax + k) * kWave + l;
-                                const index_t ix = P.walk_ix[a];
-                                out.walk_v[vbase + size_t(k) * kWave + l] = P.walk_v[a];
-                                // the system's zero slot becomes the combined one
-                                out.walk_ix[vbase + size_t(k) * kWave + l] =
-                                    ix == P.max_big_rows ? out.max_big_rows : ix;
-                            }
-                    }
-                    out.walk_step_off.push_back(out.walk_step_off.back() +
-                                                (P.walk_step_off[bb + 1] - P.walk_step_off[bb]));
-                    walk_sys.push_back(int(si));
                 }
             }
-            out.phase_big.push_back(index_t(walk_sys.size()));
-        }
+        });
         if (any_big)
             for (int k = 0; k < kWalkDepth; ++k) out.push_idle_step();
-        out.val.resize(out.val.size() + size_t(kSweepChunk) * kWave, F(0));
-        out.col.resize(out.col.size() + size_t(kSweepChunk) * kWave, kPadCol);
         out.big_pos.assign(size_t(walk_sys.size()) + 1, 0);
         out.big_lvl_off.assign(size_t(walk_sys.size()) + 1, 0);
         out.big_lvl.assign(1, 0);

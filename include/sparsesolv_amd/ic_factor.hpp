@@ -134,6 +134,24 @@ inline void parallel_chunks(index_t count, index_t grain, Fn fn)
         if (e) std::rethrow_exception(e);
 }
 
+// fn(i) for every i in [0, count), the items handed out one at a time in
+// increasing order to up to setup_threads() threads (items of uneven cost:
+// put the costly ones first).  fn must not throw.
+template <typename Fn>
+inline void parallel_items(index_t count, Fn fn)
+{
+    const index_t nt = std::max<index_t>(1, std::min<index_t>(setup_threads(), count));
+    std::atomic<index_t> next{0};
+    auto run = [&] {
+        for (index_t i; (i = next.fetch_add(1)) < count;) fn(i);
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(size_t(nt));
+    for (index_t t = 1; t < nt; ++t) pool.emplace_back(run);
+    run();
+    for (auto& th : pool) th.join();
+}
+
 // B[perm i][perm j] = A[i][j], each row sorted by column (unique columns).
 template <typename T>
 CsrHost<T> permute_csr(index_t n, const index_t* rp, const index_t* ci, const T* v,

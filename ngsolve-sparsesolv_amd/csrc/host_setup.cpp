@@ -197,6 +197,13 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
     std::vector<HostPlan<T, F>> plans(static_cast<size_t>(count));
     // independent systems: planned on host threads, errors re-raised in order
     std::vector<std::string> errors(static_cast<size_t>(count));
+    // largest systems first (by entries), so the last to finish is a small
+    // one; the plans keep the caller's order
+    std::vector<int> queue(static_cast<size_t>(count));
+    for (int s = 0; s < count; ++s) queue[size_t(s)] = s;
+    std::stable_sort(queue.begin(), queue.end(), [&](int a, int b) {
+        return row_ptr[a][n[a]] > row_ptr[b][n[b]];
+    });
     std::atomic<int> next{0};
     auto work = [&] {
         // the systems run side by side: each plans on one thread (the cap
@@ -204,7 +211,8 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
         int& cap = sparsesolv_amd::setup_thread_cap();
         const int saved = cap;
         if (count > 1) cap = 1;
-        for (int s; (s = next.fetch_add(1)) < count;) {
+        for (int k; (k = next.fetch_add(1)) < count;) {
+            const int s = queue[size_t(k)];
             try {
                 sparsesolv_amd::IcSettings own = settings;
                 if (settings.batch_block_size) own.abmc_block_size = settings.batch_block_size[s];
@@ -262,9 +270,30 @@ void build_batch(BatchPlan<T, F>& bp, int count, const int* n, const int* const*
             lf.push_back(&P.fwd);
             lb.push_back(&P.bwd);
         }
-        bp.spmv = sparsesolv_amd::combine_spmv(sp, 4, bp.row0);
-        bp.fwd = sparsesolv_amd::SweepLayout<F>::combine(lf, bp.row0, bp.slice_sys_f, bp.walk_sys_f);
-        bp.bwd = sparsesolv_amd::SweepLayout<F>::combine(lb, bp.row0, bp.slice_sys_b, bp.walk_sys_b);
+        // the three layouts side by side (each also copies its systems on
+        // threads of its own: much of a combine is first-touch of new memory)
+        bp.row0 = sparsesolv_amd::combined_rows(sp, 4);
+        std::exception_ptr err_f, err_b;
+        std::thread tf([&] {
+            try {
+                bp.fwd = sparsesolv_amd::SweepLayout<F>::combine(lf, bp.row0, bp.slice_sys_f, bp.walk_sys_f);
+            } catch (...) {
+                err_f = std::current_exception();
+            }
+        });
+        std::thread tb([&] {
+            try {
+                bp.bwd = sparsesolv_amd::SweepLayout<F>::combine(lb, bp.row0, bp.slice_sys_b, bp.walk_sys_b);
+            } catch (...) {
+                err_b = std::current_exception();
+            }
+        });
+        std::vector<int> rows;
+        bp.spmv = sparsesolv_amd::combine_spmv(sp, 4, rows);
+        tf.join();
+        tb.join();
+        if (err_f) std::rethrow_exception(err_f);
+        if (err_b) std::rethrow_exception(err_b);
         const int N = bp.row0.back();
         bp.invd.assign(size_t(N), F(0));
         bp.gather_in.assign(size_t(N), -1);
