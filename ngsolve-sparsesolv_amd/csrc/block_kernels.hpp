@@ -6,7 +6,8 @@
 // right-hand side r in workgroup g, the contributions of rows g*kBlock + t
 // (stepping by G*kBlock) in thread t, in the same order, with block_sum's
 // tree -- as workgroup g of the one-vector kernel does -- so a block solve
-// reproduces the one-vector solves bit for bit (tile_reduce).  Element-wise
+// reproduces the one-vector solves bit for bit (tile_accumulate,
+// tile_finish).  Element-wise
 // kernels take the flat, coalesced mapping (thread e, element e; G*kBlock is
 // a multiple of W, so a thread keeps one right-hand side).  The control steps
 // run as W one-workgroup kernels between producer and consumer: a fused
@@ -23,10 +24,14 @@ namespace ssa {
 // thread t owns rows g*kBlock + t (stepping by G*kBlock) and adds their
 // contributions in that order, then block_sum combines the threads.  The
 // memory side stays coalesced: a tile of kBlock rows is processed element by
-// element (consecutive threads, consecutive elements), each contribution is
-// written to an LDS tile, and after a barrier thread t adds the
-// contributions of its row.  Columns go CP at a time (the tile is
-// kBlock x (CP+1) values, one pad column against bank conflicts).
+// element (consecutive threads, consecutive elements), CP columns at a time.
+// In pass h a thread keeps one column, h*CP + (t & (CP-1)), and the rows
+// j*(kBlock/CP) + t/CP (j < CP) of every tile; it accumulates each
+// (row, column) pair in its own register, acc[h*CP + j], tile after tile --
+// the owner's sequence of additions, done by another thread.  tile_finish
+// hands every sum to its owning thread through LDS once at the end (a
+// kBlock x (CP+1) tile per pass, one pad column against bank conflicts)
+// instead of once per tile.
 template <typename V, int W>
 constexpr int pass_cols()
 {
@@ -34,11 +39,23 @@ constexpr int pass_cols()
     return W <= lim ? W : lim >= 8 ? 8 : lim >= 4 ? 4 : 1;
 }
 
+// columns per exchange in tile_finish (an LDS tile of kBlock x (FK+1)
+// values, at most 10 KB for FP64 and 20 KB for complex<double>)
+template <typename V, int W>
+constexpr int finish_cols()
+{
+    constexpr int CP = pass_cols<V, W>();
+    constexpr int lim = sizeof(V) >= 16 ? 2 : sizeof(V) >= 8 ? 4 : 8;
+    return CP < lim ? CP : lim;
+}
+template <typename V, int W>
+constexpr int tile_words() { return kBlock * (finish_cols<V, W>() + 1); }
+
 // One tile: rows [base, base + rows) (rows <= kBlock).  fn(e, h) does the
 // element-wise work of element e (index into the interleaved vectors; its
 // column is h*CP + (threadIdx.x & (CP-1))) and returns the contribution.
 template <typename V, int W, typename Fn>
-__device__ inline void tile_reduce(index_t base, index_t rows, V* tile, V (&s)[W], Fn&& fn)
+__device__ inline void tile_accumulate(index_t base, index_t rows, V (&acc)[W], Fn&& fn)
 {
     constexpr int CP = pass_cols<V, W>();
     const int t = threadIdx.x;
@@ -47,18 +64,38 @@ __device__ inline void tile_reduce(index_t base, index_t rows, V* tile, V (&s)[W
         const int c = h * CP + (t & (CP - 1));
 #pragma unroll
         for (int j = 0; j < CP; ++j) {
-            const int el = j * kBlock + t;
-            const int lr = el / CP;
-            V v = zero<V>();
-            if (lr < rows) v = fn(size_t(base + lr) * W + c, h);
-            tile[lr * (CP + 1) + (el & (CP - 1))] = v;
+            const int lr = (j * kBlock + t) / CP;
+            if (lr < rows) acc[h * CP + j] = add(acc[h * CP + j], fn(size_t(base + lr) * W + c, h));
         }
-        __syncthreads();
-        if (t < rows) {
+    }
+}
+
+// s[c] of thread t <- the sum of row t, column c (tile_accumulate's acc of
+// the thread that kept it), FK columns per exchange; `tile` holds
+// tile_words<V, W>() values
+template <typename V, int W>
+__device__ inline void tile_finish(const V (&acc)[W], V* tile, V (&s)[W])
+{
+    constexpr int CP = pass_cols<V, W>();
+    constexpr int FK = finish_cols<V, W>();
+    const int t = threadIdx.x;
+    const int ct = t & (CP - 1);
 #pragma unroll
-            for (int k = 0; k < CP; ++k) s[h * CP + k] = add(s[h * CP + k], tile[t * (CP + 1) + k]);
+    for (int h = 0; h < W / CP; ++h) {
+#pragma unroll
+        for (int k0 = 0; k0 < CP; k0 += FK) {
+            if (ct >= k0 && ct < k0 + FK) {
+#pragma unroll
+                for (int j = 0; j < CP; ++j) {
+                    const int lr = (j * kBlock + t) / CP;
+                    tile[lr * (FK + 1) + (ct - k0)] = acc[h * CP + j];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < FK; ++k) s[h * CP + k0 + k] = tile[t * (FK + 1) + k];
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -77,16 +114,17 @@ __global__ __launch_bounds__(kBlock) void k_dot_blk(index_t n, const TA* __restr
                                                    T* __restrict__ partials)
 {
     __shared__ T red[kWavesPerBlock * W];
-    __shared__ T tile[kBlock * (pass_cols<T, W>() + 1)];
-    T d[W];
+    __shared__ T tile[tile_words<T, W>()];
+    T acc[W], d[W];
 #pragma unroll
-    for (int r = 0; r < W; ++r) d[r] = zero<T>();
+    for (int r = 0; r < W; ++r) acc[r] = zero<T>();
     for (index_t base = index_t(blockIdx.x) * kBlock; base < n; base += index_t(gridDim.x) * kBlock) {
         const index_t rows = n - base < kBlock ? n - base : kBlock;
-        tile_reduce<T, W>(base, rows, tile, d, [&](size_t e, int) {
+        tile_accumulate<T, W>(base, rows, acc, [&](size_t e, int) {
             return mul(conj_if(cvt<T>(a[e]), conj != 0), cvt<T>(b[e]));
         });
     }
+    tile_finish<T, W>(acc, tile, d);
     const T t = block_sum_multi<T, W>(d, red);
     if (threadIdx.x < W) partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
@@ -99,15 +137,17 @@ __global__ __launch_bounds__(kBlock) void k_norms_blk(index_t n, const T* __rest
                                                      double* __restrict__ pb)
 {
     __shared__ double red[kWavesPerBlock * W];
-    __shared__ double tile[kBlock * (pass_cols<double, W>() + 1)];
-    double sa[W], sb[W];
+    __shared__ double tile[tile_words<double, W>()];
+    double aa[W], ab[W], sa[W], sb[W];
 #pragma unroll
-    for (int r = 0; r < W; ++r) sa[r] = sb[r] = 0.0;
+    for (int r = 0; r < W; ++r) aa[r] = ab[r] = 0.0;
     for (index_t base = index_t(blockIdx.x) * kBlock; base < n; base += index_t(gridDim.x) * kBlock) {
         const index_t rows = n - base < kBlock ? n - base : kBlock;
-        tile_reduce<double, W>(base, rows, tile, sa, [&](size_t e, int) { return norm2(a[e]); });
-        if (b) tile_reduce<double, W>(base, rows, tile, sb, [&](size_t e, int) { return norm2(b[e]); });
+        tile_accumulate<double, W>(base, rows, aa, [&](size_t e, int) { return norm2(a[e]); });
+        if (b) tile_accumulate<double, W>(base, rows, ab, [&](size_t e, int) { return norm2(b[e]); });
     }
+    tile_finish<double, W>(aa, tile, sa);
+    if (b) tile_finish<double, W>(ab, tile, sb);
     double t = block_sum_multi<double, W>(sa, red);
     if (threadIdx.x < W) pa[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
     if (b) {
@@ -129,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_blk(index_t n, T* __restri
     if (all_stopped<W>(st)) return;
     __shared__ double red[kWavesPerBlock * W];
     constexpr int CP = pass_cols<double, W>();
-    __shared__ double tile[kBlock * (CP + 1)];
+    __shared__ double tile[tile_words<double, W>()];
     // the column a thread updates in each pass, its alpha and whether it runs
     T alpha[W / CP];
     bool run[W / CP];
@@ -139,12 +179,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_blk(index_t n, T* __restri
         run[h] = st[c].status == 0;
         alpha[h] = from_pair<T>(st[c].alpha);
     }
-    double s[W];
+    double acc[W], s[W];
 #pragma unroll
-    for (int q = 0; q < W; ++q) s[q] = 0.0;
+    for (int q = 0; q < W; ++q) acc[q] = 0.0;
     for (index_t base = index_t(blockIdx.x) * kBlock; base < n; base += index_t(gridDim.x) * kBlock) {
         const index_t rows = n - base < kBlock ? n - base : kBlock;
-        tile_reduce<double, W>(base, rows, tile, s, [&](size_t e, int h) {
+        tile_accumulate<double, W>(base, rows, acc, [&](size_t e, int h) {
             if (!run[h]) return 0.0;
             x[e] = fma_(alpha[h], p[e], x[e]);
             const T ri = fms_(alpha[h], Ap[e], r[e]);
@@ -153,6 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_blk(index_t n, T* __restri
             return norm2(ri);
         });
     }
+    tile_finish<double, W>(acc, tile, s);
     const double t = block_sum_multi<double, W>(s, red);
     if (threadIdx.x < W) partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
@@ -180,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
     if (all_stopped<W>(st)) return;
     __shared__ double red[kWavesPerBlock * W];
     constexpr int CP = pass_cols<double, W>();
-    __shared__ double tile[kBlock * (CP + 1)];
+    __shared__ double tile[tile_words<double, W>()];
     T alpha[W / CP], beta[W / CP];
     bool run[W / CP], copy[W / CP];
 #pragma unroll
@@ -191,12 +232,12 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
         alpha[h] = from_pair<T>(st[c].alpha);
         beta[h] = from_pair<T>(st[c].beta);
     }
-    double s[W];
+    double acc[W], s[W];
 #pragma unroll
-    for (int k = 0; k < W; ++k) s[k] = 0.0;
+    for (int k = 0; k < W; ++k) acc[k] = 0.0;
     for (index_t base = index_t(blockIdx.x) * kBlock; base < n; base += index_t(gridDim.x) * kBlock) {
         const index_t rows = n - base < kBlock ? n - base : kBlock;
-        tile_reduce<double, W>(base, rows, tile, s, [&](size_t e, int h) {
+        tile_accumulate<double, W>(base, rows, acc, [&](size_t e, int h) {
             if (!run[h]) return 0.0;
             const T rte = cvt<T>(rt[e]);
             const T pe = fma_(beta[h], p[e], rte);
@@ -210,6 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
             return norm2(ri);
         });
     }
+    tile_finish<double, W>(acc, tile, s);
     const double t = block_sum_multi<double, W>(s, red);
     if (threadIdx.x < W) partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
