@@ -4,7 +4,8 @@ test_oracle_tiny.py): one and two rows, and row counts on either side of
 the 64-row slice and the 256-row workgroup, through every entry point that
 takes a system -- solves on the ABMC and level paths, mixed precision,
 COCR, SGS-MRTR, the SpMV, blocks of right-hand sides and a batch of such
-systems (each equal to its own solve bit for bit).  An empty system solves
+systems (each equal to its own solve bit for bit), and the preconditioner
+with clamped pivots.  An empty system solves
 as the oracle's does (converged after the one breakdown check, x empty);
 the reference itself does not accept one."""
 import math
@@ -114,3 +115,17 @@ def test_empty_system_solves_as_the_oracle(oracle):
     xo, ro = oracle.solve_iccg(A, np.zeros(0), oracle.config(use_abmc=1))
     assert x.shape == (0,)
     assert res.converged == ro["converged"] and res.iterations == ro["iterations"]
+
+
+@pytest.mark.parametrize("use_abmc", [1, 0])
+def test_clamped_pivots_apply_matches_oracle(oracle, use_abmc):
+    # every second pivot of uncoupled [[1, 1], [1, 1]] blocks at shift 1.0
+    # vanishes and is clamped to 1e-10 (inverse 1e10)
+    import scipy.sparse as sp
+    A = P.CSR.from_scipy(sp.kron(sp.identity(3000), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr())
+    cfg = _lib.default_config(use_abmc=use_abmc, shift_parameter=1.0)
+    x = rhs(A.n, 7)
+    y = _lib.IcHandle(A, cfg).apply(x)
+    want = oracle.ic(A, oracle.config(use_abmc=use_abmc, shift_parameter=1.0)).apply(x)
+    assert np.max(np.abs(want)) > 1e9
+    assert rel(y, want) < 1e-12

@@ -28,6 +28,11 @@ CASES = [
     ("pairs_restart", lambda: P.CSR.from_scipy(
         sp.kron(sp.identity(10000), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr()),
      dict(shift_parameter=1.0, auto_shift=1)),
+    # the same pairs without the auto-shift: every second pivot is 0 and is
+    # clamped to zero_diagonal_replacement (ic_preconditioner.hpp:1261-1267)
+    ("pairs_clamp", lambda: P.CSR.from_scipy(
+        sp.kron(sp.identity(3000), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr()),
+     dict(shift_parameter=1.0)),
 ]
 
 

@@ -1,7 +1,8 @@
 """The oracle on tiny and ragged systems, bit for bit against the
 reference's headers compiled here (oracle/_ref): one and two rows, and row
 counts on either side of the 64-row slice and the 256-row workgroup, for
-ICCG on the ABMC and level paths, COCR, SGS and SGS-MRTR.  (An empty
+ICCG on the ABMC and level paths, COCR, SGS and SGS-MRTR; and the
+pivot clamp on a system whose pivots vanish.  (An empty
 system is outside what the reference accepts: its SparseMatrixView asserts
 non-null column indices.)"""
 import numpy as np
@@ -37,3 +38,22 @@ def test_tiny_systems_match_reference_build(oracle, ref, name):
     r = same(oracle, ref, "solve_cocr", Z, z, use_abmc=1, abmc_num_colors=8)
     assert r["converged"]
     np.testing.assert_array_equal(oracle.sgs_apply(Z, z), ref.sgs_apply(Z, z))
+
+
+def test_pivot_clamp_matches_reference_build(oracle, ref):
+    # uncoupled 2x2 blocks [[1, 1], [1, 1]] at shift 1.0 without the
+    # auto-shift: every second pivot is 1 - 1 = 0, clamped to
+    # zero_diagonal_replacement (ic_preconditioner.hpp:1261-1267)
+    import scipy.sparse as sp
+    from sparsesolv_amd import problems as P
+    A = P.CSR.from_scipy(sp.kron(sp.identity(300), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr())
+    for abmc in (1, 0):
+        fo = oracle.ic(A, oracle.config(use_abmc=abmc, shift_parameter=1.0))
+        fr = ref.ic(A, ref.config(use_abmc=abmc, shift_parameter=1.0))
+        a, b = fo.factor(), fr.factor()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])
+        d = a["L_values"][a["L_row_ptr"][1:] - 1]
+        assert np.sum(d == 1e-10) == 300
+        x = rhs(A.n, 7)
+        np.testing.assert_array_equal(fo.apply(x), fr.apply(x))
