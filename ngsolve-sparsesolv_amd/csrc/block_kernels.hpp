@@ -107,9 +107,13 @@ __device__ inline int pass_column(int h)
     return h * CP + int(threadIdx.x & (CP - 1));
 }
 
-// partial sums of conj?(a_i) b_i, accumulated in T (operands converted)
+// partial sums of conj?(a_i) b_i, accumulated in T (operands converted).
+// Complex at W = 16: held to 128 VGPRs (4 waves per SIMD, so the kMaxGrid
+// workgroups are resident at once; 130 VGPRs gave 3 and a second, quarter
+// full round): eddy78 (q, qt) dot 134 -> 120 us
 template <typename T, typename TA, typename TB, int W>
-__global__ __launch_bounds__(kBlock) void k_dot_blk(index_t n, const TA* __restrict__ a,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(is_cplx<T>::value && W >= 16 ? 4 : 1))) void k_dot_blk(index_t n, const TA* __restrict__ a,
                                                    const TB* __restrict__ b, int conj,
                                                    T* __restrict__ partials)
 {

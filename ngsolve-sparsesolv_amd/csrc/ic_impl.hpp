@@ -85,11 +85,17 @@ constexpr int kMaxGrid = 1024;  // vector kernels: partial count bound
 // small-colour kernel (register budget of its large workgroup); with W
 // right-hand sides per lane at most 8, with at most 64 gathers (a
 // wavefront's limit of outstanding loads) and 512 bytes of gathered values
-// per lane in flight.  The layouts pad their entry arrays by kSweepChunk.
+// per lane in flight -- 256 bytes at W = 16 with 8-byte values: a batch of
+// four 16-value complex<float> rows took 202 VGPRs (2 waves per SIMD, so a
+// 600-workgroup colour ran in more than one round), two take 126-143 (3-4
+// waves; eddy78 colour kernel 32.5 -> 31.9 us).  Complex<double> keeps 2
+// (one entry a batch was slower: FP64 block 3.23 -> 3.29 s).  The layouts
+// pad their entry arrays by kSweepChunk.
 template <typename P, int W> constexpr int sweep_chunk()
 {
     if (W == 1) return sizeof(P) > 8 ? 8 : 16;
-    constexpr int by_count = 64 / W, by_bytes = int(512 / (W * sizeof(P)));
+    constexpr int budget = (W >= 16 && sizeof(P) == 8) ? 256 : 512;
+    constexpr int by_count = 64 / W, by_bytes = int(budget / (W * sizeof(P)));
     constexpr int c = by_count < by_bytes ? by_count : by_bytes;
     return c > 8 ? 8 : c < 1 ? 1 : c;
 }
