@@ -688,7 +688,7 @@ struct IcImpl final : IcBase {
     // neighbours: no oversized block, at most two passes of the workgroup
     int tail_threads(int BR) const
     {
-        if (W > 1) return ssa::tail_threads<P, 1, 4>();
+        if (W > 1) return BR >= 16 ? ssa::tail_threads<P, 16, 4>() : ssa::tail_threads<P, 1, 4>();
         return BR >= 16 ? ssa::tail_threads<P, 16>() : ssa::tail_threads<P, 1>();
     }
 

@@ -469,9 +469,18 @@ __global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, index_
 // wavefronts share each colour's slices, a workgroup barrier separates the
 // colours (global writes of a workgroup are visible to it after the
 // barrier).  NT = 1024 for real values, 512 for complex values or 16-row
-// blocks, 256 with several right-hand sides per lane (register budget).
+// blocks; with several right-hand sides per lane 512 (at most 256 VGPRs,
+// which those kernels fit without spills), and 256 for complex<double>
+// values or 16-row blocks (register budget).  A colour of a 16-vector
+// block is a chain of dependent loads of ~15 us per pass of the
+// workgroup's wavefronts: at 256 threads the eddy78 tail (9 slices) took
+// three passes.
 template <typename P, int BR, int W = 1>
-constexpr int tail_threads() { return W > 1 ? 256 : (sizeof(P) > 8 || BR >= 16) ? 512 : 1024; }
+constexpr int tail_threads()
+{
+    if (W > 1) return (sizeof(P) > 8 || BR >= 16) ? 256 : 512;
+    return (sizeof(P) > 8 || BR >= 16) ? 512 : 1024;
+}
 constexpr int kTailMaxPhases = 16;
 struct TailPhases {
     int count;
