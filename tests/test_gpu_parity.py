@@ -236,6 +236,35 @@ def test_initial_guess_and_zero_rhs():
     assert r0.converged and np.all(x0 == 0)
 
 
+@pytest.mark.parametrize("norm_type,custom", [(0, 1.0), (1, 1.0), (2, 3.5), (2, 1e-3)])
+@pytest.mark.parametrize("case", ["lap", "zlap"])
+def test_norm_types_with_initial_guess_match_oracle(oracle, case, norm_type, custom):
+    # NormType RHS / InitialResidual / Custom (core/types.hpp:42-46,
+    # iterative_solver.hpp:155-160): the normaliser of the convergence test
+    # and of the residual history; the oracle is pinned to the reference
+    # build on the same cases (test_oracle_golden.py)
+    if case == "lap":
+        s = P.poisson(9)
+    else:
+        A, b = complex_shifted_laplace(9)
+        s = P.System(A, b, "zlap9", {})
+    method = _lib.METHOD_ICCG if case == "lap" else _lib.METHOD_ICCOCR
+    kw = dict(use_abmc=1, abmc_num_colors=4 if case == "lap" else 8, norm_type=norm_type,
+              custom_norm=custom, save_residual_history=1, max_iterations=2000)
+    x0 = P.uniform(21, s.A.n, -1.0, 1.0)
+    if s.A.is_complex:
+        x0 = x0 + 1j * P.uniform(22, s.A.n, -1.0, 1.0)
+    cfg = _lib.default_config(**kw)
+    x, res, hist = _lib.IcHandle(s.A, cfg).solve(method, s.b, x0, cfg)
+    fn = oracle.solve_iccg if case == "lap" else oracle.solve_cocr
+    xo, ro = fn(s.A, s.b, oracle.config(**kw), x0=x0)
+    assert bool(res.converged) == ro["converged"]
+    assert abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+    assert rel(x, xo) < 1e-9
+    m = min(len(hist), len(ro["residual_history"]), 6)
+    np.testing.assert_allclose(hist[:m], ro["residual_history"][:m], rtol=1e-10)
+
+
 def test_maxiter_zero():
     s = P.poisson(6)
     cfg = _lib.default_config(use_abmc=1, max_iterations=0)

@@ -130,3 +130,31 @@ def test_oracle_sgs_matches_reference_build(oracle, ref, case):
         xr, rr = ref.solve_sgs(A, b, ref.config(**cfg), method)
         assert ro["iterations"] == rr["iterations"]
         np.testing.assert_array_equal(xo, xr)
+
+
+@pytest.mark.parametrize("norm_type,custom", [(1, 1.0), (2, 3.5), (2, 1e-3)])
+@pytest.mark.parametrize("case", ["lap", "zlap", "eddy"])
+def test_oracle_norm_types_match_reference_build(oracle, ref, case, norm_type, custom):
+    # NormType::InitialResidual / Custom (core/types.hpp:42-46,
+    # iterative_solver.hpp:155-160) with a nonzero initial guess, so that
+    # ||r_0|| differs from ||b||
+    if case == "lap":
+        s = P.poisson(9)
+    elif case == "eddy":
+        s = P.eddy(5)
+    else:
+        import scipy.sparse as sp
+        A = P.laplace7(9).to_scipy().astype(complex) + sp.identity(9 ** 3) * 0.3j
+        z = P.gaussian(6, 2 * 9 ** 3)
+        s = P.System(P.CSR.from_scipy(A), z[0::2] + 1j * z[1::2], "zlap9", {})
+    fn = "solve_iccg" if case == "lap" else "solve_cocr"
+    kw = dict(use_abmc=1, abmc_num_colors=4 if case == "lap" else 8, norm_type=norm_type,
+              custom_norm=custom, save_residual_history=1, max_iterations=2000)
+    x0 = P.uniform(21, s.A.n, -1.0, 1.0)
+    if s.A.is_complex:
+        x0 = x0 + 1j * P.uniform(22, s.A.n, -1.0, 1.0)
+    xo, ro = getattr(oracle, fn)(s.A, s.b, oracle.config(**kw), x0=x0)
+    xr, rr = getattr(ref, fn)(s.A, s.b, ref.config(**kw), x0=x0)
+    assert ro["iterations"] == rr["iterations"] and ro["converged"] == rr["converged"]
+    np.testing.assert_array_equal(ro["residual_history"], rr["residual_history"])
+    np.testing.assert_array_equal(xo, xr)
