@@ -33,6 +33,19 @@ CASES = [
     ("pairs_clamp", lambda: P.CSR.from_scipy(
         sp.kron(sp.identity(3000), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr()),
      dict(shift_parameter=1.0)),
+    # the auto-shift's own settings away from their defaults
+    # (core/solver_config.hpp:81-100): a larger first increment, a lower
+    # shift cap, a looser pivot test and a trial limit that ends the search
+    ("cc6_auto_params", lambda: P.curlcurl(6).A,
+     dict(auto_shift=1, shift_parameter=0.5, shift_increment=0.2, max_shift_value=3.0,
+          min_diagonal_threshold=1e-3, max_shift_trials=3)),
+    ("pairs_auto_params", lambda: P.CSR.from_scipy(
+        sp.kron(sp.identity(300), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr()),
+     dict(shift_parameter=1.0, auto_shift=1, shift_increment=0.01, max_shift_value=1.02,
+          max_shift_trials=10)),
+    ("pairs_clamp_repl", lambda: P.CSR.from_scipy(
+        sp.kron(sp.identity(300), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr()),
+     dict(shift_parameter=1.0, zero_diagonal_replacement=1e-4)),
 ]
 
 

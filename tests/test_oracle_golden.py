@@ -158,3 +158,54 @@ def test_oracle_norm_types_match_reference_build(oracle, ref, case, norm_type, c
     assert ro["iterations"] == rr["iterations"] and ro["converged"] == rr["converged"]
     np.testing.assert_array_equal(ro["residual_history"], rr["residual_history"])
     np.testing.assert_array_equal(xo, xr)
+
+
+@pytest.mark.parametrize("case", ["cc6_params", "pairs_params", "pairs_repl"])
+def test_oracle_shift_settings_match_reference_build(oracle, ref, case):
+    # auto-shift increment, cap, pivot threshold, trial limit and the clamp
+    # value away from their defaults (core/solver_config.hpp:81-100,
+    # ic_preconditioner.hpp:1377-1412): factor and solve bit for bit
+    import scipy.sparse as sp
+    if case == "cc6_params":
+        s = P.curlcurl(6)
+        A, b = s.A, s.b
+        kw = dict(auto_shift=1, shift_parameter=0.5, shift_increment=0.2, max_shift_value=3.0,
+                  min_diagonal_threshold=1e-3, max_shift_trials=3)
+    else:
+        A = P.CSR.from_scipy(sp.kron(sp.identity(300), np.array([[1.0, 1.0], [1.0, 1.0]])).tocsr())
+        b = P.uniform(3, A.n, -1.0, 1.0)
+        kw = (dict(shift_parameter=1.0, auto_shift=1, shift_increment=0.01, max_shift_value=1.02,
+                   max_shift_trials=10) if case == "pairs_params"
+              else dict(shift_parameter=1.0, zero_diagonal_replacement=1e-4))
+    kw = dict(use_abmc=1, save_residual_history=1, max_iterations=200, **kw)
+    fo = oracle.ic(A, oracle.config(**kw)).factor()
+    fr = ref.ic(A, ref.config(**kw)).factor()
+    for k in ("ordering", "L_values", "inv_diag"):
+        np.testing.assert_array_equal(fo[k], fr[k])
+    assert fo["actual_shift"] == fr["actual_shift"]
+    xo, ro = oracle.solve_iccg(A, b, oracle.config(**kw))
+    xr, rr = ref.solve_iccg(A, b, ref.config(**kw))
+    assert ro["iterations"] == rr["iterations"] and ro["actual_shift"] == rr["actual_shift"]
+    np.testing.assert_array_equal(ro["residual_history"], rr["residual_history"])
+    np.testing.assert_array_equal(xo, xr)
+
+
+@pytest.mark.parametrize("fn", ["solve_iccg", "solve_cocr"])
+def test_oracle_conjugate_matches_reference_build(oracle, ref, fn):
+    # conjugate=True: the Hermitian inner product in CG / COCR's dots
+    # (iterative_solver.hpp:284-322) with the IC(0) preconditioner, on a
+    # Hermitian positive definite system (Laplacian + a skew-symmetric
+    # imaginary part) and on a complex-symmetric one
+    import scipy.sparse as sp
+    L = P.laplace7(7).to_scipy()
+    S = sp.triu(L, 1) * 0.3j
+    for A in (P.CSR.from_scipy((L.astype(complex) + S - S.T).tocsr()),
+              P.CSR.from_scipy((L.astype(complex) + 0.4j * sp.identity(343)).tocsr())):
+        z = P.gaussian(9, 2 * A.n)
+        b = z[0::2] + 1j * z[1::2]
+        kw = dict(use_abmc=1, conjugate=1, save_residual_history=1, max_iterations=400)
+        xo, ro = getattr(oracle, fn)(A, b, oracle.config(**kw))
+        xr, rr = getattr(ref, fn)(A, b, ref.config(**kw))
+        assert ro["iterations"] == rr["iterations"] and ro["converged"] == rr["converged"]
+        np.testing.assert_array_equal(ro["residual_history"], rr["residual_history"])
+        np.testing.assert_array_equal(xo, xr)
