@@ -265,6 +265,40 @@ def test_norm_types_with_initial_guess_match_oracle(oracle, case, norm_type, cus
     np.testing.assert_allclose(hist[:m], ro["residual_history"][:m], rtol=1e-10)
 
 
+@pytest.mark.parametrize("system", ["hermitian", "complex_symmetric"])
+@pytest.mark.parametrize("method", ["iccg", "cocr"])
+def test_conjugate_with_ic_matches_oracle(oracle, system, method):
+    # conjugate=True (Hermitian dots) with the ABMC IC(0) preconditioner; the
+    # oracle is pinned to the reference build on these systems
+    # (test_oracle_golden.py).  Where the oracle converges: the full bar;
+    # where it does not (the unconjugated IC(0) of a Hermitian matrix is no
+    # Hermitian preconditioner): the early residual history and the best
+    # residual's true value
+    import scipy.sparse as sp
+    L = P.laplace7(7).to_scipy()
+    if system == "hermitian":
+        S = sp.triu(L, 1) * 0.3j
+        A = P.CSR.from_scipy((L.astype(complex) + S - S.T).tocsr())
+    else:
+        A = P.CSR.from_scipy((L.astype(complex) + 0.4j * sp.identity(343)).tocsr())
+    z = P.gaussian(9, 2 * A.n)
+    b = z[0::2] + 1j * z[1::2]
+    kw = dict(use_abmc=1, conjugate=1, save_residual_history=1, max_iterations=400)
+    cfg = _lib.default_config(**kw)
+    code = _lib.METHOD_ICCG if method == "iccg" else _lib.METHOD_ICCOCR
+    x, res, hist = _lib.IcHandle(A, cfg).solve(code, b, None, cfg)
+    fn = oracle.solve_iccg if method == "iccg" else oracle.solve_cocr
+    xo, ro = fn(A, b, oracle.config(**kw))
+    assert bool(res.converged) == ro["converged"]
+    np.testing.assert_allclose(hist[:10], ro["residual_history"][:10], rtol=1e-8)
+    if ro["converged"]:
+        assert abs(res.iterations - ro["iterations"]) <= it_tol(ro["iterations"])
+        assert rel(x, xo) < 1e-9
+    else:
+        t = np.linalg.norm(b - A.to_scipy() @ x) / np.linalg.norm(b)
+        assert abs(t - res.final_residual) <= 1e-8 * t
+
+
 def test_maxiter_zero():
     s = P.poisson(6)
     cfg = _lib.default_config(use_abmc=1, max_iterations=0)
