@@ -17,6 +17,12 @@ many solves).
                     classes, each class tuned on its largest system, and the
                     per-system choice for ssa_batch_create_each.
 
+Rows per wavefront are not a separate dimension: every kernel gives a
+wavefront 64 rows (SELL-64 slices of the SpMV, one row per lane in the
+sweeps, the rows of an ABMC block in consecutive lanes), and what a
+wavefront carries per row is the block width -- the right-hand sides per
+lane that autotune_width chooses.
+
 No reference counterpart: the reference takes abmc_block_size and
 abmc_num_colors from the caller and solves one vector per call.
 """
