@@ -115,11 +115,25 @@ __device__ inline void ctrl_beta_op(SolverState* st, const SolveParams& prm, T v
     if (st->iter >= st->maxit) stop(st, kMaxIter, 0, st->maxit, st->norm_r);
 }
 
+// Thread t adds partials t, t + B, t + 2B, ... (B threads) in index order,
+// then block_sum combines the threads.  Eight of a thread's partials are
+// loaded before they are added, so their loads are in flight together (the
+// additions, and so the bits, are the same as one at a time).
 template <typename T>
 __device__ inline T sum_partials(const T* __restrict__ p, int np, T* red)
 {
+    constexpr int U = 8;
+    const int B = int(blockDim.x);
     T s = zero<T>();
-    for (int i = threadIdx.x; i < np; i += blockDim.x) s = add(s, p[i]);
+    int i = int(threadIdx.x);
+    for (; i + (U - 1) * B < np; i += U * B) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = p[i + u * B];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s = add(s, v[u]);
+    }
+    for (; i < np; i += B) s = add(s, p[i]);
     return block_sum(s, red);
 }
 
