@@ -29,7 +29,10 @@ system run as the plain FP64 GPU path) with its time per right-hand side.
 `roofline`: the kernel with the largest share of an iteration, from an extra
 solve of the same block per kernel class in which HIP events on the
 library's stream bracket every launch of that class (ssa_ic_trace; these
-solves run eagerly, cut to `--trace-iterations` iterations): algorithmic
+solves run eagerly, over the timed solve's own iterations unless
+`--trace-iterations` caps them, so their launches are the timed solve's
+mix -- the early iterations, whose residual improves, also store the best
+iterate in the update kernel, the later ones rarely do): algorithmic
 bytes per launch / mean launch time, against the 8 TB/s HBM peak.
 `traffic` is the HBM bytes per launch that rocprofv3's FETCH_SIZE +
 WRITE_SIZE counters read for the same kernel, from the counter summary under
@@ -294,13 +297,17 @@ def run_gpu(name, steps, warmup, budget=None, trace_iterations=0):
         model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], s, w["method"] == "cocr",
                                     s // 2 if mixed else None)
     trace = {}
-    if trace_iterations > 0:
-        tcfg = _lib.default_config(**dict(kw, max_iterations=trace_iterations))
+    if trace_iterations != 0:
+        # the traced solve runs the timed solve's iterations (its mix of
+        # improving iterations, whose update also stores the best iterate)
+        # unless a cap is given
+        tits = kw.get("max_iterations", 1000) if trace_iterations < 0 else trace_iterations
+        tcfg = _lib.default_config(**dict(kw, max_iterations=tits))
 
         def one():
             x_d.zero_()
             return h.solve_device(code, b_d.data_ptr(), x_d.data_ptr(), tcfg).iterations
-        trace = trace_kernels(h, one, trace_iterations)
+        trace = trace_kernels(h, one, tits)
     out = {
         "workload": name, "desc": w["desc"], "method": w["method"], "n": A.n, "nnz": A.nnz,
         "nnz_l": info["nnz_l"], "colours": info["num_colors"],
@@ -514,14 +521,15 @@ def run_multi(system, steps, warmup, nrhs=16, trace_iterations=0, budget=None):
     sec = float(np.mean(times))
     its = [int(r.iterations) for r in res]
     trace = {}
-    if trace_iterations > 0:
-        tcfg = _lib.default_config(**dict(kw, precision=1, max_iterations=trace_iterations))
+    if trace_iterations != 0:
+        tits = kw.get("max_iterations", 1000) if trace_iterations < 0 else trace_iterations
+        tcfg = _lib.default_config(**dict(kw, precision=1, max_iterations=tits))
 
         def one():
             X_d.zero_()
             rr = hm.solve_multi_device(code, nrhs, B_d.data_ptr(), X_d.data_ptr(), tcfg)
             return max(r.iterations for r in rr)
-        trace = trace_kernels(hm, one, trace_iterations)
+        trace = trace_kernels(hm, one, tits)
     model = bytes_per_iteration(A.n, A.nnz, info["nnz_l"], 16, True, 8, nrhs)
     it_mean = float(np.mean(its))
     out = {
@@ -746,8 +754,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--budget", type=int, default=None,
                     help="iteration budget per solve instead of the workloads' own")
-    ap.add_argument("--trace-iterations", type=int, default=60,
-                    help="iterations of each traced solve (0: no in-solve kernel timing)")
+    ap.add_argument("--trace-iterations", type=int, default=-1,
+                    help="iterations of each traced solve (-1: the timed solve's own budget; "
+                         "0: no in-solve kernel timing)")
     ap.add_argument("--extras", default=None,
                     help="comma-separated extra workloads to run (default: all of them)")
     ap.add_argument("--detail", default=None, help="also write every figure as JSON to this path")
