@@ -81,23 +81,38 @@ struct DevBuf {
 };
 
 constexpr int kMaxGrid = 1024;  // vector kernels: partial count bound
-// sweep entries per batch: 16 for real, 8 for complex values, 8 in the
-// small-colour kernel (register budget of its large workgroup); with W
-// right-hand sides per lane at most 8, with at most 64 gathers (a
-// wavefront's limit of outstanding loads) and 512 bytes of gathered values
-// per lane in flight -- 256 bytes at W = 16 with 8-byte values: a batch of
-// four 16-value complex<float> rows took 202 VGPRs (2 waves per SIMD, so a
-// 600-workgroup colour ran in more than one round), two take 126-143 (3-4
-// waves; eddy78 colour kernel 32.5 -> 31.9 us).  Complex<double> keeps 2
-// (one entry a batch was slower: FP64 block 3.23 -> 3.29 s).  The layouts
-// pad their entry arrays by kSweepChunk.
+// sweep entries per batch with W right-hand sides per lane and `budget`
+// bytes of gathered values per lane in flight (1 to 8)
+template <typename P, int W, int budget> constexpr int chunk_of()
+{
+    constexpr int c = int(budget / (W * sizeof(P)));
+    return c > 8 ? 8 : c < 1 ? 1 : c;
+}
+// sweep entries per batch: 16 for real, 8 for complex values; with W
+// right-hand sides per lane at most 64 / W gathers and 512 bytes of
+// gathered values per lane in flight -- 256 bytes at W = 16 with 8-byte
+// values: a batch of four 16-value
+// complex<float> rows took 202 VGPRs (2 waves per SIMD, so a 600-workgroup
+// colour ran in more than one round), two take 126-143 (3-4 waves; eddy78
+// colour kernel 32.5 -> 31.9 us).  Complex<double> keeps 2 (one entry a
+// batch was slower: FP64 block 3.23 -> 3.29 s).  The layouts pad their
+// entry arrays by kSweepChunk.
 template <typename P, int W> constexpr int sweep_chunk()
 {
     if (W == 1) return sizeof(P) > 8 ? 8 : 16;
-    constexpr int budget = (W >= 16 && sizeof(P) == 8) ? 256 : 512;
-    constexpr int by_count = 64 / W, by_bytes = int(budget / (W * sizeof(P)));
-    constexpr int c = by_count < by_bytes ? by_count : by_bytes;
-    return c > 8 ? 8 : c < 1 ? 1 : c;
+    constexpr int c = chunk_of<P, W, (W >= 16 && sizeof(P) == 8) ? 256 : 512>();
+    return c < 64 / W ? c : 64 / W;
+}
+// Colours of at most kSmallColourSlices slices hold at most one wavefront
+// per SIMD (256 CUs x 4 SIMDs, a wavefront per slice): occupancy costs them
+// nothing, while each batch is a memory round trip on a lane's dependent
+// chain, so with several right-hand sides their batches take up to 1024
+// bytes of gathered values a lane (a lone wave's 256 VGPRs and its AGPRs;
+// 512 bytes for complex<double>, whose 8-row blocks spilled to scratch).
+constexpr index_t kSmallColourSlices = 1024;
+template <typename P, int W> constexpr int sweep_chunk_small()
+{
+    return W == 1 ? sweep_chunk<P, 1>() : chunk_of<P, W, (sizeof(P) > 8 ? 512 : 1024)>();
 }
 // right-hand sides one launch carries (larger blocks are split)
 constexpr int kMaxRhs = 16;
@@ -692,9 +707,17 @@ struct IcImpl final : IcBase {
         return BR >= 16 ? ssa::tail_threads<P, 16>() : ssa::tail_threads<P, 1>();
     }
 
+    // A run of small colours shares one workgroup, i.e. one CU's address
+    // rate: with several right-hand sides (a cache line per lane and entry)
+    // a colour joins only while its gathers are few -- the eddy78 16-vector
+    // forward tail, 0.8 MB of gathers, took 39 us in one workgroup and 21
+    // us as two launches of a wavefront per workgroup; the backward one, 0.1
+    // MB, 15 us against 19.
+    static constexpr double kTailGatherBytes = 128.0 * 1024.0;
     bool small_phase(const DevSweep& D, index_t ph) const
     {
-        return D.phase_big[ph + 1] == D.phase_big[ph] &&
+        return (W == 1 || double(D.phase_entries[ph]) * W * sizeof(P) <= kTailGatherBytes) &&
+               D.phase_big[ph + 1] == D.phase_big[ph] &&
                D.phase_slices[ph + 1] - D.phase_slices[ph] <= 2 * (tail_threads(D.BR) / 64);
     }
 
@@ -851,7 +874,11 @@ struct IcImpl final : IcBase {
     void tail_w(const ssa::TailPhases& tp, const ssa::SweepArrays& A, const P* in, P* out,
                 const SolverState* st)
     {
-        constexpr int CH = WV == 1 ? 8 : sweep_chunk<P, WV>();
+        // one workgroup: batches as wide as the register budget of its
+        // waves allows (256 threads: a wave a SIMD, as in small colours;
+        // 512: two, 512 bytes a lane)
+        constexpr int TT = ssa::tail_threads<P, BRV, WV>();
+        constexpr int CH = WV == 1 ? 8 : TT > 256 ? chunk_of<P, WV, 512>() : sweep_chunk_small<P, WV>();
         ssa::k_sweep_tail<P, F, BRV, FWD, CH, WV><<<1, ssa::tail_threads<P, BRV, WV>(), 0, lstream()>>>(
             tp, A, in, out, st);
     }
@@ -870,26 +897,37 @@ struct IcImpl final : IcBase {
     }
 
     // one colour's regular slices (and the external part of its oversized
-    // blocks) with WV right-hand sides per lane
+    // blocks, gw workgroups) with WV right-hand sides per lane.  A small
+    // colour of several right-hand sides runs a wavefront per workgroup:
+    // its gathers (W values of a row per lane, a cache line per lane) are
+    // bound by the address rate of the CUs they run on, so they go to four
+    // times as many CUs.
     template <bool FWD, int BRV, int WV>
-    void sweep_w(int g, index_t s0, index_t s1, index_t slot0, index_t stride,
+    void sweep_w(int gw, index_t s0, index_t s1, index_t slot0, index_t stride,
                  const ssa::SweepArrays& A, const P* in, P* out, const ssa::WalkExt& we,
                  const SolverState* st)
     {
+        const index_t ns = s1 - s0;
+        if (WV > 1 && gw == 0 && ns <= kSmallColourSlices) {
+            ssa::k_sweep<P, F, BRV, FWD, sweep_chunk_small<P, WV>(), WV, 1><<<int(ns), 64, 0, lstream()>>>(
+                s0, s1, slot0, stride, A, in, out, we, st);
+            return;
+        }
+        const int g = int((ns + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
         ssa::k_sweep<P, F, BRV, FWD, sweep_chunk<P, WV>(), WV><<<g, ssa::kBlock, 0, lstream()>>>(
             s0, s1, slot0, stride, A, in, out, we, st);
     }
 
     template <bool FWD, int BRV>
-    void sweep_br(int g, index_t s0, index_t s1, index_t slot0, index_t stride,
+    void sweep_br(int gw, index_t s0, index_t s1, index_t slot0, index_t stride,
                   const ssa::SweepArrays& A, const P* in, P* out, const ssa::WalkExt& we,
                   const SolverState* st)
     {
         switch (W) {
-        case 1: sweep_w<FWD, BRV, 1>(g, s0, s1, slot0, stride, A, in, out, we, st); break;
-        case 4: sweep_w<FWD, BRV, 4>(g, s0, s1, slot0, stride, A, in, out, we, st); break;
-        case 8: sweep_w<FWD, BRV, 8>(g, s0, s1, slot0, stride, A, in, out, we, st); break;
-        case 16: sweep_w<FWD, BRV, 16>(g, s0, s1, slot0, stride, A, in, out, we, st); break;
+        case 1: sweep_w<FWD, BRV, 1>(gw, s0, s1, slot0, stride, A, in, out, we, st); break;
+        case 4: sweep_w<FWD, BRV, 4>(gw, s0, s1, slot0, stride, A, in, out, we, st); break;
+        case 8: sweep_w<FWD, BRV, 8>(gw, s0, s1, slot0, stride, A, in, out, we, st); break;
+        case 16: sweep_w<FWD, BRV, 16>(gw, s0, s1, slot0, stride, A, in, out, we, st); break;
         default: throw SsaError(SSA_ERR_ARGUMENT, "unsupported block width");
         }
     }
@@ -931,13 +969,12 @@ struct IcImpl final : IcBase {
         if (part == kWalkOnly || part == kExtOnly) s1 = s0;   // no regular slices
         if (part == kWalkOnly || part == kRegularOnly) gw = 0;
         if (s1 > s0 || gw > 0) {
-            const int g = int((s1 - s0 + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
             const ssa::SweepArrays A = sweep_arrays(D);
             // timed as the colour kernel: colours without oversized blocks
             const int cls = b1 == b0 ? SSA_TRACE_COLOUR : -2;
             const double bytes = colour_bytes(D, ph);
 #define SSA_SWEEP(BRV) \
-    sweep_br<FWD, BRV>(g, s0, s1, D.phase_slot[ph], D.phase_stride[ph], A, in, out, we, st)
+    sweep_br<FWD, BRV>(gw, s0, s1, D.phase_slot[ph], D.phase_stride[ph], A, in, out, we, st)
             traced(cls, s, bytes, [&] {
                 switch (D.BR == 0 ? 1 : D.BR) {
                 case 1: SSA_SWEEP(1); break;

@@ -441,25 +441,27 @@ __device__ inline int all_stopped(const SolverState* __restrict__ st)
     return stopped;
 }
 
-// Slice s0 + k has its entries at slot slot0 + k * stride.
-template <typename P, typename F, int BR, bool FWD, int CH, int W>
-__global__ __launch_bounds__(kBlock) void k_sweep(index_t s0, index_t s1, index_t slot0,
-                                                  index_t stride, SweepArrays A,
-                                                  const P* __restrict__ in, P* out,
-                                                  WalkExt walk,
-                                                  const SolverState* __restrict__ st)
+// Slice s0 + k has its entries at slot slot0 + k * stride.  WPB slices
+// (wavefronts) per workgroup; the external part takes WPB * 64 rows a
+// workgroup.
+template <typename P, typename F, int BR, bool FWD, int CH, int W, int WPB = kWavesPerBlock>
+__global__ __launch_bounds__(WPB * 64) void k_sweep(index_t s0, index_t s1, index_t slot0,
+                                                    index_t stride, SweepArrays A,
+                                                    const P* __restrict__ in, P* out,
+                                                    WalkExt walk,
+                                                    const SolverState* __restrict__ st)
 {
-    const index_t nreg = (s1 - s0 + kWavesPerBlock - 1) / kWavesPerBlock;
+    const index_t nreg = (s1 - s0 + WPB - 1) / WPB;
     const index_t lb = index_t(W == 1 ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
     if (lb >= nreg) {
         // batches: computed for every system (the results of a stopped
         // system's sweep are never read)
         if (!A.slice_sys && all_stopped<W>(st)) return;
-        walk_external<P, F, FWD, W>(walk, walk.t0 + size_t(lb - nreg) * kBlock + threadIdx.x, in, out);
+        walk_external<P, F, FWD, W>(walk, walk.t0 + size_t(lb - nreg) * (WPB * 64) + threadIdx.x, in, out);
         return;
     }
-    const index_t wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    const index_t s = s0 + lb * kWavesPerBlock + wave;
+    const index_t wave = WPB == 1 ? 0 : __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const index_t s = s0 + lb * WPB + wave;
     if (s >= s1) return;                       // whole wavefront: no shuffle below
     const int stopped = all_stopped<W>(st + (A.slice_sys ? A.slice_sys[s] : 0));
     sweep_slice<P, F, BR, FWD, CH, W>(s, slot0 + (s - s0) * stride, stride, A, in, out, stopped);
