@@ -16,7 +16,7 @@
 //    colour -> blocks -> rows; blocks of one colour are independent, rows of
 //    a block are sequential.  Each row of a regular block (at most BR rows)
 //    gets a lane; the rows of a block sit in consecutive lanes of one slice,
-//    in processing order (ascending rows for the forward sweep over L,
+//    within one run of kBlockAlign lanes, in processing order (ascending rows for the forward sweep over L,
 //    descending for the backward sweep over L^T).  Blocks are sorted by
 //    their longest row and packed into 64-lane slices; a slice holds
 //    K = its longest row entry slots, laid out like SpmvLayout, and every
@@ -64,6 +64,10 @@ constexpr int walk_record_words(int kmax, int fs) { return ((kmax + 1) * fs + (2
 // entries a sweep kernel loads per batch; the entry arrays end with this
 // many padding slots so a batch never needs a bounds guard
 constexpr int kSweepChunk = 16;
+// the rows of a regular block (at most 16) never straddle a multiple of
+// kBlockAlign lanes of a slice: a sweep that gives a wavefront 16 rows of a
+// slice (several lanes a row) resolves every block within that wavefront
+constexpr int kBlockAlign = 16;
 // blocks a colour's longest-row sort reorders among: 16 blocks (64 rows of
 // 4-row blocks) keep the slices that run together spatially close
 // (64^3 curl-curl ICCG 72.0 -> 69.0 ms per 300 iterations, 48^3 Laplacian
@@ -323,6 +327,8 @@ struct SweepLayout {
                     const index_t b = keyed[q].second;
                     const index_t b0 = S.block_offsets[b], b1 = S.block_offsets[b + 1];
                     const int sz = int(b1 - b0);
+                    // a block never crosses a multiple of kBlockAlign lanes
+                    if (used % kBlockAlign + sz > kBlockAlign) used += kBlockAlign - used % kBlockAlign;
                     if (used + sz > kWave) break;
                     for (int r = 0; r < sz; ++r) {
                         sl.row[used + r] = forward ? b0 + r : b1 - 1 - r;

@@ -114,6 +114,21 @@ template <typename P, int W> constexpr int sweep_chunk_small()
 {
     return W == 1 ? sweep_chunk<P, 1>() : chunk_of<P, W, (sizeof(P) > 8 ? 512 : 1024)>();
 }
+// lanes a row in the split colour kernel: as many as the row has 16-byte
+// words, at most 4 (16 complex<float> values: 4 lanes of 32 bytes, one
+// 128-B line a row per load pair)
+template <typename P, int W> constexpr int split_lanes()
+{
+    constexpr int words = int(W * sizeof(P) / 16);
+    return words >= 4 ? 4 : words >= 2 ? 2 : 1;
+}
+// entries a batch in the split colour kernel: 256 bytes of a lane's values
+// (8 entries of 4 complex<float>: 140 VGPRs, 3 waves per SIMD; 512 bytes of
+// complex<double> took 254 VGPRs)
+template <typename P, int W> constexpr int split_chunk()
+{
+    return chunk_of<P, W / split_lanes<P, W>(), 256>();
+}
 // right-hand sides one launch carries (larger blocks are split)
 constexpr int kMaxRhs = 16;
 // the block width that carries nrhs right-hand sides: 1, 4, 8 or 16
@@ -897,21 +912,32 @@ struct IcImpl final : IcBase {
     }
 
     // one colour's regular slices (and the external part of its oversized
-    // blocks, gw workgroups) with WV right-hand sides per lane.  A small
-    // colour of several right-hand sides runs a wavefront per workgroup:
-    // its gathers (W values of a row per lane, a cache line per lane) are
-    // bound by the address rate of the CUs they run on, so they go to four
-    // times as many CUs.
+    // blocks, gw workgroups) with WV right-hand sides per lane.  Several
+    // right-hand sides without an external part take split_lanes lanes a
+    // row (k_sweep_split); a small colour of them runs a wavefront per
+    // workgroup, since its gathers are bound by the address rate of the CUs
+    // they run on.
     template <bool FWD, int BRV, int WV>
     void sweep_w(int gw, index_t s0, index_t s1, index_t slot0, index_t stride,
                  const ssa::SweepArrays& A, const P* in, P* out, const ssa::WalkExt& we,
                  const SolverState* st)
     {
         const index_t ns = s1 - s0;
-        if (WV > 1 && gw == 0 && ns <= kSmallColourSlices) {
-            ssa::k_sweep<P, F, BRV, FWD, sweep_chunk_small<P, WV>(), WV, 1><<<int(ns), 64, 0, lstream()>>>(
-                s0, s1, slot0, stride, A, in, out, we, st);
-            return;
+        if constexpr (WV > 1) {
+            if (gw == 0) {
+                constexpr int L = split_lanes<P, WV>();
+                constexpr int CH = split_chunk<P, WV>();
+                static_assert(64 / L % sparsesolv_amd::kBlockAlign == 0, "a wavefront's rows hold whole blocks");
+                const size_t waves = size_t(ns) * L;
+                if (ns <= kSmallColourSlices)
+                    ssa::k_sweep_split<P, F, BRV, FWD, CH, WV, L, 1><<<unsigned(waves), 64, 0, lstream()>>>(
+                        s0, s1, slot0, stride, A, in, out, st);
+                else
+                    ssa::k_sweep_split<P, F, BRV, FWD, CH, WV, L, ssa::kWavesPerBlock>
+                        <<<unsigned((waves + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock), ssa::kBlock, 0,
+                           lstream()>>>(s0, s1, slot0, stride, A, in, out, st);
+                return;
+            }
         }
         const int g = int((ns + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + gw;
         ssa::k_sweep<P, F, BRV, FWD, sweep_chunk<P, WV>(), WV><<<g, ssa::kBlock, 0, lstream()>>>(

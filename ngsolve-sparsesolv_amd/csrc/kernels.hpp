@@ -429,6 +429,165 @@ __device__ inline void sweep_slice(index_t s, index_t o0, index_t stride, const 
     }
 }
 
+// A row's W values as L lanes hold them: lane q of the row takes the
+// 16-byte words q, q + L, q + 2L, ... of the row (RL = W / L values), so
+// one load instruction reads 16 L contiguous bytes of every row it touches.
+template <typename P, int W, int L>
+__device__ inline void load_part(const P* __restrict__ row, int q, P (&d)[W / L])
+{
+    constexpr int kWords = int(W * sizeof(P) / 16) / L;
+    static_assert(kWords * L * 16 == int(W * sizeof(P)), "a row is L lanes of whole dwordx4 words");
+    uint4 w[kWords];
+    const uint4* p = reinterpret_cast<const uint4*>(row) + q;
+#pragma unroll
+    for (int k = 0; k < kWords; ++k) w[k] = p[k * L];
+    __builtin_memcpy(d, w, sizeof(w));
+}
+
+template <typename P, int W, int L>
+__device__ inline void store_part(P* row, int q, const P (&d)[W / L])
+{
+    constexpr int kWords = int(W * sizeof(P) / 16) / L;
+    uint4 w[kWords];
+    __builtin_memcpy(w, d, sizeof(w));
+    uint4* p = reinterpret_cast<uint4*>(row) + q;
+#pragma unroll
+    for (int k = 0; k < kWords; ++k) p[k * L] = w[k];
+}
+
+// sweep_slice with L lanes a row: wavefront `part` of the slice takes its
+// lanes part*64/L .. (part+1)*64/L - 1 (whole blocks: the layout keeps a
+// block within kBlockAlign = 16 lanes, and 64 / L is a multiple of 16),
+// lane rl * L + q of the wavefront holding row rl's words q, q + L, ...
+// With W values of a row in one lane, a gather instruction touches a cache
+// line per lane; here L lanes share one, so the CUs' address units move L
+// times the bytes per instruction.  Each right-hand side sees the same
+// operations in the same order as in sweep_slice (entries in slot order,
+// then the block's earlier positions), so the results are bit for bit.
+template <typename P, typename F, int BR, bool FWD, int CH, int W, int L>
+__device__ inline void sweep_slice_split(index_t s, int part, index_t o0, index_t stride,
+                                         const SweepArrays& A, const P* __restrict__ in, P* out,
+                                         int stopped)
+{
+    constexpr int RL = W / L, RPW = 64 / L;
+    const F* __restrict__ val = static_cast<const F*>(A.val);
+    const index_t* __restrict__ col = A.col;
+    const int lane = threadIdx.x & 63;
+    const int rl = lane / L, q = lane % L;
+    const int sl = part * RPW + rl;        // the row's lane in the slice layout
+    const size_t li = size_t(s) * 64 + sl;
+    const index_t row = A.lane_row[li];
+    const index_t info = A.lane_info[li];
+    index_t c[CH];
+    F v[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        c[j] = kPadColumn;
+        v[j] = zero<F>();
+        if (j < stride) {
+            const size_t at = (size_t(o0) + j) * 64 + sl;
+            c[j] = col[at];
+            v[j] = val[at];
+        }
+    }
+    if (stopped) {
+        if (info < 0) out[0] = cvt<P>(v[0]);   // keeps the loads above the test
+        return;
+    }
+    const int pos = info & 255;
+    const index_t o1 = o0 + __builtin_amdgcn_readfirstlane(info >> 8);
+    const bool live = row >= 0;
+    const index_t rr = live ? row : 0;
+    constexpr bool kLateIn = !FWD;
+    P xin[RL], acc[RL];
+    if (kLateIn) {
+#pragma unroll
+        for (int r = 0; r < RL; ++r) xin[r] = zero<P>();
+    } else {
+        load_part<P, W, L>(in + size_t(rr) * W, q, xin);
+    }
+#pragma unroll
+    for (int r = 0; r < RL; ++r) acc[r] = FWD ? xin[r] : zero<P>();
+    const P dinv = cvt<P>(static_cast<const F*>(A.invd)[rr]);
+    F intv[BR > 1 ? BR - 1 : 1];
+#pragma unroll
+    for (int j = 0; j < (BR > 1 ? BR - 1 : 1); ++j) intv[j] = zero<F>();
+    for (index_t k0 = o0;;) {
+        P g[CH][RL];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            if (k0 + j < o1) {
+                const index_t cj = c[j] >= 0 ? c[j] : 0;
+                load_part<P, W, L>(out + size_t(cj) * W, q, g[j]);
+            } else {
+                c[j] = kPadColumn;
+#pragma unroll
+                for (int r = 0; r < RL; ++r) g[j][r] = zero<P>();
+            }
+        }
+        const index_t k1 = k0 + CH;
+        index_t cn[CH];
+        F vn[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            cn[j] = kPadColumn;
+            vn[j] = zero<F>();
+        }
+        if (k1 < o1) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                if (k1 + j < o1) {
+                    const size_t at = (size_t(k1) + j) * 64 + sl;
+                    cn[j] = col[at];
+                    vn[j] = val[at];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const P vj = cvt<P>(v[j]);
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const P t = fms_(vj, g[j][r], acc[r]);
+                acc[r] = select_(c[j] >= 0, t, acc[r]);
+            }
+#pragma unroll
+            for (int p = 0; p < BR - 1; ++p) intv[p] = select_(c[j] == -1 - p, v[j], intv[p]);
+        }
+        k0 = k1;
+        if (k0 >= o1) break;
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            c[j] = cn[j];
+            v[j] = vn[j];
+        }
+    }
+    if (kLateIn) load_part<P, W, L>(in + size_t(rr) * W, q, xin);
+    const int base = rl - pos;
+    P res[RL];
+#pragma unroll
+    for (int r = 0; r < RL; ++r) res[r] = zero<P>();
+#pragma unroll
+    for (int j = 0; j < BR; ++j) {
+        if (live && pos == j) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) res[r] = FWD ? mul(acc[r], dinv) : fma_(acc[r], dinv, xin[r]);
+            store_part<P, W, L>(out + size_t(row) * W, q, res);
+        }
+        if (j < BR - 1) {
+            // the same right-hand sides (lane q) of position j's row;
+            // clamped for idle lanes and short blocks at the wavefront's end
+            const int src = min(base + j, RPW - 1) * L + q;
+            const P cj = cvt<P>(intv[j]);
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const P vj = shfl(res[r], src);
+                if (pos > j) acc[r] = fms_(cj, vj, acc[r]);
+            }
+        }
+    }
+}
+
 // One colour: slices s0..s1, four per workgroup; extra workgroups compute
 // the external part of the colour's oversized blocks.  W right-hand sides
 // per lane; the colour is skipped once all of them have stopped.
@@ -465,6 +624,23 @@ __global__ __launch_bounds__(WPB * 64) void k_sweep(index_t s0, index_t s1, inde
     if (s >= s1) return;                       // whole wavefront: no shuffle below
     const int stopped = all_stopped<W>(st + (A.slice_sys ? A.slice_sys[s] : 0));
     sweep_slice<P, F, BR, FWD, CH, W>(s, slot0 + (s - s0) * stride, stride, A, in, out, stopped);
+}
+
+// One colour of a block of W right-hand sides, L wavefronts a slice
+// (sweep_slice_split), WPB wavefronts a workgroup; no external part.
+template <typename P, typename F, int BR, bool FWD, int CH, int W, int L, int WPB>
+__global__ __launch_bounds__(WPB * 64) void k_sweep_split(index_t s0, index_t s1, index_t slot0,
+                                                          index_t stride, SweepArrays A,
+                                                          const P* __restrict__ in, P* out,
+                                                          const SolverState* __restrict__ st)
+{
+    const index_t wv = index_t(blockIdx.x) * WPB +
+                       (WPB == 1 ? 0 : __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)));
+    const index_t s = s0 + wv / L;
+    if (s >= s1) return;                       // whole wavefront: no shuffle below
+    const int stopped = all_stopped<W>(st + (A.slice_sys ? A.slice_sys[s] : 0));
+    sweep_slice_split<P, F, BR, FWD, CH, W, L>(s, int(wv % L), slot0 + (s - s0) * stride, stride, A,
+                                               in, out, stopped);
 }
 
 // Several consecutive small colours in one workgroup of NT threads: the

@@ -106,6 +106,9 @@ void emulate_sweep(const SweepLayout<T>& L, bool fwd, const std::vector<T>& invd
                 posv[l] = pos;
                 CHECK(row < n);
                 CHECK(pos >= 0 && pos < BR && l - pos >= 0);
+                // the block starts in this lane's run of kBlockAlign lanes
+                // (the split sweep resolves it within one wavefront)
+                CHECK((l - pos) / kBlockAlign == l / kBlockAlign);
                 const bool live = row >= 0;
                 xin[l] = live ? in[size_t(row)] : T(0);
                 dinv[l] = live ? invd[size_t(row)] : T(0);
