@@ -103,13 +103,16 @@ template <typename P, int W> constexpr int sweep_chunk()
     constexpr int c = chunk_of<P, W, (W >= 16 && sizeof(P) == 8) ? 256 : 512>();
     return c < 64 / W ? c : 64 / W;
 }
-// Colours of at most kSmallColourSlices slices hold at most one wavefront
-// per SIMD (256 CUs x 4 SIMDs, a wavefront per slice): occupancy costs them
-// nothing, while each batch is a memory round trip on a lane's dependent
-// chain, so with several right-hand sides their batches take up to 1024
-// bytes of gathered values a lane (a lone wave's 256 VGPRs and its AGPRs;
-// 512 bytes for complex<double>, whose 8-row blocks spilled to scratch).
+// A colour of at most kSmallColourSlices slices (256 CUs x 4 SIMDs) leaves
+// SIMDs idle: with several right-hand sides its wavefronts each get a
+// workgroup (and a CU's address unit) of their own.  Eddy78, 16 right-hand
+// sides: the forward colours of 210 and 52 slices 21 -> 14 and 19 -> 13 us
+// with a lane a row, 9.4 and 8.4 us with four lanes a row.
 constexpr index_t kSmallColourSlices = 1024;
+// entries a batch for a lone wavefront per SIMD (the small-colour tail at 256
+// threads): up to 1024 bytes of gathered values a lane (256 VGPRs and the
+// AGPRs; 512 bytes for complex<double>, whose 8-row blocks spilled to
+// scratch at 1024)
 template <typename P, int W> constexpr int sweep_chunk_small()
 {
     return W == 1 ? sweep_chunk<P, 1>() : chunk_of<P, W, (sizeof(P) > 8 ? 512 : 1024)>();
@@ -916,7 +919,9 @@ struct IcImpl final : IcBase {
     // right-hand sides without an external part take split_lanes lanes a
     // row (k_sweep_split); a small colour of them runs a wavefront per
     // workgroup, since its gathers are bound by the address rate of the CUs
-    // they run on.
+    // they run on.  A colour with an external part keeps one launch: the
+    // eddy78 forward one took 30.6 us, against 19.7 + 14.2 us as a split
+    // launch and an external-only one.
     template <bool FWD, int BRV, int WV>
     void sweep_w(int gw, index_t s0, index_t s1, index_t slot0, index_t stride,
                  const ssa::SweepArrays& A, const P* in, P* out, const ssa::WalkExt& we,
