@@ -125,12 +125,15 @@ template <typename P, int W> constexpr int split_lanes()
     constexpr int words = int(W * sizeof(P) / 16);
     return words >= 4 ? 4 : words >= 2 ? 2 : 1;
 }
-// entries a batch in the split colour kernel: 256 bytes of a lane's values
-// (8 entries of 4 complex<float>: 140 VGPRs, 3 waves per SIMD; 512 bytes of
-// complex<double> took 254 VGPRs)
+// entries a batch in the split colour kernel: 128 bytes of a lane's values
+// in flight.  Eddy78, 16 complex<float> right-hand sides: 8 entries a batch
+// (140 VGPRs, 3 waves per SIMD) 23.3 us per colour launch, 4 (84 VGPRs, 5
+// waves) 21.0 us, 2 (58 VGPRs, 8 waves) 21.2 us with slower small colours;
+// complex<double>, 2 entries, against 4 and 1: 49.2 against 50.4 and 51.1 us
+// in the forward colours.
 template <typename P, int W> constexpr int split_chunk()
 {
-    return chunk_of<P, W / split_lanes<P, W>(), 256>();
+    return chunk_of<P, W / split_lanes<P, W>(), 128>();
 }
 // right-hand sides one launch carries (larger blocks are split)
 constexpr int kMaxRhs = 16;
