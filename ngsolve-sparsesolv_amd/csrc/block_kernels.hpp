@@ -20,43 +20,6 @@
 
 namespace ssa {
 
-// a vector store with the non-temporal hint (streams not read again before
-// they have left the caches)
-template <typename T>
-__device__ inline void store_nt(T* p, const T& v)
-{
-    if constexpr (sizeof(T) == 16) {
-        typedef double v2d __attribute__((ext_vector_type(2)));
-        v2d w;
-        __builtin_memcpy(&w, &v, 16);
-        __builtin_nontemporal_store(w, reinterpret_cast<v2d*>(p));
-    } else if constexpr (sizeof(T) == 8) {
-        double w;
-        __builtin_memcpy(&w, &v, 8);
-        __builtin_nontemporal_store(w, reinterpret_cast<double*>(p));
-    } else {
-        *p = v;
-    }
-}
-
-// a vector load with the non-temporal hint
-template <typename T>
-__device__ inline T load_nt(const T* p)
-{
-    T v;
-    if constexpr (sizeof(T) == 16) {
-        typedef double v2d __attribute__((ext_vector_type(2)));
-        const v2d w = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
-        __builtin_memcpy(&v, &w, 16);
-    } else if constexpr (sizeof(T) == 8) {
-        const double w = __builtin_nontemporal_load(reinterpret_cast<const double*>(p));
-        __builtin_memcpy(&v, &w, 8);
-    } else {
-        v = *p;
-    }
-    return v;
-}
-
 // Reductions over a block keep the one-vector kernel's tree: in workgroup g,
 // thread t owns rows g*kBlock + t (stepping by G*kBlock) and adds their
 // contributions in that order, then block_sum combines the threads.  The
@@ -286,9 +249,9 @@ __global__ __launch_bounds__(kBlock) void k_cocr_update_blk(index_t n, T* __rest
             const T xe = load_nt(x + e);
             if (copy[h]) store_nt(best + e, xe);
             store_nt(x + e, fma_(alpha[h], pe, xe));
-            const T ri = fms_(alpha[h], q[e], load_nt(r + e));
+            const T ri = fms_(alpha[h], load_nt(q + e), load_nt(r + e));
             store_nt(r + e, ri);
-            rt[e] = cvt<TR>(fms_(alpha[h], cvt<T>(qt[e]), rte));
+            rt[e] = cvt<TR>(fms_(alpha[h], cvt<T>(load_nt(qt + e)), rte));
             return norm2(ri);
         });
     }
@@ -347,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_cocr_q_blk(size_t total, const T* __
     const T beta = from_pair<T>(st[k].beta);
     for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < total;
          e += size_t(gridDim.x) * kBlock) {
-        const T qi = fma_(beta, q[e], t[e]);
+        const T qi = fma_(beta, q[e], load_nt(t + e));
         store_nt(q + e, qi);
         if (q_low) q_low[e] = cvt<P>(qi);
     }

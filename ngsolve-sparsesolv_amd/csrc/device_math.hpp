@@ -26,6 +26,49 @@ template <> struct is_cplx<c64> { static constexpr bool value = true; };
 
 template <typename T> __host__ __device__ inline T zero() { return T{}; }
 
+// a vector store with the non-temporal hint (streams not read again before
+// they have left the caches)
+template <typename T>
+__device__ inline void store_nt(T* p, const T& v)
+{
+    if constexpr (sizeof(T) == 16) {
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        v2d w;
+        __builtin_memcpy(&w, &v, 16);
+        __builtin_nontemporal_store(w, reinterpret_cast<v2d*>(p));
+    } else if constexpr (sizeof(T) == 8) {
+        double w;
+        __builtin_memcpy(&w, &v, 8);
+        __builtin_nontemporal_store(w, reinterpret_cast<double*>(p));
+    } else {
+        static_assert(sizeof(T) == 4, "16-, 8- or 4-byte values");
+        int w;
+        __builtin_memcpy(&w, &v, 4);
+        __builtin_nontemporal_store(w, reinterpret_cast<int*>(p));
+    }
+}
+
+// a vector load with the non-temporal hint
+template <typename T>
+__device__ inline T load_nt(const T* p)
+{
+    T v;
+    if constexpr (sizeof(T) == 16) {
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        const v2d w = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
+        __builtin_memcpy(&v, &w, 16);
+    } else if constexpr (sizeof(T) == 8) {
+        const double w = __builtin_nontemporal_load(reinterpret_cast<const double*>(p));
+        __builtin_memcpy(&v, &w, 8);
+    } else {
+        static_assert(sizeof(T) == 4, "16-, 8- or 4-byte values");
+        const int w = __builtin_nontemporal_load(reinterpret_cast<const int*>(p));
+        __builtin_memcpy(&v, &w, 4);
+    }
+    return v;
+}
+
+
 // ---- conversions ------------------------------------------------------
 template <typename To, typename From> __device__ inline To cvt(const From& a) { return To(a); }
 template <> __device__ inline c128 cvt<c128, c128>(const c128& a) { return a; }
