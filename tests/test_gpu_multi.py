@@ -76,6 +76,21 @@ def test_multi_auto_shift_curlcurl_block16():
     check_against_singles(s.A, rhs_block(s.A.n, 5, 80), cfg, _lib.METHOD_ICCG)
 
 
+@pytest.mark.parametrize("case", ["curl6_b16_w16", "eddy6_b8_w16_mixed"])
+def test_multi_split_sweep_block_sizes(case):
+    # 16 right-hand sides: four lanes a row, 16 rows a wavefront, so 16-row
+    # blocks fill a wavefront and 8-row blocks pair up within one
+    if case == "curl6_b16_w16":
+        s = P.curlcurl(6)
+        cfg = _lib.default_config(use_abmc=1, auto_shift=1, abmc_block_size=16, max_iterations=400)
+        check_against_singles(s.A, rhs_block(s.A.n, 16, 85), cfg, _lib.METHOD_ICCG)
+    else:
+        s = P.eddy(6)
+        cfg = _lib.default_config(use_abmc=1, abmc_block_size=8, abmc_num_colors=6, max_iterations=600,
+                                  precision=_lib.PRECISION_MIXED)
+        check_against_singles(s.A, rhs_block(s.A.n, 16, 86, complex_=True), cfg, _lib.METHOD_ICCOCR)
+
+
 def test_multi_walk_of_oversized_block():
     # 48^3 Laplacian, B=4: one 366-row straggler block walked per right-hand side
     s = P.poisson(48)
