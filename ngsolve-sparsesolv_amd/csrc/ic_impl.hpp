@@ -919,12 +919,12 @@ struct IcImpl final : IcBase {
 
     // one colour's regular slices (and the external part of its oversized
     // blocks, gw workgroups) with WV right-hand sides per lane.  Several
-    // right-hand sides without an external part take split_lanes lanes a
-    // row (k_sweep_split); a small colour of them runs a wavefront per
-    // workgroup, since its gathers are bound by the address rate of the CUs
-    // they run on.  A colour with an external part keeps one launch: the
-    // eddy78 forward one took 30.6 us, against 19.7 + 14.2 us as a split
-    // launch and an external-only one.
+    // right-hand sides take split_lanes lanes a row (k_sweep_split, the
+    // external part in workgroups after the slices': the eddy78 forward
+    // colour with the oversized block 30.6 us with a lane a row, 24.2 us
+    // split, 19.7 + 14.2 us as two launches); a small colour of
+    // them without an external part runs a wavefront per workgroup, since
+    // its gathers are bound by the address rate of the CUs they run on.
     template <bool FWD, int BRV, int WV>
     void sweep_w(int gw, index_t s0, index_t s1, index_t slot0, index_t stride,
                  const ssa::SweepArrays& A, const P* in, P* out, const ssa::WalkExt& we,
@@ -932,18 +932,18 @@ struct IcImpl final : IcBase {
     {
         const index_t ns = s1 - s0;
         if constexpr (WV > 1) {
-            if (gw == 0) {
+            if (ns > 0) {
                 constexpr int L = split_lanes<P, WV>();
                 constexpr int CH = split_chunk<P, WV>();
                 static_assert(64 / L % sparsesolv_amd::kBlockAlign == 0, "a wavefront's rows hold whole blocks");
                 const size_t waves = size_t(ns) * L;
-                if (ns <= kSmallColourSlices)
+                if (gw == 0 && ns <= kSmallColourSlices)
                     ssa::k_sweep_split<P, F, BRV, FWD, CH, WV, L, 1><<<unsigned(waves), 64, 0, lstream()>>>(
-                        s0, s1, slot0, stride, A, in, out, st);
+                        s0, s1, slot0, stride, A, in, out, we, st);
                 else
                     ssa::k_sweep_split<P, F, BRV, FWD, CH, WV, L, ssa::kWavesPerBlock>
-                        <<<unsigned((waves + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock), ssa::kBlock, 0,
-                           lstream()>>>(s0, s1, slot0, stride, A, in, out, st);
+                        <<<unsigned((waves + ssa::kWavesPerBlock - 1) / ssa::kWavesPerBlock) + unsigned(gw),
+                           ssa::kBlock, 0, lstream()>>>(s0, s1, slot0, stride, A, in, out, we, st);
                 return;
             }
         }

@@ -627,13 +627,23 @@ __global__ __launch_bounds__(WPB * 64) void k_sweep(index_t s0, index_t s1, inde
 }
 
 // One colour of a block of W right-hand sides, L wavefronts a slice
-// (sweep_slice_split), WPB wavefronts a workgroup; no external part.
+// (sweep_slice_split), WPB wavefronts a workgroup; workgroups past the
+// slices' compute the external part of the colour's oversized blocks
+// (WPB * 64 rows a workgroup).
 template <typename P, typename F, int BR, bool FWD, int CH, int W, int L, int WPB>
 __global__ __launch_bounds__(WPB * 64) void k_sweep_split(index_t s0, index_t s1, index_t slot0,
                                                           index_t stride, SweepArrays A,
                                                           const P* __restrict__ in, P* out,
+                                                          WalkExt walk,
                                                           const SolverState* __restrict__ st)
 {
+    const index_t nreg = ((s1 - s0) * L + WPB - 1) / WPB;
+    if (index_t(blockIdx.x) >= nreg) {
+        if (!A.slice_sys && all_stopped<W>(st)) return;
+        walk_external<P, F, FWD, W>(walk, walk.t0 + size_t(index_t(blockIdx.x) - nreg) * (WPB * 64) + threadIdx.x,
+                                    in, out);
+        return;
+    }
     const index_t wv = index_t(blockIdx.x) * WPB +
                        (WPB == 1 ? 0 : __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)));
     const index_t s = s0 + wv / L;
