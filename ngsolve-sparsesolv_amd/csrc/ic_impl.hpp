@@ -135,6 +135,13 @@ template <typename P, int W> constexpr int split_chunk()
 {
     return chunk_of<P, W / split_lanes<P, W>(), 128>();
 }
+// One-vector SpMV and sweeps load the matrix / factor entries with the
+// non-temporal hint when they stream more than this (a quarter of the 256 MB
+// last-level cache): eddy78 (356 MB of A, 192 MB of factor a sweep) one-
+// vector COCR 0.472 -> 0.423 s, the 64-system batch 0.507 -> 0.483 s; the
+// 48^3 Laplacian (9 MB, cached from iteration to iteration) 0.0109 ->
+// 0.0115 s and its SGS-MRTR 0.049 -> 0.057 s with the hint.
+constexpr size_t kStreamNtBytes = size_t(64) << 20;
 // right-hand sides one launch carries (larger blocks are split)
 constexpr int kMaxRhs = 16;
 // the block width that carries nrhs right-hand sides: 1, 4, 8 or 16
@@ -700,16 +707,17 @@ struct IcImpl final : IcBase {
         const index_t* col = A_col.as<index_t>();
         const index_t* rows = A_row.as<index_t>();
         const T* val = A_val.as<T>();
+        const int nt = WV == 1 && A_val.bytes + A_col.bytes > kStreamNtBytes ? 1 : 0;
         // read x once, write y, the matrix entries and row pointers once
         const double bytes = double(nnz) * (sizeof(T) + 4.0) + 4.0 * (double(n) + 1.0) +
                              double(n) * WV * (sizeof(TX) + sizeof(T));
         traced(SSA_TRACE_SPMV, s, bytes, [&] {
             if (dot == 0)
-                ssa::k_spmv<T, 0, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+                ssa::k_spmv<T, 0, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy, nt);
             else if (dot == 1)
-                ssa::k_spmv<T, 1, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+                ssa::k_spmv<T, 1, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy, nt);
             else
-                ssa::k_spmv<T, 2, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy);
+                ssa::k_spmv<T, 2, WV, TX><<<g, ssa::kBlock, 0, s>>>(off, val, col, rows, A_nslices, xin, y, part, st, ctrl, sy, nt);
         });
     }
 
@@ -717,7 +725,8 @@ struct IcImpl final : IcBase {
     {
         return ssa::SweepArrays{D.lrow.as<index_t>(), D.linfo.as<index_t>(),
                                 D.val.p, D.col.as<index_t>(), invd.p,
-                                D.batched ? D.slice_sys.as<int>() : nullptr};
+                                D.batched ? D.slice_sys.as<int>() : nullptr,
+                                W == 1 && D.val.bytes + D.col.bytes > kStreamNtBytes ? 1 : 0};
     }
 
     // a colour small enough to share a single-workgroup kernel with its

@@ -10,6 +10,8 @@
 // workgroup (control.hpp).
 #pragma once
 
+#include <type_traits>
+
 #include "control.hpp"
 #include "device_math.hpp"
 #include "solver_state.hpp"
@@ -109,7 +111,8 @@ __device__ inline T block_sum_multi(const T (&v)[W], T* scratch)
 // and applies it to all W vectors; the partials of right-hand side r are
 // [r*gridDim.x, (r+1)*gridDim.x).  Right-hand sides that have stopped are
 // still multiplied (their results are not read).  The fused convergence
-// check (`check`) is for one vector only.  x may be stored narrower (TX:
+// check (`check`) is for one vector only.  With `stream_nt` (one vector) the
+// matrix entries load with the non-temporal hint.  x may be stored narrower (TX:
 // the preconditioner's precision); its values are widened exactly before
 // they are multiplied, so the products are those of the widened vector.
 // ---------------------------------------------------------------------------
@@ -118,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const index_t* __restrict__ slice_off, const T* __restrict__ val,
     const index_t* __restrict__ col, const index_t* __restrict__ rows, index_t nslices,
     const TX* __restrict__ x, T* __restrict__ y, T* __restrict__ partials,
-    const SolverState* __restrict__ st, Ctrl check, Systems sy)
+    const SolverState* __restrict__ st, Ctrl check, Systems sy, int stream_nt)
 {
     const unsigned lb = W == 1 ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const int sys = sy.spmv_sys ? sy.spmv_sys[lb] : 0;
@@ -145,15 +148,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 #pragma unroll
         for (int r = 0; r < W; ++r) acc[r] = zero<T>();
         if constexpr (W == 1) {
+            // two copies of the loop, so the hinted and the plain loads are
+            // never merged into one
+            const auto rows_loop = [&](auto nt) {
 #pragma unroll 4
-            for (index_t k = o0; k < o1; ++k) {
-                const size_t at = size_t(k) * 64 + lane;
-                const T v = val[at];
-                TX g[W];
-                load_row<TX, W>(x + size_t(col[at]) * W, g);
-#pragma unroll
-                for (int r = 0; r < W; ++r) acc[r] = fma_(v, cvt<T>(g[r]), acc[r]);
-            }
+                for (index_t k = o0; k < o1; ++k) {
+                    const size_t at = size_t(k) * 64 + lane;
+                    const T v = nt.value ? load_nt(val + at) : val[at];
+                    const index_t c = nt.value ? load_nt(col + at) : col[at];
+                    acc[0] = fma_(v, cvt<T>(x[c]), acc[0]);
+                }
+            };
+            if (stream_nt) rows_loop(std::true_type{});
+            else rows_loop(std::false_type{});
         } else {
             // one entry's W gathered values fill the registers: the next
             // entry's column and value load while this entry's gathers are
@@ -276,6 +283,7 @@ struct SweepArrays {
     const index_t* col;
     const void* invd;
     const int* slice_sys;   // batches: system of every slice (null: one system)
+    int stream_nt;          // entries loaded with the non-temporal hint
 };
 
 // One wavefront, one slice, whose entries start at slot o0 (wavefront-
@@ -287,7 +295,7 @@ struct SweepArrays {
 // first K entries are gathered.  W right-hand sides (interleaved rows):
 // each entry is loaded once and applied to all of them; CH entries per
 // batch.
-template <typename P, typename F, int BR, bool FWD, int CH, int W>
+template <typename P, typename F, int BR, bool FWD, int CH, int W, bool NT = false>
 __device__ inline void sweep_slice(index_t s, index_t o0, index_t stride, const SweepArrays& A,
                                    const P* __restrict__ in, P* out, int stopped)
 {
@@ -306,8 +314,8 @@ __device__ inline void sweep_slice(index_t s, index_t o0, index_t stride, const 
         v[j] = zero<F>();
         if (j < stride) {
             const size_t at = (size_t(o0) + j) * 64 + lane;
-            c[j] = col[at];
-            v[j] = val[at];
+            c[j] = NT ? load_nt(col + at) : col[at];
+            v[j] = NT ? load_nt(val + at) : val[at];
         }
     }
     if (stopped) {
@@ -371,8 +379,8 @@ __device__ inline void sweep_slice(index_t s, index_t o0, index_t stride, const 
             for (int j = 0; j < CH; ++j) {
                 if (k1 + j < o1) {
                     const size_t at = (size_t(k1) + j) * 64 + lane;
-                    cn[j] = col[at];
-                    vn[j] = val[at];
+                    cn[j] = NT ? load_nt(col + at) : col[at];
+                    vn[j] = NT ? load_nt(val + at) : val[at];
                 }
             }
         }
@@ -398,8 +406,8 @@ __device__ inline void sweep_slice(index_t s, index_t o0, index_t stride, const 
                 v[j] = vn[j];
             } else if (k0 + j < o1) {
                 const size_t at = (size_t(k0) + j) * 64 + lane;
-                c[j] = col[at];
-                v[j] = val[at];
+                c[j] = NT ? load_nt(col + at) : col[at];
+                v[j] = NT ? load_nt(val + at) : val[at];
             }
         }
     }
@@ -623,7 +631,10 @@ __global__ __launch_bounds__(WPB * 64) void k_sweep(index_t s0, index_t s1, inde
     const index_t s = s0 + lb * WPB + wave;
     if (s >= s1) return;                       // whole wavefront: no shuffle below
     const int stopped = all_stopped<W>(st + (A.slice_sys ? A.slice_sys[s] : 0));
-    sweep_slice<P, F, BR, FWD, CH, W>(s, slot0 + (s - s0) * stride, stride, A, in, out, stopped);
+    if (A.stream_nt)
+        sweep_slice<P, F, BR, FWD, CH, W, true>(s, slot0 + (s - s0) * stride, stride, A, in, out, stopped);
+    else
+        sweep_slice<P, F, BR, FWD, CH, W>(s, slot0 + (s - s0) * stride, stride, A, in, out, stopped);
 }
 
 // One colour of a block of W right-hand sides, L wavefronts a slice
@@ -688,8 +699,12 @@ __global__ __launch_bounds__((tail_threads<P, BR, W>())) void k_sweep_tail(TailP
     for (int ph = 0; ph < T.count; ++ph) {
         const index_t sp = T.slices[ph], stride = T.stride[ph];
         for (index_t s = sp + w; s < T.slices[ph + 1]; s += kWaves)
-            sweep_slice<P, F, BR, FWD, CH, W>(s, T.slot[ph] + (s - sp) * stride, stride, A, in, out,
-                                              A.slice_sys ? st[A.slice_sys[s]].status : 0);
+            if (A.stream_nt)
+                sweep_slice<P, F, BR, FWD, CH, W, true>(s, T.slot[ph] + (s - sp) * stride, stride, A, in, out,
+                                                        A.slice_sys ? st[A.slice_sys[s]].status : 0);
+            else
+                sweep_slice<P, F, BR, FWD, CH, W>(s, T.slot[ph] + (s - sp) * stride, stride, A, in, out,
+                                                  A.slice_sys ? st[A.slice_sys[s]].status : 0);
         __syncthreads();
     }
 }
