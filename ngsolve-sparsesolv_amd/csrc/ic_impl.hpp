@@ -91,12 +91,12 @@ template <typename P, int W, int budget> constexpr int chunk_of()
 // sweep entries per batch: 16 for real, 8 for complex values; with W
 // right-hand sides per lane at most 64 / W gathers and 512 bytes of
 // gathered values per lane in flight -- 256 bytes at W = 16 with 8-byte
-// values: a batch of four 16-value
-// complex<float> rows took 202 VGPRs (2 waves per SIMD, so a 600-workgroup
-// colour ran in more than one round), two take 126-143 (3-4 waves; eddy78
-// colour kernel 32.5 -> 31.9 us).  Complex<double> keeps 2 (one entry a
-// batch was slower: FP64 block 3.23 -> 3.29 s).  The layouts pad their
-// entry arrays by kSweepChunk.
+// values (the lane-a-row colour kernel, which blocks of right-hand sides
+// now run only for an external part alone; when it ran their colours, a
+// batch of four 16-value complex<float> rows took 202 VGPRs, 2 waves per
+// SIMD, and two 126-143, 3-4 waves: eddy78 colour kernel 32.5 -> 31.9 us;
+// complex<double> with one entry a batch: FP64 block 3.23 -> 3.29 s).  The
+// layouts pad their entry arrays by kSweepChunk.
 template <typename P, int W> constexpr int sweep_chunk()
 {
     if (W == 1) return sizeof(P) > 8 ? 8 : 16;
