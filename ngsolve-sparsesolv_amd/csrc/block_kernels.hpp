@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_cocr_q_blk(size_t total, const T* __
     const T beta = from_pair<T>(st[k].beta);
     for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < total;
          e += size_t(gridDim.x) * kBlock) {
-        const T qi = fma_(beta, q[e], load_nt(t + e));
+        const T qi = fma_(beta, load_nt(q + e), load_nt(t + e));
         store_nt(q + e, qi);
         if (q_low) q_low[e] = cvt<P>(qi);
     }
